@@ -1,0 +1,314 @@
+/*
+ * ttdft_chfsi.h -- C ABI of the MI355X ChFSI engine (libttdft_chfsi.so).
+ *
+ * Drop-in boundary for the reference's Chebyshev-filtered subspace iteration
+ * path.  Every entry point takes plain pointers and sizes; nothing here names
+ * a torch, PETSc or HIP type.  Each declaration cites the reference interface
+ * it replaces (paths relative to the reference repository).
+ *
+ * Conventions
+ *   - Status: every function returns int; 0 = success, negative = error class
+ *     (TTDFT_E*).  The message of the last failure on the calling thread is
+ *     returned by ttdft_last_error().  No entry point aborts the process
+ *     (the reference prints and calls std::terminate, MultCuda.cu:57-60).
+ *   - Context: one ttdft_ctx = one device + one HIP stream.  Calls on a
+ *     context are stream-ordered and not thread-safe; distinct contexts are
+ *     independent.  Functions that return host-visible results synchronise
+ *     the stream before returning.
+ *   - Wavefunction blocks ("X blocks") live in device memory and are stored
+ *     basis-major: X[i*ld + j] is the coefficient of basis function i in
+ *     state j, ld >= k.  (The reference stores the same block column-major
+ *     in a PETSc MATDENSE; ttdft_block_upload_colmajor converts.)  Element
+ *     type is TTDFT_F64 (double) or TTDFT_F32 (float).
+ *   - Basis index I = x + Rx*y + Rx*Ry*z (ProjectHamiltonianSparse.cc:420-433).
+ *   - Small k x k matrices are column-major, FP64.
+ *   - 1-D basis tables are column-major (points x functions), FP64, device.
+ *   - 3-D grids are x-fastest, FP64, device: rho[qx + Px*qy + Px*Py*qz].
+ */
+#ifndef TTDFT_CHFSI_H
+#define TTDFT_CHFSI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes, dtypes, modes ---------------------------------------- */
+enum {
+    TTDFT_OK = 0,
+    TTDFT_EBADARG = -1,   /* invalid argument (null pointer, size, mode)   */
+    TTDFT_EHIP = -2,      /* HIP runtime / kernel launch failure           */
+    TTDFT_ENOTSPD = -3,   /* overlap matrix not positive definite          */
+    TTDFT_ENOCONV = -4,   /* dense eigensolver did not converge            */
+    TTDFT_ENOMEM = -5,    /* device allocation failed                      */
+    TTDFT_ELIB = -6       /* rocBLAS / rocSOLVER failure                   */
+};
+
+enum { TTDFT_F64 = 0, TTDFT_F32 = 1 };
+
+/* Dense contraction modes for the Gram / projection / rotation GEMMs. */
+enum {
+    TTDFT_GEMM_F64 = 0,    /* FP64 data, FP64 MFMA                              */
+    TTDFT_GEMM_BF16X3 = 1  /* FP32 data, split-BF16 3-term MFMA, FP32 accumulate */
+};
+
+/* Which Chebyshev iterate ttdft_cheb_filter returns. */
+enum {
+    TTDFT_DEGREE_REFERENCE = 0, /* Y_{m-1}: what ChebyShevFilter.cc:163-176 writes back */
+    TTDFT_DEGREE_EXACT = 1      /* Y_m: textbook degree-m filter                       */
+};
+
+/* Which device copies ttdft_ham_upload keeps (bit mask). */
+enum { TTDFT_HAM_F64 = 1, TTDFT_HAM_F32 = 2 };
+
+/* Kernel classes whose launch times the context records when profiling. */
+enum {
+    TTDFT_KSTAT_SPMM = 0,    /* H_loc SpMM with fused recurrence epilogue */
+    TTDFT_KSTAT_NLPROJ = 1,  /* W = D C X (nonlocal projection)           */
+    TTDFT_KSTAT_GRAM = 2,    /* X^T Y                                     */
+    TTDFT_KSTAT_ROTATE = 3,  /* X Q                                       */
+    TTDFT_KSTAT_DENSITY = 4, /* fused density contraction                 */
+    TTDFT_KSTAT_COUNT = 5
+};
+
+typedef struct ttdft_ctx_s *ttdft_ctx;
+typedef struct ttdft_ham_s *ttdft_ham;
+
+/* ---- errors ---------------------------------------------------------------- */
+/* Message of the last failure on this thread ("" if none). */
+const char *ttdft_last_error(void);
+
+/* ---- device / context ------------------------------------------------------ */
+/* replaces device_get_device_count, eigensolver/DeviceUtils.cu:26-28 */
+int ttdft_get_device_count(int *count);
+
+/* Binds `device`, creates a stream and the dense-solver handle.
+ * replaces device_set_device (DeviceUtils.cu:30-31), the device binding of
+ * DeviceCommUtils (DeviceCommUtils.cc:69-71) and MultTM::MultTM (MultTm.cu:31-84) */
+int ttdft_ctx_create(int device, ttdft_ctx *out);
+/* replaces MultTM::~MultTM (MultTm.cu:469-473) */
+int ttdft_ctx_destroy(ttdft_ctx ctx);
+/* Make the context issue its work on an existing HIP stream (e.g. the
+ * caller's); NULL restores the context's own stream. */
+int ttdft_ctx_set_stream(ttdft_ctx ctx, void *hip_stream);
+int ttdft_ctx_synchronize(ttdft_ctx ctx);
+/* Enable per-launch HIP-event timing of the kernel classes TTDFT_KSTAT_*. */
+int ttdft_ctx_set_profiling(ttdft_ctx ctx, int enable);
+/* Launch count, summed milliseconds and algorithmic bytes / flops recorded
+ * for one kernel class since the last reset (synchronises the stream). */
+int ttdft_ctx_kernel_stats(ttdft_ctx ctx, int kernel_class, int64_t *launches,
+                           double *total_ms, double *bytes, double *flops);
+int ttdft_ctx_reset_stats(ttdft_ctx ctx);
+
+/* ---- device memory helpers (for callers without their own allocator) ------ */
+int ttdft_malloc(ttdft_ctx ctx, size_t bytes, void **dptr);
+int ttdft_free(ttdft_ctx ctx, void *dptr);
+int ttdft_memcpy_h2d(ttdft_ctx ctx, void *dst, const void *src, size_t bytes);
+int ttdft_memcpy_d2h(ttdft_ctx ctx, void *dst, const void *src, size_t bytes);
+int ttdft_memcpy_d2d(ttdft_ctx ctx, void *dst, const void *src, size_t bytes);
+
+/* Upload a host column-major n x k FP64 block (PETSc MATDENSE layout, lda)
+ * into a device basis-major block of dtype `dtype` (ld >= k). */
+int ttdft_block_upload_colmajor(ttdft_ctx ctx, const double *host, int64_t lda,
+                                int64_t n, int32_t k, void *dX, int64_t ldx,
+                                int dtype);
+/* Inverse of the above: device basis-major block -> host column-major FP64. */
+int ttdft_block_download_colmajor(ttdft_ctx ctx, const void *dX, int64_t ldx,
+                                  int dtype, int64_t n, int32_t k, double *host,
+                                  int64_t lda);
+/* Element-type conversion of a device block (FP64 <-> FP32; same dtype copies). */
+int ttdft_block_convert(ttdft_ctx ctx, const void *src, int64_t lds, int src_dtype,
+                        void *dst, int64_t ldd, int dst_dtype, int64_t n, int32_t k);
+
+/* ---- Hamiltonian ------------------------------------------------------------ */
+/* Upload H = H_loc + C^T D C.
+ *   H_loc: n x n CSR (host arrays; rowptr int64[n+1], colind int32[nnz],
+ *          val double[nnz]); must be symmetric (as the reference requires,
+ *          Mult.h:37-42).
+ *   C    : p x n CSR of the projectors, NOT row-scaled; D = diag(inv_c_lm).
+ *          p = 0 means no nonlocal part (pointers may be NULL).
+ *   copies: TTDFT_HAM_F64 | TTDFT_HAM_F32 (device copies to keep).
+ * replaces AX::setup_system + setup_mata_on_device (MultCuda.cu:105-371) and
+ * the C_nloc / C_nloc_trans / 1/C_lm scaling of Create_Cnloc
+ * (dft/ProjectHamiltonianSparse.cc:316-325). */
+int ttdft_ham_upload(ttdft_ctx ctx, int64_t n, const int64_t *rowptr,
+                     const int32_t *colind, const double *val, int32_t p,
+                     const int64_t *c_rowptr, const int32_t *c_colind,
+                     const double *c_val, const double *inv_c_lm, int copies,
+                     ttdft_ham *out);
+/* replaces AX::free_mata_on_device / AX::~AX (MultCuda.cu:673-717) */
+int ttdft_ham_free(ttdft_ham ham);
+/* Basis size n, H_loc nnz, projector count p. */
+int ttdft_ham_info(ttdft_ham ham, int64_t *n, int64_t *nnz, int32_t *p);
+
+/* Y = H X over k states (device blocks of dtype `dtype`).
+ * replaces AX::perform_ax (MultCuda.cu:719-820) together with the two
+ * MatMatMult nonlocal products at ChebyShevFilter.cc:103-114 and
+ * main.cc:1079-1092; unlike the published perform_ax it writes Y. */
+int ttdft_ham_apply(ttdft_ctx ctx, ttdft_ham ham, const void *X, int64_t ldx,
+                    void *Y, int64_t ldy, int32_t k, int dtype);
+
+/* In-place Chebyshev filter of the k states of X (device, dtype).
+ * Filter interval [a, b], a0 = lowest wanted bound, degree m, the columns
+ * processed in num_blocks chunks, `degree_convention` TTDFT_DEGREE_*.
+ * Needs k*ldx*sizeof(dtype) bytes of context scratch.
+ * replaces ChebyShevFilter::computeFilteredSubspace (ChebyShevFilter.h:31-39,
+ * ChebyShevFilter.cc:34-188). */
+int ttdft_cheb_filter(ttdft_ctx ctx, ttdft_ham ham, void *X, int64_t ldx,
+                      int32_t k, int32_t m, double a, double b, double a0,
+                      int32_t num_blocks, int dtype, int degree_convention);
+
+/* ---- dense subspace operations -------------------------------------------- */
+/* S = X^T Y (k x k, column-major FP64, device).  X, Y: n x k device blocks.
+ * mode TTDFT_GEMM_F64 needs dtype F64; TTDFT_GEMM_BF16X3 needs dtype F32.
+ * replaces MultTM::mult (MultTm.h:35, MultTm.cu:86-224). */
+int ttdft_gram(ttdft_ctx ctx, const void *X, int64_t ldx, const void *Y,
+               int64_t ldy, int64_t n, int32_t k, int dtype, int mode,
+               double *dS);
+
+/* Cholesky-QR in place: S = X^T X = L L^T (FP64 on the device),
+ * X <- X L^{-T}.  replaces MultTM::orth (MultTm.h:37, MultTm.cu:226-467). */
+int ttdft_orth(ttdft_ctx ctx, void *X, int64_t ldx, int64_t n, int32_t k,
+               int dtype, int mode);
+
+/* All eigenpairs of the symmetric k x k matrix dA (device, column-major, lower
+ * triangle referenced; dA is not modified), ascending: d_evals (device, k),
+ * dQ (device, k x k column-major, column i <-> evals[i]).
+ * replaces EigenSolver::computeEigenPairsSeq (EigenSolver.h:33-39,
+ * EigenSolver.cc:30-184). */
+int ttdft_rr_eig(ttdft_ctx ctx, const double *dA, int32_t k, double *d_evals,
+                 double *dQ);
+
+/* X <- X Q in place (Q: k x k column-major FP64, device).
+ * replaces the rotation of main.cc:1185-1207. */
+int ttdft_rotate(ttdft_ctx ctx, void *X, int64_t ldx, int64_t n, int32_t k,
+                 const double *dQ, int dtype, int mode);
+
+/* Lanczos spectrum bounds with `maxit` steps, start vector drawn from the
+ * engine's seeded generator (splitmix64 + xoshiro256**, uniform [0,1)).
+ * upper = |lambda_max(T)| + ||f||, lower = lambda_min(T).
+ * replaces EigenSolver::computeUpperBoundWithLanczos (EigenSolver.h:41-45,
+ * EigenSolver.cc:186-388). */
+int ttdft_lanczos_bounds(ttdft_ctx ctx, ttdft_ham ham, int32_t maxit,
+                         uint64_t seed, double *upper, double *lower);
+
+/* ---- occupancies ------------------------------------------------------------ */
+/* Fermi level (Newton, tol 1e-8, <= maxit steps) and Fermi-Dirac occupancies
+ * for k eigenvalues (host arrays).
+ * replaces KSDFTEnergyFunctional::computeFermiEnergy
+ * (dft/KSDFTEnergyFunctional.cc:76-124) and main.cc:1239-1271. */
+int ttdft_fermi_occupancy(const double *evals, int32_t k, int32_t nelec,
+                          double kb, double temperature, int32_t maxit,
+                          double *fermi, double *occ);
+/* Band energy sum_i 2 f_i e_i, main.cc:1315-1318. */
+double ttdft_band_energy(const double *evals, const double *occ, int32_t k);
+
+/* ---- density ---------------------------------------------------------------- */
+/* rho = sum_i 2 occ_i psi_i^2 on the px x py x pz tensor grid of 1-D points,
+ * psi_i = sum_abc X[abc][i] Bx[qx,a] By[qy,b] Bz[qz,c]; occ is a host array,
+ * Bx/By/Bz device (px x rx, ... column-major), rho device (written).
+ * replaces ProjectHamiltonianSparse::computeRhoOut
+ * (dft/ProjectHamiltonianSparse.cc:911-1221, direct form of :956-983,1081-1104)
+ * and the RhoComputer stub (dft/RhoComputer.cc:28-53). */
+int ttdft_density(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype,
+                  int32_t k, const double *occ, int32_t rx, int32_t ry,
+                  int32_t rz, const double *dBx, const double *dBy,
+                  const double *dBz, int32_t px, int32_t py, int32_t pz,
+                  double *d_rho);
+
+/* ---- initial density and grid integrals ------------------------------------ */
+/* rho(q) = sum_atoms spline(|q - R_a|) on a tensor grid (gx, gy, gz device
+ * 1-D coordinates; rho device, written).  The radial table (r, rho_r; host)
+ * is interpolated by a clamped cubic spline with zero end slopes, evaluated
+ * beyond the table with the end segments.
+ * replaces InitialGuess::initializeRhoFromRadiusFile
+ * (dft/InitialGuess.cc:573-681) as called by initialize_rho (:683-762). */
+int ttdft_initial_density(ttdft_ctx ctx, int32_t natoms, const double *xyz,
+                          int32_t nr, const double *r, const double *rho_r,
+                          int32_t px, int32_t py, int32_t pz, const double *dgx,
+                          const double *dgy, const double *dgz, double *d_rho);
+
+/* sum f[q] wx[qx] wy[qy] wz[qz] (device f, device weights), result on host.
+ * replaces KSDFTEnergyFunctional::compute3DIntegral
+ * (dft/KSDFTEnergyFunctional.cc:165-192) for separable weights. */
+int ttdft_grid_integral(ttdft_ctx ctx, const double *d_f, int32_t px,
+                        int32_t py, int32_t pz, const double *dwx,
+                        const double *dwy, const double *dwz, double *result);
+/* sum (f - g)^2 wx wy wz: the SCF density residual of main.cc:1410-1428. */
+int ttdft_grid_residual(ttdft_ctx ctx, const double *d_f, const double *d_g,
+                        int32_t px, int32_t py, int32_t pz, const double *dwx,
+                        const double *dwy, const double *dwz, double *result);
+/* f *= s in place (renormalisation of main.cc:674-680). */
+int ttdft_grid_scale(ttdft_ctx ctx, double *d_f, int64_t count, double s);
+
+/* ---- the ChFSI driver block --------------------------------------------------- */
+/* State main.cc carries between SCF iterations (lines 730-734, 685-688). */
+typedef struct {
+    double upper;          /* upperBoundUnWantedSpectrum          */
+    double lower_unwanted; /* lowerBoundUnWantedSpectrum          */
+    double lower_wanted;   /* lowerBoundWantedSpectrum            */
+    double err_upper;      /* error_in_upper_bound                */
+    double occ_energy;     /* occupied_orbital_energy             */
+    int32_t passes;        /* out: filter passes in the last call */
+    int32_t lanczos_runs;  /* out: Lanczos calls in the last call */
+} ttdft_chfsi_state;
+
+/* Fills the initial state of main.cc:733-734 and :685.  */
+int ttdft_chfsi_state_init(ttdft_chfsi_state *st);
+
+/* One SCF iteration's eigensolve on a device-resident block X:
+ * bound update (Lanczos when err_upper > 0.01), restart loop of
+ * filter -> orth -> Rayleigh-Ritz with the stop test on eig[occ_index]
+ * (relative change < 1e-3), then the rotation X <- X Q.
+ * evals: host array of k (previous iteration in, new out).
+ * precision: TTDFT_F64 (X FP64, FP64 GEMMs) or TTDFT_F32 (X FP32, FP32
+ * filter, split-BF16 GEMMs; Cholesky and Rayleigh-Ritz stay FP64).
+ * replaces the ChFSI block of main.cc:977-1237. */
+int ttdft_chfsi_step(ttdft_ctx ctx, ttdft_ham ham, void *X, int64_t ldx,
+                     int32_t k, int32_t m, int32_t num_blocks,
+                     int32_t restart_times, int32_t occ_index, int32_t scf_iter,
+                     int32_t start_iter, int32_t maxit_lanczos, uint64_t seed,
+                     int degree_convention, int precision, double *evals,
+                     ttdft_chfsi_state *st);
+
+/* Random starting subspace: X ~ N(0,1) from the seeded generator, then
+ * Cholesky-QR twice.  replaces MatSetRandom + BVOrthogonalize
+ * (main.cc:807-829); PETSc's random stream is not reproduced. */
+int ttdft_random_subspace(ttdft_ctx ctx, void *X, int64_t ldx, int64_t n,
+                          int32_t k, uint64_t seed, int dtype);
+
+/* ---- synthetic Hamiltonians (host) ------------------------------------------ */
+/* Kronecker-banded symmetric test Hamiltonian of the reference's structure
+ * (ProjectHamiltonianSparse.cc:464-548):
+ *   H = 1/(2h^2) (T_x (x) I (x) I + I (x) T_y (x) I + I (x) I (x) T_z)
+ *       + sum_r g_r P_x^r (x) P_y^r (x) P_z^r,
+ * T the sinc-DVR kinetic stencil cut to half-bandwidth b, P^r banded
+ * Gaussian-coupled potential factors, entries with |h| <= truncation
+ * dropped; plus p s-type projectors (products of 1-D Gaussians over the
+ * basis index) at seeded sites, truncated the same way, inv_c_lm = 1.
+ * Two calls: first with NULL arrays to get nnz / nnz_c, then to fill. */
+typedef struct {
+    int32_t rx, ry, rz;  /* Tucker ranks per axis (N = rx*ry*rz)      */
+    int32_t bx, by, bz;  /* half-bandwidths per axis                  */
+    int32_t p;           /* projector count                           */
+    double h;            /* grid spacing of the kinetic stencil (bohr) */
+    double truncation;   /* drop |h_ij| <= truncation                 */
+    uint64_t seed;
+} ttdft_synth_params;
+
+int ttdft_synth_hamiltonian(const ttdft_synth_params *prm, int64_t *nnz,
+                            int64_t *nnz_c, int64_t *rowptr, int32_t *colind,
+                            double *val, int64_t *c_rowptr, int32_t *c_colind,
+                            double *c_val, double *inv_c_lm);
+
+/* Host random streams (same generator the engine uses internally). */
+int ttdft_rng_uniform(uint64_t seed, int64_t count, double *out);
+int ttdft_rng_normal(uint64_t seed, int64_t count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TTDFT_CHFSI_H */

@@ -1,0 +1,280 @@
+"""HIP engine vs the CPU oracle on identical seeded inputs, through the C ABI.
+
+Tolerances: FP64 paths agree to rounding (relative 1e-12 or tighter; the
+summation order differs between the GPU kernels and the oracle); the FP32
+SpMM to 1e-5 relative; split-BF16 Gram/rotation to 5e-5 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ttdft_public_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(ctx, a, dtype=np.float64):
+    return E.DeviceArray.from_host(ctx, np.ascontiguousarray(a, dtype=dtype))
+
+
+def synth(p=5, dims=(8, 8, 8), bands=(2, 2, 3), seed=7):
+    s = E.SynthHam(*dims, *bands, p, seed=seed)
+    return s, oracle.Ham.from_synth(s)
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# ---------------------------------------------------------------- H apply
+@pytest.mark.parametrize("k", [1, 7, 64, 65, 130, 300])
+def test_ham_apply_f64(ctx, k):
+    s, H = synth()
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32)
+    X = np.random.default_rng(k).standard_normal((s.n, k))
+    dX, dY = dev(ctx, X), E.DeviceArray(ctx, (s.n, k))
+    ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, E.F64)
+    assert rel(dY.to_host(), oracle.ham_apply(H, X)) < 1e-13
+
+
+@pytest.mark.parametrize("k", [5, 64, 300])
+def test_ham_apply_f32(ctx, k):
+    s, H = synth()
+    ham = ctx.ham_upload(s, E.HAM_F32)
+    X = np.random.default_rng(k).standard_normal((s.n, k)).astype(np.float32)
+    dX, dY = dev(ctx, X, np.float32), E.DeviceArray(ctx, (s.n, k), np.float32)
+    ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, E.F32)
+    assert rel(dY.to_host().astype(np.float64), oracle.ham_apply(H, X.astype(np.float64))) < 1e-5
+
+
+def test_ham_apply_ragged_and_no_projectors(ctx):
+    """Empty rows, a single dense row, p = 0, leading dimension > k."""
+    n = 200
+    rng = np.random.default_rng(3)
+    rows = []
+    for i in range(n):
+        if i % 7 == 0:
+            rows.append([])
+        elif i == 5:
+            rows.append(list(range(n)))
+        else:
+            rows.append(sorted(set(rng.integers(0, n, size=rng.integers(1, 90)).tolist())))
+    # symmetrise the pattern with random values
+    A = np.zeros((n, n))
+    for i, cs in enumerate(rows):
+        for c in cs:
+            A[i, c] = A[c, i] = rng.standard_normal()
+    rowptr = np.zeros(n + 1, np.int64)
+    cols, vals = [], []
+    for i in range(n):
+        nz = np.nonzero(A[i])[0]
+        cols.extend(nz.tolist())
+        vals.extend(A[i, nz].tolist())
+        rowptr[i + 1] = len(cols)
+    ham = E.Hamiltonian(ctx, n, rowptr, np.array(cols, np.int32), np.array(vals), 0, None, None,
+                        None, None)
+    k, ld = 70, 80
+    X = rng.standard_normal((n, ld))
+    dX, dY = dev(ctx, X), dev(ctx, np.full((n, ld), 7.0))
+    ctx.ham_apply(ham, dX.ptr, ld, dY.ptr, ld, k, E.F64)
+    Y = dY.to_host()
+    assert rel(Y[:, :k], A @ X[:, :k]) < 1e-13
+    assert np.all(Y[:, k:] == 7.0)  # padding columns untouched
+
+
+def test_ham_upload_rejects_bad_csr(ctx):
+    with pytest.raises(E.TtdftError) as ei:
+        E.Hamiltonian(ctx, 3, np.array([0, 1, 2, 3]), np.array([0, 5, 2], np.int32),
+                      np.ones(3), 0, None, None, None, None)
+    assert ei.value.status == E.EBADARG
+
+
+# ---------------------------------------------------------------- filter
+@pytest.mark.parametrize("convention", [E.DEGREE_REFERENCE, E.DEGREE_EXACT])
+@pytest.mark.parametrize("m,nb", [(1, 1), (2, 1), (5, 1), (8, 3)])
+def test_cheb_filter_f64(ctx, convention, m, nb):
+    s, H = synth()
+    ham = ctx.ham_upload(s)
+    k = 24
+    X = np.random.default_rng(m).standard_normal((s.n, k))
+    a, b, a0 = -5.0, 20.0, -30.0
+    dX = dev(ctx, X)
+    ctx.cheb_filter(ham, dX.ptr, k, k, m, a, b, a0, nb, E.F64, convention)
+    ref = oracle.cheb_filter(H, X, m, a, b, a0, nb, convention)
+    assert rel(dX.to_host(), ref) < 1e-12
+
+
+def test_cheb_filter_f32(ctx):
+    s, H = synth()
+    ham = ctx.ham_upload(s, E.HAM_F32)
+    k, m = 40, 10
+    X = np.random.default_rng(9).standard_normal((s.n, k)).astype(np.float32)
+    dX = dev(ctx, X, np.float32)
+    ctx.cheb_filter(ham, dX.ptr, k, k, m, -5.0, 20.0, -30.0, 1, E.F32)
+    ref = oracle.cheb_filter(H, X.astype(np.float64), m, -5.0, 20.0, -30.0)
+    assert rel(dX.to_host().astype(np.float64), ref) < 5e-5
+
+
+# ---------------------------------------------------------------- dense
+@pytest.mark.parametrize("n,k", [(512, 24), (3000, 65), (5000, 300)])
+def test_gram_f64(ctx, n, k):
+    rng = np.random.default_rng(n + k)
+    X, Y = rng.standard_normal((n, k)), rng.standard_normal((n, k))
+    dS = E.DeviceArray(ctx, (k, k))
+    ctx.gram(dev(ctx, X).ptr, k, dev(ctx, Y).ptr, k, n, k, dS.ptr)
+    S = dS.to_host().T  # column-major on device
+    assert rel(S, oracle.gram(X, Y)) < 1e-13
+
+
+@pytest.mark.parametrize("n,k", [(512, 24), (5000, 300)])
+def test_gram_bf16x3(ctx, n, k):
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, k)).astype(np.float32)
+    Y = rng.standard_normal((n, k)).astype(np.float32)
+    dS = E.DeviceArray(ctx, (k, k))
+    ctx.gram(dev(ctx, X, np.float32).ptr, k, dev(ctx, Y, np.float32).ptr, k, n, k, dS.ptr,
+             E.F32, E.GEMM_BF16X3)
+    ref = X.astype(np.float64).T @ Y.astype(np.float64)
+    # error relative to the natural scale sqrt(n) of a Gram entry
+    assert np.abs(dS.to_host().T - ref).max() / np.sqrt(n) < 5e-5
+
+
+@pytest.mark.parametrize("mode", [E.GEMM_F64, E.GEMM_BF16X3])
+def test_rotate_and_orth(ctx, mode):
+    n, k = 4000, 100
+    rng = np.random.default_rng(mode)
+    f32 = mode == E.GEMM_BF16X3
+    dt, npdt = (E.F32, np.float32) if f32 else (E.F64, np.float64)
+    X = rng.standard_normal((n, k)).astype(npdt)
+    Q = rng.standard_normal((k, k))
+    dX = dev(ctx, X, npdt)
+    dQ = dev(ctx, Q.T.copy())  # column-major
+    ctx.rotate(dX.ptr, k, n, k, dQ.ptr, dt, mode)
+    ref = oracle.rotate(X.astype(np.float64), Q)
+    tol = 1e-13 if not f32 else 5e-5
+    assert rel(dX.to_host().astype(np.float64), ref) < tol
+    ctx.orth(dX.ptr, k, n, k, dt, mode)
+    Xo = dX.to_host().astype(np.float64)
+    assert np.abs(Xo.T @ Xo - np.eye(k)).max() < (1e-12 if not f32 else 1e-5)
+    # same span as the input
+    assert np.abs(Xo @ (Xo.T @ ref) - ref).max() / np.abs(ref).max() < (1e-11 if not f32 else 1e-4)
+
+
+def test_orth_matches_oracle_f64(ctx):
+    n, k = 2000, 40
+    X = np.random.default_rng(1).standard_normal((n, k))
+    dX = dev(ctx, X)
+    ctx.orth(dX.ptr, k, n, k)
+    assert rel(dX.to_host(), oracle.orth(X)) < 1e-11
+
+
+def test_orth_not_spd_reports_error(ctx):
+    n, k = 300, 8
+    X = np.random.default_rng(2).standard_normal((n, k))
+    X[:, 5] = X[:, 2]  # rank deficient
+    dX = dev(ctx, X)
+    with pytest.raises(E.TtdftError) as ei:
+        ctx.orth(dX.ptr, k, n, k)
+    assert ei.value.status in (E.ENOTSPD,)
+
+
+@pytest.mark.parametrize("k", [24, 300])
+def test_rr_eig(ctx, k):
+    A = np.random.default_rng(k).standard_normal((k, k))
+    A = A + A.T
+    dA = dev(ctx, A.T.copy())
+    dE, dQ = E.DeviceArray(ctx, (k,)), E.DeviceArray(ctx, (k, k))
+    ctx.rr_eig(dA.ptr, k, dE.ptr, dQ.ptr)
+    w = dE.to_host()
+    Q = dQ.to_host().T
+    wo, Qo = oracle.syev(A)
+    assert np.abs(w - wo).max() < 1e-12 * np.abs(wo).max()
+    assert np.abs(A @ Q - Q * w).max() < 1e-11
+    # same eigenvectors up to sign
+    assert np.abs(np.abs(np.sum(Q * Qo, axis=0)) - 1).max() < 1e-9
+
+
+# ---------------------------------------------------------------- Lanczos
+def test_lanczos_matches_oracle(ctx):
+    s, H = synth()
+    ham = ctx.ham_upload(s)
+    up, lo = ctx.lanczos_bounds(ham, 20, seed=11)
+    upo, loo = oracle.lanczos_bounds(H, 20, seed=11)
+    assert abs(up - upo) < 1e-9 * abs(upo) and abs(lo - loo) < 1e-9 * abs(loo)
+
+
+# ---------------------------------------------------------------- density
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+def test_density(ctx, dtype):
+    rng = np.random.default_rng(5)
+    rx, ry, rz = 6, 5, 7
+    px, py, pz = 70, 33, 20
+    Bx, By, Bz = (rng.standard_normal((p, r)) for p, r in ((px, rx), (py, ry), (pz, rz)))
+    k = 21
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = rng.standard_normal((rx * ry * rz, k)).astype(npdt)
+    occ = np.linspace(1.0, 0.0, k)  # last state has zero weight
+    dRho = E.DeviceArray(ctx, (pz, py, px))
+    ctx.density(dev(ctx, X, npdt).ptr, k, k, occ, (rx, ry, rz), dev(ctx, Bx.T.copy()).ptr,
+                dev(ctx, By.T.copy()).ptr, dev(ctx, Bz.T.copy()).ptr, (px, py, pz), dRho.ptr,
+                dtype)
+    ref = oracle.density(X.astype(np.float64), occ, (rx, ry, rz), Bx, By, Bz)
+    got = dRho.to_host()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+
+
+def test_initial_density_and_integrals(ctx):
+    r = np.linspace(0.0, 9.0, 901)
+    rho_r = np.exp(-r**2) * (1 + 0.3 * r)
+    xyz = np.array([[0.3, -0.2, 0.1], [1.7, 0.4, -0.9], [-1.1, 1.2, 0.6]])
+    x, w = np.polynomial.legendre.leggauss(40)
+    gx, wx = 7.0 * x, 7.0 * w
+    gy = np.linspace(-8, 8, 35)
+    wy = np.full(35, 16 / 34)
+    gz = gx[::2].copy()
+    wz = wx[::2].copy()
+    dims = (len(gx), len(gy), len(gz))
+    dR = E.DeviceArray(ctx, (dims[2], dims[1], dims[0]))
+    ctx.initial_density(xyz, r, rho_r, dev(ctx, gx).ptr, dev(ctx, gy).ptr, dev(ctx, gz).ptr,
+                        dims, dR.ptr)
+    ref = oracle.initial_density(xyz, r, rho_r, gx, gy, gz)
+    got = dR.to_host()
+    assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
+    dwx, dwy, dwz = dev(ctx, wx), dev(ctx, wy), dev(ctx, wz)
+    q = ctx.grid_integral(dR.ptr, dims, dwx.ptr, dwy.ptr, dwz.ptr)
+    assert abs(q - oracle.grid_integral(ref, wx, wy, wz)) < 1e-11 * abs(q)
+    dR2 = dev(ctx, ref * 1.01)
+    res = ctx.grid_residual(dR.ptr, dR2.ptr, dims, dwx.ptr, dwy.ptr, dwz.ptr)
+    assert abs(res - oracle.grid_integral((0.01 * ref) ** 2, wx, wy, wz)) < 1e-10 * res
+    ctx.grid_scale(dR.ptr, int(np.prod(dims)), 2.0)
+    assert np.abs(dR.to_host() - 2 * ref).max() < 1e-13 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------- driver
+def test_chfsi_step_matches_oracle(ctx):
+    s, H = synth(p=5, seed=21)
+    ham = ctx.ham_upload(s)
+    k, nocc = 16, 10
+    X0 = oracle.orth(oracle.rng_normal(5, s.n * k).reshape(s.n, k))
+    Xo = X0.copy()
+    evo = np.zeros(k)
+    sto = oracle.new_state()
+    dX = dev(ctx, X0)
+    ev = np.zeros(k)
+    st = E.new_state()
+    for scf in range(6):
+        rt = 10 if scf == 0 else 3
+        oracle.chfsi_step(H, Xo, evo, sto, m=12, scf_iter=scf, occ_index=nocc, restart_times=rt)
+        ctx.chfsi_step(ham, dX.ptr, k, k, 12, ev, st, scf, nocc, rt)
+        assert st.passes == sto.passes and st.lanczos_runs == sto.lanczos_runs
+        assert np.abs(ev - evo).max() < 1e-8, scf
+        assert abs(st.upper - sto.upper) < 1e-8 * abs(sto.upper)
+    lam = np.linalg.eigvalsh(H.dense())
+    assert np.abs(ev[:nocc] - lam[:nocc]).max() < 1e-8
+    # band energy and density from the converged subspace
+    ef, occ = E.fermi_occupancy(ev, 2 * nocc - 1)
+    efo = oracle.fermi_energy(evo, 2 * nocc - 1)
+    assert abs(ef - efo) < 1e-8
+    X = dX.to_host()
+    # subspaces agree (eigenvector signs are free)
+    assert np.abs(np.abs(np.sum(X[:, :nocc] * Xo[:, :nocc], axis=0)) - 1).max() < 1e-6

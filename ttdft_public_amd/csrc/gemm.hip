@@ -1,0 +1,404 @@
+// Tall-skinny dense contractions of the subspace iteration on MFMA:
+//   Gram      S = X^T Y   (n x k)^T (n x k) -> k x k     (MultTM::mult)
+//   rotation  Z = X Q     (n x k)(k x k)    -> n x k     (orth, X <- X Q)
+// MFMA output layouts differ by type: for f64 16x16x4, element v of lane l
+// holds D[l/16 + 4v][l%16]; for f32 16x16x16 bf16 it holds D[4(l/16) + v][l%16].
+// Two arithmetic modes:
+//   F64    : FP64 blocks, v_mfma_f64_16x16x4_f64, FP64 accumulation.
+//   BF16X3 : FP32 blocks, each operand split as x = hi + lo (both bf16), the
+//            product taken as hi*hi' + hi*lo' + lo*hi' on
+//            v_mfma_f32_16x16x16_bf16 with FP32 accumulation.
+// The Gram is split over row chunks; each chunk's k x k partial is reduced
+// in FP64 by a second kernel in a fixed order (deterministic).
+#include "internal.h"
+
+namespace ttdft {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma_bf16(s4 a, s4 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// round-to-nearest-even float -> bf16 bits
+__device__ __forceinline__ short f2bf(float x) {
+    unsigned u = __builtin_bit_cast(unsigned, x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(short h) {
+    return __builtin_bit_cast(float, ((unsigned)(unsigned short)h) << 16);
+}
+__device__ __forceinline__ void split_bf16(float x, short &hi, short &lo) {
+    hi = f2bf(x);
+    lo = f2bf(x - bf2f(hi));
+}
+
+// ---------------------------------------------------------------------------
+// Gram, FP64.  Block = 4 waves = 64x64 tile of S; wave = 32x32 (2x2 MFMA
+// tiles).  MFMA operand layout (16x16x4 f64): A[i][kk] in lane i + 16 kk,
+// B[kk][j] in lane j + 16 kk, D[i][j] in lane j + 16 (i%4), element i/4.
+// A[i][kk] = X[r+kk][a0+i], B[kk][j] = Y[r+kk][b0+j].
+__global__ __launch_bounds__(256) void gram_f64_kernel(const double *__restrict__ X,
+                                                       int64_t ldx,
+                                                       const double *__restrict__ Y,
+                                                       int64_t ldy, int64_t n, int32_t k,
+                                                       int64_t rows_per_chunk,
+                                                       double *__restrict__ P) {
+    const int ntile = (k + 63) / 64;
+    const int ta = blockIdx.x % ntile, tb = blockIdx.x / ntile;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int a0 = ta * 64 + (wave & 1) * 32, b0 = tb * 64 + (wave >> 1) * 32;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+    const int64_t r1 = (r0 + rows_per_chunk) < n ? (r0 + rows_per_chunk) : n;
+    const int li = lane & 15, lk = lane >> 4;
+    const bool ma0 = a0 + li < k, ma1 = a0 + 16 + li < k;
+    const bool mb0 = b0 + li < k, mb1 = b0 + 16 + li < k;
+    d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+    for (int64_t r = r0; r < r1; r += 4) {
+        const int64_t row = r + lk;
+        const bool ok = row < r1;
+        const double *xr = X + row * ldx + a0 + li;
+        const double *yr = Y + row * ldy + b0 + li;
+        const double xa0 = (ok && ma0) ? xr[0] : 0.0;
+        const double xa1 = (ok && ma1) ? xr[16] : 0.0;
+        const double yb0 = (ok && mb0) ? yr[0] : 0.0;
+        const double yb1 = (ok && mb1) ? yr[16] : 0.0;
+        acc00 = mfma_f64(xa0, yb0, acc00);
+        acc01 = mfma_f64(xa0, yb1, acc01);
+        acc10 = mfma_f64(xa1, yb0, acc10);
+        acc11 = mfma_f64(xa1, yb1, acc11);
+    }
+    double *Pc = P + (int64_t)blockIdx.y * k * k;
+    const int j = lane & 15;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = (lane >> 4) + 4 * v;
+        const int aA = a0 + i, aB = a0 + 16 + i, bA = b0 + j, bB = b0 + 16 + j;
+        if (aA < k && bA < k) Pc[aA + (int64_t)k * bA] = acc00[v];
+        if (aA < k && bB < k) Pc[aA + (int64_t)k * bB] = acc01[v];
+        if (aB < k && bA < k) Pc[aB + (int64_t)k * bA] = acc10[v];
+        if (aB < k && bB < k) Pc[aB + (int64_t)k * bB] = acc11[v];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gram, split-BF16.  MFMA 16x16x16 bf16: A[i][kk] in lane i + 16 (kk/4),
+// element kk%4; B[kk][j] in lane j + 16 (kk/4), element kk%4.  16 rows per
+// step.
+__global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float *__restrict__ X,
+                                                          int64_t ldx,
+                                                          const float *__restrict__ Y,
+                                                          int64_t ldy, int64_t n, int32_t k,
+                                                          int64_t rows_per_chunk,
+                                                          double *__restrict__ P) {
+    const int ntile = (k + 63) / 64;
+    const int ta = blockIdx.x % ntile, tb = blockIdx.x / ntile;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int a0 = ta * 64 + (wave & 1) * 32, b0 = tb * 64 + (wave >> 1) * 32;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+    const int64_t r1 = (r0 + rows_per_chunk) < n ? (r0 + rows_per_chunk) : n;
+    const int li = lane & 15, lk = lane >> 4;
+    const bool ma0 = a0 + li < k, ma1 = a0 + 16 + li < k;
+    const bool mb0 = b0 + li < k, mb1 = b0 + 16 + li < k;
+    f4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+    for (int64_t r = r0; r < r1; r += 16) {
+        s4 xh0, xl0, xh1, xl1, yh0, yl0, yh1, yl1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = r + 4 * lk + v;
+            const bool ok = row < r1;
+            const float *xr = X + row * ldx + a0 + li;
+            const float *yr = Y + row * ldy + b0 + li;
+            short h, l;
+            split_bf16((ok && ma0) ? xr[0] : 0.0f, h, l);
+            xh0[v] = h; xl0[v] = l;
+            split_bf16((ok && ma1) ? xr[16] : 0.0f, h, l);
+            xh1[v] = h; xl1[v] = l;
+            split_bf16((ok && mb0) ? yr[0] : 0.0f, h, l);
+            yh0[v] = h; yl0[v] = l;
+            split_bf16((ok && mb1) ? yr[16] : 0.0f, h, l);
+            yh1[v] = h; yl1[v] = l;
+        }
+        acc00 = mfma_bf16(xl0, yh0, acc00);
+        acc00 = mfma_bf16(xh0, yl0, acc00);
+        acc00 = mfma_bf16(xh0, yh0, acc00);
+        acc01 = mfma_bf16(xl0, yh1, acc01);
+        acc01 = mfma_bf16(xh0, yl1, acc01);
+        acc01 = mfma_bf16(xh0, yh1, acc01);
+        acc10 = mfma_bf16(xl1, yh0, acc10);
+        acc10 = mfma_bf16(xh1, yl0, acc10);
+        acc10 = mfma_bf16(xh1, yh0, acc10);
+        acc11 = mfma_bf16(xl1, yh1, acc11);
+        acc11 = mfma_bf16(xh1, yl1, acc11);
+        acc11 = mfma_bf16(xh1, yh1, acc11);
+    }
+    double *Pc = P + (int64_t)blockIdx.y * k * k;
+    const int j = lane & 15;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = 4 * (lane >> 4) + v;
+        const int aA = a0 + i, aB = a0 + 16 + i, bA = b0 + j, bB = b0 + 16 + j;
+        if (aA < k && bA < k) Pc[aA + (int64_t)k * bA] = (double)acc00[v];
+        if (aA < k && bB < k) Pc[aA + (int64_t)k * bB] = (double)acc01[v];
+        if (aB < k && bA < k) Pc[aB + (int64_t)k * bA] = (double)acc10[v];
+        if (aB < k && bB < k) Pc[aB + (int64_t)k * bB] = (double)acc11[v];
+    }
+}
+
+__global__ void reduce_partials_kernel(const double *__restrict__ P, int nchunks, int64_t count,
+                                       double *__restrict__ S) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunks; ++c) s += P[(int64_t)c * count + idx];
+    S[idx] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Rotation, FP64: Z = X Qt with Qt row-major (Qt[a*k + b] = Q[a][b]).
+// Block = 4 waves = 64 rows x 64 columns of Z; wave = 16 rows x 64 columns
+// (1x4 MFMA tiles).  A[i][kk] = X[n0+i][a+kk], B[kk][j] = Qt[a+kk][b0+j].
+__global__ __launch_bounds__(256) void rotate_f64_kernel(const double *__restrict__ X,
+                                                         int64_t ldx,
+                                                         const double *__restrict__ Qt,
+                                                         double *__restrict__ Z, int64_t ldz,
+                                                         int64_t n, int32_t k) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t n0 = (int64_t)blockIdx.x * 64 + wave * 16;
+    const int b0 = blockIdx.y * 64;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t arow = n0 + li;
+    const bool rok = arow < n;
+    const double *xr = X + (rok ? arow : 0) * ldx;
+    bool mb[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) mb[t] = b0 + 16 * t + li < k;
+    d4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = d4{0, 0, 0, 0};
+    for (int a = 0; a < k; a += 4) {
+        const int aa = a + lk;
+        const bool aok = aa < k;
+        const double xa = (rok && aok) ? xr[aa] : 0.0;
+        const double *qr = Qt + (int64_t)(aok ? aa : 0) * k + b0 + li;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double qb = (aok && mb[t]) ? qr[16 * t] : 0.0;
+            acc[t] = mfma_f64(xa, qb, acc[t]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int64_t row = n0 + lk + 4 * v;
+        if (row >= n) continue;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int b = b0 + 16 * t + li;
+            if (b < k) Z[row * ldz + b] = acc[t][v];
+        }
+    }
+}
+
+// Rotation, split-BF16: Z = X Q for FP32 X; Q given as bf16 hi/lo planes in
+// column-major order (Qh[a + k*b]).  A[i][kk] = X[n0+i][a+kk] (lane i +
+// 16 (kk/4), element kk%4), B[kk][j] = Q[a+kk][b0+j].
+__global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const float *__restrict__ X,
+                                                            int64_t ldx,
+                                                            const short *__restrict__ Qh,
+                                                            const short *__restrict__ Ql,
+                                                            float *__restrict__ Z, int64_t ldz,
+                                                            int64_t n, int32_t k) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t n0 = (int64_t)blockIdx.x * 64 + wave * 16;
+    const int b0 = blockIdx.y * 64;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t arow = n0 + li;
+    const bool rok = arow < n;
+    const float *xr = X + (rok ? arow : 0) * ldx;
+    f4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f4{0, 0, 0, 0};
+    for (int a = 0; a < k; a += 16) {
+        s4 xh, xl;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int aa = a + 4 * lk + v;
+            short h, l;
+            split_bf16((rok && aa < k) ? xr[aa] : 0.0f, h, l);
+            xh[v] = h;
+            xl[v] = l;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int b = b0 + 16 * t + li;
+            s4 qh, ql;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int aa = a + 4 * lk + v;
+                const bool ok = aa < k && b < k;
+                qh[v] = ok ? Qh[aa + (int64_t)k * b] : (short)0;
+                ql[v] = ok ? Ql[aa + (int64_t)k * b] : (short)0;
+            }
+            acc[t] = mfma_bf16(xl, qh, acc[t]);
+            acc[t] = mfma_bf16(xh, ql, acc[t]);
+            acc[t] = mfma_bf16(xh, qh, acc[t]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int64_t row = n0 + 4 * lk + v;
+        if (row >= n) continue;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int b = b0 + 16 * t + li;
+            if (b < k) Z[row * ldz + b] = acc[t][v];
+        }
+    }
+}
+
+__global__ void split_q_kernel(const double *__restrict__ Q, int64_t count,
+                               short *__restrict__ Qh, short *__restrict__ Ql) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    short h, l;
+    split_bf16((float)Q[i], h, l);
+    Qh[i] = h;
+    Ql[i] = l;
+}
+
+__global__ void transpose_small_kernel(const double *__restrict__ A, double *__restrict__ AT,
+                                       int32_t k) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)k * k) return;
+    const int i = (int)(idx % k), j = (int)(idx / k);
+    AT[j + (int64_t)k * i] = A[i + (int64_t)k * j];
+}
+
+template <typename S, typename D>
+__global__ void copy_block_kernel(const S *__restrict__ src, int64_t lds, D *__restrict__ dst,
+                                  int64_t ldd, int64_t n, int32_t k) {
+    const int64_t total = n * (int64_t)k;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx / k;
+        const int j = (int)(idx - i * k);
+        dst[i * ldd + j] = (D)src[i * lds + j];
+    }
+}
+
+}  // namespace
+
+int launch_gram(ttdft_ctx ctx, const void *X, int64_t ldx, const void *Y, int64_t ldy,
+                int64_t n, int32_t k, int dtype, int mode, double *dS) {
+    TT_CHECK(k >= 1 && n >= 1, TTDFT_EBADARG, "gram: n=%lld k=%d", (long long)n, k);
+    TT_CHECK((mode == TTDFT_GEMM_F64 && dtype == TTDFT_F64) ||
+                 (mode == TTDFT_GEMM_BF16X3 && dtype == TTDFT_F32),
+             TTDFT_EBADARG, "gram: mode %d does not match dtype %d", mode, dtype);
+    const int ntile = (k + 63) / 64;
+    const int tiles = ntile * ntile;
+    const int64_t step = (mode == TTDFT_GEMM_F64) ? 4 : 16;
+    int64_t nchunks = 2048 / tiles;
+    if (nchunks < 1) nchunks = 1;
+    const int64_t max_chunks = (n + 255) / 256;
+    if (nchunks > max_chunks) nchunks = max_chunks;
+    int64_t rows = (n + nchunks - 1) / nchunks;
+    rows = (rows + step - 1) / step * step;
+    nchunks = (n + rows - 1) / rows;
+    const int64_t kk = (int64_t)k * k;
+    TT_TRY(ctx->scratch[2].ensure((size_t)nchunks * kk * sizeof(double)));
+    double *P = (double *)ctx->scratch[2].ptr;
+    const dim3 grid((unsigned)tiles, (unsigned)nchunks);
+    PendingTiming t;
+    TT_TRY(timing_begin(ctx, TTDFT_KSTAT_GRAM, &t));
+    if (mode == TTDFT_GEMM_F64) {
+        hipLaunchKernelGGL(gram_f64_kernel, grid, dim3(256), 0, ctx->stream, (const double *)X,
+                           ldx, (const double *)Y, ldy, n, k, rows, P);
+    } else {
+        hipLaunchKernelGGL(gram_bf16x3_kernel, grid, dim3(256), 0, ctx->stream,
+                           (const float *)X, ldx, (const float *)Y, ldy, n, k, rows, P);
+    }
+    TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0,
+                       ctx->stream, P, (int)nchunks, kk, dS);
+    TT_LAUNCH_CHECK();
+    const double es = (double)dtype_size(dtype);
+    TT_TRY(timing_end(ctx, &t, 2.0 * n * k * es + kk * 8.0,
+                      2.0 * n * kk * (mode == TTDFT_GEMM_F64 ? 1.0 : 3.0)));
+    return TTDFT_OK;
+}
+
+int launch_rotate(ttdft_ctx ctx, const void *X, int64_t ldx, void *Z, int64_t ldz, int64_t n,
+                  int32_t k, const double *dQ, int dtype, int mode) {
+    TT_CHECK(k >= 1 && n >= 1, TTDFT_EBADARG, "rotate: n=%lld k=%d", (long long)n, k);
+    TT_CHECK((mode == TTDFT_GEMM_F64 && dtype == TTDFT_F64) ||
+                 (mode == TTDFT_GEMM_BF16X3 && dtype == TTDFT_F32),
+             TTDFT_EBADARG, "rotate: mode %d does not match dtype %d", mode, dtype);
+    const int64_t kk = (int64_t)k * k;
+    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((k + 63) / 64));
+    PendingTiming t;
+    if (mode == TTDFT_GEMM_F64) {
+        TT_TRY(ctx->scratch[4].ensure((size_t)kk * sizeof(double)));
+        double *Qt = (double *)ctx->scratch[4].ptr;
+        hipLaunchKernelGGL(transpose_small_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256),
+                           0, ctx->stream, dQ, Qt, k);
+        TT_LAUNCH_CHECK();
+        TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ROTATE, &t));
+        hipLaunchKernelGGL(rotate_f64_kernel, grid, dim3(256), 0, ctx->stream,
+                           (const double *)X, ldx, Qt, (double *)Z, ldz, n, k);
+        TT_LAUNCH_CHECK();
+        TT_TRY(timing_end(ctx, &t, 2.0 * n * k * 8.0 + kk * 8.0, 2.0 * n * kk));
+    } else {
+        TT_TRY(ctx->scratch[4].ensure((size_t)kk * 2 * sizeof(short)));
+        short *Qh = (short *)ctx->scratch[4].ptr;
+        short *Ql = Qh + kk;
+        hipLaunchKernelGGL(split_q_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0,
+                           ctx->stream, dQ, kk, Qh, Ql);
+        TT_LAUNCH_CHECK();
+        TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ROTATE, &t));
+        hipLaunchKernelGGL(rotate_bf16x3_kernel, grid, dim3(256), 0, ctx->stream,
+                           (const float *)X, ldx, Qh, Ql, (float *)Z, ldz, n, k);
+        TT_LAUNCH_CHECK();
+        TT_TRY(timing_end(ctx, &t, 2.0 * n * k * 4.0 + kk * 4.0, 3.0 * 2.0 * n * kk));
+    }
+    return TTDFT_OK;
+}
+
+int launch_transpose_small(ttdft_ctx ctx, const double *A, double *AT, int32_t k) {
+    const int64_t kk = (int64_t)k * k;
+    hipLaunchKernelGGL(transpose_small_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0,
+                       ctx->stream, A, AT, k);
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
+}
+
+int launch_copy_block(ttdft_ctx ctx, const void *src, int64_t lds, int src_dtype, void *dst,
+                      int64_t ldd, int dst_dtype, int64_t n, int32_t k) {
+    const int64_t total = n * (int64_t)k;
+    if (total == 0) return TTDFT_OK;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    const dim3 g((unsigned)blocks), b(256);
+    if (src_dtype == TTDFT_F64 && dst_dtype == TTDFT_F64)
+        hipLaunchKernelGGL((copy_block_kernel<double, double>), g, b, 0, ctx->stream,
+                           (const double *)src, lds, (double *)dst, ldd, n, k);
+    else if (src_dtype == TTDFT_F64 && dst_dtype == TTDFT_F32)
+        hipLaunchKernelGGL((copy_block_kernel<double, float>), g, b, 0, ctx->stream,
+                           (const double *)src, lds, (float *)dst, ldd, n, k);
+    else if (src_dtype == TTDFT_F32 && dst_dtype == TTDFT_F64)
+        hipLaunchKernelGGL((copy_block_kernel<float, double>), g, b, 0, ctx->stream,
+                           (const float *)src, lds, (double *)dst, ldd, n, k);
+    else
+        hipLaunchKernelGGL((copy_block_kernel<float, float>), g, b, 0, ctx->stream,
+                           (const float *)src, lds, (float *)dst, ldd, n, k);
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
+}
+
+}  // namespace ttdft

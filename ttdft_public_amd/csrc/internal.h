@@ -1,0 +1,159 @@
+// Internal declarations of the MI355X ChFSI engine (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/ttdft_chfsi.h"
+
+namespace ttdft {
+
+// ---- error reporting ------------------------------------------------------
+void set_error(const char *fmt, ...);
+void clear_error();
+
+#define TT_HIP(expr)                                                              \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            ::ttdft::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr,         \
+                               hipGetErrorString(_e));                            \
+            return TTDFT_EHIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define TT_CHECK(cond, code, ...)                                                 \
+    do {                                                                          \
+        if (!(cond)) {                                                            \
+            ::ttdft::set_error(__VA_ARGS__);                                      \
+            return (code);                                                        \
+        }                                                                         \
+    } while (0)
+
+#define TT_TRY(expr)                                                              \
+    do {                                                                          \
+        int _rc = (expr);                                                         \
+        if (_rc != TTDFT_OK) return _rc;                                          \
+    } while (0)
+
+#define TT_LAUNCH_CHECK()                                                         \
+    do {                                                                          \
+        hipError_t _e = hipGetLastError();                                        \
+        if (_e != hipSuccess) {                                                   \
+            ::ttdft::set_error("%s:%d kernel launch: %s", __FILE__, __LINE__,     \
+                               hipGetErrorString(_e));                            \
+            return TTDFT_EHIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+// ---- grow-only device scratch buffer ---------------------------------------
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+};
+
+// ---- kernel-class timing --------------------------------------------------
+struct KStat {
+    int64_t launches = 0;
+    double ms = 0.0;
+    double bytes = 0.0;
+    double flops = 0.0;
+};
+
+struct PendingTiming {
+    int cls;
+    hipEvent_t start, stop;
+    double bytes, flops;
+};
+
+}  // namespace ttdft
+
+struct ttdft_ctx_s {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    int profiling = 0;
+    ttdft::KStat stats[TTDFT_KSTAT_COUNT];
+    std::vector<ttdft::PendingTiming> *pending = nullptr;
+    // scratch: [0] second X block for the filter / rotation, [1] small k x k
+    // work, [2] Gram partials, [3] nonlocal projections, [4..7] misc
+    ttdft::Scratch scratch[8];
+    int *d_info = nullptr;  // rocSOLVER info
+};
+
+struct ttdft_ham_s {
+    ttdft_ctx ctx = nullptr;
+    int64_t n = 0, nnz = 0;
+    int32_t p = 0;
+    int64_t nnz_c = 0;
+    int copies = 0;
+    int32_t max_row_nnz = 0;
+    // H_loc CSR
+    int64_t *rowptr = nullptr;
+    int32_t *col = nullptr;
+    double *val64 = nullptr;
+    float *val32 = nullptr;
+    // D C (row-scaled projector rows, p x n) and C^T (n x p, unscaled)
+    int64_t *c_rowptr = nullptr;
+    int32_t *c_col = nullptr;
+    double *c_val64 = nullptr;
+    float *c_val32 = nullptr;
+    int64_t *ct_rowptr = nullptr;
+    int32_t *ct_col = nullptr;
+    double *ct_val64 = nullptr;
+    float *ct_val32 = nullptr;
+};
+
+namespace ttdft {
+
+// ---- timing helpers (no-ops unless ctx->profiling) --------------------------
+int timing_begin(ttdft_ctx ctx, int cls, PendingTiming *t);
+int timing_end(ttdft_ctx ctx, PendingTiming *t, double bytes, double flops);
+int timing_flush(ttdft_ctx ctx);
+
+// ---- kernels (launchers return TTDFT_* status) -------------------------------
+// W[p x k] = (D C) X  (nonlocal projection; D folded into c_val)
+int launch_nl_project(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
+                      int32_t k, int dtype, void *W);
+// Y = alpha*(H_loc X + C^T W) + beta*X + delta*Z   (Z may alias Y; flags select terms)
+// use_beta / use_delta select whether X / Z are read.
+int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
+                      const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
+                      int32_t k, int dtype, double alpha, double beta, double delta,
+                      int use_beta, int use_delta);
+// Full H application (nl projection + fused SpMM) with the given epilogue.
+int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, void *Y,
+             int64_t ldy, const void *Z, int64_t ldz, int32_t k, int dtype, double alpha,
+             double beta, double delta, int use_beta, int use_delta);
+
+// S = X^T Y, k x k column-major FP64 (device)
+int launch_gram(ttdft_ctx ctx, const void *X, int64_t ldx, const void *Y, int64_t ldy,
+                int64_t n, int32_t k, int dtype, int mode, double *dS);
+// Z = X Q (Z separate from X), Q k x k column-major FP64
+int launch_rotate(ttdft_ctx ctx, const void *X, int64_t ldx, void *Z, int64_t ldz,
+                  int64_t n, int32_t k, const double *dQ, int dtype, int mode);
+
+// small helpers
+int launch_copy_block(ttdft_ctx ctx, const void *src, int64_t lds, int src_dtype, void *dst,
+                      int64_t ldd, int dst_dtype, int64_t n, int32_t k);
+int launch_transpose_small(ttdft_ctx ctx, const double *A, double *AT, int32_t k);
+
+// host random streams (splitmix64 + xoshiro256**)
+void rng_uniform(uint64_t seed, int64_t count, double *out);
+void rng_normal(uint64_t seed, int64_t count, double *out);
+
+// host small symmetric eigensolver (cyclic Jacobi), ascending, for Lanczos T
+int host_syev_small(int k, const double *A, double *evals);
+
+inline size_t dtype_size(int dtype) { return dtype == TTDFT_F32 ? 4 : 8; }
+
+}  // namespace ttdft

@@ -1,0 +1,251 @@
+// Hamiltonian application kernels: H X = H_loc X + C^T (D C X), with the
+// three-term Chebyshev recurrence fused into the SpMM epilogue.
+//
+// Data layout: X, Y are basis-major blocks (X[i*ld + j], state j of basis
+// function i), so one H_loc row reads whole contiguous state rows of X and a
+// 64-wide wavefront loads 64 consecutive states of one basis function in one
+// coalesced access.  One wavefront owns one H_loc row and all k states
+// (NC = ceil(k/64) accumulator chunks per lane), so H_loc is streamed from
+// HBM exactly once per application; the row's (value, column) pairs are read
+// 64 at a time with one coalesced load and broadcast with v_readlane.
+#include "internal.h"
+
+namespace ttdft {
+namespace {
+
+__device__ __forceinline__ double bcast(double v, int l) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ int bcast(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void spmm_fused_kernel(
+    int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const T *__restrict__ val, const T *__restrict__ X, int64_t ldx,
+    const int64_t *__restrict__ ct_rowptr, const int32_t *__restrict__ ct_col,
+    const T *__restrict__ ct_val, const T *__restrict__ W, int32_t ldw, T *Y, int64_t ldy,
+    const T *Z, int64_t ldz, int32_t k, T alpha, T beta, T delta, int use_beta,
+    int use_delta) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (row >= n) return;
+    T acc[NC];
+#pragma unroll
+    for (int t = 0; t < NC; ++t) acc[t] = T(0);
+
+    const int64_t e0 = rowptr[row], e1 = rowptr[row + 1];
+    for (int64_t base = e0; base < e1; base += 64) {
+        const int cnt = (int)((e1 - base) < 64 ? (e1 - base) : 64);
+        T v = T(0);
+        int c = 0;
+        if (lane < cnt) {
+            v = val[base + lane];
+            c = col[base + lane];
+        }
+        for (int j = 0; j < cnt; ++j) {
+            const T a = bcast(v, j);
+            const T *xr = X + (int64_t)bcast(c, j) * ldx + lane;
+#pragma unroll
+            for (int t = 0; t < NC; ++t) {
+                if (lane + 64 * t < k) acc[t] += a * xr[64 * t];
+            }
+        }
+    }
+    if (ct_rowptr != nullptr) {
+        const int64_t f0 = ct_rowptr[row], f1 = ct_rowptr[row + 1];
+        for (int64_t e = f0; e < f1; ++e) {
+            const T a = ct_val[e];
+            const T *wr = W + (int64_t)ct_col[e] * ldw + lane;
+#pragma unroll
+            for (int t = 0; t < NC; ++t) {
+                if (lane + 64 * t < k) acc[t] += a * wr[64 * t];
+            }
+        }
+    }
+    T *yr = Y + row * ldy + lane;
+    const T *xr = X + row * ldx + lane;
+    const T *zr = Z + row * ldz + lane;
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
+        if (lane + 64 * t < k) {
+            T y = acc[t] * alpha;
+            if (use_beta) y = y + beta * xr[64 * t];
+            if (use_delta) y = y + delta * zr[64 * t];
+            yr[64 * t] = y;
+        }
+    }
+}
+
+// W[r][j] = sum_e (D C)[r][e] X[col_e][j]: one block per projector row, the
+// row's nonzeros split over 4 wavefronts and summed through LDS.
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void nl_project_kernel(
+    int32_t p, const int64_t *__restrict__ c_rowptr, const int32_t *__restrict__ c_col,
+    const T *__restrict__ c_val, const T *__restrict__ X, int64_t ldx, int32_t k,
+    T *__restrict__ W, int32_t ldw) {
+    __shared__ T red[4][64 * NC];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int r = blockIdx.x;
+    T acc[NC];
+#pragma unroll
+    for (int t = 0; t < NC; ++t) acc[t] = T(0);
+    const int64_t e0 = c_rowptr[r], e1 = c_rowptr[r + 1];
+    for (int64_t base = e0 + 64 * wave; base < e1; base += 256) {
+        const int cnt = (int)((e1 - base) < 64 ? (e1 - base) : 64);
+        T v = T(0);
+        int c = 0;
+        if (lane < cnt) {
+            v = c_val[base + lane];
+            c = c_col[base + lane];
+        }
+        for (int j = 0; j < cnt; ++j) {
+            const T a = bcast(v, j);
+            const T *xr = X + (int64_t)bcast(c, j) * ldx + lane;
+#pragma unroll
+            for (int t = 0; t < NC; ++t) {
+                if (lane + 64 * t < k) acc[t] += a * xr[64 * t];
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NC; ++t) red[wave][lane + 64 * t] = acc[t];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int t = 0; t < NC; ++t) {
+            if (lane + 64 * t < k) {
+                const T s = (red[0][lane + 64 * t] + red[1][lane + 64 * t]) +
+                            (red[2][lane + 64 * t] + red[3][lane + 64 * t]);
+                W[(int64_t)r * ldw + lane + 64 * t] = s;
+            }
+        }
+    }
+}
+
+template <typename T, int NC>
+int spmm_dispatch_nc(ttdft_ctx ctx, const ttdft_ham_s *H, const T *X, int64_t ldx, const T *W,
+                     T *Y, int64_t ldy, const T *Z, int64_t ldz, int32_t k, double alpha,
+                     double beta, double delta, int use_beta, int use_delta,
+                     const T *val, const T *ct_val) {
+    const dim3 grid((unsigned)((H->n + 3) / 4));
+    hipLaunchKernelGGL((spmm_fused_kernel<T, NC>), grid, dim3(256), 0, ctx->stream, H->n,
+                       H->rowptr, H->col, val, X, ldx, H->p > 0 ? H->ct_rowptr : nullptr,
+                       H->ct_col, ct_val, W, k, Y, ldy, Z, ldz, k, (T)alpha, (T)beta,
+                       (T)delta, use_beta, use_delta);
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
+}
+
+template <typename T, int NC>
+int nl_dispatch_nc(ttdft_ctx ctx, const ttdft_ham_s *H, const T *X, int64_t ldx, int32_t k,
+                   const T *c_val, T *W) {
+    hipLaunchKernelGGL((nl_project_kernel<T, NC>), dim3((unsigned)H->p), dim3(256), 0,
+                       ctx->stream, H->p, H->c_rowptr, H->c_col, c_val, X, ldx, k, W, k);
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
+}
+
+#define TT_NC_SWITCH(NCVAL, CALL)                                                 \
+    switch (NCVAL) {                                                              \
+    case 1: return CALL(1);                                                       \
+    case 2: return CALL(2);                                                       \
+    case 3: return CALL(3);                                                       \
+    case 4: return CALL(4);                                                       \
+    case 5: return CALL(5);                                                       \
+    case 6: return CALL(6);                                                       \
+    case 7: return CALL(7);                                                       \
+    case 8: return CALL(8);                                                       \
+    case 9: return CALL(9);                                                       \
+    case 10: return CALL(10);                                                     \
+    case 11: return CALL(11);                                                     \
+    case 12: return CALL(12);                                                     \
+    case 13: return CALL(13);                                                     \
+    case 14: return CALL(14);                                                     \
+    case 15: return CALL(15);                                                     \
+    case 16: return CALL(16);                                                     \
+    default: break;                                                               \
+    }
+
+}  // namespace
+
+int launch_nl_project(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
+                      int32_t k, int dtype, void *W) {
+    if (H->p == 0) return TTDFT_OK;
+    const int nc = (k + 63) / 64;
+    TT_CHECK(nc >= 1 && nc <= 16, TTDFT_EBADARG, "k=%d outside 1..1024", k);
+    if (dtype == TTDFT_F64) {
+        TT_CHECK(H->c_val64, TTDFT_EBADARG, "no FP64 copy of the projectors uploaded");
+#define CALL(NCV) nl_dispatch_nc<double, NCV>(ctx, H, (const double *)X, ldx, k, H->c_val64, (double *)W)
+        TT_NC_SWITCH(nc, CALL)
+#undef CALL
+    } else {
+        TT_CHECK(H->c_val32, TTDFT_EBADARG, "no FP32 copy of the projectors uploaded");
+#define CALL(NCV) nl_dispatch_nc<float, NCV>(ctx, H, (const float *)X, ldx, k, H->c_val32, (float *)W)
+        TT_NC_SWITCH(nc, CALL)
+#undef CALL
+    }
+    return TTDFT_EBADARG;
+}
+
+int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
+                      const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
+                      int32_t k, int dtype, double alpha, double beta, double delta,
+                      int use_beta, int use_delta) {
+    const int nc = (k + 63) / 64;
+    TT_CHECK(nc >= 1 && nc <= 16, TTDFT_EBADARG, "k=%d outside 1..1024", k);
+    if (dtype == TTDFT_F64) {
+        TT_CHECK(H->val64, TTDFT_EBADARG, "no FP64 copy of H_loc uploaded");
+#define CALL(NCV)                                                                        \
+    spmm_dispatch_nc<double, NCV>(ctx, H, (const double *)X, ldx, (const double *)W,     \
+                                  (double *)Y, ldy, (const double *)Z, ldz, k, alpha,    \
+                                  beta, delta, use_beta, use_delta, H->val64, H->ct_val64)
+        TT_NC_SWITCH(nc, CALL)
+#undef CALL
+    } else {
+        TT_CHECK(H->val32, TTDFT_EBADARG, "no FP32 copy of H_loc uploaded");
+#define CALL(NCV)                                                                        \
+    spmm_dispatch_nc<float, NCV>(ctx, H, (const float *)X, ldx, (const float *)W,        \
+                                 (float *)Y, ldy, (const float *)Z, ldz, k, alpha, beta, \
+                                 delta, use_beta, use_delta, H->val32, H->ct_val32)
+        TT_NC_SWITCH(nc, CALL)
+#undef CALL
+    }
+    return TTDFT_EBADARG;
+}
+
+int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, void *Y,
+             int64_t ldy, const void *Z, int64_t ldz, int32_t k, int dtype, double alpha,
+             double beta, double delta, int use_beta, int use_delta) {
+    const size_t es = dtype_size(dtype);
+    void *W = nullptr;
+    if (H->p > 0) {
+        TT_TRY(ctx->scratch[3].ensure((size_t)H->p * k * es));
+        W = ctx->scratch[3].ptr;
+        PendingTiming tn;
+        TT_TRY(timing_begin(ctx, TTDFT_KSTAT_NLPROJ, &tn));
+        TT_TRY(launch_nl_project(ctx, H, X, ldx, k, dtype, W));
+        TT_TRY(timing_end(ctx, &tn, (double)H->nnz_c * (es + 4) + (double)H->nnz_c * k * es,
+                          2.0 * H->nnz_c * k));
+    }
+    PendingTiming ts;
+    TT_TRY(timing_begin(ctx, TTDFT_KSTAT_SPMM, &ts));
+    TT_TRY(launch_spmm_fused(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
+                             alpha, beta, delta, use_beta, use_delta));
+    // algorithmic bytes: H_loc values + indices + row pointers, X panel read once,
+    // Y written once, the recurrence operands read once (fused epilogue).
+    const double nk = (double)H->n * k * es;
+    const double bytes = (double)H->nnz * (es + 4) + 8.0 * (H->n + 1) + nk + nk +
+                         (use_delta ? nk : 0.0);
+    const double flops = 2.0 * H->nnz * k + 5.0 * H->n * k;
+    TT_TRY(timing_end(ctx, &ts, bytes, flops));
+    return TTDFT_OK;
+}
+
+}  // namespace ttdft

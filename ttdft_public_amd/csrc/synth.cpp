@@ -1,0 +1,220 @@
+// Synthetic Kronecker-banded Hamiltonians with the structure of the projected
+// Kohn-Sham Hamiltonian in a Tucker basis (ProjectHamiltonianSparse.cc:464-548):
+//   H = 1/(2h^2) (T_x (x) I (x) I + I (x) T_y (x) I + I (x) I (x) T_z)
+//       + sum_r g_r P_x^r (x) P_y^r (x) P_z^r
+// T: sinc-DVR second-derivative stencil (pi^2/3 on the diagonal,
+// 2 (-1)^d / d^2 off it) cut to the half-bandwidth; P^r: symmetric banded
+// factors env(i) env(j) exp(-(i-j)^2 / (2 s^2)) with a smooth seeded
+// modulation env in [0.5, 1.5].  Entries with |h| <= truncation are dropped,
+// as the reference's truncation does (:541-548).  Projector rows are products
+// of 1-D Gaussians over the basis index centred at seeded sites, normalised
+// to 2-norm 1/2 and truncated the same way.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/ttdft_chfsi.h"
+
+namespace {
+
+constexpr int kTerms = 2;
+
+struct Axis {
+    int R = 0, b = 0;
+    std::vector<double> T;                 // (2b+1) stencil, index d + b
+    std::vector<double> P[kTerms];         // R x (2b+1): P[r][i*(2b+1) + (j-i+b)]
+};
+
+uint64_t splitmix(uint64_t *s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double unif(uint64_t *s) { return (double)(splitmix(s) >> 11) * 0x1.0p-53; }
+
+Axis make_axis(int R, int b, uint64_t *rng) {
+    Axis ax;
+    ax.R = R;
+    ax.b = b;
+    ax.T.resize(2 * b + 1);
+    for (int d = -b; d <= b; ++d)
+        ax.T[d + b] = d == 0 ? M_PI * M_PI / 3.0 : 2.0 * ((d & 1) ? -1.0 : 1.0) / (double)(d * d);
+    const double s = b > 0 ? b / 2.4 : 1.0;
+    for (int r = 0; r < kTerms; ++r) {
+        const double freq = 1.0 + std::floor(3.0 * unif(rng));
+        const double phase = 2.0 * M_PI * unif(rng);
+        std::vector<double> env(R);
+        for (int i = 0; i < R; ++i) env[i] = 1.0 + 0.5 * std::cos(2.0 * M_PI * freq * i / R + phase);
+        ax.P[r].assign((size_t)R * (2 * b + 1), 0.0);
+        for (int i = 0; i < R; ++i)
+            for (int d = -b; d <= b; ++d) {
+                const int j = i + d;
+                if (j < 0 || j >= R) continue;
+                ax.P[r][(size_t)i * (2 * b + 1) + d + b] =
+                    env[i] * env[j] * std::exp(-(double)(d * d) / (2.0 * s * s));
+            }
+    }
+    return ax;
+}
+
+struct Gen {
+    Axis ax[3];
+    double kin = 0.0;
+    double g[kTerms] = {-1.0, -0.5};
+    double trunc = 0.0;
+
+    // value of H[(x,y,z), (x+dx, y+dy, z+dz)] (caller guarantees in range)
+    double value(int x, int y, int z, int dx, int dy, int dz) const {
+        double v = 0.0;
+        if (dy == 0 && dz == 0) v += kin * ax[0].T[dx + ax[0].b];
+        if (dx == 0 && dz == 0) v += kin * ax[1].T[dy + ax[1].b];
+        if (dx == 0 && dy == 0) v += kin * ax[2].T[dz + ax[2].b];
+        const int wx = 2 * ax[0].b + 1, wy = 2 * ax[1].b + 1, wz = 2 * ax[2].b + 1;
+        for (int r = 0; r < kTerms; ++r)
+            v += g[r] * ax[0].P[r][(size_t)x * wx + dx + ax[0].b] *
+                 ax[1].P[r][(size_t)y * wy + dy + ax[1].b] *
+                 ax[2].P[r][(size_t)z * wz + dz + ax[2].b];
+        return v;
+    }
+
+    // count (cols == nullptr) or emit the kept entries of row I, ascending columns
+    int64_t row(int64_t I, int32_t *cols, double *vals) const {
+        const int Rx = ax[0].R, Ry = ax[1].R, Rz = ax[2].R;
+        const int x = (int)(I % Rx), y = (int)((I / Rx) % Ry), z = (int)(I / ((int64_t)Rx * Ry));
+        int64_t cnt = 0;
+        for (int dz = -ax[2].b; dz <= ax[2].b; ++dz) {
+            const int zz = z + dz;
+            if (zz < 0 || zz >= Rz) continue;
+            for (int dy = -ax[1].b; dy <= ax[1].b; ++dy) {
+                const int yy = y + dy;
+                if (yy < 0 || yy >= Ry) continue;
+                for (int dx = -ax[0].b; dx <= ax[0].b; ++dx) {
+                    const int xx = x + dx;
+                    if (xx < 0 || xx >= Rx) continue;
+                    const double v = value(x, y, z, dx, dy, dz);
+                    if (std::fabs(v) > trunc) {
+                        if (cols) {
+                            cols[cnt] = (int32_t)(xx + (int64_t)Rx * yy + (int64_t)Rx * Ry * zz);
+                            vals[cnt] = v;
+                        }
+                        ++cnt;
+                    }
+                }
+            }
+        }
+        return cnt;
+    }
+};
+
+template <typename F>
+void parallel_rows(int64_t n, F f) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    if (nt > 32) nt = 32;
+    if (n < 4096) nt = 1;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([=]() {
+            const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+            for (int64_t i = lo; i < hi; ++i) f(i);
+        });
+    for (auto &x : th) x.join();
+}
+
+struct Projector {
+    double c[3];
+    double norm;  // scale applied to the raw product
+};
+
+double gauss1(double i, double c, double s) { return std::exp(-(i - c) * (i - c) / (2.0 * s * s)); }
+
+}  // namespace
+
+extern "C" int ttdft_synth_hamiltonian(const ttdft_synth_params *prm, int64_t *nnz,
+                                       int64_t *nnz_c, int64_t *rowptr, int32_t *colind,
+                                       double *val, int64_t *c_rowptr, int32_t *c_colind,
+                                       double *c_val, double *inv_c_lm) {
+    if (!prm || !nnz || !nnz_c || prm->rx < 1 || prm->ry < 1 || prm->rz < 1 || prm->bx < 0 ||
+        prm->by < 0 || prm->bz < 0 || prm->p < 0 || !(prm->h > 0.0) || prm->truncation < 0.0)
+        return TTDFT_EBADARG;
+    const int64_t n = (int64_t)prm->rx * prm->ry * prm->rz;
+    if (n > (int64_t)INT32_MAX) return TTDFT_EBADARG;
+    uint64_t rng = prm->seed;
+    Gen G;
+    G.ax[0] = make_axis(prm->rx, prm->bx, &rng);
+    G.ax[1] = make_axis(prm->ry, prm->by, &rng);
+    G.ax[2] = make_axis(prm->rz, prm->bz, &rng);
+    G.kin = 1.0 / (2.0 * prm->h * prm->h);
+    G.trunc = prm->truncation;
+
+    // projector sites and normalisation (1-D sums of squares factorise)
+    const double ps = 1.5;
+    std::vector<Projector> proj(prm->p);
+    for (int r = 0; r < prm->p; ++r) {
+        const int R[3] = {prm->rx, prm->ry, prm->rz};
+        double ss = 1.0;
+        for (int d = 0; d < 3; ++d) {
+            proj[r].c[d] = unif(&rng) * (R[d] - 1);
+            double s1 = 0.0;
+            for (int i = 0; i < R[d]; ++i) s1 += gauss1(i, proj[r].c[d], ps) * gauss1(i, proj[r].c[d], ps);
+            ss *= s1;
+        }
+        proj[r].norm = 0.5 / std::sqrt(ss);
+    }
+    auto proj_row = [&](int r, int32_t *cols, double *vals) -> int64_t {
+        const int Rx = prm->rx, Ry = prm->ry, Rz = prm->rz;
+        int64_t cnt = 0;
+        for (int z = 0; z < Rz; ++z) {
+            const double gz = gauss1(z, proj[r].c[2], ps);
+            for (int y = 0; y < Ry; ++y) {
+                const double gy = gauss1(y, proj[r].c[1], ps) * gz;
+                for (int x = 0; x < Rx; ++x) {
+                    const double v = proj[r].norm * gauss1(x, proj[r].c[0], ps) * gy;
+                    if (std::fabs(v) > prm->truncation) {
+                        if (cols) {
+                            cols[cnt] = (int32_t)(x + (int64_t)Rx * y + (int64_t)Rx * Ry * z);
+                            vals[cnt] = v;
+                        }
+                        ++cnt;
+                    }
+                }
+            }
+        }
+        return cnt;
+    };
+
+    if (!rowptr) {  // sizing pass
+        std::vector<int64_t> cnt(n);
+        parallel_rows(n, [&](int64_t i) { cnt[i] = G.row(i, nullptr, nullptr); });
+        int64_t t = 0;
+        for (int64_t c : cnt) t += c;
+        *nnz = t;
+        int64_t tc = 0;
+        for (int r = 0; r < prm->p; ++r) tc += proj_row(r, nullptr, nullptr);
+        *nnz_c = tc;
+        return TTDFT_OK;
+    }
+    if (!colind || !val || (prm->p > 0 && (!c_rowptr || !c_colind || !c_val || !inv_c_lm)))
+        return TTDFT_EBADARG;
+    {
+        std::vector<int64_t> cnt(n);
+        parallel_rows(n, [&](int64_t i) { cnt[i] = G.row(i, nullptr, nullptr); });
+        rowptr[0] = 0;
+        for (int64_t i = 0; i < n; ++i) rowptr[i + 1] = rowptr[i] + cnt[i];
+        if (rowptr[n] != *nnz) return TTDFT_EBADARG;
+        parallel_rows(n, [&](int64_t i) { G.row(i, colind + rowptr[i], val + rowptr[i]); });
+    }
+    if (prm->p > 0) {
+        c_rowptr[0] = 0;
+        for (int r = 0; r < prm->p; ++r) {
+            const int64_t c = proj_row(r, c_colind + c_rowptr[r], c_val + c_rowptr[r]);
+            c_rowptr[r + 1] = c_rowptr[r] + c;
+            inv_c_lm[r] = 1.0;
+        }
+        if (c_rowptr[prm->p] != *nnz_c) return TTDFT_EBADARG;
+    }
+    return TTDFT_OK;
+}

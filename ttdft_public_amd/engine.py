@@ -1,0 +1,443 @@
+"""ctypes binding of the engine's C ABI (include/ttdft_chfsi.h).
+
+Device buffers are plain integers (device addresses); callers that use torch
+pass ``tensor.data_ptr()`` and make the context issue work on torch's current
+stream with :meth:`Context.use_stream`.  Every call raises
+:class:`TtdftError` on a nonzero status, with the library's message.
+
+The library must have been built (``__graft_entry__.build()``); loading fails
+loudly otherwise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libttdft_chfsi.so")
+
+OK, EBADARG, EHIP, ENOTSPD, ENOCONV, ENOMEM, ELIB = 0, -1, -2, -3, -4, -5, -6
+F64, F32 = 0, 1
+GEMM_F64, GEMM_BF16X3 = 0, 1
+DEGREE_REFERENCE, DEGREE_EXACT = 0, 1
+HAM_F64, HAM_F32 = 1, 2
+KSTAT_SPMM, KSTAT_NLPROJ, KSTAT_GRAM, KSTAT_ROTATE, KSTAT_DENSITY = range(5)
+
+_c_dp = ctypes.POINTER(ctypes.c_double)
+_c_i64p = ctypes.POINTER(ctypes.c_int64)
+_c_i32p = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_d = ctypes.c_double
+
+
+class TtdftError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"ttdft status {status}: {msg}")
+        self.status = status
+
+
+class ChfsiState(ctypes.Structure):
+    _fields_ = [
+        ("upper", ctypes.c_double),
+        ("lower_unwanted", ctypes.c_double),
+        ("lower_wanted", ctypes.c_double),
+        ("err_upper", ctypes.c_double),
+        ("occ_energy", ctypes.c_double),
+        ("passes", ctypes.c_int32),
+        ("lanczos_runs", ctypes.c_int32),
+    ]
+
+
+class SynthParams(ctypes.Structure):
+    _fields_ = [
+        ("rx", ctypes.c_int32), ("ry", ctypes.c_int32), ("rz", ctypes.c_int32),
+        ("bx", ctypes.c_int32), ("by", ctypes.c_int32), ("bz", ctypes.c_int32),
+        ("p", ctypes.c_int32),
+        ("h", ctypes.c_double),
+        ("truncation", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "ttdft_last_error": (ctypes.c_char_p, []),
+    "ttdft_get_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "ttdft_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "ttdft_ctx_destroy": (ctypes.c_int, [_vp]),
+    "ttdft_ctx_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "ttdft_ctx_synchronize": (ctypes.c_int, [_vp]),
+    "ttdft_ctx_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "ttdft_ctx_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_int, _c_i64p, _c_dp, _c_dp, _c_dp]),
+    "ttdft_ctx_reset_stats": (ctypes.c_int, [_vp]),
+    "ttdft_malloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "ttdft_free": (ctypes.c_int, [_vp, _vp]),
+    "ttdft_memcpy_h2d": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "ttdft_memcpy_d2h": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "ttdft_memcpy_d2d": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t]),
+    "ttdft_block_upload_colmajor": (ctypes.c_int, [_vp, _c_dp, _i64, _i64, _i32, _vp, _i64,
+                                                   ctypes.c_int]),
+    "ttdft_block_download_colmajor": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int, _i64, _i32,
+                                                     _c_dp, _i64]),
+    "ttdft_block_convert": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int, _vp, _i64, ctypes.c_int,
+                                           _i64, _i32]),
+    "ttdft_ham_upload": (ctypes.c_int, [_vp, _i64, _c_i64p, _c_i32p, _c_dp, _i32, _c_i64p,
+                                        _c_i32p, _c_dp, _c_dp, ctypes.c_int,
+                                        ctypes.POINTER(_vp)]),
+    "ttdft_ham_free": (ctypes.c_int, [_vp]),
+    "ttdft_ham_info": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _c_i32p]),
+    "ttdft_ham_apply": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _i64, _i32, ctypes.c_int]),
+    "ttdft_cheb_filter": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _d, _d, _d, _i32,
+                                         ctypes.c_int, ctypes.c_int]),
+    "ttdft_gram": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _i64, _i32, ctypes.c_int,
+                                  ctypes.c_int, _vp]),
+    "ttdft_orth": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i32, ctypes.c_int, ctypes.c_int]),
+    "ttdft_rr_eig": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp]),
+    "ttdft_rotate": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i32, _vp, ctypes.c_int,
+                                    ctypes.c_int]),
+    "ttdft_lanczos_bounds": (ctypes.c_int, [_vp, _vp, _i32, ctypes.c_uint64, _c_dp, _c_dp]),
+    "ttdft_fermi_occupancy": (ctypes.c_int, [_c_dp, _i32, _i32, _d, _d, _i32, _c_dp, _c_dp]),
+    "ttdft_band_energy": (ctypes.c_double, [_c_dp, _c_dp, _i32]),
+    "ttdft_density": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_int, _i32, _c_dp, _i32, _i32, _i32,
+                                     _vp, _vp, _vp, _i32, _i32, _i32, _vp]),
+    "ttdft_initial_density": (ctypes.c_int, [_vp, _i32, _c_dp, _i32, _c_dp, _c_dp, _i32, _i32,
+                                             _i32, _vp, _vp, _vp, _vp]),
+    "ttdft_grid_integral": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _c_dp]),
+    "ttdft_grid_residual": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp,
+                                           _c_dp]),
+    "ttdft_grid_scale": (ctypes.c_int, [_vp, _vp, _i64, _d]),
+    "ttdft_chfsi_state_init": (ctypes.c_int, [ctypes.POINTER(ChfsiState)]),
+    "ttdft_chfsi_step": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i32,
+                                        _i32, _i32, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                        _c_dp, ctypes.POINTER(ChfsiState)]),
+    "ttdft_random_subspace": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i32, ctypes.c_uint64,
+                                             ctypes.c_int]),
+    "ttdft_synth_hamiltonian": (ctypes.c_int, [ctypes.POINTER(SynthParams), _c_i64p, _c_i64p,
+                                               _c_i64p, _c_i32p, _c_dp, _c_i64p, _c_i32p, _c_dp,
+                                               _c_dp]),
+    "ttdft_rng_uniform": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
+    "ttdft_rng_normal": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+def lib():
+    """Load libttdft_chfsi.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        msg = lib().ttdft_last_error()
+        raise TtdftError(rc, msg.decode() if msg else "")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_c_dp)
+
+
+def _i64ptr(a):
+    return a.ctypes.data_as(_c_i64p)
+
+
+def _i32ptr(a):
+    return a.ctypes.data_as(_c_i32p)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(lib().ttdft_get_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def rng_uniform(seed, count):
+    out = np.empty(count)
+    _check(lib().ttdft_rng_uniform(seed, count, _dptr(out)))
+    return out
+
+
+def rng_normal(seed, count):
+    out = np.empty(count)
+    _check(lib().ttdft_rng_normal(seed, count, _dptr(out)))
+    return out
+
+
+def fermi_occupancy(evals, nelec, kb=3.166811429e-06, temperature=500.0, maxit=20):
+    e = np.ascontiguousarray(evals, dtype=np.float64)
+    occ = np.empty_like(e)
+    ef = ctypes.c_double()
+    _check(lib().ttdft_fermi_occupancy(_dptr(e), len(e), int(nelec), kb, temperature, maxit,
+                                       ctypes.byref(ef), _dptr(occ)))
+    return ef.value, occ
+
+
+def band_energy(evals, occ):
+    e = np.ascontiguousarray(evals, dtype=np.float64)
+    o = np.ascontiguousarray(occ, dtype=np.float64)
+    return lib().ttdft_band_energy(_dptr(e), _dptr(o), len(e))
+
+
+class SynthHam:
+    """Synthetic Kronecker-banded Hamiltonian (CSR arrays in numpy)."""
+
+    def __init__(self, rx, ry, rz, bx, by, bz, p, h=0.8, truncation=1e-6, seed=20261202):
+        prm = SynthParams(rx, ry, rz, bx, by, bz, p, h, truncation, seed)
+        nnz = ctypes.c_int64()
+        nnz_c = ctypes.c_int64()
+        _check(lib().ttdft_synth_hamiltonian(ctypes.byref(prm), ctypes.byref(nnz),
+                                             ctypes.byref(nnz_c), None, None, None, None, None,
+                                             None, None))
+        self.dims = (rx, ry, rz)
+        self.n = rx * ry * rz
+        self.p = p
+        self.rowptr = np.empty(self.n + 1, np.int64)
+        self.col = np.empty(max(nnz.value, 1), np.int32)
+        self.val = np.empty(max(nnz.value, 1), np.float64)
+        self.c_rowptr = np.zeros(p + 1, np.int64)
+        self.c_col = np.empty(max(nnz_c.value, 1), np.int32)
+        self.c_val = np.empty(max(nnz_c.value, 1), np.float64)
+        self.inv_c_lm = np.ones(max(p, 1), np.float64)
+        _check(lib().ttdft_synth_hamiltonian(ctypes.byref(prm), ctypes.byref(nnz),
+                                             ctypes.byref(nnz_c), _i64ptr(self.rowptr),
+                                             _i32ptr(self.col), _dptr(self.val),
+                                             _i64ptr(self.c_rowptr), _i32ptr(self.c_col),
+                                             _dptr(self.c_val), _dptr(self.inv_c_lm)))
+        self.nnz = nnz.value
+        self.nnz_c = nnz_c.value
+        self.col = self.col[: self.nnz]
+        self.val = self.val[: self.nnz]
+        self.c_col = self.c_col[: self.nnz_c]
+        self.c_val = self.c_val[: self.nnz_c]
+        self.inv_c_lm = self.inv_c_lm[:p]
+
+
+class Context:
+    """One device + one stream (ttdft_ctx)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        _check(lib().ttdft_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            _check(lib().ttdft_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def use_stream(self, stream_handle):
+        _check(lib().ttdft_ctx_set_stream(self.h, stream_handle))
+
+    def synchronize(self):
+        _check(lib().ttdft_ctx_synchronize(self.h))
+
+    def set_profiling(self, on=True):
+        _check(lib().ttdft_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_stats(self, cls):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        b = ctypes.c_double()
+        f = ctypes.c_double()
+        _check(lib().ttdft_ctx_kernel_stats(self.h, cls, ctypes.byref(n), ctypes.byref(ms),
+                                            ctypes.byref(b), ctypes.byref(f)))
+        return {"launches": n.value, "ms": ms.value, "bytes": b.value, "flops": f.value}
+
+    def reset_stats(self):
+        _check(lib().ttdft_ctx_reset_stats(self.h))
+
+    # ---- Hamiltonian ----
+    def ham_upload(self, s: SynthHam, copies=HAM_F64):
+        return Hamiltonian(self, s.n, s.rowptr, s.col, s.val, s.p, s.c_rowptr, s.c_col, s.c_val,
+                           s.inv_c_lm, copies)
+
+    def ham_apply(self, ham, X, ldx, Y, ldy, k, dtype=F64):
+        _check(lib().ttdft_ham_apply(self.h, ham.h, X, ldx, Y, ldy, k, dtype))
+
+    def cheb_filter(self, ham, X, ldx, k, m, a, b, a0, num_blocks=1, dtype=F64,
+                    convention=DEGREE_REFERENCE):
+        _check(lib().ttdft_cheb_filter(self.h, ham.h, X, ldx, k, m, a, b, a0, num_blocks, dtype,
+                                       convention))
+
+    def gram(self, X, ldx, Y, ldy, n, k, dS, dtype=F64, mode=GEMM_F64):
+        _check(lib().ttdft_gram(self.h, X, ldx, Y, ldy, n, k, dtype, mode, dS))
+
+    def orth(self, X, ldx, n, k, dtype=F64, mode=GEMM_F64):
+        _check(lib().ttdft_orth(self.h, X, ldx, n, k, dtype, mode))
+
+    def rr_eig(self, dA, k, d_evals, dQ):
+        _check(lib().ttdft_rr_eig(self.h, dA, k, d_evals, dQ))
+
+    def rotate(self, X, ldx, n, k, dQ, dtype=F64, mode=GEMM_F64):
+        _check(lib().ttdft_rotate(self.h, X, ldx, n, k, dQ, dtype, mode))
+
+    def lanczos_bounds(self, ham, maxit=20, seed=20261202):
+        up = ctypes.c_double()
+        lo = ctypes.c_double()
+        _check(lib().ttdft_lanczos_bounds(self.h, ham.h, maxit, seed, ctypes.byref(up),
+                                          ctypes.byref(lo)))
+        return up.value, lo.value
+
+    def density(self, X, ldx, k, occ, rdims, dBx, dBy, dBz, pdims, d_rho, dtype=F64):
+        o = np.ascontiguousarray(occ, dtype=np.float64)
+        _check(lib().ttdft_density(self.h, X, ldx, dtype, k, _dptr(o), rdims[0], rdims[1],
+                                   rdims[2], dBx, dBy, dBz, pdims[0], pdims[1], pdims[2], d_rho))
+
+    def initial_density(self, xyz, r, rho_r, dgx, dgy, dgz, pdims, d_rho):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        rho_r = np.ascontiguousarray(rho_r, dtype=np.float64)
+        _check(lib().ttdft_initial_density(self.h, len(xyz) // 3, _dptr(xyz), len(r), _dptr(r),
+                                           _dptr(rho_r), pdims[0], pdims[1], pdims[2], dgx, dgy,
+                                           dgz, d_rho))
+
+    def grid_integral(self, d_f, pdims, dwx, dwy, dwz):
+        out = ctypes.c_double()
+        _check(lib().ttdft_grid_integral(self.h, d_f, pdims[0], pdims[1], pdims[2], dwx, dwy, dwz,
+                                         ctypes.byref(out)))
+        return out.value
+
+    def grid_residual(self, d_f, d_g, pdims, dwx, dwy, dwz):
+        out = ctypes.c_double()
+        _check(lib().ttdft_grid_residual(self.h, d_f, d_g, pdims[0], pdims[1], pdims[2], dwx, dwy,
+                                         dwz, ctypes.byref(out)))
+        return out.value
+
+    def grid_scale(self, d_f, count, s):
+        _check(lib().ttdft_grid_scale(self.h, d_f, count, s))
+
+    def chfsi_step(self, ham, X, ldx, k, m, evals, state, scf_iter, occ_index, restart_times,
+                   num_blocks=1, start_iter=0, maxit_lanczos=20, seed=20261202,
+                   convention=DEGREE_REFERENCE, precision=F64):
+        assert evals.dtype == np.float64 and evals.flags.c_contiguous and len(evals) == k
+        _check(lib().ttdft_chfsi_step(self.h, ham.h, X, ldx, k, m, num_blocks, restart_times,
+                                      occ_index, scf_iter, start_iter, maxit_lanczos, seed,
+                                      convention, precision, _dptr(evals), ctypes.byref(state)))
+        return state
+
+    def random_subspace(self, X, ldx, n, k, seed=20261202, dtype=F64):
+        _check(lib().ttdft_random_subspace(self.h, X, ldx, n, k, seed, dtype))
+
+    def block_upload_colmajor(self, host_cm, dX, ldx, dtype=F64):
+        a = np.asfortranarray(host_cm, dtype=np.float64)
+        n, k = a.shape
+        _check(lib().ttdft_block_upload_colmajor(self.h, a.ctypes.data_as(_c_dp), n, n, k, dX,
+                                                 ldx, dtype))
+
+    def block_download_colmajor(self, dX, ldx, n, k, dtype=F64):
+        out = np.empty((n, k), order="F")
+        _check(lib().ttdft_block_download_colmajor(self.h, dX, ldx, dtype, n, k,
+                                                   out.ctypes.data_as(_c_dp), n))
+        return out
+
+
+def new_state():
+    st = ChfsiState()
+    _check(lib().ttdft_chfsi_state_init(ctypes.byref(st)))
+    return st
+
+
+class Hamiltonian:
+    """Device copy of H = H_loc + C^T D C (ttdft_ham)."""
+
+    def __init__(self, ctx, n, rowptr, col, val, p, c_rowptr, c_col, c_val, inv_c_lm,
+                 copies=HAM_F64):
+        self.ctx = ctx
+        rowptr = np.ascontiguousarray(rowptr, np.int64)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        h = _vp()
+        if p > 0:
+            c_rowptr = np.ascontiguousarray(c_rowptr, np.int64)
+            c_col = np.ascontiguousarray(c_col, np.int32)
+            c_val = np.ascontiguousarray(c_val, np.float64)
+            inv_c_lm = np.ascontiguousarray(inv_c_lm, np.float64)
+            args = (_i64ptr(c_rowptr), _i32ptr(c_col), _dptr(c_val), _dptr(inv_c_lm))
+        else:
+            args = (None, None, None, None)
+        _check(lib().ttdft_ham_upload(ctx.h, n, _i64ptr(rowptr), _i32ptr(col), _dptr(val), p,
+                                      *args, copies, ctypes.byref(h)))
+        self.h = h
+        self.n = n
+
+    def info(self):
+        n = ctypes.c_int64()
+        nnz = ctypes.c_int64()
+        p = ctypes.c_int32()
+        _check(lib().ttdft_ham_info(self.h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(p)))
+        return n.value, nnz.value, p.value
+
+    def free(self):
+        if self.h:
+            _check(lib().ttdft_ham_free(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """Device buffer owned through the library's allocator (ttdft_malloc)."""
+
+    def __init__(self, ctx, shape, dtype=np.float64):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        _check(lib().ttdft_malloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = cls(ctx, a.shape, a.dtype)
+        _check(lib().ttdft_memcpy_h2d(ctx.h, d.ptr, a.ctypes.data_as(_vp), a.nbytes))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().ttdft_memcpy_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        _check(lib().ttdft_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(_vp), a.nbytes))
+
+    def free(self):
+        if self.ptr:
+            _check(lib().ttdft_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
