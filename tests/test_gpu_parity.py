@@ -121,7 +121,8 @@ def test_gram_f64(ctx, n, k):
     rng = np.random.default_rng(n + k)
     X, Y = rng.standard_normal((n, k)), rng.standard_normal((n, k))
     dS = E.DeviceArray(ctx, (k, k))
-    ctx.gram(dev(ctx, X).ptr, k, dev(ctx, Y).ptr, k, n, k, dS.ptr)
+    dX, dY = dev(ctx, X), dev(ctx, Y)  # keep the buffers alive through the call
+    ctx.gram(dX.ptr, k, dY.ptr, k, n, k, dS.ptr)
     S = dS.to_host().T  # column-major on device
     assert rel(S, oracle.gram(X, Y)) < 1e-13
 
@@ -132,8 +133,8 @@ def test_gram_bf16x3(ctx, n, k):
     X = rng.standard_normal((n, k)).astype(np.float32)
     Y = rng.standard_normal((n, k)).astype(np.float32)
     dS = E.DeviceArray(ctx, (k, k))
-    ctx.gram(dev(ctx, X, np.float32).ptr, k, dev(ctx, Y, np.float32).ptr, k, n, k, dS.ptr,
-             E.F32, E.GEMM_BF16X3)
+    dX, dY = dev(ctx, X, np.float32), dev(ctx, Y, np.float32)
+    ctx.gram(dX.ptr, k, dY.ptr, k, n, k, dS.ptr, E.F32, E.GEMM_BF16X3)
     ref = X.astype(np.float64).T @ Y.astype(np.float64)
     # error relative to the natural scale sqrt(n) of a Gram entry
     assert np.abs(dS.to_host().T - ref).max() / np.sqrt(n) < 5e-5
@@ -171,7 +172,7 @@ def test_orth_matches_oracle_f64(ctx):
 def test_orth_not_spd_reports_error(ctx):
     n, k = 300, 8
     X = np.random.default_rng(2).standard_normal((n, k))
-    X[:, 5] = X[:, 2]  # rank deficient
+    X[:, 5] = 0.0  # singular overlap: pivot 5 is exactly zero
     dX = dev(ctx, X)
     with pytest.raises(E.TtdftError) as ei:
         ctx.orth(dX.ptr, k, n, k)
@@ -215,9 +216,10 @@ def test_density(ctx, dtype):
     X = rng.standard_normal((rx * ry * rz, k)).astype(npdt)
     occ = np.linspace(1.0, 0.0, k)  # last state has zero weight
     dRho = E.DeviceArray(ctx, (pz, py, px))
-    ctx.density(dev(ctx, X, npdt).ptr, k, k, occ, (rx, ry, rz), dev(ctx, Bx.T.copy()).ptr,
-                dev(ctx, By.T.copy()).ptr, dev(ctx, Bz.T.copy()).ptr, (px, py, pz), dRho.ptr,
-                dtype)
+    dX = dev(ctx, X, npdt)
+    dBx, dBy, dBz = dev(ctx, Bx.T.copy()), dev(ctx, By.T.copy()), dev(ctx, Bz.T.copy())
+    ctx.density(dX.ptr, k, k, occ, (rx, ry, rz), dBx.ptr, dBy.ptr, dBz.ptr, (px, py, pz),
+                dRho.ptr, dtype)
     ref = oracle.density(X.astype(np.float64), occ, (rx, ry, rz), Bx, By, Bz)
     got = dRho.to_host()
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
@@ -235,8 +237,8 @@ def test_initial_density_and_integrals(ctx):
     wz = wx[::2].copy()
     dims = (len(gx), len(gy), len(gz))
     dR = E.DeviceArray(ctx, (dims[2], dims[1], dims[0]))
-    ctx.initial_density(xyz, r, rho_r, dev(ctx, gx).ptr, dev(ctx, gy).ptr, dev(ctx, gz).ptr,
-                        dims, dR.ptr)
+    dgx, dgy, dgz = dev(ctx, gx), dev(ctx, gy), dev(ctx, gz)
+    ctx.initial_density(xyz, r, rho_r, dgx.ptr, dgy.ptr, dgz.ptr, dims, dR.ptr)
     ref = oracle.initial_density(xyz, r, rho_r, gx, gy, gz)
     got = dR.to_host()
     assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()

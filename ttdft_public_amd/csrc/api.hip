@@ -51,7 +51,10 @@ void clear_error() { g_err.clear(); }
 int Scratch::ensure(size_t need) {
     if (need <= bytes) return TTDFT_OK;
     if (ptr) {
-        hipError_t e = hipFree(ptr);
+        // the old buffer may still be read by queued kernels
+        hipError_t e = hipDeviceSynchronize();
+        (void)e;
+        e = hipFree(ptr);
         (void)e;
         ptr = nullptr;
         bytes = 0;
