@@ -1,14 +1,19 @@
 // Synthetic Kronecker-banded Hamiltonians with the structure of the projected
 // Kohn-Sham Hamiltonian in a Tucker basis (ProjectHamiltonianSparse.cc:464-548):
 //   H = 1/(2h^2) (T_x (x) I (x) I + I (x) T_y (x) I + I (x) I (x) T_z)
-//       + sum_r g_r P_x^r (x) P_y^r (x) P_z^r
+//       - V0 F_x (x) F_y (x) F_z + g P_x (x) P_y (x) P_z
 // T: sinc-DVR second-derivative stencil (pi^2/3 on the diagonal,
-// 2 (-1)^d / d^2 off it) cut to the half-bandwidth; P^r: symmetric banded
-// factors env(i) env(j) exp(-(i-j)^2 / (2 s^2)) with a smooth seeded
-// modulation env in [0.5, 1.5].  Entries with |h| <= truncation are dropped,
-// as the reference's truncation does (:541-548).  Projector rows are products
-// of 1-D Gaussians over the basis index centred at seeded sites, normalised
-// to 2-norm 1/2 and truncated the same way.
+// 2 (-1)^d / d^2 off it) cut to the half-bandwidth, which sets the top of the
+// spectrum; F: diagonal smooth box over the central 60% of each axis (a
+// potential well holding a few hundred shallow bound states); P: symmetric
+// banded factors env(i) env(j) exp(-(i-j)^2 / (2 s^2)) with a slowly varying
+// seeded envelope env in [0.75, 1.25] and a wide s, so every entry of the
+// Kronecker band survives the truncation while the term stays a small
+// perturbation (|g| sum |P|^3 well below 1 Ha).  Entries with
+// |h| <= truncation are dropped, as the reference's truncation does
+// (:541-548).  Projector rows are products of 1-D Gaussians over the basis
+// index centred at seeded sites, normalised to 2-norm 1/2 and truncated the
+// same way.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -19,12 +24,11 @@
 
 namespace {
 
-constexpr int kTerms = 2;
-
 struct Axis {
     int R = 0, b = 0;
-    std::vector<double> T;                 // (2b+1) stencil, index d + b
-    std::vector<double> P[kTerms];         // R x (2b+1): P[r][i*(2b+1) + (j-i+b)]
+    std::vector<double> T;  // (2b+1) stencil, index d + b
+    std::vector<double> F;  // R: diagonal well profile
+    std::vector<double> P;  // R x (2b+1): P[i*(2b+1) + (j-i+b)]
 };
 
 uint64_t splitmix(uint64_t *s) {
@@ -42,28 +46,31 @@ Axis make_axis(int R, int b, uint64_t *rng) {
     ax.T.resize(2 * b + 1);
     for (int d = -b; d <= b; ++d)
         ax.T[d + b] = d == 0 ? M_PI * M_PI / 3.0 : 2.0 * ((d & 1) ? -1.0 : 1.0) / (double)(d * d);
-    const double s = b > 0 ? b / 2.4 : 1.0;
-    for (int r = 0; r < kTerms; ++r) {
-        const double freq = 1.0 + std::floor(3.0 * unif(rng));
-        const double phase = 2.0 * M_PI * unif(rng);
-        std::vector<double> env(R);
-        for (int i = 0; i < R; ++i) env[i] = 1.0 + 0.5 * std::cos(2.0 * M_PI * freq * i / R + phase);
-        ax.P[r].assign((size_t)R * (2 * b + 1), 0.0);
-        for (int i = 0; i < R; ++i)
-            for (int d = -b; d <= b; ++d) {
-                const int j = i + d;
-                if (j < 0 || j >= R) continue;
-                ax.P[r][(size_t)i * (2 * b + 1) + d + b] =
-                    env[i] * env[j] * std::exp(-(double)(d * d) / (2.0 * s * s));
-            }
-    }
+    // well: Fermi-function box of half-width 0.3 R around the axis centre
+    ax.F.resize(R);
+    const double c = 0.5 * (R - 1), hw = 0.3 * R;
+    for (int i = 0; i < R; ++i) ax.F[i] = 1.0 / (1.0 + std::exp(std::fabs(i - c) - hw));
+    const double s = 10.0;
+    const double freq = 1.0 + std::floor(3.0 * unif(rng));
+    const double phase = 2.0 * M_PI * unif(rng);
+    std::vector<double> env(R);
+    for (int i = 0; i < R; ++i) env[i] = 1.0 + 0.25 * std::cos(2.0 * M_PI * freq * i / R + phase);
+    ax.P.assign((size_t)R * (2 * b + 1), 0.0);
+    for (int i = 0; i < R; ++i)
+        for (int d = -b; d <= b; ++d) {
+            const int j = i + d;
+            if (j < 0 || j >= R) continue;
+            ax.P[(size_t)i * (2 * b + 1) + d + b] =
+                env[i] * env[j] * std::exp(-(double)(d * d) / (2.0 * s * s));
+        }
     return ax;
 }
 
 struct Gen {
     Axis ax[3];
     double kin = 0.0;
-    double g[kTerms] = {-1.0, -0.5};
+    double well = 2.5;   // V0 (Ha)
+    double g = -3e-5;    // banded coupling strength
     double trunc = 0.0;
 
     // value of H[(x,y,z), (x+dx, y+dy, z+dz)] (caller guarantees in range)
@@ -72,11 +79,10 @@ struct Gen {
         if (dy == 0 && dz == 0) v += kin * ax[0].T[dx + ax[0].b];
         if (dx == 0 && dz == 0) v += kin * ax[1].T[dy + ax[1].b];
         if (dx == 0 && dy == 0) v += kin * ax[2].T[dz + ax[2].b];
+        if (dx == 0 && dy == 0 && dz == 0) v -= well * ax[0].F[x] * ax[1].F[y] * ax[2].F[z];
         const int wx = 2 * ax[0].b + 1, wy = 2 * ax[1].b + 1, wz = 2 * ax[2].b + 1;
-        for (int r = 0; r < kTerms; ++r)
-            v += g[r] * ax[0].P[r][(size_t)x * wx + dx + ax[0].b] *
-                 ax[1].P[r][(size_t)y * wy + dy + ax[1].b] *
-                 ax[2].P[r][(size_t)z * wz + dz + ax[2].b];
+        v += g * ax[0].P[(size_t)x * wx + dx + ax[0].b] * ax[1].P[(size_t)y * wy + dy + ax[1].b] *
+             ax[2].P[(size_t)z * wz + dz + ax[2].b];
         return v;
     }
 

@@ -197,7 +197,7 @@ def band_energy(evals, occ):
 class SynthHam:
     """Synthetic Kronecker-banded Hamiltonian (CSR arrays in numpy)."""
 
-    def __init__(self, rx, ry, rz, bx, by, bz, p, h=0.8, truncation=1e-6, seed=20261202):
+    def __init__(self, rx, ry, rz, bx, by, bz, p, h=2.0, truncation=1e-6, seed=20261202):
         prm = SynthParams(rx, ry, rz, bx, by, bz, p, h, truncation, seed)
         nnz = ctypes.c_int64()
         nnz_c = ctypes.c_int64()
