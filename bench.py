@@ -55,7 +55,10 @@ def basis_tables(R, P, seed):
     centers = np.linspace(-38.0, 38.0, R)
     width = 1.2 * (centers[1] - centers[0])
     B = np.exp(-((pts[:, None] - centers[None, :]) ** 2) / (2 * width**2))
-    return pts, wts, np.asfortranarray(B)
+    # orthonormal under the quadrature (as the Tucker basis is), so the density
+    # of an orthonormal block integrates to sum_i 2 f_i = N_e
+    Q, _ = np.linalg.qr(np.sqrt(wts)[:, None] * B)
+    return pts, wts, np.asfortranarray(Q / np.sqrt(wts)[:, None])
 
 
 def cpu_baseline_sample(s, threads, reps=3):
@@ -200,6 +203,7 @@ def main():
                              "peak": FP64_PEAK_TFLOPS},
         "density_tflops": tf(de),
         "charge": q64,
+        "nelec": NELEC,
     }
     if not args.no_mixed:
         wall32, st32, ev32, ef32, q32, _ = run(E.F32, args.steps, args.warmup)

@@ -138,6 +138,24 @@ int ttdft_ham_upload(ttdft_ctx ctx, int64_t n, const int64_t *rowptr,
                      const int64_t *c_rowptr, const int32_t *c_colind,
                      const double *c_val, const double *inv_c_lm, int copies,
                      ttdft_ham *out);
+/* As ttdft_ham_upload, for H_loc in a Tucker product basis with ranks
+ * (rx, ry, rz), n = rx*ry*rz, basis index x + rx*y + rx*ry*z.  The ranks let the
+ * engine detect the Kronecker band structure of H_loc (ProjectHamiltonianSparse.cc
+ * :420-491: |x-x'| <= bx, |y-y'| <= by, |z-z'| <= bz) and store it as
+ * index-free band blocks for the register-blocked band SpMM; when the band is
+ * too sparse or too wide the CSR kernel is used.  The reference passes the
+ * same ranks to Create_Hloc (ProjectHamiltonianSparse.cc:330-340). */
+int ttdft_ham_upload_tucker(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz,
+                            const int64_t *rowptr, const int32_t *colind,
+                            const double *val, int32_t p, const int64_t *c_rowptr,
+                            const int32_t *c_colind, const double *c_val,
+                            const double *inv_c_lm, int copies, ttdft_ham *out);
+
+/* Storage chosen for H_loc: *format = 0 (CSR) or 1 (Kronecker band), and the
+ * half-bandwidths found (bands[3], zero for CSR).  Introspection only; no
+ * reference counterpart. */
+int ttdft_ham_format(ttdft_ham ham, int32_t *format, int32_t *bands);
+
 /* replaces AX::free_mata_on_device / AX::~AX (MultCuda.cu:673-717) */
 int ttdft_ham_free(ttdft_ham ham);
 /* Basis size n, H_loc nnz, projector count p. */

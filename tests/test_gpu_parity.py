@@ -89,6 +89,64 @@ def test_ham_upload_rejects_bad_csr(ctx):
     assert ei.value.status == E.EBADARG
 
 
+# ---------------------------------------------------------------- band format
+@pytest.mark.parametrize("dims,bands,k", [((11, 9, 7), (3, 2, 2), 16), ((20, 10, 6), (2, 3, 1), 64),
+                                          ((12, 12, 12), (6, 6, 7), 100),
+                                          ((50, 6, 5), (6, 2, 2), 300)])
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+def test_band_spmm_matches_oracle(ctx, dims, bands, k, dtype):
+    s = E.SynthHam(*dims, *bands, 4, seed=sum(dims))
+    H = oracle.Ham.from_synth(s)
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=True)
+    fmt, b = ham.format()
+    assert fmt == 1 and b == tuple(bands)
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = np.random.default_rng(k).standard_normal((s.n, k)).astype(npdt)
+    dX, dY = dev(ctx, X, npdt), E.DeviceArray(ctx, (s.n, k), npdt)
+    ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, dtype)
+    ref = oracle.ham_apply(H, X.astype(np.float64))
+    assert rel(dY.to_host().astype(np.float64), ref) < (1e-13 if dtype == E.F64 else 1e-5)
+
+
+def test_band_filter_with_holes_matches_oracle(ctx):
+    """Truncation leaves holes inside the band: stored as zeros, same result."""
+    s = E.SynthHam(10, 10, 10, 4, 4, 4, 6, truncation=2e-5, seed=3)
+    H = oracle.Ham.from_synth(s)
+    ham = ctx.ham_upload(s, tucker=True)
+    assert ham.format()[0] == 1
+    assert s.nnz < 1000 * 9 ** 3  # some entries were truncated
+    k, m = 48, 9
+    X = np.random.default_rng(2).standard_normal((s.n, k))
+    dX = dev(ctx, X)
+    ctx.cheb_filter(ham, dX.ptr, k, k, m, -0.5, 30.0, -3.0, 2, E.F64)
+    ref = oracle.cheb_filter(H, X, m, -0.5, 30.0, -3.0, 2)
+    assert rel(dX.to_host(), ref) < 1e-12
+
+
+def test_band_rejects_sparse_band_keeps_csr(ctx):
+    """A band with too few entries stays in CSR (and stays correct)."""
+    n = 6 * 6 * 6
+    rows = [[i, (i + 7 * 36) % n] for i in range(n)]  # far coupling -> wide, empty band
+    A = np.zeros((n, n))
+    for i, cs in enumerate(rows):
+        for c in cs:
+            A[i, c] = A[c, i] = 1.0 + 0.01 * i
+    rowptr = np.zeros(n + 1, np.int64)
+    cols, vals = [], []
+    for i in range(n):
+        nz = np.nonzero(A[i])[0]
+        cols += nz.tolist()
+        vals += A[i, nz].tolist()
+        rowptr[i + 1] = len(cols)
+    ham = E.Hamiltonian(ctx, n, rowptr, np.array(cols, np.int32), np.array(vals), 0, None, None,
+                        None, None, dims=(6, 6, 6))
+    assert ham.format()[0] == 0
+    X = np.random.default_rng(1).standard_normal((n, 32))
+    dX, dY = dev(ctx, X), E.DeviceArray(ctx, (n, 32))
+    ctx.ham_apply(ham, dX.ptr, 32, dY.ptr, 32, 32)
+    assert rel(dY.to_host(), A @ X) < 1e-13
+
+
 # ---------------------------------------------------------------- filter
 @pytest.mark.parametrize("convention", [E.DEGREE_REFERENCE, E.DEGREE_EXACT])
 @pytest.mark.parametrize("m,nb", [(1, 1), (2, 1), (5, 1), (8, 3)])

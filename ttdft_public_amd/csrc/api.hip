@@ -33,6 +33,8 @@ int launch_cm_to_bm(ttdft_ctx ctx, const double *src, int64_t lda, int64_t n, in
 int launch_bm_to_cm(ttdft_ctx ctx, const void *src, int64_t lds, int dtype, int64_t n,
                     int32_t k, double *dst, int64_t lda);
 int launch_lower_transpose(ttdft_ctx ctx, const double *L, double *LT, int32_t k);
+int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz,
+               const int64_t *rowptr, const int32_t *colind, const double *val, int copies);
 
 // ---- errors ----------------------------------------------------------------
 static thread_local std::string g_err;
@@ -566,10 +568,43 @@ int ttdft_ham_upload(ttdft_ctx ctx, int64_t n, const int64_t *rowptr, const int3
     return TTDFT_OK;
 }
 
+int ttdft_ham_upload_tucker(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz,
+                            const int64_t *rowptr, const int32_t *colind, const double *val,
+                            int32_t p, const int64_t *c_rowptr, const int32_t *c_colind,
+                            const double *c_val, const double *inv_c_lm, int copies,
+                            ttdft_ham *out) {
+    clear_error();
+    TT_CHECK(rx > 0 && ry > 0 && rz > 0, TTDFT_EBADARG, "ham_upload_tucker: ranks must be > 0");
+    const int64_t n = (int64_t)rx * ry * rz;
+    TT_TRY(ttdft_ham_upload(ctx, n, rowptr, colind, val, p, c_rowptr, c_colind, c_val,
+                            inv_c_lm, copies, out));
+    const int rc = build_band(ctx, *out, rx, ry, rz, rowptr, colind, val, copies);
+    if (rc != TTDFT_OK) {
+        std::string msg = g_err;
+        ttdft_ham_free(*out);
+        *out = nullptr;
+        g_err = msg;
+    }
+    return rc;
+}
+
+int ttdft_ham_format(ttdft_ham H, int32_t *format, int32_t *bands) {
+    TT_CHECK(H, TTDFT_EBADARG, "ham is NULL");
+    if (format) *format = H->format;
+    if (bands) {
+        bands[0] = H->bx;
+        bands[1] = H->by;
+        bands[2] = H->bz;
+    }
+    return TTDFT_OK;
+}
+
 int ttdft_ham_free(ttdft_ham H) {
     if (!H) return TTDFT_OK;
     hipError_t e = hipStreamSynchronize(H->ctx->stream);
     (void)e;
+    dfree(H->band64);
+    dfree(H->band32);
     dfree(H->rowptr);
     dfree(H->col);
     dfree(H->val64);

@@ -236,8 +236,14 @@ int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, vo
     }
     PendingTiming ts;
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_SPMM, &ts));
-    TT_TRY(launch_spmm_fused(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
-                             alpha, beta, delta, use_beta, use_delta));
+    const bool band = H->format == 1 && k >= 16 &&
+                      (dtype == TTDFT_F64 ? H->band64 != nullptr : H->band32 != nullptr);
+    if (band)
+        TT_TRY(launch_band_spmm(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
+                                alpha, beta, delta, use_beta, use_delta));
+    else
+        TT_TRY(launch_spmm_fused(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
+                                 alpha, beta, delta, use_beta, use_delta));
     // algorithmic bytes: H_loc values + indices + row pointers, X panel read once,
     // Y written once, the recurrence operands read once (fused epilogue).
     const double nk = (double)H->n * k * es;

@@ -88,6 +88,10 @@ _SIGS = {
     "ttdft_ham_upload": (ctypes.c_int, [_vp, _i64, _c_i64p, _c_i32p, _c_dp, _i32, _c_i64p,
                                         _c_i32p, _c_dp, _c_dp, ctypes.c_int,
                                         ctypes.POINTER(_vp)]),
+    "ttdft_ham_upload_tucker": (ctypes.c_int, [_vp, _i32, _i32, _i32, _c_i64p, _c_i32p, _c_dp,
+                                               _i32, _c_i64p, _c_i32p, _c_dp, _c_dp, ctypes.c_int,
+                                               ctypes.POINTER(_vp)]),
+    "ttdft_ham_format": (ctypes.c_int, [_vp, _c_i32p, _c_i32p]),
     "ttdft_ham_free": (ctypes.c_int, [_vp]),
     "ttdft_ham_info": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _c_i32p]),
     "ttdft_ham_apply": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _i64, _i32, ctypes.c_int]),
@@ -270,9 +274,11 @@ class Context:
         _check(lib().ttdft_ctx_reset_stats(self.h))
 
     # ---- Hamiltonian ----
-    def ham_upload(self, s: SynthHam, copies=HAM_F64):
+    def ham_upload(self, s: SynthHam, copies=HAM_F64, tucker=True):
+        """Upload a synthetic H; with tucker=True its ranks are passed so the
+        engine can use the Kronecker band format."""
         return Hamiltonian(self, s.n, s.rowptr, s.col, s.val, s.p, s.c_rowptr, s.c_col, s.c_val,
-                           s.inv_c_lm, copies)
+                           s.inv_c_lm, copies, dims=s.dims if tucker else None)
 
     def ham_apply(self, ham, X, ldx, Y, ldy, k, dtype=F64):
         _check(lib().ttdft_ham_apply(self.h, ham.h, X, ldx, Y, ldy, k, dtype))
@@ -364,7 +370,7 @@ class Hamiltonian:
     """Device copy of H = H_loc + C^T D C (ttdft_ham)."""
 
     def __init__(self, ctx, n, rowptr, col, val, p, c_rowptr, c_col, c_val, inv_c_lm,
-                 copies=HAM_F64):
+                 copies=HAM_F64, dims=None):
         self.ctx = ctx
         rowptr = np.ascontiguousarray(rowptr, np.int64)
         col = np.ascontiguousarray(col, np.int32)
@@ -378,10 +384,23 @@ class Hamiltonian:
             args = (_i64ptr(c_rowptr), _i32ptr(c_col), _dptr(c_val), _dptr(inv_c_lm))
         else:
             args = (None, None, None, None)
-        _check(lib().ttdft_ham_upload(ctx.h, n, _i64ptr(rowptr), _i32ptr(col), _dptr(val), p,
-                                      *args, copies, ctypes.byref(h)))
+        if dims is None:
+            _check(lib().ttdft_ham_upload(ctx.h, n, _i64ptr(rowptr), _i32ptr(col), _dptr(val), p,
+                                          *args, copies, ctypes.byref(h)))
+        else:
+            assert dims[0] * dims[1] * dims[2] == n
+            _check(lib().ttdft_ham_upload_tucker(ctx.h, dims[0], dims[1], dims[2],
+                                                 _i64ptr(rowptr), _i32ptr(col), _dptr(val), p,
+                                                 *args, copies, ctypes.byref(h)))
         self.h = h
         self.n = n
+
+    def format(self):
+        """(0 = CSR | 1 = Kronecker band, (bx, by, bz))"""
+        f = ctypes.c_int32()
+        b = (ctypes.c_int32 * 3)()
+        _check(lib().ttdft_ham_format(self.h, ctypes.byref(f), b))
+        return f.value, tuple(b)
 
     def info(self):
         n = ctypes.c_int64()
