@@ -5,8 +5,8 @@
 //   1. gather the batch's coefficients state-major (Cb[s][c][b][a]),
 //   2. T1[s][qz][b][a]  = sum_c Bz[qz,c] Cb[s][c][b][a]      (batched GEMM),
 //   3. T2[s][qz][b][qx] = sum_a T1[s][qz][b][a] Bx[qx,a]     (one GEMM),
-//   4. fused last contraction + square + weighted sum over the batch, with
-//      psi never written to memory: rho tile += sum_s w_s (By T2_s)^2.
+//   4. fused last contraction on FP64 MFMA + square + weighted sum over the
+//      batch, with psi never written to memory: rho tile += sum_s w_s (By T2_s)^2.
 // The contraction order (z, then x, then y) is that of the reference's TTM
 // chain (dft/ProjectHamiltonianSparse.cc:1052-1067).
 #include "internal.h"
@@ -80,61 +80,96 @@ __global__ __launch_bounds__(256) void gemm_batched_f64_kernel(
     }
 }
 
-// rho[qz][qy][qx] (+)= sum_s w[s] (sum_b By[qy + py b] T2[s][qz][b][qx])^2
-// Block: 64 qx x 64 qy at one qz.  Dynamic LDS: By tile [ry][64], T2 tile [ry][64].
-__global__ __launch_bounds__(256) void density_square_kernel(
+// rho[qz][qy][qx] (+)= sum_s w[s] (sum_b By[qy + py b] T2[s][qz][b][qx])^2 on
+// FP64 MFMA (16x16x4).  Block = 8 waves = 128 qy x 64 qx at one qz; wave =
+// 32 qy x 32 qx (2x2 MFMA tiles).  A[i][kk] = By[qy0+i][b+kk] is the same for
+// every state and stays in registers; B[kk][j] = T2_s[b+kk][qx0+j] is staged
+// per state in LDS ([ry][64], shared by the four qy-groups).  psi never leaves
+// the accumulators: it is squared, weighted and summed over the batch there.
+// f64 MFMA layouts: A[i][kk] in lane i + 16 kk, B[kk][j] in lane j + 16 kk,
+// D[i][j] in lane j + 16 (i%4), element i/4.
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int kMaxKSteps = 32;  // ry <= 128
+
+__global__ __launch_bounds__(512) void density_square_mfma_kernel(
     const double *__restrict__ T2, const double *__restrict__ By, const double *__restrict__ w,
     int32_t S, int32_t px, int32_t py, int32_t pz, int32_t ry, double *__restrict__ rho,
     int accumulate) {
-    extern __shared__ double lds[];
-    double *Bys = lds;            // [ry][64]
-    double *Ts = lds + ry * 64;   // [ry][64]
+    extern __shared__ double Ts[];  // [ksteps*4][64]
     const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
-    const int qx0 = blockIdx.x * 64, qy0 = blockIdx.y * 64, qz = blockIdx.z;
-    for (int e = tid; e < ry * 64; e += 256) {
-        const int q = e & 63, b = e >> 6;
-        const int qy = qy0 + q;
-        Bys[b * 64 + q] = (qy < py) ? By[qy + (int64_t)py * b] : 0.0;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int qx0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int qy0 = blockIdx.y * 128 + (wave >> 1) * 32;
+    const int qz = blockIdx.z;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ksteps = (ry + 3) / 4;
+    const int kpad = ksteps * 4;
+    // A fragments for the two qy tiles, all k steps (registers)
+    double a0[kMaxKSteps], a1[kMaxKSteps];
+#pragma unroll
+    for (int t = 0; t < kMaxKSteps; ++t) {
+        a0[t] = 0.0;
+        a1[t] = 0.0;
+        if (t < ksteps) {
+            const int b = 4 * t + lk;
+            const int qa = qy0 + li, qb = qy0 + 16 + li;
+            if (b < ry && qa < py) a0[t] = By[qa + (int64_t)py * b];
+            if (b < ry && qb < py) a1[t] = By[qb + (int64_t)py * b];
+        }
     }
-    double acc[4][4] = {};
+    d4 r00 = {0, 0, 0, 0}, r01 = r00, r10 = r00, r11 = r00;
+    const int lx = (wave & 1) * 32;
     for (int s = 0; s < S; ++s) {
         __syncthreads();
         const double *T2s = T2 + (((int64_t)s * pz + qz) * ry) * px;
-        for (int e = tid; e < ry * 64; e += 256) {
+        for (int e = tid; e < kpad * 64; e += 512) {
             const int q = e & 63, b = e >> 6;
-            const int qx = qx0 + q;
-            Ts[b * 64 + q] = (qx < px) ? T2s[(int64_t)b * px + qx] : 0.0;
+            const int qx = blockIdx.x * 64 + q;
+            Ts[e] = (b < ry && qx < px) ? T2s[(int64_t)b * px + qx] : 0.0;
         }
         __syncthreads();
-        double psi[4][4] = {};
-        for (int b = 0; b < ry; ++b) {
-            double yv[4], tv[4];
+        d4 p00 = {0, 0, 0, 0}, p01 = p00, p10 = p00, p11 = p00;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) yv[v] = Bys[b * 64 + ty + 16 * v];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) tv[u] = Ts[b * 64 + tx + 16 * u];
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-#pragma unroll
-                for (int u = 0; u < 4; ++u) psi[v][u] += yv[v] * tv[u];
+        for (int t = 0; t < kMaxKSteps; ++t) {
+            if (t < ksteps) {
+                const double b0 = Ts[(4 * t + lk) * 64 + lx + li];
+                const double b1 = Ts[(4 * t + lk) * 64 + lx + 16 + li];
+                p00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0, p00, 0, 0, 0);
+                p01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b1, p01, 0, 0, 0);
+                p10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b0, p10, 0, 0, 0);
+                p11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1, p11, 0, 0, 0);
+            }
         }
         const double ws = w[s];
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc[v][u] += ws * psi[v][u] * psi[v][u];
+        for (int v = 0; v < 4; ++v) {
+            r00[v] += ws * p00[v] * p00[v];
+            r01[v] += ws * p01[v] * p01[v];
+            r10[v] += ws * p10[v] * p10[v];
+            r11[v] += ws * p11[v] * p11[v];
+        }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int qy = qy0 + ty + 16 * v;
-        if (qy >= py) continue;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int qx = qx0 + tx + 16 * u;
-            if (qx >= px) continue;
-            const int64_t o = qx + (int64_t)px * qy + (int64_t)px * py * qz;
-            rho[o] = accumulate ? rho[o] + acc[v][u] : acc[v][u];
+        const int i = lk + 4 * v;
+        const int qyA = qy0 + i, qyB = qy0 + 16 + i;
+        const int qxA = qx0 + li, qxB = qx0 + 16 + li;
+        const int64_t base = (int64_t)px * py * qz;
+        if (qyA < py && qxA < px) {
+            const int64_t o = base + qxA + (int64_t)px * qyA;
+            rho[o] = accumulate ? rho[o] + r00[v] : r00[v];
+        }
+        if (qyA < py && qxB < px) {
+            const int64_t o = base + qxB + (int64_t)px * qyA;
+            rho[o] = accumulate ? rho[o] + r01[v] : r01[v];
+        }
+        if (qyB < py && qxA < px) {
+            const int64_t o = base + qxA + (int64_t)px * qyB;
+            rho[o] = accumulate ? rho[o] + r10[v] : r10[v];
+        }
+        if (qyB < py && qxB < px) {
+            const int64_t o = base + qxB + (int64_t)px * qyB;
+            rho[o] = accumulate ? rho[o] + r11[v] : r11[v];
         }
     }
 }
@@ -213,10 +248,10 @@ int density_compute(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, int32_
         // T2(row, qx) = sum_a T1(row, a) Bx(qx, a),  row = (s, qz, b)
         TT_TRY(gemm_batched(ctx, (int64_t)cnt * pz * ry, px, rx, 1, T1, 0, rx, 1, Bx, 0, px, 1,
                             T2, 0, px));
-        const dim3 grid((unsigned)((px + 63) / 64), (unsigned)((py + 63) / 64), (unsigned)pz);
-        const size_t lds = (size_t)2 * ry * 64 * sizeof(double);
-        hipLaunchKernelGGL(density_square_kernel, grid, dim3(256), lds, ctx->stream, T2, By, wdev,
-                           cnt, px, py, pz, ry, rho, first ? 0 : 1);
+        const dim3 grid((unsigned)((px + 63) / 64), (unsigned)((py + 127) / 128), (unsigned)pz);
+        const size_t lds = (size_t)((ry + 3) / 4) * 4 * 64 * sizeof(double);
+        hipLaunchKernelGGL(density_square_mfma_kernel, grid, dim3(512), lds, ctx->stream, T2, By,
+                           wdev, cnt, px, py, pz, ry, rho, first ? 0 : 1);
         TT_LAUNCH_CHECK();
         first = 0;
         b0 += cnt;

@@ -1,0 +1,166 @@
+// ChFSI driver over the reference-shaped C++ layer: the control flow of the
+// ChFSI block of main.cc:977-1237 (bounds, restart loop of filter / orth /
+// Rayleigh-Ritz, stop test, rotation) run for a number of SCF iterations on a
+// synthetic Kronecker-banded Hamiltonian held fixed, printing what the
+// reference prints (bounds, "%dth e-vals:\t%.18e", Fermi level).
+//
+//   chfsi_driver [--dims RX RY RZ] [--bands BX BY BZ] [--proj P] [--k K]
+//                [--m M] [--nelec NE] [--scf S] [--h H] [--seed S] [--f32]
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "ttdft_host.h"
+
+using namespace ttdft_host;
+
+int main(int argc, char **argv) {
+    int dims[3] = {8, 8, 8}, bands[3] = {2, 2, 3};
+    int nproj = 5, k = 16, m = 12, nelec = 19, nscf = 6, f32 = 0;
+    double h = 2.0;
+    unsigned long long seed = 20261202ULL;
+    for (int i = 1; i < argc; ++i) {
+        if (!strcmp(argv[i], "--dims") && i + 3 < argc) {
+            for (int d = 0; d < 3; ++d) dims[d] = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--bands") && i + 3 < argc) {
+            for (int d = 0; d < 3; ++d) bands[d] = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--proj") && i + 1 < argc) {
+            nproj = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--k") && i + 1 < argc) {
+            k = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--m") && i + 1 < argc) {
+            m = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--nelec") && i + 1 < argc) {
+            nelec = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--scf") && i + 1 < argc) {
+            nscf = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--h") && i + 1 < argc) {
+            h = atof(argv[++i]);
+        } else if (!strcmp(argv[i], "--seed") && i + 1 < argc) {
+            seed = strtoull(argv[++i], nullptr, 10);
+        } else if (!strcmp(argv[i], "--f32")) {
+            f32 = 1;
+        } else {
+            fprintf(stderr, "unknown argument %s\n", argv[i]);
+            return 2;
+        }
+    }
+    try {
+        // ---- synthetic projected Hamiltonian (host CSR) ----
+        ttdft_synth_params prm{dims[0], dims[1], dims[2], bands[0], bands[1], bands[2],
+                               nproj,   h,       1e-6,    seed};
+        int64_t nnz = 0, nnz_c = 0;
+        check(ttdft_synth_hamiltonian(&prm, &nnz, &nnz_c, nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, nullptr, nullptr));
+        ProjectHamiltonianSparse H;
+        const int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
+        H.H_loc.rows = H.H_loc.cols = n;
+        H.H_loc.rowptr.resize(n + 1);
+        H.H_loc.col.resize(nnz);
+        H.H_loc.val.resize(nnz);
+        H.C_nloc.rows = nproj;
+        H.C_nloc.cols = n;
+        H.C_nloc.rowptr.resize(nproj + 1);
+        H.C_nloc.col.resize(nnz_c);
+        H.C_nloc.val.resize(nnz_c);
+        H.inv_C_lm.resize(nproj);
+        check(ttdft_synth_hamiltonian(&prm, &nnz, &nnz_c, H.H_loc.rowptr.data(),
+                                      H.H_loc.col.data(), H.H_loc.val.data(),
+                                      H.C_nloc.rowptr.data(), H.C_nloc.col.data(),
+                                      H.C_nloc.val.data(), H.inv_C_lm.data()));
+        H.rankTuckerBasisX = dims[0];
+        H.rankTuckerBasisY = dims[1];
+        H.rankTuckerBasisZ = dims[2];
+
+        DeviceContext ctx(0);
+        const int dtype = f32 ? TTDFT_F32 : TTDFT_F64;
+        MultTM multtm(ctx);
+        EigenSolver eigenSolver;
+        ChebyShevFilter chebyshevFilter;
+
+        // ---- initial subspace: seeded N(0,1), Cholesky-QR (main.cc:807-829) ----
+        std::vector<double> g((size_t)n * k), cm((size_t)n * k);
+        check(ttdft_rng_normal(5, (int64_t)g.size(), g.data()));
+        for (int64_t i = 0; i < n; ++i)
+            for (int j = 0; j < k; ++j) cm[i + n * j] = g[i * k + j];
+        DenseBlock eig_vecs(ctx, n, k, dtype);
+        eig_vecs.from_colmajor(cm);
+        multtm.orth(eig_vecs);
+
+        // ---- state of main.cc:730-734, 685-688 ----
+        std::vector<double> eig_vals(k, 0.0);
+        double upperBoundUnWantedSpectrum = 1e16, error_in_upper_bound = 1e16;
+        double lowerBoundWantedSpectrum = 0, lowerBoundUnWantedSpectrum = -1e16;
+        double occupied_orbital_energy = 1.0e9;
+        int occupied_orbital_number = (int)(nelec / 2.0 + 10);
+        if (occupied_orbital_number > k - 1) occupied_orbital_number = k - 1;
+        const int maxLanczosIter = 20;
+
+        AX ax(1, k);
+        ax.setup_system(ctx, H, dtype == TTDFT_F64 ? TTDFT_HAM_F64 : (TTDFT_HAM_F64 | TTDFT_HAM_F32));
+        DenseBlock HP(ctx, n, k, dtype);
+        std::vector<double> Q((size_t)k * k);
+        double *dQ = nullptr;
+        check(ttdft_malloc(ctx.handle(), sizeof(double) * k * k, (void **)&dQ));
+
+        for (int scfIter = 0; scfIter < nscf; ++scfIter) {
+            printf("====================================\n       Entering SCF %d\n", scfIter);
+            const int restart_times = scfIter == 0 ? 10 : 3;
+            double temp_lower_bound = 0.0;
+            if (error_in_upper_bound > 0.01) {
+                double current_spectrum;
+                eigenSolver.computeUpperBoundWithLanczos(ax, maxLanczosIter, current_spectrum,
+                                                         temp_lower_bound);
+                error_in_upper_bound =
+                    (upperBoundUnWantedSpectrum - current_spectrum) / upperBoundUnWantedSpectrum;
+                upperBoundUnWantedSpectrum =
+                    error_in_upper_bound > 0.01 ? current_spectrum : current_spectrum * 1.05;
+            }
+            if (scfIter == 0) {
+                lowerBoundWantedSpectrum = temp_lower_bound;
+                lowerBoundUnWantedSpectrum = 0.0;
+            } else {
+                lowerBoundWantedSpectrum = eig_vals[0];
+                lowerBoundUnWantedSpectrum = eig_vals[k - 1];
+            }
+            for (int restart_iter = 0; restart_iter < restart_times; ++restart_iter) {
+                printf("upperBoundUnWantedSpectrum is %.18e.\n", upperBoundUnWantedSpectrum);
+                printf("lowerBoundUnWantedSpectrum is %.18e.\n", lowerBoundUnWantedSpectrum);
+                printf("lowerBoundWantedSpectrum is %.18e.\n", lowerBoundWantedSpectrum);
+                chebyshevFilter.computeFilteredSubspace(H, ax, eig_vecs, (int)n, m,
+                                                        lowerBoundUnWantedSpectrum,
+                                                        upperBoundUnWantedSpectrum,
+                                                        lowerBoundWantedSpectrum, 1);
+                multtm.orth(eig_vecs);
+                ax.perform_ax(eig_vecs, HP);
+                std::vector<double> PtHP;
+                multtm.mult(eig_vecs, HP, PtHP);
+                eigenSolver.computeEigenPairsSeq(ctx, PtHP, k, k, eig_vals, Q, true);
+                for (int i = 0; i < k; ++i) printf("%dth e-vals:\t%.18e\n", i, eig_vals[i]);
+                printf("Projected Eigenvalues of Chebyshev Pass %d\n", restart_iter);
+                lowerBoundWantedSpectrum = eig_vals[0];
+                lowerBoundUnWantedSpectrum = eig_vals[k - 1];
+                const double cur = eig_vals[occupied_orbital_number];
+                const double err = std::fabs((cur - occupied_orbital_energy) / occupied_orbital_energy);
+                occupied_orbital_energy = cur;
+                if (err < 1.0e-3) {
+                    printf("the iteration converged at step %d.\n", restart_iter);
+                    break;
+                }
+            }
+            // rotation X <- X Q (main.cc:1185-1207)
+            check(ttdft_memcpy_h2d(ctx.handle(), dQ, Q.data(), sizeof(double) * k * k));
+            check(ttdft_rotate(ctx.handle(), eig_vecs.data(), eig_vecs.ld(), n, k, dQ, dtype,
+                               dtype == TTDFT_F64 ? TTDFT_GEMM_F64 : TTDFT_GEMM_BF16X3));
+            const double ef = computeFermiEnergy(eig_vals.data(), k, nelec, 3.166811429e-06, 500.0);
+            printf("0 :Fermi Energy: %.18e\n", ef);
+        }
+        ttdft_free(ctx.handle(), dQ);
+    } catch (const Error &e) {
+        fprintf(stderr, "chfsi_driver: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,145 @@
+// Implementation of the reference-shaped C++ host layer over the C ABI.
+#include "ttdft_host.h"
+
+#include <cstring>
+
+namespace ttdft_host {
+
+void check(int status) {
+    if (status != TTDFT_OK) throw Error(status, ttdft_last_error());
+}
+
+// ---- DeviceContext -----------------------------------------------------------
+DeviceContext::DeviceContext(int device) { check(ttdft_ctx_create(device, &ctx_)); }
+DeviceContext::~DeviceContext() { ttdft_ctx_destroy(ctx_); }
+void DeviceContext::synchronize() { check(ttdft_ctx_synchronize(ctx_)); }
+
+// ---- DenseBlock ----------------------------------------------------------------
+DenseBlock::DenseBlock(DeviceContext &ctx, int64_t rows, int32_t cols, int dtype)
+    : ctx_(&ctx), rows_(rows), cols_(cols), dtype_(dtype) {
+    const size_t es = dtype == TTDFT_F32 ? 4 : 8;
+    check(ttdft_malloc(ctx.handle(), (size_t)rows * cols * es, &ptr_));
+}
+DenseBlock::~DenseBlock() {
+    if (ptr_) ttdft_free(ctx_->handle(), ptr_);
+}
+void DenseBlock::from_colmajor(const std::vector<double> &host) {
+    if ((int64_t)host.size() != rows_ * cols_) throw Error(TTDFT_EBADARG, "from_colmajor: size");
+    check(ttdft_block_upload_colmajor(ctx_->handle(), host.data(), rows_, rows_, cols_, ptr_,
+                                      ld(), dtype_));
+}
+std::vector<double> DenseBlock::to_colmajor() const {
+    std::vector<double> host((size_t)rows_ * cols_);
+    check(ttdft_block_download_colmajor(ctx_->handle(), ptr_, ld(), dtype_, rows_, cols_,
+                                        host.data(), rows_));
+    return host;
+}
+
+// ---- AX ----------------------------------------------------------------------
+AX::AX(int num_bg, int num_wfns) : num_band_groups_(num_bg), num_wfns_(num_wfns) {
+    if (num_bg != 1)
+        throw Error(TTDFT_EBADARG, "one band group per GPU: the engine runs on one device");
+}
+AX::~AX() { free_mata_on_device(); }
+
+void AX::setup_system(DeviceContext &ctx, const ProjectHamiltonianSparse &H, int copies) {
+    free_mata_on_device();
+    ctx_ = &ctx;
+    const CsrMatrix &A = H.H_loc;
+    const int32_t p = (int32_t)H.C_nloc.rows;
+    const int64_t *crp = p ? H.C_nloc.rowptr.data() : nullptr;
+    const int32_t *cc = p ? H.C_nloc.col.data() : nullptr;
+    const double *cv = p ? H.C_nloc.val.data() : nullptr;
+    const double *ic = p ? H.inv_C_lm.data() : nullptr;
+    if (H.rankTuckerBasisX > 0)
+        check(ttdft_ham_upload_tucker(ctx.handle(), H.rankTuckerBasisX, H.rankTuckerBasisY,
+                                      H.rankTuckerBasisZ, A.rowptr.data(), A.col.data(),
+                                      A.val.data(), p, crp, cc, cv, ic, copies, &ham_));
+    else
+        check(ttdft_ham_upload(ctx.handle(), A.rows, A.rowptr.data(), A.col.data(), A.val.data(),
+                               p, crp, cc, cv, ic, copies, &ham_));
+}
+
+void AX::perform_ax(DenseBlock &X, DenseBlock &AXout) {
+    if (!ham_) throw Error(TTDFT_EBADARG, "perform_ax before setup_system");
+    check(ttdft_ham_apply(ctx_->handle(), ham_, X.data(), X.ld(), AXout.data(), AXout.ld(),
+                          X.cols(), X.dtype()));
+}
+
+void AX::free_mata_on_device() {
+    if (ham_) ttdft_ham_free(ham_);
+    ham_ = nullptr;
+}
+
+// ---- ChebyShevFilter -------------------------------------------------------------
+void ChebyShevFilter::computeFilteredSubspace(ProjectHamiltonianSparse &H, AX &ax, DenseBlock &X,
+                                              int basisSize, int m, double a, double b,
+                                              double a0, int num_X_blocks) {
+    (void)H;  // the operator was uploaded by ax.setup_system
+    if (basisSize != X.rows())
+        throw Error(TTDFT_EBADARG, "computeFilteredSubspace: basisSize != rows of X");
+    check(ttdft_cheb_filter(ax.context()->handle(), ax.handle(), X.data(), X.ld(), X.cols(), m,
+                            a, b, a0, num_X_blocks, X.dtype(), convention_));
+}
+
+// ---- MultTM --------------------------------------------------------------------
+void MultTM::mult(DenseBlock &X, DenseBlock &Y, std::vector<double> &result) {
+    const int k = X.cols();
+    if (Y.cols() != k || Y.rows() != X.rows()) throw Error(TTDFT_EBADARG, "mult: shape");
+    double *dS = nullptr;
+    check(ttdft_malloc(ctx_->handle(), sizeof(double) * k * k, (void **)&dS));
+    const int mode = X.dtype() == TTDFT_F64 ? TTDFT_GEMM_F64 : TTDFT_GEMM_BF16X3;
+    int rc = ttdft_gram(ctx_->handle(), X.data(), X.ld(), Y.data(), Y.ld(), X.rows(), k,
+                        X.dtype(), mode, dS);
+    result.resize((size_t)k * k);
+    if (rc == TTDFT_OK)
+        rc = ttdft_memcpy_d2h(ctx_->handle(), result.data(), dS, sizeof(double) * k * k);
+    ttdft_free(ctx_->handle(), dS);
+    check(rc);
+}
+
+void MultTM::orth(DenseBlock &X) {
+    const int mode = X.dtype() == TTDFT_F64 ? TTDFT_GEMM_F64 : TTDFT_GEMM_BF16X3;
+    check(ttdft_orth(ctx_->handle(), X.data(), X.ld(), X.rows(), X.cols(), X.dtype(), mode));
+}
+
+// ---- EigenSolver -----------------------------------------------------------------
+void EigenSolver::computeEigenPairsSeq(DeviceContext &ctx, const std::vector<double> &A,
+                                       int numberEigenPairs, int numGlobalRows,
+                                       std::vector<double> &eigVals,
+                                       std::vector<double> &eigMat, bool lapack) {
+    (void)lapack;
+    const int k = numGlobalRows;
+    if ((int64_t)A.size() != (int64_t)k * k || numberEigenPairs > k || numberEigenPairs < 1)
+        throw Error(TTDFT_EBADARG, "computeEigenPairsSeq: shape");
+    double *d = nullptr;
+    check(ttdft_malloc(ctx.handle(), sizeof(double) * (2 * (size_t)k * k + k), (void **)&d));
+    double *dA = d, *dQ = d + (size_t)k * k, *dE = dQ + (size_t)k * k;
+    std::vector<double> ev(k), Q((size_t)k * k);
+    int rc = ttdft_memcpy_h2d(ctx.handle(), dA, A.data(), sizeof(double) * k * k);
+    if (rc == TTDFT_OK) rc = ttdft_rr_eig(ctx.handle(), dA, k, dE, dQ);
+    if (rc == TTDFT_OK) rc = ttdft_memcpy_d2h(ctx.handle(), ev.data(), dE, sizeof(double) * k);
+    if (rc == TTDFT_OK)
+        rc = ttdft_memcpy_d2h(ctx.handle(), Q.data(), dQ, sizeof(double) * k * k);
+    ttdft_free(ctx.handle(), d);
+    check(rc);
+    // the reference returns the numberEigenPairs smallest pairs (EPS_SMALLEST_REAL)
+    eigVals.assign(ev.begin(), ev.begin() + numberEigenPairs);
+    eigMat.assign(Q.begin(), Q.begin() + (size_t)k * numberEigenPairs);
+}
+
+void EigenSolver::computeUpperBoundWithLanczos(AX &ax, int maxLanczosIteration, double &upperBound,
+                                               double &lowerBound, uint64_t seed) {
+    check(ttdft_lanczos_bounds(ax.context()->handle(), ax.handle(), maxLanczosIteration, seed,
+                               &upperBound, &lowerBound));
+}
+
+double computeFermiEnergy(const double *eigenValues, int numberEigenValues, int numberElectrons,
+                          double kb, double T, int maxFermiIter) {
+    double ef = 0.0;
+    check(ttdft_fermi_occupancy(eigenValues, numberEigenValues, numberElectrons, kb, T,
+                                maxFermiIter, &ef, nullptr));
+    return ef;
+}
+
+}  // namespace ttdft_host

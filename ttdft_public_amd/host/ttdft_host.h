@@ -1,0 +1,160 @@
+// C++ host layer with the reference's ChFSI call surface, over the C ABI of
+// libttdft_chfsi.so.  The classes keep the reference's names, method names,
+// argument order and meaning; PETSc Mat/Vec are replaced by repo-owned types:
+//   CsrMatrix   - host CSR (MPIAIJ / SEQAIJ in the reference)
+//   DenseBlock  - device-resident n x k wavefunction block (MATDENSE); stored
+//                 basis-major on the device, converted from/to the
+//                 reference's column-major layout by from_colmajor/to_colmajor
+//   ProjectHamiltonianSparse - holds H_loc, C_nloc (unscaled) and 1/C_lm, the
+//                 members the filter reads (dft/ProjectHamiltonianSparse.h:66-97)
+// Errors: every failing engine call throws ttdft_host::Error (status + the
+// engine's message).  Uncaught, it ends the program through std::terminate,
+// which is what the reference does on any CUDA/cuBLAS/cuSPARSE/cuSOLVER
+// failure (eigensolver/MultCuda.cu:57-60, MultTm.cu:157-161).
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ttdft_chfsi.h"
+
+namespace ttdft_host {
+
+class Error : public std::runtime_error {
+public:
+    Error(int status, const std::string &msg)
+        : std::runtime_error("ttdft status " + std::to_string(status) + ": " + msg),
+          status_(status) {}
+    int status() const { return status_; }
+
+private:
+    int status_;
+};
+
+void check(int status);
+
+struct CsrMatrix {
+    int64_t rows = 0, cols = 0;
+    std::vector<int64_t> rowptr;
+    std::vector<int32_t> col;
+    std::vector<double> val;
+    int64_t nnz() const { return rowptr.empty() ? 0 : rowptr.back(); }
+};
+
+// One device + one stream (DeviceUtils / DeviceCommUtils device binding).
+class DeviceContext {
+public:
+    explicit DeviceContext(int device = 0);
+    ~DeviceContext();
+    DeviceContext(const DeviceContext &) = delete;
+    DeviceContext &operator=(const DeviceContext &) = delete;
+    ttdft_ctx handle() const { return ctx_; }
+    void synchronize();
+
+private:
+    ttdft_ctx ctx_ = nullptr;
+};
+
+class DenseBlock {
+public:
+    DenseBlock(DeviceContext &ctx, int64_t rows, int32_t cols, int dtype = TTDFT_F64);
+    ~DenseBlock();
+    DenseBlock(const DenseBlock &) = delete;
+    DenseBlock &operator=(const DenseBlock &) = delete;
+    int64_t rows() const { return rows_; }
+    int32_t cols() const { return cols_; }
+    int dtype() const { return dtype_; }
+    void *data() const { return ptr_; }
+    int64_t ld() const { return cols_; }
+    DeviceContext &context() const { return *ctx_; }
+    // host column-major rows x cols (PETSc dense layout)
+    void from_colmajor(const std::vector<double> &host);
+    std::vector<double> to_colmajor() const;
+
+private:
+    DeviceContext *ctx_;
+    int64_t rows_;
+    int32_t cols_;
+    int dtype_;
+    void *ptr_ = nullptr;
+};
+
+struct ProjectHamiltonianSparse {
+    CsrMatrix H_loc;                // n x n, symmetric
+    CsrMatrix C_nloc;               // p x n projector rows, not row-scaled
+    std::vector<double> inv_C_lm;   // p
+    int32_t rankTuckerBasisX = 0, rankTuckerBasisY = 0, rankTuckerBasisZ = 0;
+};
+
+// eigensolver/Mult.h:30-114
+class AX {
+public:
+    AX(int num_bg, int num_wfns);
+    ~AX();
+    AX(const AX &) = delete;
+    AX &operator=(const AX &) = delete;
+    // Uploads H_loc together with the nonlocal projectors of H, so that
+    // perform_ax applies the whole operator the filter needs.
+    void setup_system(DeviceContext &ctx, const ProjectHamiltonianSparse &H,
+                      int copies = TTDFT_HAM_F64);
+    // AX = H X (written, unlike the published perform_ax, MultCuda.cu:719-820)
+    void perform_ax(DenseBlock &X, DenseBlock &AXout);
+    void free_mata_on_device();
+    ttdft_ham handle() const { return ham_; }
+    DeviceContext *context() const { return ctx_; }
+
+private:
+    int num_band_groups_;
+    int num_wfns_;
+    DeviceContext *ctx_ = nullptr;
+    ttdft_ham ham_ = nullptr;
+};
+
+// eigensolver/ChebyShevFilter.h:29-40
+class ChebyShevFilter {
+public:
+    explicit ChebyShevFilter(int degree_convention = TTDFT_DEGREE_REFERENCE)
+        : convention_(degree_convention) {}
+    void computeFilteredSubspace(ProjectHamiltonianSparse &H, AX &ax, DenseBlock &X,
+                                 int basisSize, int m, double a, double b, double a0,
+                                 int num_X_blocks);
+
+private:
+    int convention_;
+};
+
+// eigensolver/MultTm.h:33-53
+class MultTM {
+public:
+    explicit MultTM(DeviceContext &ctx) : ctx_(&ctx) {}
+    // result = X^T Y, k x k column-major (resized like MultTm.cu:138)
+    void mult(DenseBlock &X, DenseBlock &Y, std::vector<double> &result);
+    // Cholesky-QR, X <- X L^{-T}
+    void orth(DenseBlock &X);
+
+private:
+    DeviceContext *ctx_;
+};
+
+// eigensolver/EigenSolver.h:29-45
+class EigenSolver {
+public:
+    // A: numGlobalRows x numGlobalRows column-major (lower triangle used);
+    // eigVals ascending, eigMat column i <-> eigVals[i] (column-major).
+    // `lapack` selects the dense solver in the reference; the engine always
+    // uses its dense FP64 solver.
+    void computeEigenPairsSeq(DeviceContext &ctx, const std::vector<double> &A,
+                              int numberEigenPairs, int numGlobalRows,
+                              std::vector<double> &eigVals, std::vector<double> &eigMat,
+                              bool lapack);
+    void computeUpperBoundWithLanczos(AX &ax, int maxLanczosIteration, double &upperBound,
+                                      double &lowerBound, uint64_t seed = 20261202);
+};
+
+// KSDFTEnergyFunctional::computeFermiEnergy (dft/KSDFTEnergyFunctional.h:46-51)
+double computeFermiEnergy(const double *eigenValues, int numberEigenValues, int numberElectrons,
+                          double kb, double T, int maxFermiIter = 20);
+
+}  // namespace ttdft_host
