@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-density", action="store_true")
+    ap.add_argument("--mode", default="step", choices=["step", "spmm"],
+                    help="spmm: only 3 FP64 H applications and 3 copies of one 125000 x 300 "
+                         "FP64 block (copy = calibration of rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU "
@@ -103,6 +106,19 @@ def main():
     ctx = E.Context(0)
     ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32)
     n, k = s.n, K
+    fmt = ham.format()
+    log(f"H_loc storage: {'Kronecker band' if fmt[0] == 1 else 'CSR'}, half-bandwidths {fmt[1]}")
+    if args.mode == "spmm":
+        dX = E.DeviceArray(ctx, (n, k))
+        dY = E.DeviceArray(ctx, (n, k))
+        ctx.random_subspace(dX.ptr, k, n, k, seed=5)
+        for _ in range(3):
+            ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k)
+        for _ in range(3):
+            E._check(E.lib().ttdft_block_convert(ctx.h, dX.ptr, k, E.F64, dY.ptr, k, E.F64, n, k))
+        ctx.synchronize()
+        log("spmm mode done")
+        return
 
     # quadrature grid and basis tables for the density
     pts, wts, B = basis_tables(DIMS[0], QUAD, 3)
@@ -177,6 +193,7 @@ def main():
         "config": {
             "workload": "172-atom cluster ChFSI SCF step, FP64 (N=125000, k=300, m=30, "
                         "600^3 density grid)",
+            "h_loc_format": "kronecker-band" if fmt[0] == 1 else "csr",
             "n": n, "k": k, "degree": DEGREE, "nnz": s.nnz, "nnz_per_row": round(s.nnz / n, 1),
             "projectors": NPROJ, "nnz_c": s.nnz_c, "panel_width": k,
         },

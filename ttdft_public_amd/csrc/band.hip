@@ -8,7 +8,9 @@
 // (zeros where the CSR has no entry).
 //
 // One wavefront owns RW consecutive x-rows of LPW consecutive y-lines (same
-// z) and a 64-state panel (lane = state).  For every neighbouring line
+// z) and a 64-state panel (lane = state); small tiles (RW = 5, LPW = 2 in
+// FP64, 4 in FP32) keep ~100 VGPRs per wave so enough waves are resident to
+// hide the scalar-cache latency of the band values.  For every neighbouring line
 // (y',z') it loads the RW + 2bx X rows of that line's x-window into registers
 // once (coalesced 64-state rows) and feeds up to LPW * RW * (2bx+1) FMAs from
 // them; the band values are wavefront-uniform and arrive through the scalar
@@ -46,9 +48,7 @@ struct BandArgs {
     int use_beta, use_delta;
 };
 
-constexpr int kLPW = 4;
-
-template <typename T, int W, int RW>
+template <typename T, int W, int RW, int kLPW>
 __global__ __launch_bounds__(256) void band_spmm_kernel(BandArgs<T> a) {
     constexpr int BX = (W - 1) / 2;
     constexpr int NWIN = RW + W - 1;
@@ -130,12 +130,15 @@ __global__ __launch_bounds__(256) void band_spmm_kernel(BandArgs<T> a) {
     }
 }
 
+template <typename T>
+constexpr int lines_per_wave() { return sizeof(T) == 8 ? 2 : 4; }
+
 template <typename T, int W, int RW>
 int launch_tmpl(ttdft_ctx ctx, const BandArgs<T> &a) {
     int64_t blocks = (a.nwaves + 3) / 4;
     blocks = (blocks + 7) / 8 * 8;
-    hipLaunchKernelGGL((band_spmm_kernel<T, W, RW>), dim3((unsigned)blocks), dim3(256), 0,
-                       ctx->stream, a);
+    hipLaunchKernelGGL((band_spmm_kernel<T, W, RW, lines_per_wave<T>()>), dim3((unsigned)blocks),
+                       dim3(256), 0, ctx->stream, a);
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
@@ -144,7 +147,8 @@ template <typename T>
 int dispatch(ttdft_ctx ctx, const BandArgs<T> &a, int W, int RW) {
 #define TT_BAND_CASE(WV)                                                              \
     case WV:                                                                          \
-        return RW == 8 ? launch_tmpl<T, WV, 8>(ctx, a) : launch_tmpl<T, WV, 10>(ctx, a);
+        return RW == 5 ? launch_tmpl<T, WV, 5>(ctx, a)                                \
+                       : (RW == 8 ? launch_tmpl<T, WV, 8>(ctx, a) : launch_tmpl<T, WV, 10>(ctx, a));
     switch (W) {
         TT_BAND_CASE(5)
         TT_BAND_CASE(7)
@@ -171,7 +175,8 @@ int launch_band_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t
     const int Wd = 2 * H->bx + 1;
     const int npanel = (k + 63) / 64;
     const int ngx = H->rxp / H->rw;
-    const int nly = (H->ry + kLPW - 1) / kLPW;
+    const int lpw = dtype == TTDFT_F64 ? lines_per_wave<double>() : lines_per_wave<float>();
+    const int nly = (H->ry + lpw - 1) / lpw;
     const int64_t nwaves = (int64_t)npanel * ngx * nly * H->rz;
     if (dtype == TTDFT_F64) {
         TT_CHECK(H->band64, TTDFT_EBADARG, "no FP64 band copy of H_loc uploaded");
@@ -229,8 +234,9 @@ int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz
     const int W = 2 * bx + 1;
     const int64_t P = (int64_t)(2 * by + 1) * (2 * bz + 1);
     // rows per wavefront: the register block with the least padding of rx
-    const int pad8 = (8 - rx % 8) % 8, pad10 = (10 - rx % 10) % 10;
-    const int rw = pad10 <= pad8 ? 10 : 8;
+    // (ties go to the smaller block, which keeps more waves resident)
+    const int pad5 = (5 - rx % 5) % 5, pad8 = (8 - rx % 8) % 8, pad10 = (10 - rx % 10) % 10;
+    const int rw = (pad5 <= pad8 && pad5 <= pad10) ? 5 : (pad8 <= pad10 ? 8 : 10);
     const int rxp = (rx + rw - 1) / rw * rw;
     const int64_t slots = (int64_t)ry * rz * P * rxp * W;
     const int64_t nnz = rowptr[n];
