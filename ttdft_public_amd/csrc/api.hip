@@ -752,7 +752,7 @@ int ttdft_lanczos_bounds(ttdft_ctx ctx, ttdft_ham H, int32_t maxit, uint64_t see
     double *v = (double *)ctx->scratch[7].ptr, *v0 = v + n, *f = v0 + n;
     TT_HIP(hipMemcpyAsync(v, hv.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     std::vector<double> T((size_t)maxit * maxit, 0.0);
-    TT_TRY(ham_step(ctx, H, v, 1, f, 1, nullptr, 0, 1, TTDFT_F64, 1.0, 0.0, 0.0, 0, 0));
+    TT_TRY(ham_spmv(ctx, H, v, f));
     double alpha = 0.0, beta = 0.0;
     TT_TRY(dev_dot(ctx, f, v, n, &alpha));
     TT_TRY(dev_axpby(ctx, f, 1.0, v, -alpha, n));
@@ -764,7 +764,7 @@ int ttdft_lanczos_bounds(ttdft_ctx ctx, ttdft_ham H, int32_t maxit, uint64_t see
         TT_TRY(dev_axpby(ctx, f, 1.0 / beta, nullptr, 0.0, n));
         std::swap(v0, v);  // v0 <- v
         std::swap(v, f);   // v <- f (f's old storage is reused below)
-        TT_TRY(ham_step(ctx, H, v, 1, f, 1, nullptr, 0, 1, TTDFT_F64, 1.0, 0.0, 0.0, 0, 0));
+        TT_TRY(ham_spmv(ctx, H, v, f));
         TT_TRY(dev_axpby(ctx, f, 1.0, v0, -beta, n));
         TT_TRY(dev_dot(ctx, v, f, n, &alpha));
         TT_TRY(dev_axpby(ctx, f, 1.0, v, -alpha, n));

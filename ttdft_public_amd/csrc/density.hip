@@ -89,56 +89,69 @@ __global__ __launch_bounds__(256) void gemm_batched_f64_kernel(
 // f64 MFMA layouts: A[i][kk] in lane i + 16 kk, B[kk][j] in lane j + 16 kk,
 // D[i][j] in lane j + 16 (i%4), element i/4.
 typedef double d4 __attribute__((ext_vector_type(4)));
-constexpr int kMaxKSteps = 32;  // ry <= 128
 
+// KS = number of 4-deep K steps (ceil(ry/4) rounded up to a supported size).
+// The next state's T2 tile is prefetched into registers while the current
+// state's MFMAs run, then written to the other LDS buffer (one barrier per
+// state).
+template <int KS>
 __global__ __launch_bounds__(512) void density_square_mfma_kernel(
     const double *__restrict__ T2, const double *__restrict__ By, const double *__restrict__ w,
     int32_t S, int32_t px, int32_t py, int32_t pz, int32_t ry, double *__restrict__ rho,
     int accumulate) {
-    extern __shared__ double Ts[];  // [ksteps*4][64]
+    constexpr int TILE = KS * 4 * 64;          // doubles per state tile
+    constexpr int PER_THREAD = (TILE + 511) / 512;
+    extern __shared__ double Ts[];             // [2][TILE]
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int qx0 = blockIdx.x * 64 + (wave & 1) * 32;
     const int qy0 = blockIdx.y * 128 + (wave >> 1) * 32;
     const int qz = blockIdx.z;
     const int li = lane & 15, lk = lane >> 4;
-    const int ksteps = (ry + 3) / 4;
-    const int kpad = ksteps * 4;
-    // A fragments for the two qy tiles, all k steps (registers)
-    double a0[kMaxKSteps], a1[kMaxKSteps];
+    double a0[KS], a1[KS];
 #pragma unroll
-    for (int t = 0; t < kMaxKSteps; ++t) {
-        a0[t] = 0.0;
-        a1[t] = 0.0;
-        if (t < ksteps) {
-            const int b = 4 * t + lk;
-            const int qa = qy0 + li, qb = qy0 + 16 + li;
-            if (b < ry && qa < py) a0[t] = By[qa + (int64_t)py * b];
-            if (b < ry && qb < py) a1[t] = By[qb + (int64_t)py * b];
-        }
+    for (int t = 0; t < KS; ++t) {
+        const int b = 4 * t + lk;
+        const int qa = qy0 + li, qb = qy0 + 16 + li;
+        a0[t] = (b < ry && qa < py) ? By[qa + (int64_t)py * b] : 0.0;
+        a1[t] = (b < ry && qb < py) ? By[qb + (int64_t)py * b] : 0.0;
     }
+    auto fetch = [&](int s, double *regs) {
+        const double *T2s = T2 + (((int64_t)s * pz + qz) * ry) * px;
+#pragma unroll
+        for (int u = 0; u < PER_THREAD; ++u) {
+            const int e = tid + 512 * u;
+            const int q = e & 63, b = e >> 6;
+            const int qx = blockIdx.x * 64 + q;
+            regs[u] = (e < TILE && b < ry && qx < px) ? T2s[(int64_t)b * px + qx] : 0.0;
+        }
+    };
+    auto store = [&](int buf, const double *regs) {
+#pragma unroll
+        for (int u = 0; u < PER_THREAD; ++u) {
+            const int e = tid + 512 * u;
+            if (e < TILE) Ts[buf * TILE + e] = regs[u];
+        }
+    };
+    double pre[PER_THREAD];
+    fetch(0, pre);
+    store(0, pre);
+    __syncthreads();
     d4 r00 = {0, 0, 0, 0}, r01 = r00, r10 = r00, r11 = r00;
     const int lx = (wave & 1) * 32;
     for (int s = 0; s < S; ++s) {
-        __syncthreads();
-        const double *T2s = T2 + (((int64_t)s * pz + qz) * ry) * px;
-        for (int e = tid; e < kpad * 64; e += 512) {
-            const int q = e & 63, b = e >> 6;
-            const int qx = blockIdx.x * 64 + q;
-            Ts[e] = (b < ry && qx < px) ? T2s[(int64_t)b * px + qx] : 0.0;
-        }
-        __syncthreads();
+        const int buf = s & 1;
+        if (s + 1 < S) fetch(s + 1, pre);
+        const double *Tb = Ts + buf * TILE;
         d4 p00 = {0, 0, 0, 0}, p01 = p00, p10 = p00, p11 = p00;
 #pragma unroll
-        for (int t = 0; t < kMaxKSteps; ++t) {
-            if (t < ksteps) {
-                const double b0 = Ts[(4 * t + lk) * 64 + lx + li];
-                const double b1 = Ts[(4 * t + lk) * 64 + lx + 16 + li];
-                p00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0, p00, 0, 0, 0);
-                p01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b1, p01, 0, 0, 0);
-                p10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b0, p10, 0, 0, 0);
-                p11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1, p11, 0, 0, 0);
-            }
+        for (int t = 0; t < KS; ++t) {
+            const double b0 = Tb[(4 * t + lk) * 64 + lx + li];
+            const double b1 = Tb[(4 * t + lk) * 64 + lx + 16 + li];
+            p00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0, p00, 0, 0, 0);
+            p01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b1, p01, 0, 0, 0);
+            p10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b0, p10, 0, 0, 0);
+            p11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1, p11, 0, 0, 0);
         }
         const double ws = w[s];
 #pragma unroll
@@ -148,6 +161,8 @@ __global__ __launch_bounds__(512) void density_square_mfma_kernel(
             r10[v] += ws * p10[v] * p10[v];
             r11[v] += ws * p11[v] * p11[v];
         }
+        if (s + 1 < S) store(buf ^ 1, pre);
+        __syncthreads();
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -172,6 +187,28 @@ __global__ __launch_bounds__(512) void density_square_mfma_kernel(
             rho[o] = accumulate ? rho[o] + r11[v] : r11[v];
         }
     }
+}
+
+template <int KS>
+int launch_square(ttdft_ctx ctx, const double *T2, const double *By, const double *w, int32_t S,
+                  int32_t px, int32_t py, int32_t pz, int32_t ry, double *rho, int accumulate) {
+    const dim3 grid((unsigned)((px + 63) / 64), (unsigned)((py + 127) / 128), (unsigned)pz);
+    const size_t lds = (size_t)2 * KS * 4 * 64 * sizeof(double);
+    hipLaunchKernelGGL((density_square_mfma_kernel<KS>), grid, dim3(512), lds, ctx->stream, T2,
+                       By, w, S, px, py, pz, ry, rho, accumulate);
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
+}
+
+int density_square(ttdft_ctx ctx, const double *T2, const double *By, const double *w, int32_t S,
+                   int32_t px, int32_t py, int32_t pz, int32_t ry, double *rho, int accumulate) {
+    const int ks = (ry + 3) / 4;
+    if (ks <= 4) return launch_square<4>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
+    if (ks <= 8) return launch_square<8>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
+    if (ks <= 13) return launch_square<13>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
+    if (ks <= 18) return launch_square<18>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
+    if (ks <= 24) return launch_square<24>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
+    return launch_square<32>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
 }
 
 int gemm_batched(ttdft_ctx ctx, int64_t M, int64_t N, int64_t K, int64_t batch,
@@ -248,11 +285,7 @@ int density_compute(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, int32_
         // T2(row, qx) = sum_a T1(row, a) Bx(qx, a),  row = (s, qz, b)
         TT_TRY(gemm_batched(ctx, (int64_t)cnt * pz * ry, px, rx, 1, T1, 0, rx, 1, Bx, 0, px, 1,
                             T2, 0, px));
-        const dim3 grid((unsigned)((px + 63) / 64), (unsigned)((py + 127) / 128), (unsigned)pz);
-        const size_t lds = (size_t)((ry + 3) / 4) * 4 * 64 * sizeof(double);
-        hipLaunchKernelGGL(density_square_mfma_kernel, grid, dim3(512), lds, ctx->stream, T2, By,
-                           wdev, cnt, px, py, pz, ry, rho, first ? 0 : 1);
-        TT_LAUNCH_CHECK();
+        TT_TRY(density_square(ctx, T2, By, wdev, cnt, px, py, pz, ry, rho, first ? 0 : 1));
         first = 0;
         b0 += cnt;
     }

@@ -140,6 +140,8 @@ int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_
                       const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
                       int32_t k, int dtype, double alpha, double beta, double delta,
                       int use_beta, int use_delta);
+// y = H x for one FP64 vector (Lanczos): shuffle-reduced row SpMV.
+int ham_spmv(ttdft_ctx ctx, const ttdft_ham_s *H, const double *x, double *y);
 // Band-format SpMM with the same epilogue (format 1 Hamiltonians).
 int launch_band_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                      const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,

@@ -82,6 +82,28 @@ __global__ __launch_bounds__(256) void spmm_fused_kernel(
     }
 }
 
+// Single-vector H_loc application for the Lanczos bound estimator:
+// y = H_loc x + C^T w.  One wavefront per row; lanes stride over the row's
+// nonzeros (coalesced value/index loads, gathered x) and the partial sums are
+// combined with a wavefront shuffle reduction.
+__global__ __launch_bounds__(256) void spmv_kernel(
+    int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ x,
+    const int64_t *__restrict__ ct_rowptr, const int32_t *__restrict__ ct_col,
+    const double *__restrict__ ct_val, const double *__restrict__ w, double *__restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    double s = 0.0;
+    for (int64_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64) s += val[e] * x[col[e]];
+    if (ct_rowptr)
+        for (int64_t e = ct_rowptr[row] + lane; e < ct_rowptr[row + 1]; e += 64)
+            s += ct_val[e] * w[ct_col[e]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) y[row] = s;
+}
+
 // W[r][j] = sum_e (D C)[r][e] X[col_e][j]: one block per projector row, the
 // row's nonzeros split over 4 wavefronts and summed through LDS.
 template <typename T, int NC>
@@ -218,6 +240,21 @@ int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_
 #undef CALL
     }
     return TTDFT_EBADARG;
+}
+
+int ham_spmv(ttdft_ctx ctx, const ttdft_ham_s *H, const double *x, double *y) {
+    TT_CHECK(H->val64, TTDFT_EBADARG, "no FP64 copy of H_loc uploaded");
+    double *W = nullptr;
+    if (H->p > 0) {
+        TT_TRY(ctx->scratch[3].ensure((size_t)H->p * sizeof(double)));
+        W = (double *)ctx->scratch[3].ptr;
+        TT_TRY(launch_nl_project(ctx, H, x, 1, 1, TTDFT_F64, W));
+    }
+    hipLaunchKernelGGL(spmv_kernel, dim3((unsigned)((H->n + 3) / 4)), dim3(256), 0, ctx->stream,
+                       H->n, H->rowptr, H->col, H->val64, x, H->p > 0 ? H->ct_rowptr : nullptr,
+                       H->ct_col, H->ct_val64, W, y);
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
 }
 
 int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, void *Y,
