@@ -78,6 +78,22 @@ def cpu_baseline_sample(s, threads, reps=3):
     return float(np.median(times))
 
 
+def profiled_traffic(kernel_tag):
+    """HBM bytes per launch of the FP64 band SpMM from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a block copy)."""
+    path = os.path.join(ROOT, "profiles", "spmm_band_fp64_172atom_hbm_traffic.txt")
+    try:
+        cur = None
+        for ln in open(path):
+            if ln.startswith("kernel "):
+                cur = ln
+            elif cur and kernel_tag in cur and ln.strip().startswith("traffic"):
+                return float(ln.split()[1]) * 1e9
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,7 +220,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic("band_spmm_kernel<double"),
             "launch_ms": round(spmm_ms, 4),
             "bytes_per_launch": spmm_bytes,
             "flop_frac_fp64": round(spmm_flops / (spmm_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
