@@ -220,7 +220,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": profiled_traffic("band_spmm_kernel<double"),
+            "traffic": profiled_traffic("brick_spmm_kernel<double"),
             "launch_ms": round(spmm_ms, 4),
             "bytes_per_launch": spmm_bytes,
             "flop_frac_fp64": round(spmm_flops / (spmm_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),

@@ -92,7 +92,10 @@ def test_ham_upload_rejects_bad_csr(ctx):
 # ---------------------------------------------------------------- band format
 @pytest.mark.parametrize("dims,bands,k", [((11, 9, 7), (3, 2, 2), 16), ((20, 10, 6), (2, 3, 1), 64),
                                           ((12, 12, 12), (6, 6, 7), 100),
-                                          ((50, 6, 5), (6, 2, 2), 300)])
+                                          ((50, 6, 5), (6, 2, 2), 300),
+                                          ((70, 5, 4), (3, 1, 1), 40),    # two x-chunks
+                                          ((6, 7, 5), (1, 2, 2), 33),     # padded narrow chunk
+                                          ((17, 9, 8), (7, 3, 2), 81)])   # widest x band
 @pytest.mark.parametrize("dtype", [E.F64, E.F32])
 def test_band_spmm_matches_oracle(ctx, dims, bands, k, dtype):
     s = E.SynthHam(*dims, *bands, 4, seed=sum(dims))

@@ -603,8 +603,8 @@ int ttdft_ham_free(ttdft_ham H) {
     if (!H) return TTDFT_OK;
     hipError_t e = hipStreamSynchronize(H->ctx->stream);
     (void)e;
-    dfree(H->band64);
-    dfree(H->band32);
+    dfree(H->mf64);
+    dfree(H->mf32);
     dfree(H->rowptr);
     dfree(H->col);
     dfree(H->val64);

@@ -1,24 +1,26 @@
-// Register-blocked SpMM for H_loc with Kronecker band structure.
+// Matrix-core SpMM for H_loc with Kronecker band structure.
 //
-// In a Tucker product basis the projected Hamiltonian couples
-// (x,y,z) with (x',y',z') only for |x-x'| <= bx, |y-y'| <= by, |z-z'| <= bz
+// In a Tucker product basis the projected Hamiltonian couples (x,y,z) with
+// (x',y',z') only for |x-x'| <= bx, |y-y'| <= by, |z-z'| <= bz
 // (ProjectHamiltonianSparse.cc:464-548 builds each row as a Kronecker dyad and
-// truncates it).  Such an H_loc is stored index-free: for every x-line
-// (y,z) and every (dy,dz) pair, an rx x (2bx+1) block of band values
-// (zeros where the CSR has no entry).
+// truncates it).  The rows are grouped into 4x4 "bricks" (4 consecutive x, 4
+// consecutive y, one z).  For one brick and one neighbouring x-line (y',z'),
+// the coupling is a dense 16 x (4 + 2bx) block: 16 brick rows against the x'
+// window of that line.  It is padded to 16 x 4KS and stored pre-swizzled in
+// the A-operand layout of the 16x16x4 MFMA.  HX for a brick and a 16-state
+// tile is then a sum of KS MFMAs per neighbouring line, so the flops of the
+// band run on the matrix cores in FP64 or FP32 instead of the VALU.
 //
-// One wavefront owns RW consecutive x-rows of LPW consecutive y-lines (same
-// z) and a 64-state panel (lane = state); small tiles (RW = 5, LPW = 2 in
-// FP64, 4 in FP32) keep ~100 VGPRs per wave so enough waves are resident to
-// hide the scalar-cache latency of the band values.  For every neighbouring line
-// (y',z') it loads the RW + 2bx X rows of that line's x-window into registers
-// once (coalesced 64-state rows) and feeds up to LPW * RW * (2bx+1) FMAs from
-// them; the band values are wavefront-uniform and arrive through the scalar
-// cache.  No LDS and no barriers: waves are independent.  The nonlocal term
-// C^T W and the Chebyshev three-term recurrence are applied in the epilogue,
-// as in the CSR kernel.  Blocks are renumbered so consecutive logical blocks
-// land on the same XCD (blocks are dispatched round-robin over the 8 XCDs),
-// keeping neighbouring lines' X rows in one L2.
+// A workgroup owns one y-brick row (4 y), one z, a chunk of up to 16 x-bricks
+// (one wavefront each) and 16*NST states.  For every neighbouring line it
+// stages the line's X window (all x-bricks of the chunk plus the 2bx halo,
+// 16*NST states) in LDS once; every wave reads its B operands from there.
+// The stage is double-buffered in LDS; the global loads of X run two lines
+// ahead through a pair of register buffers and those of the A fragments one
+// line ahead, so HBM/L2 latency hides behind MFMA work.  The nonlocal term C^T W and the Chebyshev three-term
+// recurrence are applied in the epilogue, as in the CSR kernel.  Workgroups are
+// renumbered so consecutive logical workgroups land on the same XCD (dispatch
+// is round-robin over the 8 XCDs), keeping neighbouring lines' X in one L2.
 #include <algorithm>
 #include <thread>
 #include <vector>
@@ -28,9 +30,33 @@
 namespace ttdft {
 namespace {
 
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4_t mfma16x16x4(double a, double b, d4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma16x16x4(float a, float b, f4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 template <typename T>
-struct BandArgs {
-    const T *band;
+struct Acc4;
+template <>
+struct Acc4<double> {
+    typedef d4_t type;
+};
+template <>
+struct Acc4<float> {
+    typedef f4_t type;
+};
+
+// states per workgroup = 16 * kStateTiles<T>
+template <typename T>
+constexpr int kStateTiles() { return sizeof(T) == 8 ? 4 : 5; }
+
+template <typename T>
+struct BrickArgs {
+    const T *mf;
     const T *X;
     int64_t ldx;
     const int64_t *ct_rowptr;
@@ -42,166 +68,249 @@ struct BandArgs {
     int64_t ldy;
     const T *Z;
     int64_t ldz;
-    int32_t k, rx, ry, rz, by, bz, rxp, ngx, nly, npanel;
-    int64_t nwaves;
+    int32_t k, rx, ry, rz, bx, by, bz;
+    int32_t gxn, hyn;  // x-bricks, y-bricks
+    int32_t gxb, nxc;  // x-bricks per workgroup (waves), x-chunks
+    int32_t nsg;       // state groups
+    int64_t nblocks;
     T alpha, beta, delta;
     int use_beta, use_delta;
 };
 
-template <typename T, int W, int RW, int kLPW>
-__global__ __launch_bounds__(256) void band_spmm_kernel(BandArgs<T> a) {
-    constexpr int BX = (W - 1) / 2;
-    constexpr int NWIN = RW + W - 1;
+// KS: 4-column MFMA steps covering the x' window (4 KS >= 4 + 2 bx).
+// NST: 16-state tiles per workgroup.  PF: staged X elements per thread.
+template <typename T, int KS, int NST, int PF>
+__global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
+    typedef typename Acc4<T>::type acc_t;
+    constexpr int S = 16 * NST;
+    extern __shared__ __align__(16) unsigned char brick_lds[];
+    T *Xs = reinterpret_cast<T *>(brick_lds);
+    const int nthr = a.gxb * 64;
+    const int TR = 4 * (a.gxb - 1) + 4 * KS;  // staged x' rows
+    const int TILE = TR * S;
+    const int lane = threadIdx.x & 63;
+    const int wl = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+
     const unsigned nb = gridDim.x;  // a multiple of 8
     const unsigned bid = blockIdx.x;
     const unsigned L = (bid & 7u) * (nb >> 3) + (bid >> 3);
-    const int lane = threadIdx.x & 63;
-    const int64_t wid = (int64_t)L * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (wid >= a.nwaves) return;
-    int64_t t = wid;
-    const int panel = (int)(t % a.npanel);
-    t /= a.npanel;
-    const int gx = (int)(t % a.ngx);
-    t /= a.ngx;
-    const int ly = (int)(t % a.nly);
-    const int z = (int)(t / a.nly);
-    const int xr0 = gx * RW, y0 = ly * kLPW;
-    const int c = panel * 64 + lane;
-    const bool cok = c < a.k;
-    const int wy = 2 * a.by + 1;
-    const int P = wy * (2 * a.bz + 1);
+    if ((int64_t)L >= a.nblocks) return;
+    int64_t t = L;
+    const int sg = (int)(t % a.nsg);
+    t /= a.nsg;
+    const int xc = (int)(t % a.nxc);
+    t /= a.nxc;
+    const int h = (int)(t % a.hyn);
+    const int z = (int)(t / a.hyn);
+    const int s0 = sg * S;
+    const int g0 = xc * a.gxb;
+    const int g = g0 + wl;
+    const bool active = g < a.gxn;
+    const int xw0 = 4 * g0 - a.bx;  // x' of staged row 0
+    const int NPY = 2 * a.by + 4;
+    const int NP = NPY * (2 * a.bz + 1);
 
-    T acc[kLPW][RW];
+    acc_t acc[NST];
 #pragma unroll
-    for (int l = 0; l < kLPW; ++l)
-#pragma unroll
-        for (int r = 0; r < RW; ++r) acc[l][r] = T(0);
+    for (int st = 0; st < NST; ++st) acc[st] = acc_t{0, 0, 0, 0};
 
+    const int y0 = 4 * h;
     const int zlo = z - a.bz < 0 ? 0 : z - a.bz;
     const int zhi = z + a.bz > a.rz - 1 ? a.rz - 1 : z + a.bz;
     const int ylo = y0 - a.by < 0 ? 0 : y0 - a.by;
-    const int yhi = y0 + kLPW - 1 + a.by > a.ry - 1 ? a.ry - 1 : y0 + kLPW - 1 + a.by;
-    for (int zz = zlo; zz <= zhi; ++zz) {
-        for (int yy = ylo; yy <= yhi; ++yy) {
-            const T *xb = a.X + ((int64_t)(zz * a.ry + yy) * a.rx) * a.ldx + c;
-            T xw[NWIN];
+    const int yhi = y0 + 3 + a.by > a.ry - 1 ? a.ry - 1 : y0 + 3 + a.by;
+    const int ny = yhi - ylo + 1;
+    const int nlines = ny * (zhi - zlo + 1);
+
+    T ra[PF], rb[PF];
+    // Loads are unconditional (clamped, always-valid addresses; out-of-range
+    // elements are zeroed after the load): a load under a branch would make the
+    // compiler wait for every outstanding load before the MFMAs.
+    auto fetch = [&](int li_, T *r) {
+        const bool line_ok = li_ < nlines;
+        const int lc = line_ok ? li_ : nlines - 1;
+        const int yy = ylo + lc % ny, zz = zlo + lc / ny;
+        const T *xb = a.X + ((int64_t)(zz * a.ry + yy) * a.rx) * a.ldx;
 #pragma unroll
-            for (int j = 0; j < NWIN; ++j) {
-                const int xp = xr0 - BX + j;
-                xw[j] = (cok && xp >= 0 && xp < a.rx) ? xb[(int64_t)xp * a.ldx] : T(0);
-            }
-#pragma unroll
-            for (int l = 0; l < kLPW; ++l) {
-                const int y = y0 + l;
-                const int dy = yy - y;
-                if (y < a.ry && dy >= -a.by && dy <= a.by) {
-                    const int pair = (dy + a.by) + wy * (zz - z + a.bz);
-                    const T *Al =
-                        a.band + (((int64_t)(y + a.ry * z) * P + pair) * a.rxp + xr0) * W;
-#pragma unroll
-                    for (int r = 0; r < RW; ++r)
-#pragma unroll
-                        for (int d = 0; d < W; ++d) acc[l][r] += Al[r * W + d] * xw[r + d];
-                }
-            }
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + nthr * u;
+            const int row = e / S, s = e - row * S;
+            const int xp = xw0 + row, st = s0 + s;
+            const bool ok = line_ok && e < TILE && xp >= 0 && xp < a.rx && st < a.k;
+            const int xpc = xp < 0 ? 0 : (xp >= a.rx ? a.rx - 1 : xp);
+            const int stc = st < a.k ? st : a.k - 1;
+            const T v = xb[(int64_t)xpc * a.ldx + stc];
+            r[u] = ok ? v : T(0);
         }
+    };
+    auto stage = [&](int buf, const T *r) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + nthr * u;
+            if (e < TILE) Xs[buf * TILE + e] = r[u];
+        }
+    };
+    // A fragments of line li_ (wave-private, from HBM/L2): loaded one line ahead
+    auto load_a = [&](int li_, T *av) {
+        const int lc = li_ < nlines ? li_ : nlines - 1;
+        const int gc = active ? g : a.gxn - 1;
+        const int yy = ylo + lc % ny, zz = zlo + lc / ny;
+        const int tt = yy - y0;  // in [-by, 3 + by] by construction
+        const int p = (tt + a.by) + NPY * (zz - z + a.bz);
+        const T *Ab = a.mf + ((((int64_t)(z * a.hyn + h) * a.gxn + gc) * NP + p) * KS) * 64 + lane;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) av[ks] = Ab[ks * 64];
+    };
+    auto compute = [&](int li_, const T *av) {
+        const T *Xb = Xs + (li_ & 1) * TILE + (4 * wl + lk) * S + li;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int st = 0; st < NST; ++st)
+                acc[st] = mfma16x16x4(av[ks], Xb[4 * ks * S + 16 * st], acc[st]);
+    };
+
+    // Line i is staged in LDS buffer i&1; the X windows of lines i+1 and i+2
+    // and the A fragments of line i+1 are in flight in registers while line i
+    // is multiplied.
+    T aa[KS], ab[KS];
+    fetch(0, rb);
+    stage(0, rb);
+    fetch(1, ra);
+    load_a(0, aa);
+    __syncthreads();
+    for (int li_ = 0; li_ < nlines; li_ += 2) {
+        fetch(li_ + 2, rb);
+        load_a(li_ + 1, ab);
+        if (active) compute(li_, aa);
+        if (li_ + 1 < nlines) stage(1, ra);
+        __syncthreads();
+        if (li_ + 1 >= nlines) break;
+        fetch(li_ + 3, ra);
+        load_a(li_ + 2, aa);
+        if (active) compute(li_ + 1, ab);
+        if (li_ + 2 < nlines) stage(0, rb);
+        __syncthreads();
     }
-    if (!cok) return;
+    if (!active) return;
+
+    // D layout of the 16x16x4 MFMA: FP64 element v of lane l is D[l/16 + 4v][l%16],
+    // FP32 element v is D[4(l/16) + v][l%16].  D rows are brick rows i =
+    // (x - 4g) + 4 (y - 4h), D columns are states.
 #pragma unroll
-    for (int l = 0; l < kLPW; ++l) {
-        const int y = y0 + l;
-        if (y >= a.ry) continue;
+    for (int st = 0; st < NST; ++st) {
+        const int s = s0 + 16 * st + li;
 #pragma unroll
-        for (int r = 0; r < RW; ++r) {
-            const int x = xr0 + r;
-            if (x >= a.rx) continue;
-            const int64_t row = x + (int64_t)a.rx * (y + (int64_t)a.ry * z);
-            T v = acc[l][r];
-            if (a.ct_rowptr) {
-                const int64_t f0 = a.ct_rowptr[row], f1 = a.ct_rowptr[row + 1];
-                for (int64_t e = f0; e < f1; ++e)
-                    v += a.ct_val[e] * a.Wnl[(int64_t)a.ct_col[e] * a.ldw + c];
+        for (int v = 0; v < 4; ++v) {
+            const int i = sizeof(T) == 8 ? (lk + 4 * v) : (4 * lk + v);
+            const int x = 4 * g + (i & 3), y = y0 + (i >> 2);
+            if (x < a.rx && y < a.ry && s < a.k) {
+                const int64_t row = x + (int64_t)a.rx * (y + (int64_t)a.ry * z);
+                T val = acc[st][v];
+                if (a.ct_rowptr) {
+                    const int64_t f0 = a.ct_rowptr[row], f1 = a.ct_rowptr[row + 1];
+                    for (int64_t e = f0; e < f1; ++e)
+                        val += a.ct_val[e] * a.Wnl[(int64_t)a.ct_col[e] * a.ldw + s];
+                }
+                T out = val * a.alpha;
+                if (a.use_beta) out = out + a.beta * a.X[row * a.ldx + s];
+                if (a.use_delta) out = out + a.delta * a.Z[row * a.ldz + s];
+                a.Y[row * a.ldy + s] = out;
             }
-            T out = v * a.alpha;
-            if (a.use_beta) out = out + a.beta * a.X[row * a.ldx + c];
-            if (a.use_delta) out = out + a.delta * a.Z[row * a.ldz + c];
-            a.Y[row * a.ldy + c] = out;
         }
     }
 }
 
-template <typename T>
-constexpr int lines_per_wave() { return sizeof(T) == 8 ? 2 : 4; }
+// staged elements per thread: TR * S / (64 gxb) <= NST (gxb - 1 + KS) / gxb
+constexpr int pf_for(int nst, int ks, int gxb_min) {
+    return (nst * (gxb_min - 1 + ks) + gxb_min - 1) / gxb_min;
+}
 
-template <typename T, int W, int RW>
-int launch_tmpl(ttdft_ctx ctx, const BandArgs<T> &a) {
-    int64_t blocks = (a.nwaves + 3) / 4;
-    blocks = (blocks + 7) / 8 * 8;
-    hipLaunchKernelGGL((band_spmm_kernel<T, W, RW, lines_per_wave<T>()>), dim3((unsigned)blocks),
-                       dim3(256), 0, ctx->stream, a);
+template <typename T, int KS>
+int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a) {
+    constexpr int NST = kStateTiles<T>();
+    const int S = 16 * NST;
+    const int TR = 4 * (a.gxb - 1) + 4 * KS;
+    const size_t lds = (size_t)2 * TR * S * sizeof(T);
+    const int nthr = 64 * a.gxb;
+    const int pf = (TR * S + nthr - 1) / nthr;
+    int64_t blocks = (a.nblocks + 7) / 8 * 8;
+    TT_CHECK(lds <= 160 * 1024, TTDFT_EBADARG, "band SpMM: LDS stage too large");
+    // two variants: wide chunks (>= 12 x-bricks) and narrow ones (>= 4)
+    constexpr int PF_WIDE = pf_for(NST, KS, 12), PF_NARROW = pf_for(NST, KS, 4);
+    if (a.gxb >= 12 && pf <= PF_WIDE) {
+        auto kf = brick_spmm_kernel<T, KS, NST, PF_WIDE>;
+        TT_HIP(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL(kf, dim3((unsigned)blocks), dim3(nthr), lds, ctx->stream, a);
+    } else {
+        TT_CHECK(pf <= PF_NARROW, TTDFT_EBADARG, "band SpMM: staging exceeds the register buffer");
+        auto kf = brick_spmm_kernel<T, KS, NST, PF_NARROW>;
+        TT_HIP(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL(kf, dim3((unsigned)blocks), dim3(nthr), lds, ctx->stream, a);
+    }
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
 
 template <typename T>
-int dispatch(ttdft_ctx ctx, const BandArgs<T> &a, int W, int RW) {
-#define TT_BAND_CASE(WV)                                                              \
-    case WV:                                                                          \
-        return RW == 5 ? launch_tmpl<T, WV, 5>(ctx, a)                                \
-                       : (RW == 8 ? launch_tmpl<T, WV, 8>(ctx, a) : launch_tmpl<T, WV, 10>(ctx, a));
-    switch (W) {
-        TT_BAND_CASE(5)
-        TT_BAND_CASE(7)
-        TT_BAND_CASE(9)
-        TT_BAND_CASE(11)
-        TT_BAND_CASE(13)
-        TT_BAND_CASE(15)
+int launch_brick(ttdft_ctx ctx, BrickArgs<T> a, int ks) {
+    a.nsg = (a.k + 16 * kStateTiles<T>() - 1) / (16 * kStateTiles<T>());
+    a.nblocks = (int64_t)a.nsg * a.nxc * a.hyn * a.rz;
+    switch (ks) {
+    case 2:
+        return launch_ks<T, 2>(ctx, a);
+    case 3:
+        return launch_ks<T, 3>(ctx, a);
+    case 4:
+        return launch_ks<T, 4>(ctx, a);
+    case 5:
+        return launch_ks<T, 5>(ctx, a);
     default:
         break;
     }
-#undef TT_BAND_CASE
-    set_error("band SpMM: unsupported band width %d", W);
+    set_error("band SpMM: unsupported x half-bandwidth %d", a.bx);
     return TTDFT_EBADARG;
 }
 
 }  // namespace
 
-bool band_supported(int32_t W) { return W >= 5 && W <= 15 && (W & 1); }
-
 int launch_band_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                      const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
                      int32_t k, int dtype, double alpha, double beta, double delta,
                      int use_beta, int use_delta) {
-    const int Wd = 2 * H->bx + 1;
-    const int npanel = (k + 63) / 64;
-    const int ngx = H->rxp / H->rw;
-    const int lpw = dtype == TTDFT_F64 ? lines_per_wave<double>() : lines_per_wave<float>();
-    const int nly = (H->ry + lpw - 1) / lpw;
-    const int64_t nwaves = (int64_t)npanel * ngx * nly * H->rz;
     if (dtype == TTDFT_F64) {
-        TT_CHECK(H->band64, TTDFT_EBADARG, "no FP64 band copy of H_loc uploaded");
-        BandArgs<double> a{H->band64, (const double *)X, ldx,
-                           H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col, H->ct_val64,
-                           (const double *)W, k, (double *)Y, ldy, (const double *)Z, ldz,
-                           k, H->rx, H->ry, H->rz, H->by, H->bz, H->rxp, ngx, nly, npanel,
-                           nwaves, alpha, beta, delta, use_beta, use_delta};
-        return dispatch<double>(ctx, a, Wd, H->rw);
+        TT_CHECK(H->mf64, TTDFT_EBADARG, "no FP64 band copy of H_loc uploaded");
+        BrickArgs<double> a{H->mf64, (const double *)X, ldx,
+                            H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col, H->ct_val64,
+                            (const double *)W, k, (double *)Y, ldy, (const double *)Z, ldz,
+                            k, H->rx, H->ry, H->rz, H->bx, H->by, H->bz, H->gxn, H->hyn,
+                            H->gxb, H->nxc, 0, 0, alpha, beta, delta, use_beta, use_delta};
+        return launch_brick<double>(ctx, a, H->ks);
     }
-    TT_CHECK(H->band32, TTDFT_EBADARG, "no FP32 band copy of H_loc uploaded");
-    BandArgs<float> a{H->band32, (const float *)X, ldx,
-                      H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col, H->ct_val32,
-                      (const float *)W, k, (float *)Y, ldy, (const float *)Z, ldz,
-                      k, H->rx, H->ry, H->rz, H->by, H->bz, H->rxp, ngx, nly, npanel,
-                      nwaves, (float)alpha, (float)beta, (float)delta, use_beta, use_delta};
-    return dispatch<float>(ctx, a, Wd, H->rw);
+    TT_CHECK(H->mf32, TTDFT_EBADARG, "no FP32 band copy of H_loc uploaded");
+    BrickArgs<float> a{H->mf32, (const float *)X, ldx,
+                       H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col, H->ct_val32,
+                       (const float *)W, k, (float *)Y, ldy, (const float *)Z, ldz,
+                       k, H->rx, H->ry, H->rz, H->bx, H->by, H->bz, H->gxn, H->hyn,
+                       H->gxb, H->nxc, 0, 0, (float)alpha, (float)beta, (float)delta,
+                       use_beta, use_delta};
+    return launch_brick<float>(ctx, a, H->ks);
 }
 
 // ---------------------------------------------------------------------------
-// Host: detect the band and build the band blocks from CSR.
+// Host: detect the band and scatter the CSR values into MFMA A fragments.
+//
+// Fragment layout: mf[((((z * hyn + h) * gxn + g) * NP + p) * KS + ks) * 64 + lane]
+// holds A[i][kk] of the 16x16x4 MFMA (lane = i + 16 kk) for brick (g, h, z),
+// neighbouring line p = (y' - 4h + by) + (2by + 4)(z' - z + bz) and x' =
+// 4g - bx + 4ks + kk; brick row i = (x - 4g) + 4 (y - 4h).  Zero where H_loc
+// has no entry.
 int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz,
                const int64_t *rowptr, const int32_t *colind, const double *val, int copies) {
     const int64_t n = (int64_t)rx * ry * rz;
-    // half-bandwidths (parallel max over rows)
     unsigned nt = std::thread::hardware_concurrency();
     if (nt == 0) nt = 1;
     if (nt > 32) nt = 32;
@@ -212,7 +321,8 @@ int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz
             th.emplace_back([&, t]() {
                 int bx = 0, by = 0, bz = 0;
                 for (int64_t i = n * t / nt; i < n * (t + 1) / nt; ++i) {
-                    const int x = (int)(i % rx), y = (int)((i / rx) % ry), z = (int)(i / ((int64_t)rx * ry));
+                    const int x = (int)(i % rx), y = (int)((i / rx) % ry),
+                              z = (int)(i / ((int64_t)rx * ry));
                     for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
                         const int64_t j = colind[e];
                         const int xj = (int)(j % rx), yj = (int)((j / rx) % ry),
@@ -231,50 +341,54 @@ int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz
     const int bx = *std::max_element(mx.begin(), mx.end());
     const int by = *std::max_element(my.begin(), my.end());
     const int bz = *std::max_element(mz.begin(), mz.end());
-    const int W = 2 * bx + 1;
-    const int64_t P = (int64_t)(2 * by + 1) * (2 * bz + 1);
-    // rows per wavefront: the register block with the least padding of rx
-    // (ties go to the smaller block, which keeps more waves resident)
-    const int pad5 = (5 - rx % 5) % 5, pad8 = (8 - rx % 8) % 8, pad10 = (10 - rx % 10) % 10;
-    const int rw = (pad5 <= pad8 && pad5 <= pad10) ? 5 : (pad8 <= pad10 ? 8 : 10);
-    const int rxp = (rx + rw - 1) / rw * rw;
-    const int64_t slots = (int64_t)ry * rz * P * rxp * W;
-    const int64_t nnz = rowptr[n];
-    const double fill = slots > 0 ? (double)nnz / (double)slots : 0.0;
-    if (!band_supported(W) || fill < 0.15 || rx < 4) return TTDFT_OK;  // keep CSR
-    std::vector<double> band((size_t)slots, 0.0);
+    const int64_t band_slots = n * (int64_t)(2 * bx + 1) * (2 * by + 1) * (2 * bz + 1);
+    const double fill = band_slots > 0 ? (double)rowptr[n] / (double)band_slots : 0.0;
+    if (bx < 1 || bx > 7 || fill < 0.15 || rx < 4) return TTDFT_OK;  // keep CSR
+
+    const int KS = (4 + 2 * bx + 3) / 4;
+    const int gxn = (rx + 3) / 4, hyn = (ry + 3) / 4;
+    const int NPY = 2 * by + 4, NP = NPY * (2 * bz + 1);
+    const int64_t slots = (int64_t)rz * hyn * gxn * NP * KS * 64;
+    std::vector<double> mf((size_t)slots, 0.0);
     {
+        // rows of one thread fill distinct brick rows: no two threads write a slot
         std::vector<std::thread> th;
         for (unsigned t = 0; t < nt; ++t)
             th.emplace_back([&, t]() {
-                for (int64_t i = n * t / nt; i < n * (t + 1) / nt; ++i) {
-                    const int x = (int)(i % rx), y = (int)((i / rx) % ry), z = (int)(i / ((int64_t)rx * ry));
-                    const int64_t line = y + (int64_t)ry * z;
-                    for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+                for (int64_t r = n * t / nt; r < n * (t + 1) / nt; ++r) {
+                    const int x = (int)(r % rx), y = (int)((r / rx) % ry),
+                              z = (int)(r / ((int64_t)rx * ry));
+                    const int g = x >> 2, h = y >> 2;
+                    const int i16 = (x & 3) + 4 * (y & 3);
+                    const int64_t base = (((int64_t)z * hyn + h) * gxn + g) * NP;
+                    for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
                         const int64_t j = colind[e];
-                        const int dx = (int)(j % rx) - x, dy = (int)((j / rx) % ry) - y,
-                                  dz = (int)(j / ((int64_t)rx * ry)) - z;
-                        const int64_t pair = (dy + by) + (int64_t)(2 * by + 1) * (dz + bz);
-                        band[((line * P + pair) * rxp + x) * W + dx + bx] += val[e];
+                        const int xj = (int)(j % rx), yj = (int)((j / rx) % ry),
+                                  zj = (int)(j / ((int64_t)rx * ry));
+                        const int p = (yj - 4 * h + by) + NPY * (zj - z + bz);
+                        const int c = xj - 4 * g + bx;  // in [0, 4 KS)
+                        mf[(size_t)(((base + p) * KS + (c >> 2)) * 64 + i16 + 16 * (c & 3))] +=
+                            val[e];
                     }
                 }
             });
         for (auto &x : th) x.join();
     }
     if (copies & TTDFT_HAM_F64) {
-        TT_HIP(hipMalloc(&H->band64, sizeof(double) * band.size()));
-        TT_HIP(hipMemcpyAsync(H->band64, band.data(), sizeof(double) * band.size(),
+        TT_HIP(hipMalloc(&H->mf64, sizeof(double) * mf.size()));
+        TT_HIP(hipMemcpyAsync(H->mf64, mf.data(), sizeof(double) * mf.size(),
                               hipMemcpyHostToDevice, ctx->stream));
         TT_HIP(hipStreamSynchronize(ctx->stream));
     }
     if (copies & TTDFT_HAM_F32) {
-        std::vector<float> b32(band.size());
-        for (size_t q = 0; q < band.size(); ++q) b32[q] = (float)band[q];
-        TT_HIP(hipMalloc(&H->band32, sizeof(float) * b32.size()));
-        TT_HIP(hipMemcpyAsync(H->band32, b32.data(), sizeof(float) * b32.size(),
+        std::vector<float> m32(mf.size());
+        for (size_t q = 0; q < mf.size(); ++q) m32[q] = (float)mf[q];
+        TT_HIP(hipMalloc(&H->mf32, sizeof(float) * m32.size()));
+        TT_HIP(hipMemcpyAsync(H->mf32, m32.data(), sizeof(float) * m32.size(),
                               hipMemcpyHostToDevice, ctx->stream));
         TT_HIP(hipStreamSynchronize(ctx->stream));
     }
+    const int nxc = (gxn + 15) / 16;
     H->format = 1;
     H->rx = rx;
     H->ry = ry;
@@ -282,8 +396,11 @@ int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz
     H->bx = bx;
     H->by = by;
     H->bz = bz;
-    H->rw = rw;
-    H->rxp = rxp;
+    H->ks = KS;
+    H->gxn = gxn;
+    H->hyn = hyn;
+    H->nxc = nxc;
+    H->gxb = std::max(4, (gxn + nxc - 1) / nxc);
     return TTDFT_OK;
 }
 

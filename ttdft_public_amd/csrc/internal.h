@@ -111,16 +111,16 @@ struct ttdft_ham_s {
     int32_t *ct_col = nullptr;
     double *ct_val64 = nullptr;
     float *ct_val32 = nullptr;
-    // Kronecker band storage of H_loc (format 1): values at
-    // band[((line * P + pair) * rxp + x) * W + dx + bx], line = y + ry z,
-    // pair = (dy + by) + (2 by + 1)(dz + bz), zero where H_loc has no entry
+    // Kronecker band storage of H_loc (format 1): 16x16x4 MFMA A fragments of
+    // 4x4 row bricks against each neighbouring x-line (layout in band.hip)
     int format = 0;
     int32_t rx = 0, ry = 0, rz = 0;
     int32_t bx = 0, by = 0, bz = 0;
-    int32_t rw = 0;   // rows per wavefront (x-direction register block)
-    int32_t rxp = 0;  // rx padded to a multiple of rw
-    double *band64 = nullptr;
-    float *band32 = nullptr;
+    int32_t ks = 0;             // 4-column MFMA steps per x' window
+    int32_t gxn = 0, hyn = 0;   // x-bricks, y-bricks
+    int32_t gxb = 0, nxc = 0;   // x-bricks per workgroup, x-chunks
+    double *mf64 = nullptr;
+    float *mf32 = nullptr;
 };
 
 namespace ttdft {
@@ -147,7 +147,6 @@ int launch_band_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t
                      const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
                      int32_t k, int dtype, double alpha, double beta, double delta,
                      int use_beta, int use_delta);
-bool band_supported(int32_t W);
 // Full H application (nl projection + fused SpMM) with the given epilogue.
 int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, void *Y,
              int64_t ldy, const void *Z, int64_t ldz, int32_t k, int dtype, double alpha,

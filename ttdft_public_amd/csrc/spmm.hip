@@ -274,7 +274,7 @@ int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, vo
     PendingTiming ts;
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_SPMM, &ts));
     const bool band = H->format == 1 && k >= 16 &&
-                      (dtype == TTDFT_F64 ? H->band64 != nullptr : H->band32 != nullptr);
+                      (dtype == TTDFT_F64 ? H->mf64 != nullptr : H->mf32 != nullptr);
     if (band)
         TT_TRY(launch_band_spmm(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
                                 alpha, beta, delta, use_beta, use_delta));
