@@ -287,6 +287,46 @@ def test_density(ctx, dtype):
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
 
 
+def test_density_chunked_with_sparse_weights(ctx, monkeypatch):
+    """Several qz chunks, zero weights scattered among the states, ry > rx, rz."""
+    monkeypatch.setenv("TTDFT_DENSITY_PLANES", "4")
+    rng = np.random.default_rng(11)
+    rx, ry, rz = 9, 17, 5
+    px, py, pz = 40, 31, 23
+    Bx, By, Bz = (rng.standard_normal((p, r)) for p, r in ((px, rx), (py, ry), (pz, rz)))
+    k = 37
+    X = rng.standard_normal((rx * ry * rz, k + 3))  # ld > k
+    occ = rng.uniform(0.0, 1.0, k)
+    occ[[0, 5, 6, 20, 36]] = 0.0
+    dRho = E.DeviceArray(ctx, (pz, py, px))
+    dX = dev(ctx, X)
+    dBx, dBy, dBz = dev(ctx, Bx.T.copy()), dev(ctx, By.T.copy()), dev(ctx, Bz.T.copy())
+    ctx.density(dX.ptr, k + 3, k, occ, (rx, ry, rz), dBx.ptr, dBy.ptr, dBz.ptr, (px, py, pz),
+                dRho.ptr, E.F64)
+    ref = oracle.density(X[:, :k].copy(), occ, (rx, ry, rz), Bx, By, Bz)
+    got = dRho.to_host()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+
+
+@pytest.mark.parametrize("ry", [33, 50, 70, 97, 128])
+def test_density_wide_y(ctx, ry):
+    """y ranks spanning 3..8 MFMA row tiles (the y contraction's tile groups)."""
+    rng = np.random.default_rng(ry)
+    rx, rz = 3, 2
+    px, py, pz = 7, 19, 5
+    Bx, By, Bz = (rng.standard_normal((p, r)) for p, r in ((px, rx), (py, ry), (pz, rz)))
+    k = 22
+    X = rng.standard_normal((rx * ry * rz, k))
+    occ = rng.uniform(0.0, 1.0, k)
+    dRho = E.DeviceArray(ctx, (pz, py, px))
+    dX = dev(ctx, X)
+    dBx, dBy, dBz = dev(ctx, Bx.T.copy()), dev(ctx, By.T.copy()), dev(ctx, Bz.T.copy())
+    ctx.density(dX.ptr, k, k, occ, (rx, ry, rz), dBx.ptr, dBy.ptr, dBz.ptr, (px, py, pz),
+                dRho.ptr, E.F64)
+    ref = oracle.density(X, occ, (rx, ry, rz), Bx, By, Bz)
+    assert np.linalg.norm(dRho.to_host() - ref) / np.linalg.norm(ref) < 1e-12
+
+
 def test_initial_density_and_integrals(ctx):
     r = np.linspace(0.0, 9.0, 901)
     rho_r = np.exp(-r**2) * (1 + 0.3 * r)

@@ -1,226 +1,203 @@
 // Electron density at tensor-grid points,
 //   rho(qx,qy,qz) = sum_i w_i psi_i(qx,qy,qz)^2,   w_i = 2 f_i,
-//   psi_i = sum_abc C_i[a,b,c] Bx[qx,a] By[qy,b] Bz[qz,c],
-// computed for batches of S states:
-//   1. gather the batch's coefficients state-major (Cb[s][c][b][a]),
-//   2. T1[s][qz][b][a]  = sum_c Bz[qz,c] Cb[s][c][b][a]      (batched GEMM),
-//   3. T2[s][qz][b][qx] = sum_a T1[s][qz][b][a] Bx[qx,a]     (one GEMM),
-//   4. fused last contraction on FP64 MFMA + square + weighted sum over the
-//      batch, with psi never written to memory: rho tile += sum_s w_s (By T2_s)^2.
+//   psi_i = sum_abc C_i[a,b,c] Bx[qx,a] By[qy,b] Bz[qz,c].
 // The contraction order (z, then x, then y) is that of the reference's TTM
-// chain (dft/ProjectHamiltonianSparse.cc:1052-1067).
+// chain (dft/ProjectHamiltonianSparse.cc:1052-1067).  The y step is regrouped
+// through the y-factor density matrix of every (qz,qx) column,
+//   M[qz,qx](b,b') = sum_i w_i T2_i[qz,qx](b) T2_i[qz,qx](b'),
+//   rho(qx,qy,qz)  = sum_{b<=b'} (2 - [b=b']) By[qy,b] By[qy,b'] M[qz,qx](b,b'),
+// which needs 2 R^2 Q^2 k + Q^3 R(R+1) flops instead of the 2 R Q^3 k of
+// squaring every psi_i (R ranks, Q points per axis, k states): at R = 50,
+// Q = 600, k = 300 about 4x fewer, and all states are summed in one pass.
+// Every step is a dense FP64 matrix product on the matrix cores:
+//   1. Cg[c][a][b][s]  = sqrt(w_s) X[a + rx b + rx ry c][idx_s]   (gather)
+//   2. T1[qz][a][b][s] = sum_c Bz[qz,c] Cg[c][a][b][s]   (rocBLAS GEMM)
+//   3. T2[qz][qx][b][s]= sum_a Bx[qx,a] T1[qz][a][b][s]  (rocBLAS GEMM per plane)
+//   4. m[qz][qx][pair] = upper triangle of T2[qz][qx] T2[qz][qx]^T, packed
+//                        (syrk_pack_kernel, FP64 MFMA)
+//   5. rho[qz]^T       = m[qz] E^T,  E[qy][pair] = (2 - [b=b']) By[qy,b] By[qy,b']
+//                        (rocBLAS GEMM per plane)
+// Steps 2-5 run over chunks of qz planes to bound the workspace.
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
 #include "internal.h"
 
 namespace ttdft {
 namespace {
 
+// Cg[((c rx + a) ry + b) K + s] = sqrt(w_s) X[(a + rx b + rx ry c) ldx + idx_s]
 template <typename T>
-__global__ void gather_states_kernel(const T *__restrict__ X, int64_t ldx, int32_t s0,
-                                     int32_t S, int64_t N, double *__restrict__ Cb) {
-    const int64_t total = N * S;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t I = idx / S;
-        const int s = (int)(idx - I * S);
-        Cb[(int64_t)s * N + I] = (double)X[I * ldx + s0 + s];
+__global__ void gather_scaled_kernel(const T *__restrict__ X, int64_t ldx,
+                                     const int32_t *__restrict__ idx,
+                                     const double *__restrict__ sw, int32_t K, int32_t rx,
+                                     int32_t ry, int64_t N, double *__restrict__ Cg) {
+    const int64_t total = N * K;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t J = e / K;  // (c rx + a) ry + b
+        const int s = (int)(e - J * K);
+        const int b = (int)(J % ry);
+        const int64_t ca = J / ry;
+        const int a = (int)(ca % rx);
+        const int64_t c = ca / rx;
+        const int64_t I = a + (int64_t)rx * (b + (int64_t)ry * c);
+        Cg[e] = sw[s] * (double)X[I * ldx + idx[s]];
     }
 }
 
-// C[b](i,j) = sum_kk A[b](i,kk) B[b](kk,j), all FP64, generic strides.
-// 64x64 tile, BK = 16, 256 threads, 4x4 outputs per thread.
-__global__ __launch_bounds__(256) void gemm_batched_f64_kernel(
-    int64_t M, int64_t N, int64_t K, const double *__restrict__ A, int64_t sab, int64_t sar,
-    int64_t sac, const double *__restrict__ B, int64_t sbb, int64_t sbr, int64_t sbc,
-    double *__restrict__ C, int64_t scb, int64_t ldc) {
-    __shared__ double As[16][64 + 1];
-    __shared__ double Bs[16][64 + 1];
-    const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
-    const int64_t i0 = (int64_t)blockIdx.y * 64, j0 = (int64_t)blockIdx.x * 64;
-    const int64_t bz = blockIdx.z;
-    A += bz * sab;
-    B += bz * sbb;
-    C += bz * scb;
-    double acc[4][4] = {};
-    for (int64_t k0 = 0; k0 < K; k0 += 16) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int e = tid + 256 * t;        // 0..1023
-            const int ii = e & 63, kk = e >> 6; // A tile: i fastest
-            const int64_t gi = i0 + ii, gk = k0 + kk;
-            As[kk][ii] = (gi < M && gk < K) ? A[gi * sar + gk * sac] : 0.0;
-            const int jj = e & 63;
-            const int64_t gj = j0 + jj;
-            Bs[kk][jj] = (gj < N && gk < K) ? B[gk * sbr + gj * sbc] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            double a[4], b[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) a[v] = As[kk][ty + 16 * v];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) b[u] = Bs[kk][tx + 16 * u];
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-#pragma unroll
-                for (int u = 0; u < 4; ++u) acc[v][u] += a[v] * b[u];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int64_t gi = i0 + ty + 16 * v;
-        if (gi >= M) continue;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int64_t gj = j0 + tx + 16 * u;
-            if (gj < N) C[gi * ldc + gj] = acc[v][u];
-        }
-    }
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+constexpr int tri_col(int p) {  // column b' of upper-triangle pair p (column-major order)
+    int bp = 0;
+    while ((bp + 1) * (bp + 2) / 2 <= p) ++bp;
+    return bp;
 }
 
-// rho[qz][qy][qx] (+)= sum_s w[s] (sum_b By[qy + py b] T2[s][qz][b][qx])^2 on
-// FP64 MFMA (16x16x4).  Block = 8 waves = 128 qy x 64 qx at one qz; wave =
-// 32 qy x 32 qx (2x2 MFMA tiles).  A[i][kk] = By[qy0+i][b+kk] is the same for
-// every state and stays in registers; B[kk][j] = T2_s[b+kk][qx0+j] is staged
-// per state in LDS ([ry][64], shared by the four qy-groups).  psi never leaves
-// the accumulators: it is squared, weighted and summed over the batch there.
-// f64 MFMA layouts: A[i][kk] in lane i + 16 kk, B[kk][j] in lane j + 16 kk,
-// D[i][j] in lane j + 16 (i%4), element i/4.
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-// KS = number of 4-deep K steps (ceil(ry/4) rounded up to a supported size).
-// The next state's T2 tile is prefetched into registers while the current
-// state's MFMAs run, then written to the other LDS buffer (one barrier per
-// state).
-template <int KS>
-__global__ __launch_bounds__(512) void density_square_mfma_kernel(
-    const double *__restrict__ T2, const double *__restrict__ By, const double *__restrict__ w,
-    int32_t S, int32_t px, int32_t py, int32_t pz, int32_t ry, double *__restrict__ rho,
-    int accumulate) {
-    constexpr int TILE = KS * 4 * 64;          // doubles per state tile
-    constexpr int PER_THREAD = (TILE + 511) / 512;
-    extern __shared__ double Ts[];             // [2][TILE]
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int qx0 = blockIdx.x * 64 + (wave & 1) * 32;
-    const int qy0 = blockIdx.y * 128 + (wave >> 1) * 32;
-    const int qz = blockIdx.z;
-    const int li = lane & 15, lk = lane >> 4;
-    double a0[KS], a1[KS];
+// Packed upper triangle of M = V V^T for one column t per wave:
+//   V[b][s] = T2[t R K + b K + s] (R <= 16 RT, K states),
+//   m[t np + b'(b'+1)/2 + b] = M[b][b'],  b <= b'.
+// The 16x16 output tiles (bi, bj), bi <= bj, are numbered column-major; the
+// wave computes tiles [P0, P0 + NP).  Per 16-state chunk, lane (i, kq) holds
+// V[16 bi + i][c + 4 kq + e] for e = 0..3; MFMA step e contracts the four
+// states {c + 4 kq + e}: both operands come from the same registers
+// (A[i][kq] and B[kq][j] have the same lane layout), and the order in which
+// states are summed does not matter.  The next chunk is loaded while the
+// current one is multiplied; loads are unconditional (clamped address, value
+// zeroed) so the compiler waits only for the chunk in use.
+template <int RT, int P0, int NP>
+__global__ __launch_bounds__(256) void syrk_pack_kernel(const double *__restrict__ T2, int32_t R,
+                                                         int32_t K, int64_t ncols,
+                                                         double *__restrict__ mp) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ncols) return;  // no barriers: waves are independent
+    const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+    const double *V = T2 + t * R * K;
+    d4_t acc[NP];
 #pragma unroll
-    for (int t = 0; t < KS; ++t) {
-        const int b = 4 * t + lk;
-        const int qa = qy0 + li, qb = qy0 + 16 + li;
-        a0[t] = (b < ry && qa < py) ? By[qa + (int64_t)py * b] : 0.0;
-        a1[t] = (b < ry && qb < py) ? By[qb + (int64_t)py * b] : 0.0;
+    for (int q = 0; q < NP; ++q) acc[q] = d4_t{0, 0, 0, 0};
+    double cur[RT][4], nxt[RT][4];
+    auto load = [&](int c, double (*v)[4]) {
+#pragma unroll
+        for (int bi = 0; bi < RT; ++bi) {
+            const int row = 16 * bi + i;
+            const int rc = row < R ? row : R - 1;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int s = c + 4 * kq + e;
+                const int sc = s < K ? s : K - 1;
+                const double x = V[(int64_t)rc * K + sc];
+                v[bi][e] = (row < R && s < K) ? x : 0.0;
+            }
+        }
+    };
+    load(0, cur);
+    for (int c = 0; c < K; c += 16) {
+        load(c + 16 < K ? c + 16 : c, nxt);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const int p = P0 + q;
+                const int bj = tri_col(p), bi = p - bj * (bj + 1) / 2;
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[bi][e], cur[bj][e], acc[q], 0,
+                                                              0, 0);
+            }
+#pragma unroll
+        for (int bi = 0; bi < RT; ++bi)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cur[bi][e] = nxt[bi][e];
     }
-    auto fetch = [&](int s, double *regs) {
-        const double *T2s = T2 + (((int64_t)s * pz + qz) * ry) * px;
+    // FP64 16x16x4 D layout: element v of lane l is D[l/16 + 4v][l%16]
+    const int64_t np = (int64_t)R * (R + 1) / 2;
+    double *out = mp + t * np;
 #pragma unroll
-        for (int u = 0; u < PER_THREAD; ++u) {
-            const int e = tid + 512 * u;
-            const int q = e & 63, b = e >> 6;
-            const int qx = blockIdx.x * 64 + q;
-            regs[u] = (e < TILE && b < ry && qx < px) ? T2s[(int64_t)b * px + qx] : 0.0;
-        }
-    };
-    auto store = [&](int buf, const double *regs) {
-#pragma unroll
-        for (int u = 0; u < PER_THREAD; ++u) {
-            const int e = tid + 512 * u;
-            if (e < TILE) Ts[buf * TILE + e] = regs[u];
-        }
-    };
-    double pre[PER_THREAD];
-    fetch(0, pre);
-    store(0, pre);
-    __syncthreads();
-    d4 r00 = {0, 0, 0, 0}, r01 = r00, r10 = r00, r11 = r00;
-    const int lx = (wave & 1) * 32;
-    for (int s = 0; s < S; ++s) {
-        const int buf = s & 1;
-        if (s + 1 < S) fetch(s + 1, pre);
-        const double *Tb = Ts + buf * TILE;
-        d4 p00 = {0, 0, 0, 0}, p01 = p00, p10 = p00, p11 = p00;
-#pragma unroll
-        for (int t = 0; t < KS; ++t) {
-            const double b0 = Tb[(4 * t + lk) * 64 + lx + li];
-            const double b1 = Tb[(4 * t + lk) * 64 + lx + 16 + li];
-            p00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b0, p00, 0, 0, 0);
-            p01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], b1, p01, 0, 0, 0);
-            p10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b0, p10, 0, 0, 0);
-            p11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], b1, p11, 0, 0, 0);
-        }
-        const double ws = w[s];
+    for (int q = 0; q < NP; ++q) {
+        const int p = P0 + q;
+        const int bj = tri_col(p), bi = p - bj * (bj + 1) / 2;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            r00[v] += ws * p00[v] * p00[v];
-            r01[v] += ws * p01[v] * p01[v];
-            r10[v] += ws * p10[v] * p10[v];
-            r11[v] += ws * p11[v] * p11[v];
-        }
-        if (s + 1 < S) store(buf ^ 1, pre);
-        __syncthreads();
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int i = lk + 4 * v;
-        const int qyA = qy0 + i, qyB = qy0 + 16 + i;
-        const int qxA = qx0 + li, qxB = qx0 + 16 + li;
-        const int64_t base = (int64_t)px * py * qz;
-        if (qyA < py && qxA < px) {
-            const int64_t o = base + qxA + (int64_t)px * qyA;
-            rho[o] = accumulate ? rho[o] + r00[v] : r00[v];
-        }
-        if (qyA < py && qxB < px) {
-            const int64_t o = base + qxB + (int64_t)px * qyA;
-            rho[o] = accumulate ? rho[o] + r01[v] : r01[v];
-        }
-        if (qyB < py && qxA < px) {
-            const int64_t o = base + qxA + (int64_t)px * qyB;
-            rho[o] = accumulate ? rho[o] + r10[v] : r10[v];
-        }
-        if (qyB < py && qxB < px) {
-            const int64_t o = base + qxB + (int64_t)px * qyB;
-            rho[o] = accumulate ? rho[o] + r11[v] : r11[v];
+            const int b = 16 * bi + kq + 4 * v, bp = 16 * bj + i;
+            if (b <= bp && bp < R) out[(int64_t)bp * (bp + 1) / 2 + b] = acc[q][v];
         }
     }
 }
 
-template <int KS>
-int launch_square(ttdft_ctx ctx, const double *T2, const double *By, const double *w, int32_t S,
-                  int32_t px, int32_t py, int32_t pz, int32_t ry, double *rho, int accumulate) {
-    const dim3 grid((unsigned)((px + 63) / 64), (unsigned)((py + 127) / 128), (unsigned)pz);
-    const size_t lds = (size_t)2 * KS * 4 * 64 * sizeof(double);
-    hipLaunchKernelGGL((density_square_mfma_kernel<KS>), grid, dim3(512), lds, ctx->stream, T2,
-                       By, w, S, px, py, pz, ry, rho, accumulate);
+template <int RT, int P0, int NP>
+int launch_syrk_group(ttdft_ctx ctx, const double *T2, int32_t R, int32_t K, int64_t ncols,
+                      double *mp) {
+    const int64_t blocks = (ncols + 3) / 4;
+    hipLaunchKernelGGL((syrk_pack_kernel<RT, P0, NP>), dim3((unsigned)blocks), dim3(256), 0,
+                       ctx->stream, T2, R, K, ncols, mp);
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
 
-int density_square(ttdft_ctx ctx, const double *T2, const double *By, const double *w, int32_t S,
-                   int32_t px, int32_t py, int32_t pz, int32_t ry, double *rho, int accumulate) {
-    const int ks = (ry + 3) / 4;
-    if (ks <= 4) return launch_square<4>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
-    if (ks <= 8) return launch_square<8>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
-    if (ks <= 13) return launch_square<13>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
-    if (ks <= 18) return launch_square<18>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
-    if (ks <= 24) return launch_square<24>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
-    return launch_square<32>(ctx, T2, By, w, S, px, py, pz, ry, rho, accumulate);
+// all upper-triangle tiles of an R x R result, at most 12 tiles per wave
+int launch_syrk_pack(ttdft_ctx ctx, const double *T2, int32_t R, int32_t K, int64_t ncols,
+                     double *mp) {
+    switch ((R + 15) / 16) {
+    case 1:
+        return launch_syrk_group<1, 0, 1>(ctx, T2, R, K, ncols, mp);
+    case 2:
+        return launch_syrk_group<2, 0, 3>(ctx, T2, R, K, ncols, mp);
+    case 3:
+        return launch_syrk_group<3, 0, 6>(ctx, T2, R, K, ncols, mp);
+    case 4:
+        return launch_syrk_group<4, 0, 10>(ctx, T2, R, K, ncols, mp);
+    case 5:
+        TT_TRY((launch_syrk_group<5, 0, 8>(ctx, T2, R, K, ncols, mp)));
+        return launch_syrk_group<5, 8, 7>(ctx, T2, R, K, ncols, mp);
+    case 6:
+        TT_TRY((launch_syrk_group<6, 0, 11>(ctx, T2, R, K, ncols, mp)));
+        return launch_syrk_group<6, 11, 10>(ctx, T2, R, K, ncols, mp);
+    case 7:
+        TT_TRY((launch_syrk_group<7, 0, 10>(ctx, T2, R, K, ncols, mp)));
+        TT_TRY((launch_syrk_group<7, 10, 9>(ctx, T2, R, K, ncols, mp)));
+        return launch_syrk_group<7, 19, 9>(ctx, T2, R, K, ncols, mp);
+    case 8:
+        TT_TRY((launch_syrk_group<8, 0, 12>(ctx, T2, R, K, ncols, mp)));
+        TT_TRY((launch_syrk_group<8, 12, 12>(ctx, T2, R, K, ncols, mp)));
+        return launch_syrk_group<8, 24, 12>(ctx, T2, R, K, ncols, mp);
+    default:
+        break;
+    }
+    set_error("density: ry=%d outside 1..128", R);
+    return TTDFT_EBADARG;
 }
 
-int gemm_batched(ttdft_ctx ctx, int64_t M, int64_t N, int64_t K, int64_t batch,
-                 const double *A, int64_t sab, int64_t sar, int64_t sac, const double *B,
-                 int64_t sbb, int64_t sbr, int64_t sbc, double *C, int64_t scb, int64_t ldc) {
-    const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64), (unsigned)batch);
-    TT_CHECK((M + 63) / 64 < 65536 && batch < 65536, TTDFT_EBADARG, "gemm grid too large");
-    hipLaunchKernelGGL(gemm_batched_f64_kernel, grid, dim3(256), 0, ctx->stream, M, N, K, A, sab,
-                       sar, sac, B, sbb, sbr, sbc, C, scb, ldc);
-    TT_LAUNCH_CHECK();
-    return TTDFT_OK;
+// E[qy][pair(b,b')] = (b == b' ? 1 : 2) By[qy,b] By[qy,b'];  By column-major py x R
+__global__ void pair_table_kernel(const double *__restrict__ By, int32_t py, int32_t R,
+                                  double *__restrict__ E) {
+    const int64_t np = (int64_t)R * (R + 1) / 2;
+    const int64_t total = (int64_t)py * np;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int qy = (int)(e / np);
+        const int pr = (int)(e - (int64_t)qy * np);
+        int bp = (int)((sqrt(8.0 * pr + 1.0) - 1.0) * 0.5);
+        while ((bp + 1) * (bp + 2) / 2 <= pr) ++bp;
+        while (bp * (bp + 1) / 2 > pr) --bp;
+        const int b = pr - bp * (bp + 1) / 2;
+        const double v = By[qy + (int64_t)py * b] * By[qy + (int64_t)py * bp];
+        E[e] = b == bp ? v : 2.0 * v;
+    }
 }
+
+unsigned grid_for(int64_t total) {
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    return (unsigned)blocks;
+}
+
+#define TT_BLAS(call)                                                                     \
+    do {                                                                                  \
+        const rocblas_status st_ = (call);                                                \
+        TT_CHECK(st_ == rocblas_status_success, TTDFT_ELIB, "%s: rocBLAS status %d", #call, \
+                 (int)st_);                                                               \
+    } while (0)
 
 }  // namespace
 
@@ -232,68 +209,87 @@ int density_compute(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, int32_
     const int64_t N = (int64_t)rx * ry * rz;
     // states with a nonzero weight
     std::vector<int32_t> idx;
-    std::vector<double> wts;
+    std::vector<double> sw;
     for (int i = 0; i < k; ++i) {
         const double w = 2.0 * occ[i];
+        TT_CHECK(w >= 0.0, TTDFT_EBADARG, "density: negative occupancy %g of state %d", occ[i], i);
         if (w != 0.0) {
             idx.push_back(i);
-            wts.push_back(w);
+            sw.push_back(std::sqrt(w));
         }
     }
-    const int64_t t2_per_state = (int64_t)pz * ry * px;
-    const int64_t t1_per_state = (int64_t)pz * rx * ry;
-    int32_t S = 16;
-    const int64_t budget = (int64_t)4 << 30;  // bytes for T2 of one batch
-    while (S > 1 && (int64_t)S * t2_per_state * 8 > budget) S /= 2;
-    TT_TRY(ctx->scratch[5].ensure((size_t)S * (N + t1_per_state + t2_per_state) * 8 + 8 * S));
-    double *Cb = (double *)ctx->scratch[5].ptr;
-    double *T1 = Cb + (int64_t)S * N;
-    double *T2 = T1 + (int64_t)S * t1_per_state;
-    double *wdev = T2 + (int64_t)S * t2_per_state;
     if (idx.empty()) {
         TT_HIP(hipMemsetAsync(rho, 0, sizeof(double) * (size_t)px * py * pz, ctx->stream));
         return TTDFT_OK;
     }
+    const int32_t K = (int32_t)idx.size();
+    const int64_t rxy = (int64_t)rx * ry;
+    const int64_t np = (int64_t)ry * (ry + 1) / 2;
+    TT_CHECK(rxy * K < ((int64_t)1 << 31), TTDFT_EBADARG,
+             "density: rx*ry*states = %lld exceeds the 32-bit GEMM dimension",
+             (long long)(rxy * K));
+    // per-qz-plane workspace: T1, T2, packed M
+    const int64_t per_plane = rxy * K + (int64_t)px * ry * K + (int64_t)px * np;
+    const int64_t budget = (int64_t)6 << 30;  // bytes for the chunk buffers
+    int64_t QC = budget / (8 * per_plane);
+    if (QC < 1) QC = 1;
+    if (QC > pz) QC = pz;
+    // TTDFT_DENSITY_PLANES caps the planes per chunk (smaller workspace)
+    if (const char *env = std::getenv("TTDFT_DENSITY_PLANES")) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v > 0 && v < QC) QC = v;
+    }
+    const size_t fixed = (size_t)N * K + (size_t)py * np + 2 * (size_t)K;
+    TT_TRY(ctx->scratch[5].ensure(8 * (fixed + (size_t)QC * per_plane)));
+    double *Cg = (double *)ctx->scratch[5].ptr;
+    double *E = Cg + (size_t)N * K;
+    double *swd = E + (size_t)py * np;
+    int32_t *idxd = (int32_t *)(swd + K);
+    double *T1 = swd + 2 * (size_t)K;
+    double *T2 = T1 + (size_t)QC * rxy * K;
+    double *mp = T2 + (size_t)QC * px * ry * K;
+
     PendingTiming t;
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_DENSITY, &t));
-    int first = 1;
-    for (size_t b0 = 0; b0 < idx.size();) {
-        // states of this batch must be consecutive in X for the gather; split
-        // runs of consecutive indices
-        int32_t s0 = idx[b0];
-        int32_t cnt = 1;
-        while ((size_t)(b0 + cnt) < idx.size() && cnt < S && idx[b0 + cnt] == s0 + cnt) ++cnt;
-        TT_HIP(hipMemcpyAsync(wdev, wts.data() + b0, sizeof(double) * cnt, hipMemcpyHostToDevice,
-                              ctx->stream));
-        {
-            int64_t blocks = (N * cnt + 255) / 256;
-            if (blocks > 65536) blocks = 65536;
-            if (dtype == TTDFT_F64)
-                hipLaunchKernelGGL((gather_states_kernel<double>), dim3((unsigned)blocks),
-                                   dim3(256), 0, ctx->stream, (const double *)X, ldx, s0, cnt, N,
-                                   Cb);
-            else
-                hipLaunchKernelGGL((gather_states_kernel<float>), dim3((unsigned)blocks),
-                                   dim3(256), 0, ctx->stream, (const float *)X, ldx, s0, cnt, N,
-                                   Cb);
-            TT_LAUNCH_CHECK();
-        }
-        // T1[s](qz, ab) = sum_c Bz(qz, c) Cb[s](c, ab)
-        const int64_t rxy = (int64_t)rx * ry;
-        TT_TRY(gemm_batched(ctx, pz, rxy, rz, cnt, Bz, 0, 1, pz, Cb, N, rxy, 1, T1,
-                            t1_per_state, rxy));
-        // T2(row, qx) = sum_a T1(row, a) Bx(qx, a),  row = (s, qz, b)
-        TT_TRY(gemm_batched(ctx, (int64_t)cnt * pz * ry, px, rx, 1, T1, 0, rx, 1, Bx, 0, px, 1,
-                            T2, 0, px));
-        TT_TRY(density_square(ctx, T2, By, wdev, cnt, px, py, pz, ry, rho, first ? 0 : 1));
-        first = 0;
-        b0 += cnt;
+    TT_HIP(hipMemcpyAsync(swd, sw.data(), sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
+    TT_HIP(hipMemcpyAsync(idxd, idx.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice,
+                          ctx->stream));
+    if (dtype == TTDFT_F64)
+        hipLaunchKernelGGL(gather_scaled_kernel<double>, dim3(grid_for(N * K)), dim3(256), 0,
+                           ctx->stream, (const double *)X, ldx, idxd, swd, K, rx, ry, N, Cg);
+    else
+        hipLaunchKernelGGL(gather_scaled_kernel<float>, dim3(grid_for(N * K)), dim3(256), 0,
+                           ctx->stream, (const float *)X, ldx, idxd, swd, K, rx, ry, N, Cg);
+    TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(pair_table_kernel, dim3(grid_for((int64_t)py * np)), dim3(256), 0,
+                       ctx->stream, By, py, ry, E);
+    TT_LAUNCH_CHECK();
+
+    const double one = 1.0, zero = 0.0;
+    for (int64_t q0 = 0; q0 < pz; q0 += QC) {
+        const int qc = (int)std::min<int64_t>(QC, pz - q0);
+        // 2. T1[qzl][a][b][s] = sum_c Cg[c][a][b][s] Bz[q0 + qzl, c]
+        TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
+                              (rocblas_int)(rxy * K), qc, rz, &one, Cg, (rocblas_int)(rxy * K),
+                              Bz + q0, pz, &zero, T1, (rocblas_int)(rxy * K)));
+        // 3. T2[qzl][qx][(b, s)] = sum_a T1[qzl][a][(b, s)] Bx[qx, a], one GEMM per plane
+        TT_BLAS(rocblas_dgemm_strided_batched(
+            ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
+            (rocblas_int)((int64_t)ry * K), px, rx, &one, T1, (rocblas_int)((int64_t)ry * K),
+            rxy * K, Bx, px, 0, &zero, T2, (rocblas_int)((int64_t)ry * K),
+            (int64_t)px * ry * K, qc));
+        // 4. packed upper triangles of T2[qzl][qx] T2[qzl][qx]^T
+        TT_TRY(launch_syrk_pack(ctx, T2, ry, K, (int64_t)qc * px, mp));
+        // 5. rho[q0 + qzl](qx, qy) = sum_pair m[qzl][qx][pair] E[qy][pair]
+        TT_BLAS(rocblas_dgemm_strided_batched(
+            ctx->blas, rocblas_operation_transpose, rocblas_operation_none, px, py,
+            (rocblas_int)np, &one, mp, (rocblas_int)np, (int64_t)px * np, E, (rocblas_int)np, 0,
+            &zero, rho + q0 * (int64_t)px * py, px, (int64_t)px * py, qc));
     }
-    const double nst = (double)idx.size();
-    const double flops = 2.0 * nst *
-                         ((double)rx * ry * rz * pz + (double)pz * rx * ry * px +
-                          (double)px * py * pz * ry) +
-                         3.0 * nst * px * py * pz;
+    const double nst = (double)K;
+    const double flops = 2.0 * nst * ((double)N * pz + (double)pz * rx * ry * px) +
+                         nst * (double)ry * (ry + 1) * px * pz +
+                         2.0 * (double)np * px * py * pz;
     const double bytes = (double)N * k * dtype_size(dtype) + 8.0 * px * py * pz;
     TT_TRY(timing_end(ctx, &t, bytes, flops));
     return TTDFT_OK;
