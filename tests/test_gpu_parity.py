@@ -47,6 +47,21 @@ def test_ham_apply_f32(ctx, k):
     assert rel(dY.to_host().astype(np.float64), oracle.ham_apply(H, X.astype(np.float64))) < 1e-5
 
 
+@pytest.mark.parametrize("k", [1, 70, 300])
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+def test_ham_apply_wide_projectors(ctx, k, dtype):
+    """Few projectors with ~1300 nonzeros each: the projection is split over blocks."""
+    s, H = synth(p=4, dims=(20, 18, 16), bands=(2, 2, 2), seed=1)
+    assert s.nnz_c // 4 > 1024
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=True)
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = np.random.default_rng(k).standard_normal((s.n, k)).astype(npdt)
+    dX, dY = dev(ctx, X, npdt), E.DeviceArray(ctx, (s.n, k), npdt)
+    ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, dtype)
+    ref = oracle.ham_apply(H, X.astype(np.float64))
+    assert rel(dY.to_host().astype(np.float64), ref) < (1e-13 if dtype == E.F64 else 1e-5)
+
+
 def test_ham_apply_ragged_and_no_projectors(ctx):
     """Empty rows, a single dense row, p = 0, leading dimension > k."""
     n = 200
@@ -258,8 +273,9 @@ def test_rr_eig(ctx, k):
 
 
 # ---------------------------------------------------------------- Lanczos
-def test_lanczos_matches_oracle(ctx):
-    s, H = synth()
+@pytest.mark.parametrize("dims,p", [((8, 8, 8), 5), ((20, 18, 16), 4)])
+def test_lanczos_matches_oracle(ctx, dims, p):
+    s, H = synth(p=p, dims=dims, bands=(2, 2, 2))
     ham = ctx.ham_upload(s)
     up, lo = ctx.lanczos_bounds(ham, 20, seed=11)
     upo, loo = oracle.lanczos_bounds(H, 20, seed=11)

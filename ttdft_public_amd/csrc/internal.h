@@ -132,8 +132,10 @@ int timing_flush(ttdft_ctx ctx);
 
 // ---- kernels (launchers return TTDFT_* status) -------------------------------
 // W[p x k] = (D C) X  (nonlocal projection; D folded into c_val)
+// W needs nl_workspace(H, k) elements (W itself, then partial sums).
 int launch_nl_project(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                       int32_t k, int dtype, void *W);
+int64_t nl_workspace(const ttdft_ham_s *H, int32_t k);
 // Y = alpha*(H_loc X + C^T W) + beta*X + delta*Z   (Z may alias Y; flags select terms)
 // use_beta / use_delta select whether X / Z are read.
 int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,

@@ -104,51 +104,98 @@ __global__ __launch_bounds__(256) void spmv_kernel(
     if (lane == 0) y[row] = s;
 }
 
-// W[r][j] = sum_e (D C)[r][e] X[col_e][j]: one block per projector row, the
-// row's nonzeros split over 4 wavefronts and summed through LDS.
-template <typename T, int NC>
+// W[r][j] = sum_e (D C)[r][e] X[col_e][j].  A projector row has a few
+// thousand nonzeros and there are only a few hundred rows, so one block per
+// row would leave most CUs idle: block (r, panel, split) takes 64 states and
+// one of nsplit slices of row r's nonzeros (4 wavefronts over the slice,
+// summed through LDS) and writes a partial sum; nl_reduce_kernel adds the
+// nsplit partials in a fixed order (deterministic).  X rows are gathered 8 at
+// a time so their loads are in flight together.
+template <typename T>
 __global__ __launch_bounds__(256) void nl_project_kernel(
-    int32_t p, const int64_t *__restrict__ c_rowptr, const int32_t *__restrict__ c_col,
-    const T *__restrict__ c_val, const T *__restrict__ X, int64_t ldx, int32_t k,
-    T *__restrict__ W, int32_t ldw) {
-    __shared__ T red[4][64 * NC];
+    int32_t p, int32_t npanel, int32_t nsplit, const int64_t *__restrict__ c_rowptr,
+    const int32_t *__restrict__ c_col, const T *__restrict__ c_val, const T *__restrict__ X,
+    int64_t ldx, int32_t k, T *__restrict__ Wp) {
+    __shared__ T red[4][64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int r = blockIdx.x;
-    T acc[NC];
-#pragma unroll
-    for (int t = 0; t < NC; ++t) acc[t] = T(0);
-    const int64_t e0 = c_rowptr[r], e1 = c_rowptr[r + 1];
-    for (int64_t base = e0 + 64 * wave; base < e1; base += 256) {
-        const int cnt = (int)((e1 - base) < 64 ? (e1 - base) : 64);
+    int b = blockIdx.x;
+    const int split = b % nsplit;
+    b /= nsplit;
+    const int panel = b % npanel;
+    const int r = b / npanel;
+    const int c = panel * 64 + lane;
+    const int cc = c < k ? c : k - 1;  // clamped: every lane loads a valid element
+    const int64_t e0 = c_rowptr[r], len = c_rowptr[r + 1] - e0;
+    const int64_t lo = e0 + len * split / nsplit, hi = e0 + len * (split + 1) / nsplit;
+    T acc = T(0);
+    for (int64_t base = lo + 64 * wave; base < hi; base += 256) {
+        const int cnt = (int)((hi - base) < 64 ? (hi - base) : 64);
         T v = T(0);
-        int c = 0;
+        int col = 0;
         if (lane < cnt) {
             v = c_val[base + lane];
-            c = c_col[base + lane];
+            col = c_col[base + lane];
         }
-        for (int j = 0; j < cnt; ++j) {
-            const T a = bcast(v, j);
-            const T *xr = X + (int64_t)bcast(c, j) * ldx + lane;
+        int j = 0;
+        for (; j + 8 <= cnt; j += 8) {
+            T x[8];
 #pragma unroll
-            for (int t = 0; t < NC; ++t) {
-                if (lane + 64 * t < k) acc[t] += a * xr[64 * t];
-            }
+            for (int u = 0; u < 8; ++u) x[u] = X[(int64_t)bcast(col, j + u) * ldx + cc];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += bcast(v, j + u) * x[u];
         }
+        for (; j < cnt; ++j) acc += bcast(v, j) * X[(int64_t)bcast(col, j) * ldx + cc];
     }
-#pragma unroll
-    for (int t = 0; t < NC; ++t) red[wave][lane + 64 * t] = acc[t];
+    red[wave][lane] = acc;
     __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-        for (int t = 0; t < NC; ++t) {
-            if (lane + 64 * t < k) {
-                const T s = (red[0][lane + 64 * t] + red[1][lane + 64 * t]) +
-                            (red[2][lane + 64 * t] + red[3][lane + 64 * t]);
-                W[(int64_t)r * ldw + lane + 64 * t] = s;
-            }
-        }
+    if (wave == 0 && c < k)
+        Wp[((int64_t)split * p + r) * k + c] =
+            (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+template <typename T>
+__global__ void nl_reduce_kernel(int64_t pk, int32_t nsplit, const T *__restrict__ Wp,
+                                 T *__restrict__ W) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < pk;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        T s = Wp[e];
+        for (int q = 1; q < nsplit; ++q) s += Wp[(int64_t)q * pk + e];
+        W[e] = s;
     }
+}
+
+// slices per projector row: about 4096 blocks in all, at least 256 nonzeros a slice
+int nl_splits(const ttdft_ham_s *H, int32_t k) {
+    if (H->p == 0) return 1;
+    const int64_t npanel = (k + 63) / 64;
+    int64_t ns = (4096 + H->p * npanel - 1) / (H->p * npanel);
+    const int64_t cap = H->nnz_c / H->p / 256;
+    if (ns > cap) ns = cap;
+    if (ns < 1) ns = 1;
+    if (ns > 64) ns = 64;
+    return (int)ns;
+}
+
+template <typename T>
+int nl_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *X, int64_t ldx, int32_t k,
+              const T *c_val, T *W) {
+    const int npanel = (k + 63) / 64;
+    const int ns = nl_splits(H, k);
+    T *Wp = ns > 1 ? W + (int64_t)H->p * k : W;
+    hipLaunchKernelGGL(nl_project_kernel<T>, dim3((unsigned)((int64_t)H->p * npanel * ns)),
+                       dim3(256), 0, ctx->stream, H->p, npanel, ns, H->c_rowptr, H->c_col, c_val,
+                       X, ldx, k, Wp);
+    TT_LAUNCH_CHECK();
+    if (ns > 1) {
+        const int64_t pk = (int64_t)H->p * k;
+        int64_t blocks = (pk + 255) / 256;
+        if (blocks > 4096) blocks = 4096;
+        hipLaunchKernelGGL(nl_reduce_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                           pk, ns, Wp, W);
+        TT_LAUNCH_CHECK();
+    }
+    return TTDFT_OK;
 }
 
 template <typename T, int NC>
@@ -161,15 +208,6 @@ int spmm_dispatch_nc(ttdft_ctx ctx, const ttdft_ham_s *H, const T *X, int64_t ld
                        H->rowptr, H->col, val, X, ldx, H->p > 0 ? H->ct_rowptr : nullptr,
                        H->ct_col, ct_val, W, k, Y, ldy, Z, ldz, k, (T)alpha, (T)beta,
                        (T)delta, use_beta, use_delta);
-    TT_LAUNCH_CHECK();
-    return TTDFT_OK;
-}
-
-template <typename T, int NC>
-int nl_dispatch_nc(ttdft_ctx ctx, const ttdft_ham_s *H, const T *X, int64_t ldx, int32_t k,
-                   const T *c_val, T *W) {
-    hipLaunchKernelGGL((nl_project_kernel<T, NC>), dim3((unsigned)H->p), dim3(256), 0,
-                       ctx->stream, H->p, H->c_rowptr, H->c_col, c_val, X, ldx, k, W, k);
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
@@ -197,23 +235,21 @@ int nl_dispatch_nc(ttdft_ctx ctx, const ttdft_ham_s *H, const T *X, int64_t ldx,
 
 }  // namespace
 
+int64_t nl_workspace(const ttdft_ham_s *H, int32_t k) {
+    const int ns = nl_splits(H, k);
+    return (int64_t)H->p * k * (ns > 1 ? 1 + ns : 1);
+}
+
 int launch_nl_project(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                       int32_t k, int dtype, void *W) {
     if (H->p == 0) return TTDFT_OK;
-    const int nc = (k + 63) / 64;
-    TT_CHECK(nc >= 1 && nc <= 16, TTDFT_EBADARG, "k=%d outside 1..1024", k);
+    TT_CHECK(k >= 1, TTDFT_EBADARG, "k=%d < 1", k);
     if (dtype == TTDFT_F64) {
         TT_CHECK(H->c_val64, TTDFT_EBADARG, "no FP64 copy of the projectors uploaded");
-#define CALL(NCV) nl_dispatch_nc<double, NCV>(ctx, H, (const double *)X, ldx, k, H->c_val64, (double *)W)
-        TT_NC_SWITCH(nc, CALL)
-#undef CALL
-    } else {
-        TT_CHECK(H->c_val32, TTDFT_EBADARG, "no FP32 copy of the projectors uploaded");
-#define CALL(NCV) nl_dispatch_nc<float, NCV>(ctx, H, (const float *)X, ldx, k, H->c_val32, (float *)W)
-        TT_NC_SWITCH(nc, CALL)
-#undef CALL
+        return nl_launch<double>(ctx, H, (const double *)X, ldx, k, H->c_val64, (double *)W);
     }
-    return TTDFT_EBADARG;
+    TT_CHECK(H->c_val32, TTDFT_EBADARG, "no FP32 copy of the projectors uploaded");
+    return nl_launch<float>(ctx, H, (const float *)X, ldx, k, H->c_val32, (float *)W);
 }
 
 int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
@@ -246,7 +282,7 @@ int ham_spmv(ttdft_ctx ctx, const ttdft_ham_s *H, const double *x, double *y) {
     TT_CHECK(H->val64, TTDFT_EBADARG, "no FP64 copy of H_loc uploaded");
     double *W = nullptr;
     if (H->p > 0) {
-        TT_TRY(ctx->scratch[3].ensure((size_t)H->p * sizeof(double)));
+        TT_TRY(ctx->scratch[3].ensure((size_t)nl_workspace(H, 1) * sizeof(double)));
         W = (double *)ctx->scratch[3].ptr;
         TT_TRY(launch_nl_project(ctx, H, x, 1, 1, TTDFT_F64, W));
     }
@@ -263,7 +299,7 @@ int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, vo
     const size_t es = dtype_size(dtype);
     void *W = nullptr;
     if (H->p > 0) {
-        TT_TRY(ctx->scratch[3].ensure((size_t)H->p * k * es));
+        TT_TRY(ctx->scratch[3].ensure((size_t)nl_workspace(H, k) * es));
         W = ctx->scratch[3].ptr;
         PendingTiming tn;
         TT_TRY(timing_begin(ctx, TTDFT_KSTAT_NLPROJ, &tn));
