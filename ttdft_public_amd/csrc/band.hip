@@ -15,9 +15,9 @@
 // (one wavefront each) and 16*NST states.  For every neighbouring line it
 // stages the line's X window (all x-bricks of the chunk plus the 2bx halo,
 // 16*NST states) in LDS once; every wave reads its B operands from there.
-// The stage is double-buffered in LDS; the global loads of X run two lines
-// ahead through a pair of register buffers and those of the A fragments one
-// line ahead, so HBM/L2 latency hides behind MFMA work.  The nonlocal term C^T W and the Chebyshev three-term
+// The stage is double-buffered in LDS; the global loads of X and of the A
+// fragments run two lines ahead through register buffers, so HBM/L2 latency
+// hides behind MFMA work.  The nonlocal term C^T W and the Chebyshev three-term
 // recurrence are applied in the epilogue, as in the CSR kernel.  Workgroups are
 // renumbered so consecutive logical workgroups land on the same XCD (dispatch
 // is round-robin over the 8 XCDs), keeping neighbouring lines' X in one L2.
@@ -70,7 +70,8 @@ struct BrickArgs {
     int64_t ldz;
     int32_t k, rx, ry, rz, bx, by, bz;
     int32_t gxn, hyn;  // x-bricks, y-bricks
-    int32_t gxb, nxc;  // x-bricks per workgroup (waves), x-chunks
+    int32_t gxb, nxc;  // x-bricks per workgroup, x-chunks
+    int32_t nsplit;    // trailing x-bricks whose state tiles are split over two waves
     int32_t nsg;       // state groups
     int64_t nblocks;
     T alpha, beta, delta;
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     constexpr int S = 16 * NST;
     extern __shared__ __align__(16) unsigned char brick_lds[];
     T *Xs = reinterpret_cast<T *>(brick_lds);
-    const int nthr = a.gxb * 64;
+    const int nthr = (a.gxb + a.nsplit) * 64;
     const int TR = 4 * (a.gxb - 1) + 4 * KS;  // staged x' rows
     const int TILE = TR * S;
     const int lane = threadIdx.x & 63;
@@ -104,8 +105,20 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     const int h = (int)(t % a.hyn);
     const int z = (int)(t / a.hyn);
     const int s0 = sg * S;
+    // Waves are spread round-robin over the 4 SIMDs and the MFMA work of a
+    // line ends at a barrier, so the SIMD with the most tile work sets the
+    // pace.  With gxb = 1 or 2 (mod 4) the state tiles of the last one or two
+    // x-bricks are split over two waves each (mode 1: lower tiles, mode 2:
+    // upper tiles), which evens out the MFMA work per SIMD.
+    constexpr int NH = NST - NST / 2;  // tiles of the lower half
+    int bl = wl, mode = 0;
+    if (wl >= a.gxb - a.nsplit) {
+        const int q = wl - (a.gxb - a.nsplit);
+        bl = a.gxb - a.nsplit + (q >> 1);
+        mode = 1 + (q & 1);
+    }
     const int g0 = xc * a.gxb;
-    const int g = g0 + wl;
+    const int g = g0 + bl;
     const bool active = g < a.gxn;
     const int xw0 = 4 * g0 - a.bx;  // x' of staged row 0
     const int NPY = 2 * a.by + 4;
@@ -124,12 +137,18 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     const int nlines = ny * (zhi - zlo + 1);
 
     T ra[PF], rb[PF];
-    // Loads are unconditional (clamped, always-valid addresses; out-of-range
-    // elements are zeroed after the load): a load under a branch would make the
-    // compiler wait for every outstanding load before the MFMAs.
+    // Loads are unconditional from clamped, always-valid addresses, and the
+    // out-of-range elements are zeroed only when the line is staged: a load
+    // under a branch, or a select right after it, would make the compiler wait
+    // for the load there instead of two lines later.
+    auto elem_ok = [&](int li_, int u) {
+        const int e = threadIdx.x + nthr * u;
+        const int row = e / S, s = e - row * S;
+        const int xp = xw0 + row;
+        return li_ < nlines && e < TILE && xp >= 0 && xp < a.rx && s0 + s < a.k;
+    };
     auto fetch = [&](int li_, T *r) {
-        const bool line_ok = li_ < nlines;
-        const int lc = line_ok ? li_ : nlines - 1;
+        const int lc = li_ < nlines ? li_ : nlines - 1;
         const int yy = ylo + lc % ny, zz = zlo + lc / ny;
         const T *xb = a.X + ((int64_t)(zz * a.ry + yy) * a.rx) * a.ldx;
 #pragma unroll
@@ -137,18 +156,17 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
             const int e = threadIdx.x + nthr * u;
             const int row = e / S, s = e - row * S;
             const int xp = xw0 + row, st = s0 + s;
-            const bool ok = line_ok && e < TILE && xp >= 0 && xp < a.rx && st < a.k;
             const int xpc = xp < 0 ? 0 : (xp >= a.rx ? a.rx - 1 : xp);
             const int stc = st < a.k ? st : a.k - 1;
-            const T v = xb[(int64_t)xpc * a.ldx + stc];
-            r[u] = ok ? v : T(0);
+            r[u] = xb[(int64_t)xpc * a.ldx + stc];
         }
     };
-    auto stage = [&](int buf, const T *r) {
+    auto stage = [&](int li_, const T *r) {
+        T *dst = Xs + (li_ & 1) * TILE;
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             const int e = threadIdx.x + nthr * u;
-            if (e < TILE) Xs[buf * TILE + e] = r[u];
+            if (e < TILE) dst[e] = elem_ok(li_, u) ? r[u] : T(0);
         }
     };
     // A fragments of line li_ (wave-private, from HBM/L2): loaded one line ahead
@@ -163,35 +181,53 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
         for (int ks = 0; ks < KS; ++ks) av[ks] = Ab[ks * 64];
     };
     auto compute = [&](int li_, const T *av) {
-        const T *Xb = Xs + (li_ & 1) * TILE + (4 * wl + lk) * S + li;
+        const T *Xb = Xs + (li_ & 1) * TILE + (4 * bl + lk) * S + li;
+        if (mode == 0) {
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
+            for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-            for (int st = 0; st < NST; ++st)
-                acc[st] = mfma16x16x4(av[ks], Xb[4 * ks * S + 16 * st], acc[st]);
+                for (int st = 0; st < NST; ++st)
+                    acc[st] = mfma16x16x4(av[ks], Xb[4 * ks * S + 16 * st], acc[st]);
+        } else if (mode == 1) {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                for (int st = 0; st < NH; ++st)
+                    acc[st] = mfma16x16x4(av[ks], Xb[4 * ks * S + 16 * st], acc[st]);
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                for (int st = NH; st < NST; ++st)
+                    acc[st] = mfma16x16x4(av[ks], Xb[4 * ks * S + 16 * st], acc[st]);
+        }
     };
 
-    // Line i is staged in LDS buffer i&1; the X windows of lines i+1 and i+2
-    // and the A fragments of line i+1 are in flight in registers while line i
-    // is multiplied.
-    T aa[KS], ab[KS];
-    fetch(0, rb);
-    stage(0, rb);
-    fetch(1, ra);
-    load_a(0, aa);
+    // Line i is staged in LDS buffer i&1.  While it is multiplied, the X
+    // windows of lines i+1 and i+2 (register sets xr[(i+1)&1], xr[i&1]) and the
+    // A fragments of lines i+1 and i+2 (sets av[(i+1)%3], av[(i+2)%3]) are in
+    // flight, so both loads have two lines of MFMA work to arrive.  The loop
+    // is unrolled by 6 so every register set is addressed statically.
+    T xr[2][PF];
+    T av[3][KS];
+    fetch(0, xr[0]);
+    stage(0, xr[0]);
+    fetch(1, xr[1]);
+    load_a(0, av[0]);
+    load_a(1, av[1]);
     __syncthreads();
-    for (int li_ = 0; li_ < nlines; li_ += 2) {
-        fetch(li_ + 2, rb);
-        load_a(li_ + 1, ab);
-        if (active) compute(li_, aa);
-        if (li_ + 1 < nlines) stage(1, ra);
-        __syncthreads();
-        if (li_ + 1 >= nlines) break;
-        fetch(li_ + 3, ra);
-        load_a(li_ + 2, aa);
-        if (active) compute(li_ + 1, ab);
-        if (li_ + 2 < nlines) stage(0, rb);
-        __syncthreads();
+    for (int base = 0; base < nlines; base += 6) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const int li_ = base + j;
+            if (li_ < nlines) {
+                fetch(li_ + 2, xr[j & 1]);
+                load_a(li_ + 2, av[(j + 2) % 3]);
+                if (active) compute(li_, av[j % 3]);
+                if (li_ + 1 < nlines) stage(li_ + 1, xr[(j + 1) & 1]);
+                __syncthreads();
+            }
+        }
     }
     if (!active) return;
 
@@ -200,6 +236,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     // (x - 4g) + 4 (y - 4h), D columns are states.
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
+        if ((mode == 1 && st >= NH) || (mode == 2 && st < NH)) continue;
         const int s = s0 + 16 * st + li;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -233,7 +270,7 @@ int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a) {
     const int S = 16 * NST;
     const int TR = 4 * (a.gxb - 1) + 4 * KS;
     const size_t lds = (size_t)2 * TR * S * sizeof(T);
-    const int nthr = 64 * a.gxb;
+    const int nthr = 64 * (a.gxb + a.nsplit);
     const int pf = (TR * S + nthr - 1) / nthr;
     int64_t blocks = (a.nblocks + 7) / 8 * 8;
     TT_CHECK(lds <= 160 * 1024, TTDFT_EBADARG, "band SpMM: LDS stage too large");
@@ -258,6 +295,7 @@ int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a) {
 template <typename T>
 int launch_brick(ttdft_ctx ctx, BrickArgs<T> a, int ks) {
     a.nsg = (a.k + 16 * kStateTiles<T>() - 1) / (16 * kStateTiles<T>());
+    a.nsplit = a.gxb % 4 == 1 ? 1 : (a.gxb % 4 == 2 ? 2 : 0);
     a.nblocks = (int64_t)a.nsg * a.nxc * a.hyn * a.rz;
     switch (ks) {
     case 2:
@@ -287,7 +325,7 @@ int launch_band_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t
                             H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col, H->ct_val64,
                             (const double *)W, k, (double *)Y, ldy, (const double *)Z, ldz,
                             k, H->rx, H->ry, H->rz, H->bx, H->by, H->bz, H->gxn, H->hyn,
-                            H->gxb, H->nxc, 0, 0, alpha, beta, delta, use_beta, use_delta};
+                            H->gxb, H->nxc, 0, 0, 0, alpha, beta, delta, use_beta, use_delta};
         return launch_brick<double>(ctx, a, H->ks);
     }
     TT_CHECK(H->mf32, TTDFT_EBADARG, "no FP32 band copy of H_loc uploaded");
@@ -295,7 +333,7 @@ int launch_band_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t
                        H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col, H->ct_val32,
                        (const float *)W, k, (float *)Y, ldy, (const float *)Z, ldz,
                        k, H->rx, H->ry, H->rz, H->bx, H->by, H->bz, H->gxn, H->hyn,
-                       H->gxb, H->nxc, 0, 0, (float)alpha, (float)beta, (float)delta,
+                       H->gxb, H->nxc, 0, 0, 0, (float)alpha, (float)beta, (float)delta,
                        use_beta, use_delta};
     return launch_brick<float>(ctx, a, H->ks);
 }
