@@ -22,7 +22,9 @@
 // renumbered so consecutive logical workgroups land on the same XCD (dispatch
 // is round-robin over the 8 XCDs), keeping neighbouring lines' X in one L2.
 #include <algorithm>
+#include <cstdint>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -78,19 +80,120 @@ struct BrickArgs {
     int use_beta, use_delta;
 };
 
+typedef __attribute__((address_space(3))) void lds_void;
+
+// s_waitcnt immediate: vmcnt in bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8].
+// Waits until at most n vector-memory operations of this wave are outstanding
+// and all LDS operations are done.
+#define TT_WAIT_VM_LDS(n)                                                                   \
+    do {                                                                                    \
+        asm volatile("" ::: "memory");                                                      \
+        __builtin_amdgcn_s_waitcnt(((n) & 15) | ((((n) >> 4) & 3) << 14) | (7 << 4) | (0 << 8)); \
+    } while (0)
+#define TT_WAIT_ALL() __builtin_amdgcn_s_waitcnt(0)
+
+// Workgroup barrier without the memory fence of __syncthreads(): that fence
+// would also wait for the line loads still in flight into LDS.  The waits
+// the barrier needs are issued explicitly before it; the empty asm keeps the
+// compiler from moving memory operations across.
+__device__ __forceinline__ void line_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+
+// N LDS loads of T from byte offsets OFF0 + i STRIDE past p, completed inside
+// the asm block (s_waitcnt lgkmcnt(0)); every output is used by the caller.
+// Written as asm because the compiler cannot tell these reads apart from the
+// global->LDS loads into the other line buffers and would otherwise wait for
+// all of those to finish first.
+#define TT_DSR(I) #I
+template <typename T, int N, int OFF0, int STRIDE>
+__device__ __forceinline__ void lds_read_n(const T *p, T (&v)[N]) {
+    static_assert(N >= 1 && N <= 5, "1..5 loads");
+    static_assert(OFF0 + (N - 1) * STRIDE < 65536, "ds offset field");
+    const unsigned a = (unsigned)(uintptr_t)p;
+#define TT_R(i) "ds_read_" TT_OP " %" #i ", %[a] offset:%[o" #i "]\n\t"
+    if constexpr (sizeof(T) == 8) {
+#define TT_OP "b64"
+        if constexpr (N == 1)
+            asm volatile(TT_R(0) "s_waitcnt lgkmcnt(0)" : "=v"(v[0]) : [a] "v"(a), [o0] "i"(OFF0));
+        else if constexpr (N == 2)
+            asm volatile(TT_R(0) TT_R(1) "s_waitcnt lgkmcnt(0)"
+                         : "=v"(v[0]), "=v"(v[1])
+                         : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE));
+        else if constexpr (N == 3)
+            asm volatile(TT_R(0) TT_R(1) TT_R(2) "s_waitcnt lgkmcnt(0)"
+                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2])
+                         : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
+                           [o2] "i"(OFF0 + 2 * STRIDE));
+        else if constexpr (N == 4)
+            asm volatile(TT_R(0) TT_R(1) TT_R(2) TT_R(3) "s_waitcnt lgkmcnt(0)"
+                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3])
+                         : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
+                           [o2] "i"(OFF0 + 2 * STRIDE), [o3] "i"(OFF0 + 3 * STRIDE));
+        else
+            asm volatile(TT_R(0) TT_R(1) TT_R(2) TT_R(3) TT_R(4) "s_waitcnt lgkmcnt(0)"
+                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3]), "=v"(v[4])
+                         : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
+                           [o2] "i"(OFF0 + 2 * STRIDE), [o3] "i"(OFF0 + 3 * STRIDE),
+                           [o4] "i"(OFF0 + 4 * STRIDE));
+#undef TT_OP
+    } else {
+#define TT_OP "b32"
+        if constexpr (N == 1)
+            asm volatile(TT_R(0) "s_waitcnt lgkmcnt(0)" : "=v"(v[0]) : [a] "v"(a), [o0] "i"(OFF0));
+        else if constexpr (N == 2)
+            asm volatile(TT_R(0) TT_R(1) "s_waitcnt lgkmcnt(0)"
+                         : "=v"(v[0]), "=v"(v[1])
+                         : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE));
+        else if constexpr (N == 3)
+            asm volatile(TT_R(0) TT_R(1) TT_R(2) "s_waitcnt lgkmcnt(0)"
+                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2])
+                         : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
+                           [o2] "i"(OFF0 + 2 * STRIDE));
+        else if constexpr (N == 4)
+            asm volatile(TT_R(0) TT_R(1) TT_R(2) TT_R(3) "s_waitcnt lgkmcnt(0)"
+                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3])
+                         : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
+                           [o2] "i"(OFF0 + 2 * STRIDE), [o3] "i"(OFF0 + 3 * STRIDE));
+        else
+            asm volatile(TT_R(0) TT_R(1) TT_R(2) TT_R(3) TT_R(4) "s_waitcnt lgkmcnt(0)"
+                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3]), "=v"(v[4])
+                         : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
+                           [o2] "i"(OFF0 + 2 * STRIDE), [o3] "i"(OFF0 + 3 * STRIDE),
+                           [o4] "i"(OFF0 + 4 * STRIDE));
+#undef TT_OP
+    }
+#undef TT_R
+}
+#undef TT_DSR
+
 // KS: 4-column MFMA steps covering the x' window (4 KS >= 4 + 2 bx).
-// NST: 16-state tiles per workgroup.  PF: staged X elements per thread.
-template <typename T, int KS, int NST, int PF>
+// NST: 16-state tiles per workgroup.  DB: bytes per lane of one global->LDS
+// load (16 when a row of X is 16-byte aligned, else 4).
+template <typename T, int KS, int NST, int DB, int DEPTH>
 __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     typedef typename Acc4<T>::type acc_t;
     constexpr int S = 16 * NST;
-    extern __shared__ __align__(16) unsigned char brick_lds[];
-    T *Xs = reinterpret_cast<T *>(brick_lds);
-    const int nthr = (a.gxb + a.nsplit) * 64;
+    constexpr int ES = sizeof(T);
+    constexpr int CH = 64 * DB;                  // bytes one wave loads per instruction
+    constexpr int TRMAX = 4 * 15 + 4 * KS;       // staged rows at 16 x-bricks
+    constexpr int BUF = (TRMAX * S * ES + CH - 1) / CH * CH;
+    constexpr int NQ = DB == 16 ? 4 : 16;        // loads per wave per line (fixed count)
+    constexpr int VE = DB >= ES ? DB / ES : 1;   // elements per lane-load
+    constexpr int NB = DEPTH + 1;  // LDS buffers: lines i .. i+DEPTH
+    static_assert(DEPTH == 2 || DEPTH == 3, "2 or 3 lines in flight");
+    __shared__ __attribute__((aligned(16))) unsigned char lb0[BUF];
+    __shared__ __attribute__((aligned(16))) unsigned char lb1[BUF];
+    __shared__ __attribute__((aligned(16))) unsigned char lb2[BUF];
+    __shared__ __attribute__((aligned(16))) unsigned char lb3[NB == 4 ? BUF : 16];
+    const int nw = a.gxb + a.nsplit;
     const int TR = 4 * (a.gxb - 1) + 4 * KS;  // staged x' rows
-    const int TILE = TR * S;
+    const int nch = (TR * S * ES + CH - 1) / CH;
     const int lane = threadIdx.x & 63;
-    const int wl = threadIdx.x >> 6;
+    const int wl = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int li = lane & 15, lk = lane >> 4;
 
     const unsigned nb = gridDim.x;  // a multiple of 8
@@ -136,99 +239,164 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     const int ny = yhi - ylo + 1;
     const int nlines = ny * (zhi - zlo + 1);
 
-    T ra[PF], rb[PF];
-    // Loads are unconditional from clamped, always-valid addresses, and the
-    // out-of-range elements are zeroed only when the line is staged: a load
-    // under a branch, or a select right after it, would make the compiler wait
-    // for the load there instead of two lines later.
-    auto elem_ok = [&](int li_, int u) {
-        const int e = threadIdx.x + nthr * u;
+    // The staged tile of a line is rows x' = xw0 .. xw0 + TR - 1 of that line
+    // by states s0 .. s0 + S - 1, row-major.  This wave loads byte chunks
+    // q = wl + i nw (i < NQ; past the last chunk it repeats the last one,
+    // which writes the same bytes again).  A lane's byte offset inside the
+    // line of X is the same for every line: computed once.  Rows outside
+    // 0..rx-1 and states past the end load clamped, finite values; the A
+    // fragments are zero against those rows and those states are never
+    // stored.
+    int32_t goff[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        int q = wl + i * nw;
+        q = q < nch ? q : nch - 1;
+        const int off = q * CH + lane * DB;  // byte offset in the tile
+        const int e = off / ES;
         const int row = e / S, s = e - row * S;
         const int xp = xw0 + row;
-        return li_ < nlines && e < TILE && xp >= 0 && xp < a.rx && s0 + s < a.k;
-    };
-    auto fetch = [&](int li_, T *r) {
+        const int xpc = xp < 0 ? 0 : (xp >= a.rx ? a.rx - 1 : xp);
+        int st = s0 + s;
+        if (st > (int)a.ldx - VE) st = (int)a.ldx - VE;
+        goff[i] = (xpc * (int)a.ldx + st) * ES + (off % ES);
+    }
+    auto line_of = [&](int li_, int &yy, int &zz) {
         const int lc = li_ < nlines ? li_ : nlines - 1;
-        const int yy = ylo + lc % ny, zz = zlo + lc / ny;
-        const T *xb = a.X + ((int64_t)(zz * a.ry + yy) * a.rx) * a.ldx;
+        yy = ylo + lc % ny;
+        zz = zlo + lc / ny;
+    };
+    auto fetch = [&](int li_, unsigned char *buf) {
+        int yy, zz;
+        line_of(li_, yy, zz);
+        const char *lbase =
+            reinterpret_cast<const char *>(a.X) + ((int64_t)(zz * a.ry + yy) * a.rx) * a.ldx * ES;
 #pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const int e = threadIdx.x + nthr * u;
-            const int row = e / S, s = e - row * S;
-            const int xp = xw0 + row, st = s0 + s;
-            const int xpc = xp < 0 ? 0 : (xp >= a.rx ? a.rx - 1 : xp);
-            const int stc = st < a.k ? st : a.k - 1;
-            r[u] = xb[(int64_t)xpc * a.ldx + stc];
+        for (int i = 0; i < NQ; ++i) {
+            int q = wl + i * nw;
+            q = q < nch ? q : nch - 1;
+#if defined(__HIP_DEVICE_COMPILE__)
+            // (the host pass of a .hip file rejects the 16-byte form of this builtin)
+            __builtin_amdgcn_global_load_lds(lbase + goff[i], (lds_void *)(buf + q * CH), DB, 0,
+                                             0);
+#else
+            (void)lbase;
+            (void)buf;
+            (void)q;
+#endif
         }
     };
-    auto stage = [&](int li_, const T *r) {
-        T *dst = Xs + (li_ & 1) * TILE;
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const int e = threadIdx.x + nthr * u;
-            if (e < TILE) dst[e] = elem_ok(li_, u) ? r[u] : T(0);
-        }
-    };
-    // A fragments of line li_ (wave-private, from HBM/L2): loaded one line ahead
+    // A fragments of a line (wave-private, from HBM/L2)
     auto load_a = [&](int li_, T *av) {
-        const int lc = li_ < nlines ? li_ : nlines - 1;
+        int yy, zz;
+        line_of(li_, yy, zz);
         const int gc = active ? g : a.gxn - 1;
-        const int yy = ylo + lc % ny, zz = zlo + lc / ny;
         const int tt = yy - y0;  // in [-by, 3 + by] by construction
         const int p = (tt + a.by) + NPY * (zz - z + a.bz);
         const T *Ab = a.mf + ((((int64_t)(z * a.hyn + h) * a.gxn + gc) * NP + p) * KS) * 64 + lane;
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) av[ks] = Ab[ks * 64];
     };
-    auto compute = [&](int li_, const T *av) {
-        const T *Xb = Xs + (li_ & 1) * TILE + (4 * bl + lk) * S + li;
-        if (mode == 0) {
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-                for (int st = 0; st < NST; ++st)
-                    acc[st] = mfma16x16x4(av[ks], Xb[4 * ks * S + 16 * st], acc[st]);
-        } else if (mode == 1) {
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-                for (int st = 0; st < NH; ++st)
-                    acc[st] = mfma16x16x4(av[ks], Xb[4 * ks * S + 16 * st], acc[st]);
-        } else {
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-                for (int st = NH; st < NST; ++st)
-                    acc[st] = mfma16x16x4(av[ks], Xb[4 * ks * S + 16 * st], acc[st]);
+    // reads the B operands of k-step ks (compile-time after unrolling is not
+    // enough for asm immediates, so dispatch on ks)
+    auto read_ks = [&](const T *Xb, auto &b, int ks, auto lo_c, auto hi_c) {
+        constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
+        constexpr int NT = HI - LO;
+        switch (ks) {
+        case 1:
+            if constexpr (KS > 1) lds_read_n<T, NT, (4 * 1 * S + 16 * LO) * ES, 16 * ES>(Xb, b[1 < KS ? 1 : 0]);
+            break;
+        case 2:
+            if constexpr (KS > 2) lds_read_n<T, NT, (4 * 2 * S + 16 * LO) * ES, 16 * ES>(Xb, b[2 < KS ? 2 : 0]);
+            break;
+        case 3:
+            if constexpr (KS > 3) lds_read_n<T, NT, (4 * 3 * S + 16 * LO) * ES, 16 * ES>(Xb, b[3 < KS ? 3 : 0]);
+            break;
+        case 4:
+            if constexpr (KS > 4) lds_read_n<T, NT, (4 * 4 * S + 16 * LO) * ES, 16 * ES>(Xb, b[4 < KS ? 4 : 0]);
+            break;
+        default:
+            break;
         }
     };
-
-    // Line i is staged in LDS buffer i&1.  While it is multiplied, the X
-    // windows of lines i+1 and i+2 (register sets xr[(i+1)&1], xr[i&1]) and the
-    // A fragments of lines i+1 and i+2 (sets av[(i+1)%3], av[(i+2)%3]) are in
-    // flight, so both loads have two lines of MFMA work to arrive.  The loop
-    // is unrolled by 6 so every register set is addressed statically.
-    T xr[2][PF];
-    T av[3][KS];
-    fetch(0, xr[0]);
-    stage(0, xr[0]);
-    fetch(1, xr[1]);
-    load_a(0, av[0]);
-    load_a(1, av[1]);
-    __syncthreads();
-    for (int base = 0; base < nlines; base += 6) {
+    // The MFMAs of k-step ks are issued before the B operands of ks+1 are
+    // read, so the LDS latency overlaps their execution.
+    auto mfma_tiles = [&](const T *Xb, const T *av, auto lo_c, auto hi_c) {
+        constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
+        constexpr int NT = HI - LO;
+        T b[KS][NT];
+        lds_read_n<T, NT, 16 * LO * ES, 16 * ES>(Xb, b[0]);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) {
-            const int li_ = base + j;
-            if (li_ < nlines) {
-                fetch(li_ + 2, xr[j & 1]);
-                load_a(li_ + 2, av[(j + 2) % 3]);
-                if (active) compute(li_, av[j % 3]);
-                if (li_ + 1 < nlines) stage(li_ + 1, xr[(j + 1) & 1]);
-                __syncthreads();
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                acc[LO + t] = mfma16x16x4(av[ks], b[ks][t], acc[LO + t]);
+            if constexpr (true) {
+                if (ks + 1 < KS) read_ks(Xb, b, ks + 1, lo_c, hi_c);
             }
         }
+    };
+    auto compute = [&](const unsigned char *buf, const T *av) {
+        const T *Xb = reinterpret_cast<const T *>(buf) + (4 * bl + lk) * S + li;
+        if (mode == 0)
+            mfma_tiles(Xb, av, std::integral_constant<int, 0>(), std::integral_constant<int, NST>());
+        else if (mode == 1)
+            mfma_tiles(Xb, av, std::integral_constant<int, 0>(), std::integral_constant<int, NH>());
+        else
+            mfma_tiles(Xb, av, std::integral_constant<int, NH>(), std::integral_constant<int, NST>());
+    };
+
+    // Line i is in LDS buffer i % NB.  While it is multiplied, the loads of
+    // lines i+1 .. i+DEPTH (X straight into LDS, A fragments into register
+    // sets av[(i+d) % NB]) are in flight: each has DEPTH lines of MFMA work to
+    // arrive.  Every wave issues NQ + KS vector-memory operations per line in
+    // a fixed order, so before the barrier that ends line i, waiting until at
+    // most (DEPTH-1)(NQ+KS) are outstanding (those of lines i+2 .. i+DEPTH)
+    // completes this wave's loads of line i+1.  The loop is unrolled by NB so
+    // buffers and register sets are addressed statically; buffers are picked
+    // by compile-time index so every LDS access is known to touch one buffer.
+    auto buf_of = [&](auto bc) -> unsigned char * {
+        if constexpr (decltype(bc)::value == 0)
+            return lb0;
+        else if constexpr (decltype(bc)::value == 1)
+            return lb1;
+        else if constexpr (decltype(bc)::value == 2)
+            return lb2;
+        else
+            return lb3;
+    };
+    constexpr int WAIT = (DEPTH - 1) * (NQ + KS);
+    T av[NB][KS];
+    fetch(0, lb0);
+    load_a(0, av[0]);
+    fetch(1, lb1);
+    load_a(1, av[1]);
+    if constexpr (DEPTH == 3) {
+        fetch(2, lb2);
+        load_a(2, av[2]);
     }
+    TT_WAIT_VM_LDS(WAIT);
+    line_barrier();
+    auto step = [&](int li_, auto jc) {
+        constexpr int J = decltype(jc)::value;
+        fetch(li_ + DEPTH, buf_of(std::integral_constant<int, (J + DEPTH) % NB>()));
+        load_a(li_ + DEPTH, av[(J + DEPTH) % NB]);
+        if (active) compute(buf_of(std::integral_constant<int, J % NB>()), av[J % NB]);
+        TT_WAIT_VM_LDS(WAIT);
+        line_barrier();
+    };
+    for (int base = 0; base < nlines; base += NB) {
+        step(base, std::integral_constant<int, 0>());
+        if (base + 1 >= nlines) break;
+        step(base + 1, std::integral_constant<int, 1>());
+        if (base + 2 >= nlines) break;
+        step(base + 2, std::integral_constant<int, 2>());
+        if constexpr (NB == 4) {
+            if (base + 3 >= nlines) break;
+            step(base + 3, std::integral_constant<int, 3>());
+        }
+    }
+    TT_WAIT_ALL();  // no load into LDS may outlive the workgroup
     if (!active) return;
 
     // D layout of the 16x16x4 MFMA: FP64 element v of lane l is D[l/16 + 4v][l%16],
@@ -259,37 +427,31 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     }
 }
 
-// staged elements per thread: TR * S / (64 gxb) <= NST (gxb - 1 + KS) / gxb
-constexpr int pf_for(int nst, int ks, int gxb_min) {
-    return (nst * (gxb_min - 1 + ks) + gxb_min - 1) / gxb_min;
+template <typename T, int KS, int DB>
+int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a) {
+    constexpr int NST = kStateTiles<T>();
+    // Two lines in flight.  (Three fit in LDS for FP64 but measured slower
+    // on the 172-atom workload: 8.2 ms against 7.7 ms per H application.)
+    constexpr int DEPTH = 2;
+    const int nthr = 64 * (a.gxb + a.nsplit);
+    const int64_t blocks = (a.nblocks + 7) / 8 * 8;
+    hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH>), dim3((unsigned)blocks),
+                       dim3(nthr), 0, ctx->stream, a);
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
 }
 
 template <typename T, int KS>
 int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a) {
-    constexpr int NST = kStateTiles<T>();
-    const int S = 16 * NST;
-    const int TR = 4 * (a.gxb - 1) + 4 * KS;
-    const size_t lds = (size_t)2 * TR * S * sizeof(T);
-    const int nthr = 64 * (a.gxb + a.nsplit);
-    const int pf = (TR * S + nthr - 1) / nthr;
-    int64_t blocks = (a.nblocks + 7) / 8 * 8;
-    TT_CHECK(lds <= 160 * 1024, TTDFT_EBADARG, "band SpMM: LDS stage too large");
-    // two variants: wide chunks (>= 12 x-bricks) and narrow ones (>= 4)
-    constexpr int PF_WIDE = pf_for(NST, KS, 12), PF_NARROW = pf_for(NST, KS, 4);
-    if (a.gxb >= 12 && pf <= PF_WIDE) {
-        auto kf = brick_spmm_kernel<T, KS, NST, PF_WIDE>;
-        TT_HIP(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-        hipLaunchKernelGGL(kf, dim3((unsigned)blocks), dim3(nthr), lds, ctx->stream, a);
-    } else {
-        TT_CHECK(pf <= PF_NARROW, TTDFT_EBADARG, "band SpMM: staging exceeds the register buffer");
-        auto kf = brick_spmm_kernel<T, KS, NST, PF_NARROW>;
-        TT_HIP(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-        hipLaunchKernelGGL(kf, dim3((unsigned)blocks), dim3(nthr), lds, ctx->stream, a);
-    }
-    TT_LAUNCH_CHECK();
-    return TTDFT_OK;
+    TT_CHECK(a.gxb >= 1 && a.gxb + a.nsplit <= 16, TTDFT_EBADARG, "band SpMM: %d x-bricks per chunk",
+             a.gxb);
+    TT_CHECK((int64_t)a.rx * a.ldx * (int64_t)sizeof(T) < ((int64_t)1 << 31), TTDFT_EBADARG,
+             "band SpMM: one x-line of X exceeds 2 GiB");
+    // 16-byte loads need every X row 16-byte aligned
+    const bool wide = (a.ldx * (int64_t)sizeof(T)) % 16 == 0 &&
+                      (reinterpret_cast<uintptr_t>(a.X) & 15) == 0;
+    if (wide) return launch_db<T, KS, 16>(ctx, a);
+    return launch_db<T, KS, 4>(ctx, a);
 }
 
 template <typename T>
