@@ -105,6 +105,8 @@ __device__ __forceinline__ void line_barrier() {
 
 // N LDS loads of T from byte offsets OFF0 + i STRIDE past p, completed inside
 // the asm block (s_waitcnt lgkmcnt(0)); every output is used by the caller.
+// The outputs are early-clobber ("=&v"): a load may return while later loads
+// of the block still read the address register, so no output may share it.
 // Written as asm because the compiler cannot tell these reads apart from the
 // global->LDS loads into the other line buffers and would otherwise wait for
 // all of those to finish first.
@@ -118,24 +120,24 @@ __device__ __forceinline__ void lds_read_n(const T *p, T (&v)[N]) {
     if constexpr (sizeof(T) == 8) {
 #define TT_OP "b64"
         if constexpr (N == 1)
-            asm volatile(TT_R(0) "s_waitcnt lgkmcnt(0)" : "=v"(v[0]) : [a] "v"(a), [o0] "i"(OFF0));
+            asm volatile(TT_R(0) "s_waitcnt lgkmcnt(0)" : "=&v"(v[0]) : [a] "v"(a), [o0] "i"(OFF0));
         else if constexpr (N == 2)
             asm volatile(TT_R(0) TT_R(1) "s_waitcnt lgkmcnt(0)"
-                         : "=v"(v[0]), "=v"(v[1])
+                         : "=&v"(v[0]), "=&v"(v[1])
                          : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE));
         else if constexpr (N == 3)
             asm volatile(TT_R(0) TT_R(1) TT_R(2) "s_waitcnt lgkmcnt(0)"
-                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2])
+                         : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2])
                          : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
                            [o2] "i"(OFF0 + 2 * STRIDE));
         else if constexpr (N == 4)
             asm volatile(TT_R(0) TT_R(1) TT_R(2) TT_R(3) "s_waitcnt lgkmcnt(0)"
-                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3])
+                         : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3])
                          : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
                            [o2] "i"(OFF0 + 2 * STRIDE), [o3] "i"(OFF0 + 3 * STRIDE));
         else
             asm volatile(TT_R(0) TT_R(1) TT_R(2) TT_R(3) TT_R(4) "s_waitcnt lgkmcnt(0)"
-                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3]), "=v"(v[4])
+                         : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4])
                          : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
                            [o2] "i"(OFF0 + 2 * STRIDE), [o3] "i"(OFF0 + 3 * STRIDE),
                            [o4] "i"(OFF0 + 4 * STRIDE));
@@ -143,24 +145,24 @@ __device__ __forceinline__ void lds_read_n(const T *p, T (&v)[N]) {
     } else {
 #define TT_OP "b32"
         if constexpr (N == 1)
-            asm volatile(TT_R(0) "s_waitcnt lgkmcnt(0)" : "=v"(v[0]) : [a] "v"(a), [o0] "i"(OFF0));
+            asm volatile(TT_R(0) "s_waitcnt lgkmcnt(0)" : "=&v"(v[0]) : [a] "v"(a), [o0] "i"(OFF0));
         else if constexpr (N == 2)
             asm volatile(TT_R(0) TT_R(1) "s_waitcnt lgkmcnt(0)"
-                         : "=v"(v[0]), "=v"(v[1])
+                         : "=&v"(v[0]), "=&v"(v[1])
                          : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE));
         else if constexpr (N == 3)
             asm volatile(TT_R(0) TT_R(1) TT_R(2) "s_waitcnt lgkmcnt(0)"
-                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2])
+                         : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2])
                          : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
                            [o2] "i"(OFF0 + 2 * STRIDE));
         else if constexpr (N == 4)
             asm volatile(TT_R(0) TT_R(1) TT_R(2) TT_R(3) "s_waitcnt lgkmcnt(0)"
-                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3])
+                         : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3])
                          : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
                            [o2] "i"(OFF0 + 2 * STRIDE), [o3] "i"(OFF0 + 3 * STRIDE));
         else
             asm volatile(TT_R(0) TT_R(1) TT_R(2) TT_R(3) TT_R(4) "s_waitcnt lgkmcnt(0)"
-                         : "=v"(v[0]), "=v"(v[1]), "=v"(v[2]), "=v"(v[3]), "=v"(v[4])
+                         : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4])
                          : [a] "v"(a), [o0] "i"(OFF0), [o1] "i"(OFF0 + STRIDE),
                            [o2] "i"(OFF0 + 2 * STRIDE), [o3] "i"(OFF0 + 3 * STRIDE),
                            [o4] "i"(OFF0 + 4 * STRIDE));
