@@ -10,9 +10,9 @@
 // squaring every psi_i (R ranks, Q points per axis, k states): at R = 50,
 // Q = 600, k = 300 about 4x fewer, and all states are summed in one pass.
 // Every step is a dense FP64 matrix product on the matrix cores:
-//   1. Cg[c][a][b][s]  = sqrt(w_s) X[a + rx b + rx ry c][idx_s]   (gather)
-//   2. T1[qz][a][b][s] = sum_c Bz[qz,c] Cg[c][a][b][s]   (rocBLAS GEMM)
-//   3. T2[qz][qx][b][s]= sum_a Bx[qx,a] T1[qz][a][b][s]  (rocBLAS GEMM per plane)
+//   1. Cg[c][a][s][b]  = sqrt(w_s) X[a + rx b + rx ry c][idx_s]   (gather)
+//   2. T1[qz][a][s][b] = sum_c Bz[qz,c] Cg[c][a][s][b]   (rocBLAS GEMM)
+//   3. T2[qz][qx][s][b]= sum_a Bx[qx,a] T1[qz][a][s][b]  (rocBLAS GEMM per plane)
 //   4. m[qz][qx][pair] = upper triangle of T2[qz][qx] T2[qz][qx]^T, packed
 //                        (syrk_pack_kernel, FP64 MFMA)
 //   5. rho[qz]^T       = m[qz] E^T,  E[qy][pair] = (2 - [b=b']) By[qy,b] By[qy,b']
@@ -27,7 +27,9 @@
 namespace ttdft {
 namespace {
 
-// Cg[((c rx + a) ry + b) K + s] = sqrt(w_s) X[(a + rx b + rx ry c) ldx + idx_s]
+// Cg[((c rx + a) K + s) ry + b] = sqrt(w_s) X[(a + rx b + rx ry c) ldx + idx_s]
+// (states before b: one state's y-factor coefficients are contiguous all the
+// way to the T2 columns read by syrk_pack_kernel)
 template <typename T>
 __global__ void gather_scaled_kernel(const T *__restrict__ X, int64_t ldx,
                                      const int32_t *__restrict__ idx,
@@ -36,10 +38,10 @@ __global__ void gather_scaled_kernel(const T *__restrict__ X, int64_t ldx,
     const int64_t total = N * K;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t J = e / K;  // (c rx + a) ry + b
-        const int s = (int)(e - J * K);
-        const int b = (int)(J % ry);
-        const int64_t ca = J / ry;
+        const int b = (int)(e % ry);
+        const int64_t J = e / ry;  // (c rx + a) K + s
+        const int s = (int)(J % K);
+        const int64_t ca = J / K;
         const int a = (int)(ca % rx);
         const int64_t c = ca / rx;
         const int64_t I = a + (int64_t)rx * (b + (int64_t)ry * c);
@@ -56,58 +58,80 @@ constexpr int tri_col(int p) {  // column b' of upper-triangle pair p (column-ma
 }
 
 // Packed upper triangle of M = V V^T for one column t per wave:
-//   V[b][s] = T2[t R K + b K + s] (R <= 16 RT, K states),
+//   V[s][b] = T2[t R K + s R + b] (R <= 16 RT, K states),
 //   m[t np + b'(b'+1)/2 + b] = M[b][b'],  b <= b'.
 // The 16x16 output tiles (bi, bj), bi <= bj, are numbered column-major; the
 // wave computes tiles [P0, P0 + NP).  Per 16-state chunk, lane (i, kq) holds
-// V[16 bi + i][c + 4 kq + e] for e = 0..3; MFMA step e contracts the four
+// V[c + 4 kq + e][16 bi + i] for e = 0..3; MFMA step e contracts the four
 // states {c + 4 kq + e}: both operands come from the same registers
 // (A[i][kq] and B[kq][j] have the same lane layout), and the order in which
-// states are summed does not matter.  The next chunk is loaded while the
-// current one is multiplied; loads are unconditional (clamped address, value
-// zeroed) so the compiler waits only for the chunk in use.
+// states are summed does not matter.  For a fixed state the 16 lanes of a row
+// block read 16 consecutive b.
+// The kernel streams T2 once from HBM; to keep enough bytes in flight, two
+// chunks are loaded ahead of the one being multiplied (three register sets,
+// loop unrolled by 3).  Lane offsets inside a chunk are fixed and computed
+// once; rows past R load a valid row whose products land only in tiles that
+// are never stored, and states past K (last chunk only) are zeroed, reading
+// into the 16 R doubles of slack that follow the T2 buffer.
 template <int RT, int P0, int NP>
 __global__ __launch_bounds__(256) void syrk_pack_kernel(const double *__restrict__ T2, int32_t R,
                                                          int32_t K, int64_t ncols,
                                                          double *__restrict__ mp) {
+    constexpr int NBLK = tri_col(P0 + NP - 1) + 1;  // row blocks the tiles use
+    static_assert(NBLK <= RT, "tile range");
     const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= ncols) return;  // no barriers: waves are independent
     const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
     const double *V = T2 + t * R * K;
+    int32_t off[NBLK][4];
+#pragma unroll
+    for (int bi = 0; bi < NBLK; ++bi) {
+        const int row = 16 * bi + i;
+        const int rc = row < R ? row : R - 1;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) off[bi][e] = (4 * kq + e) * R + rc;
+    }
     d4_t acc[NP];
 #pragma unroll
     for (int q = 0; q < NP; ++q) acc[q] = d4_t{0, 0, 0, 0};
-    double cur[RT][4], nxt[RT][4];
-    auto load = [&](int c, double (*v)[4]) {
+    double b0[NBLK][4], b1[NBLK][4], b2[NBLK][4];
+    const int nchunk = (K + 15) / 16;
+    auto load = [&](int ch, double (*v)[4]) {
+        const double *Vc = V + (int64_t)(ch < nchunk ? ch : nchunk - 1) * 16 * R;
 #pragma unroll
-        for (int bi = 0; bi < RT; ++bi) {
-            const int row = 16 * bi + i;
-            const int rc = row < R ? row : R - 1;
+        for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int s = c + 4 * kq + e;
-                const int sc = s < K ? s : K - 1;
-                const double x = V[(int64_t)rc * K + sc];
-                v[bi][e] = (row < R && s < K) ? x : 0.0;
-            }
-        }
+            for (int e = 0; e < 4; ++e) v[bi][e] = Vc[off[bi][e]];
     };
-    load(0, cur);
-    for (int c = 0; c < K; c += 16) {
-        load(c + 16 < K ? c + 16 : c, nxt);
+    auto mul = [&](int ch, double (*v)[4]) {
+        const int c = ch * 16;
+        if (c + 16 > K) {  // last chunk: states past K contribute nothing
+#pragma unroll
+            for (int bi = 0; bi < NBLK; ++bi)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (c + 4 * kq + e >= K) v[bi][e] = 0.0;
+        }
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
                 const int p = P0 + q;
                 const int bj = tri_col(p), bi = p - bj * (bj + 1) / 2;
-                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[bi][e], cur[bj][e], acc[q], 0,
-                                                              0, 0);
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[bi][e], v[bj][e], acc[q], 0, 0, 0);
             }
-#pragma unroll
-        for (int bi = 0; bi < RT; ++bi)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) cur[bi][e] = nxt[bi][e];
+    };
+    load(0, b0);
+    load(1, b1);
+    for (int ch = 0; ch < nchunk; ch += 3) {
+        load(ch + 2, b2);
+        mul(ch, b0);
+        if (ch + 1 >= nchunk) break;
+        load(ch + 3, b0);
+        mul(ch + 1, b1);
+        if (ch + 2 >= nchunk) break;
+        load(ch + 4, b1);
+        mul(ch + 2, b2);
     }
     // FP64 16x16x4 D layout: element v of lane l is D[l/16 + 4v][l%16]
     const int64_t np = (int64_t)R * (R + 1) / 2;
@@ -134,7 +158,7 @@ int launch_syrk_group(ttdft_ctx ctx, const double *T2, int32_t R, int32_t K, int
     return TTDFT_OK;
 }
 
-// all upper-triangle tiles of an R x R result, at most 12 tiles per wave
+// all upper-triangle tiles of an R x R result, at most 11 tiles per wave
 int launch_syrk_pack(ttdft_ctx ctx, const double *T2, int32_t R, int32_t K, int64_t ncols,
                      double *mp) {
     switch ((R + 15) / 16) {
@@ -157,9 +181,10 @@ int launch_syrk_pack(ttdft_ctx ctx, const double *T2, int32_t R, int32_t K, int6
         TT_TRY((launch_syrk_group<7, 10, 9>(ctx, T2, R, K, ncols, mp)));
         return launch_syrk_group<7, 19, 9>(ctx, T2, R, K, ncols, mp);
     case 8:
-        TT_TRY((launch_syrk_group<8, 0, 12>(ctx, T2, R, K, ncols, mp)));
-        TT_TRY((launch_syrk_group<8, 12, 12>(ctx, T2, R, K, ncols, mp)));
-        return launch_syrk_group<8, 24, 12>(ctx, T2, R, K, ncols, mp);
+        TT_TRY((launch_syrk_group<8, 0, 9>(ctx, T2, R, K, ncols, mp)));
+        TT_TRY((launch_syrk_group<8, 9, 9>(ctx, T2, R, K, ncols, mp)));
+        TT_TRY((launch_syrk_group<8, 18, 9>(ctx, T2, R, K, ncols, mp)));
+        return launch_syrk_group<8, 27, 9>(ctx, T2, R, K, ncols, mp);
     default:
         break;
     }
@@ -239,7 +264,7 @@ int density_compute(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, int32_
         const long v = std::strtol(env, nullptr, 10);
         if (v > 0 && v < QC) QC = v;
     }
-    const size_t fixed = (size_t)N * K + (size_t)py * np + 2 * (size_t)K;
+    const size_t fixed = (size_t)N * K + (size_t)py * np + 2 * (size_t)K + 16 * (size_t)ry;
     TT_TRY(ctx->scratch[5].ensure(8 * (fixed + (size_t)QC * per_plane)));
     double *Cg = (double *)ctx->scratch[5].ptr;
     double *E = Cg + (size_t)N * K;
@@ -247,7 +272,7 @@ int density_compute(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, int32_
     int32_t *idxd = (int32_t *)(swd + K);
     double *T1 = swd + 2 * (size_t)K;
     double *T2 = T1 + (size_t)QC * rxy * K;
-    double *mp = T2 + (size_t)QC * px * ry * K;
+    double *mp = T2 + (size_t)QC * px * ry * K + 16 * (size_t)ry;  // 16 ry doubles of slack after T2
 
     PendingTiming t;
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_DENSITY, &t));
@@ -268,11 +293,11 @@ int density_compute(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, int32_
     const double one = 1.0, zero = 0.0;
     for (int64_t q0 = 0; q0 < pz; q0 += QC) {
         const int qc = (int)std::min<int64_t>(QC, pz - q0);
-        // 2. T1[qzl][a][b][s] = sum_c Cg[c][a][b][s] Bz[q0 + qzl, c]
+        // 2. T1[qzl][a][s][b] = sum_c Cg[c][a][s][b] Bz[q0 + qzl, c]
         TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
                               (rocblas_int)(rxy * K), qc, rz, &one, Cg, (rocblas_int)(rxy * K),
                               Bz + q0, pz, &zero, T1, (rocblas_int)(rxy * K)));
-        // 3. T2[qzl][qx][(b, s)] = sum_a T1[qzl][a][(b, s)] Bx[qx, a], one GEMM per plane
+        // 3. T2[qzl][qx][(s, b)] = sum_a T1[qzl][a][(s, b)] Bx[qx, a], one GEMM per plane
         TT_BLAS(rocblas_dgemm_strided_batched(
             ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
             (rocblas_int)((int64_t)ry * K), px, rx, &one, T1, (rocblas_int)((int64_t)ry * K),
