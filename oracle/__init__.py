@@ -113,6 +113,13 @@ def lib():
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                      ctypes.c_int, _dp, ctypes.POINTER(OrcChfsiState)]
+        L.orc_assemble_hloc.argtypes = [ctypes.c_int] * 6 + [_dp] * 10 + [ctypes.c_double,
+                                                                           _i64p, _i32p, _dp]
+        L.orc_assemble_hloc.restype = ctypes.c_int64
+        L.orc_assemble_projectors.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                              _i64p, _i32p, _dp]
+        L.orc_assemble_projectors.restype = ctypes.c_int64
         L.orc_rng_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
         L.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
         _lib = L
@@ -354,3 +361,45 @@ def rng_normal(seed, count):
     out = np.empty(count)
     lib().orc_rng_normal(seed, count, _d(out))
     return out
+
+
+def assemble_hloc(X, DX, Y, DY, Z, DZ, PX, PY, PZ, G, truncation):
+    """H_loc rows (dft/ProjectHamiltonianSparse.cc:330-575) as (rowptr, col, val).
+
+    X..DZ are r x r (row-major [I][J]); PX is (rvx, rx, rx) = MatPotX[p][I][J];
+    G is the Veff Tucker core with shape (rvx, rvy, rvz), stored mode 0 fastest."""
+    rx, ry, rz = X.shape[0], Y.shape[0], Z.shape[0]
+    rvx, rvy, rvz = PX.shape[0], PY.shape[0], PZ.shape[0]
+    mats = [np.ascontiguousarray(m, dtype=np.float64) for m in (X, DX, Y, DY, Z, DZ, PX, PY, PZ)]
+    g = np.ascontiguousarray(np.asarray(G, dtype=np.float64).reshape(rvx, rvy, rvz)
+                             .transpose(2, 1, 0)).reshape(-1)  # p fastest
+    args = [rx, ry, rz, rvx, rvy, rvz] + [_d(m) for m in mats] + [_d(g), float(truncation)]
+    L = lib()
+    nnz = L.orc_assemble_hloc(*args, None, None, None)
+    n = rx * ry * rz
+    rowptr = np.zeros(n + 1, np.int64)
+    col = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1))
+    L.orc_assemble_hloc(*args, _i64(rowptr), _i32(col), _d(val))
+    return rowptr, col[:nnz], val[:nnz]
+
+
+def assemble_projectors(cores, mx, my, mz, truncation):
+    """Projector rows (dft/ProjectHamiltonianSparse.cc:196-278) as (rowptr, col, val).
+
+    cores[i] has shape (r_i, r_i, r_i) indexed [a, b, c]; mx[i] is rx x r_i."""
+    p = len(cores)
+    ranks = np.array([c.shape[0] for c in cores], np.int32)
+    rx, ry, rz = mx[0].shape[0], my[0].shape[0], mz[0].shape[0]
+    cc = np.concatenate([np.asarray(c, np.float64).transpose(2, 1, 0).reshape(-1) for c in cores])
+    fx = np.concatenate([np.asarray(m, np.float64).T.reshape(-1) for m in mx])  # column-major
+    fy = np.concatenate([np.asarray(m, np.float64).T.reshape(-1) for m in my])
+    fz = np.concatenate([np.asarray(m, np.float64).T.reshape(-1) for m in mz])
+    L = lib()
+    args = [p, _i32(ranks), rx, ry, rz, _d(cc), _d(fx), _d(fy), _d(fz), float(truncation)]
+    nnz = L.orc_assemble_projectors(*args, None, None, None)
+    rowptr = np.zeros(p + 1, np.int64)
+    col = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1))
+    L.orc_assemble_projectors(*args, _i64(rowptr), _i32(col), _d(val))
+    return rowptr, col[:nnz], val[:nnz]

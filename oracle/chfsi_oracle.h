@@ -165,6 +165,26 @@ int orc_chfsi_step(const orc_ham *H, double *X, int64_t ldx, int k, int m,
 void orc_rng_uniform(uint64_t seed, int64_t count, double *out);
 void orc_rng_normal(uint64_t seed, int64_t count, double *out);
 
+/* Hamiltonian assembly (assembly_oracle.c).
+ * orc_assemble_hloc: H_loc of dft/ProjectHamiltonianSparse.cc:330-575 from the
+ * 1-D overlap (X, Y, Z) and kinetic (DX, DY, DZ) matrices, row-major r x r
+ * (the reference's MatX[I][J]), the 1-D potential matrices PX[(p rx + I) rx + J]
+ * (MatPotX[p][I][J]) and the Veff Tucker core G[p + rvx (q + rvy r)].
+ * orc_assemble_projectors: the rows of C_nloc (:196-278).
+ * Both return the number of kept entries and fill rowptr/col/val when rowptr
+ * is not NULL.  Pinned: Kronecker-product closed forms and a dense einsum of
+ * the defining formula (test_oracle_assembly.py). */
+int64_t orc_assemble_hloc(int rx, int ry, int rz, int rvx, int rvy, int rvz,
+                          const double *X, const double *DX, const double *Y,
+                          const double *DY, const double *Z, const double *DZ,
+                          const double *PX, const double *PY, const double *PZ,
+                          const double *G, double truncation, int64_t *rowptr,
+                          int32_t *col, double *val);
+int64_t orc_assemble_projectors(int p, const int32_t *ranks, int rx, int ry, int rz,
+                                const double *cores, const double *mx, const double *my,
+                                const double *mz, double truncation, int64_t *rowptr,
+                                int32_t *col, double *val);
+
 #ifdef __cplusplus
 }
 #endif
