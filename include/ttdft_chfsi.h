@@ -70,7 +70,8 @@ enum {
     TTDFT_KSTAT_GRAM = 2,    /* X^T Y                                     */
     TTDFT_KSTAT_ROTATE = 3,  /* X Q                                       */
     TTDFT_KSTAT_DENSITY = 4, /* fused density contraction                 */
-    TTDFT_KSTAT_COUNT = 5
+    TTDFT_KSTAT_ASSEMBLY = 5, /* H_loc assembly (device part of each pass) */
+    TTDFT_KSTAT_COUNT = 6
 };
 
 typedef struct ttdft_ctx_s *ttdft_ctx;
@@ -320,6 +321,34 @@ int ttdft_synth_hamiltonian(const ttdft_synth_params *prm, int64_t *nnz,
                             int64_t *nnz_c, int64_t *rowptr, int32_t *colind,
                             double *val, int64_t *c_rowptr, int32_t *c_colind,
                             double *c_val, double *inv_c_lm);
+
+/* ---- Hamiltonian assembly (GPU) ------------------------------------------
+ * ttdft_assemble_hloc replaces the host-side row loops of
+ * dft/ProjectHamiltonianSparse.cc:330-575 (Create_Hloc) and :1223-1592
+ * (Compute_nnz_loc):
+ *   H[(xI,yI,zI),(x,y,z)] = 1/2 (DX Y Z + X DY Z + X Y DZ)
+ *                          + sum_{pqr} G[p,q,r] PX[p][xI][x] PY[q][yI][y] PZ[r][zI][z],
+ * row/column index x + rx (y + ry z), entries with |h| > truncation kept,
+ * columns ascending.  X..DZ are row-major r x r (MatX[I][J]); PX is
+ * [rvx][rx][rx] (MatPotX[p][I][J]); G is the Veff Tucker core with p fastest,
+ * G[p + rvx (q + rvy r)].
+ * ttdft_assemble_projectors replaces :83-328 (Create_Cnloc, rows :196-278):
+ * row i is the Tucker reconstruction of projector i, core cores_i (r_i^3,
+ * first index fastest) and column-major factors mx_i (rx x r_i), my_i, mz_i,
+ * each concatenated over i, truncated the same way.
+ * Both are called twice: with rowptr NULL to get *nnz, then with host arrays
+ * rowptr[n+1] (p+1), col[nnz], val[nnz] to fill.  Inputs and outputs are host
+ * memory; the work runs on the context's device. */
+int ttdft_assemble_hloc(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz, int32_t rvx,
+                        int32_t rvy, int32_t rvz, const double *X, const double *DX,
+                        const double *Y, const double *DY, const double *Z, const double *DZ,
+                        const double *PX, const double *PY, const double *PZ, const double *G,
+                        double truncation, int64_t *nnz, int64_t *rowptr, int32_t *col,
+                        double *val);
+int ttdft_assemble_projectors(ttdft_ctx ctx, int32_t p, const int32_t *ranks, int32_t rx,
+                              int32_t ry, int32_t rz, const double *cores, const double *mx,
+                              const double *my, const double *mz, double truncation,
+                              int64_t *nnz, int64_t *rowptr, int32_t *col, double *val);
 
 /* Host random streams (same generator the engine uses internally). */
 int ttdft_rng_uniform(uint64_t seed, int64_t count, double *out);

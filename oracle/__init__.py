@@ -116,6 +116,10 @@ def lib():
         L.orc_assemble_hloc.argtypes = [ctypes.c_int] * 6 + [_dp] * 10 + [ctypes.c_double,
                                                                            _i64p, _i32p, _dp]
         L.orc_assemble_hloc.restype = ctypes.c_int64
+        L.orc_assemble_hloc_rows.argtypes = ([ctypes.c_int] * 6 + [_dp] * 10
+                                             + [ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
+                                                _i64p, _i32p, _dp])
+        L.orc_assemble_hloc_rows.restype = ctypes.c_int64
         L.orc_assemble_projectors.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double,
                                               _i64p, _i32p, _dp]
@@ -381,6 +385,24 @@ def assemble_hloc(X, DX, Y, DY, Z, DZ, PX, PY, PZ, G, truncation):
     col = np.zeros(max(nnz, 1), np.int32)
     val = np.zeros(max(nnz, 1))
     L.orc_assemble_hloc(*args, _i64(rowptr), _i32(col), _d(val))
+    return rowptr, col[:nnz], val[:nnz]
+
+
+def assemble_hloc_rows(X, DX, Y, DY, Z, DZ, PX, PY, PZ, G, truncation, row0, row1):
+    """Rows row0 <= i < row1 of assemble_hloc (rowptr relative to row0)."""
+    rx, ry, rz = X.shape[0], Y.shape[0], Z.shape[0]
+    rvx, rvy, rvz = PX.shape[0], PY.shape[0], PZ.shape[0]
+    mats = [np.ascontiguousarray(m, dtype=np.float64) for m in (X, DX, Y, DY, Z, DZ, PX, PY, PZ)]
+    g = np.ascontiguousarray(np.asarray(G, dtype=np.float64).reshape(rvx, rvy, rvz)
+                             .transpose(2, 1, 0)).reshape(-1)
+    args = [rx, ry, rz, rvx, rvy, rvz] + [_d(m) for m in mats] + [_d(g), float(truncation),
+                                                                  int(row0), int(row1)]
+    L = lib()
+    nnz = L.orc_assemble_hloc_rows(*args, None, None, None)
+    rowptr = np.zeros(row1 - row0 + 1, np.int64)
+    col = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1))
+    L.orc_assemble_hloc_rows(*args, _i64(rowptr), _i32(col), _d(val))
     return rowptr, col[:nnz], val[:nnz]
 
 

@@ -180,6 +180,13 @@ int64_t orc_assemble_hloc(int rx, int ry, int rz, int rvx, int rvy, int rvz,
                           const double *PX, const double *PY, const double *PZ,
                           const double *G, double truncation, int64_t *rowptr,
                           int32_t *col, double *val);
+/* rows row0 <= row < row1 only; rowptr has row1 - row0 + 1 entries */
+int64_t orc_assemble_hloc_rows(int rx, int ry, int rz, int rvx, int rvy, int rvz,
+                               const double *X, const double *DX, const double *Y,
+                               const double *DY, const double *Z, const double *DZ,
+                               const double *PX, const double *PY, const double *PZ,
+                               const double *G, double truncation, int64_t row0,
+                               int64_t row1, int64_t *rowptr, int32_t *col, double *val);
 int64_t orc_assemble_projectors(int p, const int32_t *ranks, int rx, int ry, int rz,
                                 const double *cores, const double *mx, const double *my,
                                 const double *mz, double truncation, int64_t *rowptr,

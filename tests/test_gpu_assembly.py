@@ -1,0 +1,192 @@
+"""GPU Hamiltonian assembly (ttdft_assemble_hloc / ttdft_assemble_projectors)
+against the assembly oracle, through the C ABI.
+
+The potential part of an entry is summed in a different order on the device
+(three matrix products) than in the oracle (per-row mode products), so values
+agree to rounding: relative 1e-12 of the largest entry.  An entry whose value
+lies within that tolerance of the truncation threshold may be kept by one side
+and dropped by the other; every other entry, and the column order, must match
+exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ttdft_public_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def banded(rng, r, b, sym=False):
+    a = rng.standard_normal((r, r))
+    if sym:
+        a = a + a.T
+    i = np.arange(r)
+    a[np.abs(i[:, None] - i[None, :]) > b] = 0.0
+    return a
+
+
+def hloc_inputs(seed, dims, rv, band, sym=False):
+    rng = np.random.default_rng(seed)
+    out = []
+    for r in dims:
+        out += [banded(rng, r, band, sym), banded(rng, r, band, sym)]
+    for r, q in zip(dims, rv):
+        out.append(np.stack([banded(rng, r, band, sym) for _ in range(q)]))
+    out.append(rng.standard_normal(rv))
+    return out
+
+
+def keys(rowptr, col, n_cols):
+    rows = np.repeat(np.arange(len(rowptr) - 1, dtype=np.int64), np.diff(rowptr))
+    return rows * n_cols + col
+
+
+def assert_csr_match(got, ref, n_cols, truncation):
+    rp, c, v = got
+    rp0, c0, v0 = ref
+    assert rp[0] == 0 and np.all(np.diff(rp) >= 0) and rp[-1] == len(c) == len(v)
+    for i in range(len(rp) - 1):  # columns strictly ascending within every row
+        assert np.all(np.diff(c[rp[i]:rp[i + 1]]) > 0), f"row {i} not sorted"
+    tol = RTOL * max(np.abs(v0).max() if len(v0) else 0.0, 1.0)
+    k, k0 = keys(rp, c, n_cols), keys(rp0, c0, n_cols)
+    common, ia, ib = np.intersect1d(k, k0, assume_unique=True, return_indices=True)
+    if len(common):
+        assert np.abs(v[ia] - v0[ib]).max() <= tol
+    only_got = np.setdiff1d(np.arange(len(k)), ia)
+    only_ref = np.setdiff1d(np.arange(len(k0)), ib)
+    assert np.all(np.abs(v[only_got]) <= truncation + tol), "kept an entry the oracle drops"
+    assert np.all(np.abs(v0[only_ref]) <= truncation + tol), "dropped an entry the oracle keeps"
+    assert len(only_got) + len(only_ref) <= max(2, len(k0) // 100000)
+
+
+HLOC_CASES = [
+    # dims, Veff Tucker ranks, 1-D half-bandwidth, truncation
+    ((14, 12, 10), (5, 4, 3), 3, 1e-3),
+    ((8, 8, 8), (8, 8, 8), 7, 0.0),       # dense factors, nothing truncated
+    ((20, 6, 9), (1, 2, 7), 4, 1e-2),     # rank-1 mode, ragged dims
+    ((5, 17, 3), (3, 1, 2), 1, 0.5),      # heavy truncation, short z axis
+]
+
+
+@pytest.mark.parametrize("dims,rv,band,trunc", HLOC_CASES)
+def test_assemble_hloc_matches_oracle(ctx, dims, rv, band, trunc):
+    args = hloc_inputs(11, dims, rv, band)
+    n = int(np.prod(dims))
+    print(f"assemble_hloc dims={dims} rv={rv} band={band} trunc={trunc}", flush=True)
+    got = ctx.assemble_hloc(*args, trunc)
+    ref = oracle.assemble_hloc(*args, trunc)
+    assert_csr_match(got, ref, n, trunc)
+
+
+def test_assemble_hloc_chunked(ctx, monkeypatch):
+    """Seven groups (xI, y, yI) per chunk: the last chunk is ragged."""
+    monkeypatch.setenv("TTDFT_ASSEMBLE_GROUPS", "7")
+    dims, rv = (9, 7, 6), (3, 4, 2)
+    args = hloc_inputs(3, dims, rv, 2)
+    got = ctx.assemble_hloc(*args, 1e-3)
+    ref = oracle.assemble_hloc(*args, 1e-3)
+    assert_csr_match(got, ref, int(np.prod(dims)), 1e-3)
+
+
+def test_assemble_hloc_all_truncated(ctx):
+    args = hloc_inputs(5, (6, 5, 4), (2, 2, 2), 2)
+    rp, c, v = ctx.assemble_hloc(*args, 1e30)
+    assert len(c) == 0 and np.all(rp == 0)
+
+
+def test_assemble_hloc_symmetric_full_size(ctx):
+    """50^3 basis, Veff Tucker ranks 50^3 (the 172-atom workload): with
+    symmetric 1-D factors H_loc is symmetric, so the assembled matrix must
+    equal its transpose (structure exactly, away from the threshold)."""
+    dims, rv, trunc = (50, 50, 50), (50, 50, 50), 1e-2
+    args = hloc_inputs(2, dims, rv, 3, sym=True)
+    rp, c, v = ctx.assemble_hloc(*args, trunc)
+    n = int(np.prod(dims))
+    assert len(c) > 10 * n
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    k = rows * n + c
+    kt = c.astype(np.int64) * n + rows
+    order = np.argsort(kt)
+    kt, vt = kt[order], v[order]
+    assert np.all(np.diff(k) > 0)  # rows ascending, columns ascending within rows
+    common, ia, ib = np.intersect1d(k, kt, assume_unique=True, return_indices=True)
+    tol = RTOL * np.abs(v).max()
+    assert np.abs(v[ia] - vt[ib]).max() <= tol
+    asym = np.setdiff1d(np.arange(len(k)), ia)
+    assert np.all(np.abs(v[asym]) <= trunc + tol)
+    assert len(asym) <= len(k) // 100000
+
+
+def proj_inputs(seed, dims, ranks, tiny_row=None):
+    rng = np.random.default_rng(seed)
+    cores, mx, my, mz = [], [], [], []
+    grid = [np.linspace(-1, 1, r) for r in dims]
+    for i, r in enumerate(ranks):
+        s = rng.standard_normal((r, r, r))
+        if i == tiny_row:
+            s *= 1e-12
+        cores.append(s)
+        for m, g in zip((mx, my, mz), grid):
+            c = rng.uniform(-0.6, 0.6)
+            w = rng.uniform(0.15, 0.4)
+            base = np.exp(-((g - c) / w) ** 2)
+            m.append(base[:, None] * rng.standard_normal((len(g), r)))
+    return cores, mx, my, mz
+
+
+PROJ_CASES = [
+    ((14, 12, 10), [3, 1, 5, 2], 1e-4),
+    ((9, 11, 7), [2, 2, 2, 2, 2, 2, 2], 0.0),
+    ((30, 20, 25), [4, 6, 1], 1e-3),      # rows longer than one 4096-column chunk
+]
+
+
+@pytest.mark.parametrize("dims,ranks,trunc", PROJ_CASES)
+def test_assemble_projectors_matches_oracle(ctx, dims, ranks, trunc):
+    cores, mx, my, mz = proj_inputs(4, dims, ranks)
+    print(f"assemble_projectors dims={dims} ranks={ranks} trunc={trunc}", flush=True)
+    got = ctx.assemble_projectors(cores, mx, my, mz, trunc)
+    ref = oracle.assemble_projectors(cores, mx, my, mz, trunc)
+    assert_csr_match(got, ref, int(np.prod(dims)), trunc)
+
+
+def test_assemble_projectors_batched_with_empty_row(ctx, monkeypatch):
+    """Two rows per batch over five rows; row 2 falls entirely under the threshold."""
+    monkeypatch.setenv("TTDFT_ASSEMBLE_ROWS", "2")
+    dims = (12, 10, 8)
+    cores, mx, my, mz = proj_inputs(6, dims, [2, 3, 2, 1, 4], tiny_row=2)
+    got = ctx.assemble_projectors(cores, mx, my, mz, 1e-6)
+    ref = oracle.assemble_projectors(cores, mx, my, mz, 1e-6)
+    assert got[0][3] == got[0][2]
+    assert_csr_match(got, ref, int(np.prod(dims)), 1e-6)
+
+
+def test_assemble_then_apply(ctx):
+    """Assembled H_loc and projector rows, uploaded and applied, against the
+    oracle's operator built from the oracle's own assembly."""
+    dims, rv = (10, 9, 8), (4, 3, 3)
+    args = hloc_inputs(8, dims, rv, 3, sym=True)
+    cores, mx, my, mz = proj_inputs(9, dims, [2, 3, 1])
+    rp, c, v = ctx.assemble_hloc(*args, 1e-4)
+    crp, cc, cv = ctx.assemble_projectors(cores, mx, my, mz, 1e-6)
+    inv = np.array([0.5, -1.25, 2.0])
+    n = int(np.prod(dims))
+    ham = E.Hamiltonian(ctx, n, rp, c, v, 3, crp, cc, cv, inv, E.HAM_F64, dims=dims)
+    orp, oc, ov = oracle.assemble_hloc(*args, 1e-4)
+    ocrp, occ, ocv = oracle.assemble_projectors(cores, mx, my, mz, 1e-6)
+    Hd = np.zeros((n, n))
+    for i in range(n):
+        Hd[i, oc[orp[i]:orp[i + 1]]] = ov[orp[i]:orp[i + 1]]
+    Cd = np.zeros((3, n))
+    for i in range(3):
+        Cd[i, occ[ocrp[i]:ocrp[i + 1]]] = ocv[ocrp[i]:ocrp[i + 1]]
+    Hd += Cd.T @ np.diag(inv) @ Cd
+    X = np.random.default_rng(1).standard_normal((n, 16))
+    dX = E.DeviceArray.from_host(ctx, X)
+    dY = E.DeviceArray(ctx, (n, 16))
+    ctx.ham_apply(ham, dX.ptr, 16, dY.ptr, 16, 16, E.F64)
+    ref = Hd @ X
+    assert np.abs(dY.to_host() - ref).max() <= 1e-11 * np.abs(ref).max()

@@ -96,3 +96,13 @@ def test_projector_rows_match_tucker_reconstruction():
         want = np.nonzero(np.abs(dense[i]) > trunc)[0]
         assert np.array_equal(col[rowptr[i]:rowptr[i + 1]], want)
         assert np.allclose(val[rowptr[i]:rowptr[i + 1]], dense[i, want], rtol=1e-13, atol=0)
+
+
+def test_hloc_row_range_is_a_slice_of_the_full_assembly():
+    args = _inputs(np.random.default_rng(9), rx=5, ry=4, rz=3)
+    rp, c, v = oracle.assemble_hloc(*args, 0.3)
+    r0, r1 = 7, 41
+    rp2, c2, v2 = oracle.assemble_hloc_rows(*args, 0.3, r0, r1)
+    assert np.array_equal(rp2, rp[r0:r1 + 1] - rp[r0])
+    assert np.array_equal(c2, c[rp[r0]:rp[r1]])
+    assert np.array_equal(v2, v[rp[r0]:rp[r1]])

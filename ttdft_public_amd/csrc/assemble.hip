@@ -1,0 +1,507 @@
+// Hamiltonian assembly on the GPU: H_loc and the projector rows C, truncated
+// to CSR (the reference builds both row by row on the host:
+// dft/ProjectHamiltonianSparse.cc:330-575 Create_Hloc, :1223-1592
+// Compute_nnz_loc, :83-328 Create_Cnloc).
+//
+// H_loc[(xI,yI,zI),(x,y,z)] =
+//     1/2 (DX[xI,x] Y[yI,y] Z[zI,z] + X[xI,x] DY[yI,y] Z[zI,z] + X[xI,x] Y[yI,y] DZ[zI,z])
+//   + sum_{p,q,r} G[p,q,r] PX[p][xI][x] PY[q][yI][y] PZ[r][zI][z],
+// keeping the entries with |v| > truncation, columns ascending.
+//
+// The reference forms a dense length-n row per basis function (O(n^2)
+// values, 3.75e7 flop per row at rx = ry = rz = 50).  Here the potential part
+// of every entry comes out of three matrix products on the matrix cores
+// (rocBLAS), with the pairs (xI,x), (yI,y), (zI,z) as matrix indices:
+//   A[(x + rx xI) + rx^2 (q + rvy r)] = sum_p PX[p][xI][x] G[p,q,r]
+//   B[r][t],  t = (x + rx xI) + rx^2 (y + ry yI) = x + rx g,
+//             g = xI + rx (y + ry yI):   B_r = A_r PY^T
+//   Cc[(z + rz zI), t] = sum_r PZ[r][zI][z] B[r][t]      (over chunks of g)
+// and a hand-written kernel adds the kinetic term, truncates and compacts.
+// Compaction is exact and deterministic in two passes:
+//   1. count: cnt[row][y][z] = number of kept x for (row, y, z);
+//   2. per-row scan in the CSR order (z, then y) -> rel[row][y][z], row
+//      lengths -> rowptr;
+//   3. write: recompute, entry of x goes to rowptr[row] + rel + (kept x' < x).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "internal.h"
+
+namespace ttdft {
+namespace {
+
+#define TT_BLAS(call)                                                                     \
+    do {                                                                                  \
+        const rocblas_status st_ = (call);                                                \
+        TT_CHECK(st_ == rocblas_status_success, TTDFT_ELIB, "%s: rocBLAS status %d", #call, \
+                 (int)st_);                                                               \
+    } while (0)
+
+// device buffers freed on scope exit
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) {
+            hipError_t e = hipFree(p);
+            (void)e;
+        }
+    }
+    int alloc(size_t bytes) {
+        if (bytes == 0) bytes = 8;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+            return TTDFT_ENOMEM;
+        }
+        return TTDFT_OK;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+int upload_to(ttdft_ctx ctx, DevBuf &b, const void *src, size_t bytes) {
+    TT_TRY(b.alloc(bytes));
+    if (bytes) TT_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return TTDFT_OK;
+}
+
+// an environment variable that lowers a chunk size (smaller workspace)
+int64_t env_cap(const char *name, int64_t value) {
+    if (const char *env = std::getenv(name)) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v > 0 && v < value) return v;
+    }
+    return value;
+}
+
+unsigned blocks_for(int64_t threads, int per_block) {
+    int64_t b = (threads + per_block - 1) / per_block;
+    return (unsigned)std::max<int64_t>(1, b);
+}
+
+// One thread per ((zI,z) pair, group g = (xI, y, yI)); loops over x.  Threads
+// of a wavefront take consecutive pairs, so the Cc reads and the cnt writes
+// ([row][y][z], z fastest) are coalesced.  WRITE = 0 counts, 1 writes.
+template <int WRITE>
+__global__ __launch_bounds__(256) void hloc_compact_kernel(
+    const double *__restrict__ Cc, int64_t g0, int64_t ng, int rx, int ry, int rz,
+    const double *__restrict__ X, const double *__restrict__ DX, const double *__restrict__ Y,
+    const double *__restrict__ DY, const double *__restrict__ Z, const double *__restrict__ DZ,
+    double trunc, uint16_t *__restrict__ cnt, const int32_t *__restrict__ rel,
+    const int64_t *__restrict__ rowptr, int32_t *__restrict__ col, double *__restrict__ val,
+    int32_t *__restrict__ mismatch) {
+    const int64_t npair = (int64_t)rz * rz;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= npair * ng) return;
+    const int pi = (int)(tid % npair);
+    const int64_t gl = tid / npair;
+    const int zI = pi / rz, z = pi - zI * rz;  // Cc row index z + rz zI == pi
+    const int64_t g = g0 + gl;
+    const int xI = (int)(g % rx);
+    const int64_t gy = g / rx;
+    const int y = (int)(gy % ry), yI = (int)(gy / ry);
+    const int64_t row = xI + (int64_t)rx * (yI + (int64_t)ry * zI);
+    const int64_t key = (row * ry + y) * rz + z;
+    const double yv = Y[(int64_t)yI * ry + y], dyv = DY[(int64_t)yI * ry + y];
+    const double zv = Z[(int64_t)zI * rz + z], dzv = DZ[(int64_t)zI * rz + z];
+    const double *xr = X + (int64_t)xI * rx, *dxr = DX + (int64_t)xI * rx;
+    const double *cp = Cc + pi + npair * (gl * rx);
+    int64_t base = 0;
+    int limit = 0;  // entries the count pass found for (row, y, z)
+    if (WRITE) {
+        base = rowptr[row] + rel[key];
+        limit = cnt[key];
+    }
+    const int32_t col0 = (int32_t)((int64_t)rx * (y + (int64_t)ry * z));
+    int kept = 0;
+    for (int x = 0; x < rx; ++x) {
+        // the reference's dyad: 0.5 (DX Y Z + X DY Z + X Y DZ), then + potential
+        const double kin = dxr[x] * yv * zv + xr[x] * dyv * zv + xr[x] * yv * dzv;
+        const double v = 0.5 * kin + cp[npair * x];
+        if (fabs(v) > trunc) {
+            if (WRITE && kept < limit) {
+                col[base + kept] = col0 + x;
+                val[base + kept] = v;
+            }
+            ++kept;
+        }
+    }
+    if (!WRITE) cnt[key] = (uint16_t)kept;
+    // both passes evaluate the same products, so the counts agree; a
+    // difference would leave a slot unwritten and is reported to the host
+    if (WRITE && kept != limit) *mismatch = 1;
+}
+
+// Per row: exclusive scan of cnt[row][y][z] in the CSR order (z outer, y
+// inner) -> rel[row][y][z]; row length -> rowlen[row].  One block per row.
+__global__ __launch_bounds__(256) void hloc_row_scan_kernel(const uint16_t *__restrict__ cnt,
+                                                            int ry, int rz,
+                                                            int32_t *__restrict__ rel,
+                                                            int64_t *__restrict__ rowlen) {
+    extern __shared__ int32_t sc_lds[];  // ry * rz counts in (z, y) order, then 256 partials
+    const int64_t row = blockIdx.x;
+    const int L = ry * rz;
+    int32_t *v = sc_lds;
+    int32_t *part = sc_lds + L;
+    const uint16_t *c = cnt + row * L;
+    for (int e = threadIdx.x; e < L; e += blockDim.x) {
+        const int y = e / rz, z = e - y * rz;  // source index e = y rz + z
+        v[z * ry + y] = c[e];
+    }
+    __syncthreads();
+    // each thread scans a contiguous range of the (z, y) sequence
+    const int per = (L + blockDim.x - 1) / blockDim.x;
+    const int s0 = threadIdx.x * per, s1 = min(L, s0 + per);
+    int32_t sum = 0;
+    for (int s = s0; s < s1; ++s) sum += v[s];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t acc = 0;
+        for (int t = 0; t < (int)blockDim.x; ++t) {
+            const int32_t p = part[t];
+            part[t] = acc;
+            acc += p;
+        }
+        rowlen[row] = acc;
+    }
+    __syncthreads();
+    int32_t acc = part[threadIdx.x];
+    for (int s = s0; s < s1; ++s) {
+        const int32_t cv = v[s];
+        v[s] = acc;
+        acc += cv;
+    }
+    __syncthreads();
+    int32_t *r = rel + row * L;
+    for (int e = threadIdx.x; e < L; e += blockDim.x) {
+        const int y = e / rz, z = e - y * rz;
+        r[e] = v[z * ry + y];
+    }
+}
+
+// Dense projector rows: D[i][x + rx (y + ry z)] = sum_c W[i][c][x + rx y] MZ[i][z + rz c]
+// for the rows of one batch (W = S_i x_0 MX_i x_1 MY_i, padded to rmax terms).
+__global__ __launch_bounds__(256) void proj_dense_kernel(const double *__restrict__ W,
+                                                         const double *__restrict__ MZ,
+                                                         int64_t nrows, int rmax, int rx, int ry,
+                                                         int rz, double *__restrict__ D) {
+    const int64_t nxy = (int64_t)rx * ry, n = nxy * rz;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nrows * n) return;
+    const int64_t i = tid / n, j = tid - i * n;
+    const int64_t xy = j % nxy;
+    const int z = (int)(j / nxy);
+    double s = 0.0;
+    for (int c = 0; c < rmax; ++c)
+        s += W[(i * rmax + c) * nxy + xy] * MZ[(i * rmax + c) * rz + z];
+    D[tid] = s;
+}
+
+// Ordered compaction of dense rows: block (row i, chunk of 4096 columns).
+// COUNT: cnt[i][chunk]; WRITE: entries at rowptr[i] + off[i][chunk] + rank,
+// ranks from wavefront ballots and per-wave prefix sums in LDS.
+template <int WRITE>
+__global__ __launch_bounds__(256) void dense_compact_kernel(
+    const double *__restrict__ D, int64_t n, int nchunk, double trunc, int32_t *__restrict__ cnt,
+    const int64_t *__restrict__ start, int32_t *__restrict__ col, double *__restrict__ val) {
+    __shared__ int32_t wsum[4];
+    const int64_t i = blockIdx.x / nchunk;
+    const int chunk = (int)(blockIdx.x - i * nchunk);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double *row = D + i * n;
+    const int64_t c0 = (int64_t)chunk * 4096, c1 = std::min<int64_t>(n, c0 + 4096);
+    int64_t pos = WRITE ? start[(int64_t)i * nchunk + chunk] : 0;
+    int32_t total = 0;
+    for (int64_t b = c0; b < c1; b += 256) {
+        const int64_t j = b + threadIdx.x;
+        const bool keep = j < c1 && fabs(row[j]) > trunc;
+        const uint64_t m = __ballot(keep);
+        const int rank = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int before = 0, all = 0;
+        for (int t = 0; t < 4; ++t) {
+            if (t < w) before += wsum[t];
+            all += wsum[t];
+        }
+        if (WRITE && keep) {
+            col[pos + before + rank] = (int32_t)j;
+            val[pos + before + rank] = row[j];
+        }
+        pos += all;
+        total += all;
+        __syncthreads();
+    }
+    if (!WRITE && threadIdx.x == 0) cnt[(int64_t)i * nchunk + chunk] = total;
+}
+
+}  // namespace
+}  // namespace ttdft
+
+using namespace ttdft;
+
+extern "C" {
+
+int ttdft_assemble_hloc(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz, int32_t rvx,
+                        int32_t rvy, int32_t rvz, const double *X, const double *DX,
+                        const double *Y, const double *DY, const double *Z, const double *DZ,
+                        const double *PX, const double *PY, const double *PZ, const double *G,
+                        double truncation, int64_t *nnz, int64_t *rowptr, int32_t *col,
+                        double *val) {
+    clear_error();
+    TT_CHECK(ctx, TTDFT_EBADARG, "ctx is NULL");
+    TT_CHECK(rx > 0 && ry > 0 && rz > 0 && rvx > 0 && rvy > 0 && rvz > 0, TTDFT_EBADARG,
+             "assemble_hloc: ranks must be > 0");
+    TT_CHECK(X && DX && Y && DY && Z && DZ && PX && PY && PZ && G && nnz, TTDFT_EBADARG,
+             "assemble_hloc: NULL input");
+    TT_CHECK(truncation >= 0.0, TTDFT_EBADARG, "assemble_hloc: truncation < 0");
+    TT_CHECK(rx <= 65535 && (int64_t)rx * ry * rz < ((int64_t)1 << 31), TTDFT_EBADARG,
+             "assemble_hloc: rx = %d or n too large", rx);
+    TT_CHECK((int64_t)rx * rx * ry * ry < ((int64_t)1 << 31), TTDFT_EBADARG,
+             "assemble_hloc: rx^2 ry^2 exceeds the 32-bit GEMM dimension");
+    const int64_t n = (int64_t)rx * ry * rz;
+    const int64_t rx2 = (int64_t)rx * rx, ry2 = (int64_t)ry * ry, rz2 = (int64_t)rz * rz;
+    const int64_t ngroups = rx * ry2;  // g = xI + rx (y + ry yI)
+    const int64_t ntup = rx * ngroups;  // rx^2 ry^2
+    // inputs
+    DevBuf dX, dDX, dY, dDY, dZ, dDZ, dPX, dPY, dPZ, dG;
+    TT_TRY(upload_to(ctx, dX, X, sizeof(double) * rx2));
+    TT_TRY(upload_to(ctx, dDX, DX, sizeof(double) * rx2));
+    TT_TRY(upload_to(ctx, dY, Y, sizeof(double) * ry2));
+    TT_TRY(upload_to(ctx, dDY, DY, sizeof(double) * ry2));
+    TT_TRY(upload_to(ctx, dZ, Z, sizeof(double) * rz2));
+    TT_TRY(upload_to(ctx, dDZ, DZ, sizeof(double) * rz2));
+    TT_TRY(upload_to(ctx, dPX, PX, sizeof(double) * rvx * rx2));
+    TT_TRY(upload_to(ctx, dPY, PY, sizeof(double) * rvy * ry2));
+    TT_TRY(upload_to(ctx, dPZ, PZ, sizeof(double) * rvz * rz2));
+    TT_TRY(upload_to(ctx, dG, G, sizeof(double) * rvx * rvy * rvz));
+    // A = PX G, B_r = A_r PY^T
+    DevBuf dA, dB;
+    TT_TRY(dA.alloc(sizeof(double) * rx2 * rvy * rvz));
+    TT_TRY(dB.alloc(sizeof(double) * rvz * ntup));
+    const double one = 1.0, zero = 0.0;
+    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none,
+                          (rocblas_int)rx2, rvy * rvz, rvx, &one, dPX.as<double>(),
+                          (rocblas_int)rx2, dG.as<double>(), rvx, &zero, dA.as<double>(),
+                          (rocblas_int)rx2));
+    TT_BLAS(rocblas_dgemm_strided_batched(
+        ctx->blas, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)rx2,
+        (rocblas_int)ry2, rvy, &one, dA.as<double>(), (rocblas_int)rx2, rx2 * rvy,
+        dPY.as<double>(), (rocblas_int)ry2, 0, &zero, dB.as<double>(), (rocblas_int)rx2, ntup,
+        rvz));
+    // chunks of groups: Cc holds rz^2 x (rx * gc) doubles (about 1 GB)
+    int64_t gc = ((int64_t)1 << 27) / std::max<int64_t>(1, rz2 * rx);
+    gc = std::max<int64_t>(1, std::min<int64_t>(gc, ngroups));
+    gc = env_cap("TTDFT_ASSEMBLE_GROUPS", gc);  // groups (xI, y, yI) per chunk
+    DevBuf dC, dcnt, drel, drowlen, dflag;
+    TT_TRY(dflag.alloc(sizeof(int32_t)));
+    TT_HIP(hipMemsetAsync(dflag.p, 0, sizeof(int32_t), ctx->stream));
+    // deterministic products: the count and write passes must see equal values
+    rocblas_atomics_mode atomics = rocblas_atomics_not_allowed;
+    TT_BLAS(rocblas_get_atomics_mode(ctx->blas, &atomics));
+    TT_BLAS(rocblas_set_atomics_mode(ctx->blas, rocblas_atomics_not_allowed));
+    struct AtomicsRestore {
+        rocblas_handle h;
+        rocblas_atomics_mode m;
+        ~AtomicsRestore() { rocblas_set_atomics_mode(h, m); }
+    } restore{ctx->blas, atomics};
+    TT_TRY(dC.alloc(sizeof(double) * rz2 * rx * gc));
+    TT_TRY(dcnt.alloc(sizeof(uint16_t) * n * ry * rz));
+    TT_TRY(drel.alloc(sizeof(int32_t) * n * ry * rz));
+    TT_TRY(drowlen.alloc(sizeof(int64_t) * n));
+    auto run_pass = [&](int write, const int64_t *drowptr, int32_t *dcol, double *dval) -> int {
+        for (int64_t g0 = 0; g0 < ngroups; g0 += gc) {
+            const int64_t ng = std::min<int64_t>(gc, ngroups - g0);
+            TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
+                                  (rocblas_int)rz2, (rocblas_int)(rx * ng), rvz, &one,
+                                  dPZ.as<double>(), (rocblas_int)rz2, dB.as<double>() + rx * g0,
+                                  (rocblas_int)ntup, &zero, dC.as<double>(), (rocblas_int)rz2));
+            const unsigned nb = blocks_for(rz2 * ng, 256);
+            if (write)
+                hipLaunchKernelGGL(hloc_compact_kernel<1>, dim3(nb), dim3(256), 0, ctx->stream,
+                                   dC.as<double>(), g0, ng, rx, ry, rz, dX.as<double>(),
+                                   dDX.as<double>(), dY.as<double>(), dDY.as<double>(),
+                                   dZ.as<double>(), dDZ.as<double>(), truncation,
+                                   dcnt.as<uint16_t>(), drel.as<int32_t>(), drowptr, dcol, dval,
+                                   dflag.as<int32_t>());
+            else
+                hipLaunchKernelGGL(hloc_compact_kernel<0>, dim3(nb), dim3(256), 0, ctx->stream,
+                                   dC.as<double>(), g0, ng, rx, ry, rz, dX.as<double>(),
+                                   dDX.as<double>(), dY.as<double>(), dDY.as<double>(),
+                                   dZ.as<double>(), dDZ.as<double>(), truncation,
+                                   dcnt.as<uint16_t>(), drel.as<int32_t>(), drowptr, dcol, dval,
+                                   dflag.as<int32_t>());
+            TT_LAUNCH_CHECK();
+        }
+        return TTDFT_OK;
+    };
+    // algorithmic work of one pass: the three products and the compaction
+    // (Veff contraction read once per entry, kinetic term 5 flop per entry)
+    const double n2 = (double)n * (double)n;
+    const double flops = 2.0 * rx2 * rvx * rvy * rvz + 2.0 * ntup * rvy * rvz +
+                         2.0 * n2 * rvz + 7.0 * n2;
+    PendingTiming tm;
+    TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ASSEMBLY, &tm));
+    TT_TRY(run_pass(0, nullptr, nullptr, nullptr));
+    const size_t lds = sizeof(int32_t) * ((size_t)ry * rz + 256);
+    TT_CHECK(lds <= 64 * 1024, TTDFT_EBADARG, "assemble_hloc: ry*rz = %d too large", ry * rz);
+    hipLaunchKernelGGL(hloc_row_scan_kernel, dim3((unsigned)n), dim3(256), lds, ctx->stream,
+                       dcnt.as<uint16_t>(), ry, rz, drel.as<int32_t>(), drowlen.as<int64_t>());
+    TT_LAUNCH_CHECK();
+    TT_TRY(timing_end(ctx, &tm, 2.0 * n * ry * rz + 4.0 * n * ry * rz + 8.0 * n, flops));
+    std::vector<int64_t> len(n), rp(n + 1);
+    TT_HIP(hipMemcpyAsync(len.data(), drowlen.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    rp[0] = 0;
+    for (int64_t i = 0; i < n; ++i) rp[i + 1] = rp[i] + len[i];
+    *nnz = rp[n];
+    if (!rowptr) return TTDFT_OK;
+    std::copy(rp.begin(), rp.end(), rowptr);
+    if (!col || !val) return TTDFT_OK;
+    DevBuf drp, dcol, dval;
+    TT_TRY(upload_to(ctx, drp, rp.data(), sizeof(int64_t) * (n + 1)));
+    TT_TRY(dcol.alloc(sizeof(int32_t) * rp[n]));
+    TT_TRY(dval.alloc(sizeof(double) * rp[n]));
+    TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ASSEMBLY, &tm));
+    TT_TRY(run_pass(1, drp.as<int64_t>(), dcol.as<int32_t>(), dval.as<double>()));
+    TT_TRY(timing_end(ctx, &tm, 12.0 * rp[n] + 8.0 * (n + 1), flops));
+    int32_t mismatch = 0;
+    TT_HIP(hipMemcpyAsync(&mismatch, dflag.p, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TT_HIP(hipMemcpyAsync(col, dcol.p, sizeof(int32_t) * rp[n], hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TT_HIP(hipMemcpyAsync(val, dval.p, sizeof(double) * rp[n], hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    TT_CHECK(!mismatch, TTDFT_ELIB, "assemble_hloc: count and write passes disagree");
+    return TTDFT_OK;
+}
+
+int ttdft_assemble_projectors(ttdft_ctx ctx, int32_t p, const int32_t *ranks, int32_t rx,
+                              int32_t ry, int32_t rz, const double *cores, const double *mx,
+                              const double *my, const double *mz, double truncation,
+                              int64_t *nnz, int64_t *rowptr, int32_t *col, double *val) {
+    clear_error();
+    TT_CHECK(ctx, TTDFT_EBADARG, "ctx is NULL");
+    TT_CHECK(p >= 0 && rx > 0 && ry > 0 && rz > 0 && nnz, TTDFT_EBADARG,
+             "assemble_projectors: bad sizes");
+    TT_CHECK(p == 0 || (ranks && cores && mx && my && mz), TTDFT_EBADARG,
+             "assemble_projectors: NULL input");
+    TT_CHECK(truncation >= 0.0, TTDFT_EBADARG, "assemble_projectors: truncation < 0");
+    const int64_t n = (int64_t)rx * ry * rz, nxy = (int64_t)rx * ry;
+    TT_CHECK(n < ((int64_t)1 << 31), TTDFT_EBADARG, "assemble_projectors: n too large");
+    *nnz = 0;
+    if (p == 0) {
+        if (rowptr) rowptr[0] = 0;
+        return TTDFT_OK;
+    }
+    int rmax = 0;
+    for (int i = 0; i < p; ++i) {
+        TT_CHECK(ranks[i] > 0, TTDFT_EBADARG, "assemble_projectors: rank of row %d < 1", i);
+        rmax = std::max(rmax, ranks[i]);
+    }
+    // host: W_i[c][x + rx y] = sum_{a,b} S_i[a,b,c] MX_i[x,a] MY_i[y,b] (small), and MZ
+    // padded to rmax terms; zero terms past a row's rank
+    std::vector<double> W((size_t)p * rmax * nxy, 0.0), MZ((size_t)p * rmax * rz, 0.0);
+    {
+        size_t oc = 0, ox = 0, oy = 0, oz = 0;
+        for (int i = 0; i < p; ++i) {
+            const int r = ranks[i];
+            const double *S = cores + oc, *A = mx + ox, *B = my + oy, *C = mz + oz;
+            std::vector<double> u((size_t)r * r * rx);  // [c][b][x]
+            for (int c = 0; c < r; ++c)
+                for (int b = 0; b < r; ++b)
+                    for (int x = 0; x < rx; ++x) {
+                        double s = 0.0;
+                        for (int a = 0; a < r; ++a)
+                            s += A[x + (size_t)rx * a] * S[a + (size_t)r * (b + (size_t)r * c)];
+                        u[((size_t)c * r + b) * rx + x] = s;
+                    }
+            for (int c = 0; c < r; ++c) {
+                for (int y = 0; y < ry; ++y)
+                    for (int x = 0; x < rx; ++x) {
+                        double s = 0.0;
+                        for (int b = 0; b < r; ++b)
+                            s += B[y + (size_t)ry * b] * u[((size_t)c * r + b) * rx + x];
+                        W[((size_t)i * rmax + c) * nxy + x + (size_t)rx * y] = s;
+                    }
+                for (int z = 0; z < rz; ++z)
+                    MZ[((size_t)i * rmax + c) * rz + z] = C[z + (size_t)rz * c];
+            }
+            oc += (size_t)r * r * r;
+            ox += (size_t)rx * r;
+            oy += (size_t)ry * r;
+            oz += (size_t)rz * r;
+        }
+    }
+    DevBuf dW, dMZ;
+    TT_TRY(upload_to(ctx, dW, W.data(), sizeof(double) * W.size()));
+    TT_TRY(upload_to(ctx, dMZ, MZ.data(), sizeof(double) * MZ.size()));
+    const int nchunk = (int)((n + 4095) / 4096);
+    // rows in batches of about 1 GB of dense values
+    int64_t rb = std::max<int64_t>(1, std::min<int64_t>(p, ((int64_t)1 << 27) / n));
+    rb = env_cap("TTDFT_ASSEMBLE_ROWS", rb);
+    DevBuf dD, dcnt, dstart;
+    TT_TRY(dD.alloc(sizeof(double) * rb * n));
+    TT_TRY(dcnt.alloc(sizeof(int32_t) * (size_t)p * nchunk));
+    TT_TRY(dstart.alloc(sizeof(int64_t) * (size_t)p * nchunk));
+    auto run_pass = [&](int write, int32_t *dcol, double *dval) -> int {
+        for (int64_t i0 = 0; i0 < p; i0 += rb) {
+            const int64_t nr = std::min<int64_t>(rb, p - i0);
+            hipLaunchKernelGGL(proj_dense_kernel, dim3(blocks_for(nr * n, 256)), dim3(256), 0,
+                               ctx->stream, dW.as<double>() + i0 * rmax * nxy,
+                               dMZ.as<double>() + i0 * rmax * rz, nr, rmax, rx, ry, rz,
+                               dD.as<double>());
+            TT_LAUNCH_CHECK();
+            const unsigned nb = (unsigned)(nr * nchunk);
+            if (write)
+                hipLaunchKernelGGL(dense_compact_kernel<1>, dim3(nb), dim3(256), 0, ctx->stream,
+                                   dD.as<double>(), n, nchunk, truncation, nullptr,
+                                   dstart.as<int64_t>() + i0 * nchunk, dcol, dval);
+            else
+                hipLaunchKernelGGL(dense_compact_kernel<0>, dim3(nb), dim3(256), 0, ctx->stream,
+                                   dD.as<double>(), n, nchunk, truncation,
+                                   dcnt.as<int32_t>() + i0 * nchunk, nullptr, nullptr, nullptr);
+            TT_LAUNCH_CHECK();
+        }
+        return TTDFT_OK;
+    };
+    TT_TRY(run_pass(0, nullptr, nullptr));
+    std::vector<int32_t> cnt((size_t)p * nchunk);
+    TT_HIP(hipMemcpyAsync(cnt.data(), dcnt.p, sizeof(int32_t) * cnt.size(),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int64_t> start(cnt.size()), rp(p + 1, 0);
+    int64_t acc = 0;
+    for (int i = 0; i < p; ++i) {
+        for (int c = 0; c < nchunk; ++c) {
+            start[(size_t)i * nchunk + c] = acc;
+            acc += cnt[(size_t)i * nchunk + c];
+        }
+        rp[i + 1] = acc;
+    }
+    *nnz = acc;
+    if (!rowptr) return TTDFT_OK;
+    std::copy(rp.begin(), rp.end(), rowptr);
+    if (!col || !val) return TTDFT_OK;
+    DevBuf dcol, dval;
+    TT_HIP(hipMemcpyAsync(dstart.p, start.data(), sizeof(int64_t) * start.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+    TT_TRY(dcol.alloc(sizeof(int32_t) * acc));
+    TT_TRY(dval.alloc(sizeof(double) * acc));
+    TT_TRY(run_pass(1, dcol.as<int32_t>(), dval.as<double>()));
+    TT_HIP(hipMemcpyAsync(col, dcol.p, sizeof(int32_t) * acc, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TT_HIP(hipMemcpyAsync(val, dval.p, sizeof(double) * acc, hipMemcpyDeviceToHost, ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    return TTDFT_OK;
+}
+
+}  // extern "C"

@@ -23,7 +23,7 @@ F64, F32 = 0, 1
 GEMM_F64, GEMM_BF16X3 = 0, 1
 DEGREE_REFERENCE, DEGREE_EXACT = 0, 1
 HAM_F64, HAM_F32 = 1, 2
-KSTAT_SPMM, KSTAT_NLPROJ, KSTAT_GRAM, KSTAT_ROTATE, KSTAT_DENSITY = range(5)
+KSTAT_SPMM, KSTAT_NLPROJ, KSTAT_GRAM, KSTAT_ROTATE, KSTAT_DENSITY, KSTAT_ASSEMBLY = range(6)
 
 _c_dp = ctypes.POINTER(ctypes.c_double)
 _c_i64p = ctypes.POINTER(ctypes.c_int64)
@@ -123,6 +123,11 @@ _SIGS = {
     "ttdft_synth_hamiltonian": (ctypes.c_int, [ctypes.POINTER(SynthParams), _c_i64p, _c_i64p,
                                                _c_i64p, _c_i32p, _c_dp, _c_i64p, _c_i32p, _c_dp,
                                                _c_dp]),
+    "ttdft_assemble_hloc": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _i32, _i32] + [_c_dp] * 10
+                            + [_d, _c_i64p, _c_i64p, _c_i32p, _c_dp]),
+    "ttdft_assemble_projectors": (ctypes.c_int, [_vp, _i32, _c_i32p, _i32, _i32, _i32, _c_dp,
+                                                 _c_dp, _c_dp, _c_dp, _d, _c_i64p, _c_i64p,
+                                                 _c_i32p, _c_dp]),
     "ttdft_rng_uniform": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
     "ttdft_rng_normal": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
 }
@@ -343,6 +348,50 @@ class Context:
                                       occ_index, scf_iter, start_iter, maxit_lanczos, seed,
                                       convention, precision, _dptr(evals), ctypes.byref(state)))
         return state
+
+    def assemble_hloc(self, X, DX, Y, DY, Z, DZ, PX, PY, PZ, G, truncation):
+        """H_loc rows as (rowptr, col, val) numpy arrays, assembled on the device
+        (dft/ProjectHamiltonianSparse.cc:330-575).  X..DZ are r x r with [I][J]
+        indexing, PX is (rvx, rx, rx) = MatPotX[p][I][J], G has shape (rvx, rvy, rvz)."""
+        rx, ry, rz = X.shape[0], Y.shape[0], Z.shape[0]
+        rvx, rvy, rvz = PX.shape[0], PY.shape[0], PZ.shape[0]
+        mats = [np.ascontiguousarray(m, dtype=np.float64)
+                for m in (X, DX, Y, DY, Z, DZ, PX, PY, PZ)]
+        g = np.ascontiguousarray(np.asarray(G, dtype=np.float64).reshape(rvx, rvy, rvz)
+                                 .transpose(2, 1, 0)).reshape(-1)  # first index fastest
+        args = [self.h, rx, ry, rz, rvx, rvy, rvz] + [_dptr(m) for m in mats] + [
+            _dptr(g), float(truncation)]
+        nnz = ctypes.c_int64()
+        _check(lib().ttdft_assemble_hloc(*args, ctypes.byref(nnz), None, None, None))
+        n = rx * ry * rz
+        rowptr = np.empty(n + 1, np.int64)
+        col = np.empty(max(nnz.value, 1), np.int32)
+        val = np.empty(max(nnz.value, 1), np.float64)
+        _check(lib().ttdft_assemble_hloc(*args, ctypes.byref(nnz), _i64ptr(rowptr),
+                                         _i32ptr(col), _dptr(val)))
+        return rowptr, col[: nnz.value], val[: nnz.value]
+
+    def assemble_projectors(self, cores, mx, my, mz, truncation):
+        """Projector rows (dft/ProjectHamiltonianSparse.cc:196-278) as (rowptr, col, val),
+        assembled on the device.  cores[i] has shape (r_i, r_i, r_i) indexed [a, b, c];
+        mx[i] is rx x r_i, my[i] ry x r_i, mz[i] rz x r_i."""
+        p = len(cores)
+        ranks = np.array([c.shape[0] for c in cores], np.int32)
+        rx, ry, rz = mx[0].shape[0], my[0].shape[0], mz[0].shape[0]
+        cc = np.concatenate([np.asarray(c, np.float64).transpose(2, 1, 0).reshape(-1)
+                             for c in cores])
+        fx, fy, fz = (np.concatenate([np.asarray(m, np.float64).T.reshape(-1) for m in ms])
+                      for ms in (mx, my, mz))  # column-major factors
+        args = [self.h, p, _i32ptr(ranks), rx, ry, rz, _dptr(cc), _dptr(fx), _dptr(fy),
+                _dptr(fz), float(truncation)]
+        nnz = ctypes.c_int64()
+        _check(lib().ttdft_assemble_projectors(*args, ctypes.byref(nnz), None, None, None))
+        rowptr = np.empty(p + 1, np.int64)
+        col = np.empty(max(nnz.value, 1), np.int32)
+        val = np.empty(max(nnz.value, 1), np.float64)
+        _check(lib().ttdft_assemble_projectors(*args, ctypes.byref(nnz), _i64ptr(rowptr),
+                                               _i32ptr(col), _dptr(val)))
+        return rowptr, col[: nnz.value], val[: nnz.value]
 
     def random_subspace(self, X, ldx, n, k, seed=20261202, dtype=F64):
         _check(lib().ttdft_random_subspace(self.h, X, ldx, n, k, seed, dtype))
