@@ -350,6 +350,22 @@ int ttdft_assemble_projectors(ttdft_ctx ctx, int32_t p, const int32_t *ranks, in
                               const double *my, const double *mz, double truncation,
                               int64_t *nnz, int64_t *rowptr, int32_t *col, double *val);
 
+/* Assemble H = H_loc + C^T D C directly into device memory: ttdft_assemble_hloc
+ * and ttdft_assemble_projectors (same inputs and truncation rules) feeding
+ * ttdft_ham_upload_tucker, without the host round trip.  H_loc stays in HBM
+ * as CSR (and as Kronecker band fragments when its band is narrow enough);
+ * D = diag(inv_c_lm) scales C as in ProjectHamiltonianSparse.cc:316-325.
+ * replaces Create_Hloc + Create_Cnloc (ProjectHamiltonianSparse.cc:83-575)
+ * followed by AX::setup_system (MultCuda.cu:105-371). */
+int ttdft_ham_assemble_tucker(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz, int32_t rvx,
+                              int32_t rvy, int32_t rvz, const double *X, const double *DX,
+                              const double *Y, const double *DY, const double *Z,
+                              const double *DZ, const double *PX, const double *PY,
+                              const double *PZ, const double *G, double truncation, int32_t p,
+                              const int32_t *ranks, const double *cores, const double *mx,
+                              const double *my, const double *mz, double c_truncation,
+                              const double *inv_c_lm, int copies, ttdft_ham *out);
+
 /* Host random streams (same generator the engine uses internally). */
 int ttdft_rng_uniform(uint64_t seed, int64_t count, double *out);
 int ttdft_rng_normal(uint64_t seed, int64_t count, double *out);

@@ -190,3 +190,50 @@ def test_assemble_then_apply(ctx):
     ctx.ham_apply(ham, dX.ptr, 16, dY.ptr, 16, 16, E.F64)
     ref = Hd @ X
     assert np.abs(dY.to_host() - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def _apply(ctx, ham, X, dtype):
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    k = X.shape[1]
+    dX = E.DeviceArray.from_host(ctx, X.astype(npdt))
+    dY = E.DeviceArray(ctx, X.shape, npdt)
+    ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, dtype)
+    return dY.to_host()
+
+
+DEVICE_CASES = [
+    # dims, Veff ranks, 1-D half-bandwidth, projector ranks, expected storage
+    ((20, 16, 12), (4, 3, 5), 3, [2, 3, 1, 4], 1),   # Kronecker band fragments
+    ((10, 9, 8), (3, 3, 3), 9, [2, 2], 0),           # full coupling: CSR
+    ((12, 11, 10), (2, 4, 3), 2, [], 1),             # no projectors
+]
+
+
+@pytest.mark.parametrize("dims,rv,band,ranks,fmt", DEVICE_CASES)
+@pytest.mark.parametrize("copies", [E.HAM_F64, E.HAM_F32, E.HAM_F64 | E.HAM_F32])
+def test_ham_assemble_tucker_matches_host_path(ctx, dims, rv, band, ranks, fmt, copies):
+    """The device-resident Hamiltonian stores exactly what the host path
+    (assemble to host CSR, then ttdft_ham_upload_tucker) stores: equal
+    storage choice and sizes, and bitwise-equal products."""
+    hl = hloc_inputs(21, dims, rv, band, sym=True)
+    proj = proj_inputs(22, dims, ranks) if ranks else None
+    p = len(ranks)
+    inv = np.linspace(-1.5, 2.0, p) if p else None
+    print(f"ham_assemble_tucker dims={dims} band={band} p={p} copies={copies}", flush=True)
+    hd = ctx.ham_assemble_tucker(hl, 1e-4, proj, 1e-6, inv, copies)
+    rp, c, v = ctx.assemble_hloc(*hl, 1e-4)
+    n = int(np.prod(dims))
+    if p:
+        crp, cc, cv = ctx.assemble_projectors(*proj, 1e-6)
+    else:
+        crp, cc, cv = None, None, None
+    hh = E.Hamiltonian(ctx, n, rp, c, v, p, crp, cc, cv, inv, copies, dims=dims)
+    assert hd.format() == hh.format() and hd.format()[0] == fmt
+    assert hd.info() == hh.info()
+    X = np.random.default_rng(3).standard_normal((n, 64))
+    for dtype in (E.F64, E.F32):
+        if not copies & (E.HAM_F64 if dtype == E.F64 else E.HAM_F32):
+            continue
+        for k in (1, 20, 64):
+            Xk = np.ascontiguousarray(X[:, :k])
+            assert np.array_equal(_apply(ctx, hd, Xk, dtype), _apply(ctx, hh, Xk, dtype))

@@ -33,8 +33,6 @@ int launch_cm_to_bm(ttdft_ctx ctx, const double *src, int64_t lda, int64_t n, in
 int launch_bm_to_cm(ttdft_ctx ctx, const void *src, int64_t lds, int dtype, int64_t n,
                     int32_t k, double *dst, int64_t lda);
 int launch_lower_transpose(ttdft_ctx ctx, const double *L, double *LT, int32_t k);
-int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz,
-               const int64_t *rowptr, const int32_t *colind, const double *val, int copies);
 
 // ---- errors ----------------------------------------------------------------
 static thread_local std::string g_err;

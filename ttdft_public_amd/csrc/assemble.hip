@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -62,6 +63,12 @@ struct DevBuf {
     T *as() const {
         return static_cast<T *>(p);
     }
+    template <typename T>
+    T *release() {
+        T *q = static_cast<T *>(p);
+        p = nullptr;
+        return q;
+    }
 };
 
 int upload_to(ttdft_ctx ctx, DevBuf &b, const void *src, size_t bytes) {
@@ -94,12 +101,13 @@ __global__ __launch_bounds__(256) void hloc_compact_kernel(
     const double *__restrict__ DY, const double *__restrict__ Z, const double *__restrict__ DZ,
     double trunc, uint16_t *__restrict__ cnt, const int32_t *__restrict__ rel,
     const int64_t *__restrict__ rowptr, int32_t *__restrict__ col, double *__restrict__ val,
-    int32_t *__restrict__ mismatch) {
+    int32_t *__restrict__ flags) {
     const int64_t npair = (int64_t)rz * rz;
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= npair * ng) return;
-    const int pi = (int)(tid % npair);
-    const int64_t gl = tid / npair;
+    const bool valid = tid < npair * ng;
+    const int64_t t = valid ? tid : 0;  // lanes past the end mirror entry 0 and store nothing
+    const int pi = (int)(t % npair);
+    const int64_t gl = t / npair;
     const int zI = pi / rz, z = pi - zI * rz;  // Cc row index z + rz zI == pi
     const int64_t g = g0 + gl;
     const int xI = (int)(g % rx);
@@ -115,26 +123,43 @@ __global__ __launch_bounds__(256) void hloc_compact_kernel(
     int limit = 0;  // entries the count pass found for (row, y, z)
     if (WRITE) {
         base = rowptr[row] + rel[key];
-        limit = cnt[key];
+        limit = valid ? cnt[key] : 0;
     }
     const int32_t col0 = (int32_t)((int64_t)rx * (y + (int64_t)ry * z));
-    int kept = 0;
+    int kept = 0, dxmax = -1;
     for (int x = 0; x < rx; ++x) {
         // the reference's dyad: 0.5 (DX Y Z + X DY Z + X Y DZ), then + potential
         const double kin = dxr[x] * yv * zv + xr[x] * dyv * zv + xr[x] * yv * dzv;
         const double v = 0.5 * kin + cp[npair * x];
-        if (fabs(v) > trunc) {
+        if (valid && fabs(v) > trunc) {
             if (WRITE && kept < limit) {
                 col[base + kept] = col0 + x;
                 val[base + kept] = v;
             }
             ++kept;
+            dxmax = max(dxmax, abs(x - xI));
         }
     }
-    if (!WRITE) cnt[key] = (uint16_t)kept;
-    // both passes evaluate the same products, so the counts agree; a
-    // difference would leave a slot unwritten and is reported to the host
-    if (WRITE && kept != limit) *mismatch = 1;
+    if (!WRITE) {
+        if (valid) cnt[key] = (uint16_t)kept;
+        return;
+    }
+    // Both passes evaluate the same products, so the counts agree; a
+    // difference would leave a slot unwritten and is reported to the host.
+    if (kept != limit) flags[0] = 1;
+    // Kronecker half-bandwidths of the kept entries: wavefront maxima, then
+    // one atomic per wavefront.
+    int e0 = dxmax, e1 = kept ? abs(y - yI) : -1, e2 = kept ? abs(z - zI) : -1;
+    for (int o = 32; o > 0; o >>= 1) {
+        e0 = max(e0, __shfl_xor(e0, o));
+        e1 = max(e1, __shfl_xor(e1, o));
+        e2 = max(e2, __shfl_xor(e2, o));
+    }
+    if ((threadIdx.x & 63) == 0 && e0 >= 0) {
+        atomicMax(flags + 1, e0);
+        atomicMax(flags + 2, e1);
+        atomicMax(flags + 3, e2);
+    }
 }
 
 // Per row: exclusive scan of cnt[row][y][z] in the CSR order (z outer, y
@@ -241,67 +266,117 @@ __global__ __launch_bounds__(256) void dense_compact_kernel(
     if (!WRITE && threadIdx.x == 0) cnt[(int64_t)i * nchunk + chunk] = total;
 }
 
-}  // namespace
-}  // namespace ttdft
+// Row scaling of the projector matrix: D C with D = diag(inv_c_lm) (block per row).
+__global__ __launch_bounds__(256) void scale_rows_kernel(const int64_t *__restrict__ rowptr,
+                                                         const double *__restrict__ val,
+                                                         const double *__restrict__ d,
+                                                         double *__restrict__ out64,
+                                                         float *__restrict__ out32) {
+    const int r = blockIdx.x;
+    const double s = d[r];
+    for (int64_t e = rowptr[r] + threadIdx.x; e < rowptr[r + 1]; e += blockDim.x) {
+        const double v = val[e] * s;
+        if (out64) out64[e] = v;
+        if (out32) out32[e] = (float)v;
+    }
+}
 
-using namespace ttdft;
+// position of column j in the sorted columns [lo, hi), or -1
+__device__ inline int64_t find_col(const int32_t *__restrict__ col, int64_t lo, int64_t hi,
+                                   int32_t j) {
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        const int32_t c = col[mid];
+        if (c == j) return mid;
+        if (c < j)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return -1;
+}
 
-extern "C" {
+// C^T rows (one thread per basis function j): the projectors whose row of C
+// holds column j, in ascending order, found by binary search of each row.
+// FILL = 0 counts (ct_len[j]), 1 writes at ct_rowptr[j].
+template <int FILL>
+__global__ __launch_bounds__(256) void transpose_rows_kernel(
+    const int64_t *__restrict__ c_rowptr, const int32_t *__restrict__ c_col,
+    const double *__restrict__ c_val, int32_t p, int64_t n, int64_t *__restrict__ ct_len,
+    const int64_t *__restrict__ ct_rowptr, int32_t *__restrict__ ct_col,
+    double *__restrict__ ct_val64, float *__restrict__ ct_val32) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int64_t o = FILL ? ct_rowptr[j] : 0;
+    for (int32_t r = 0; r < p; ++r) {
+        const int64_t e = find_col(c_col, c_rowptr[r], c_rowptr[r + 1], (int32_t)j);
+        if (e < 0) continue;
+        if (FILL) {
+            ct_col[o] = r;
+            if (ct_val64) ct_val64[o] = c_val[e];
+            if (ct_val32) ct_val32[o] = (float)c_val[e];
+        }
+        ++o;
+    }
+    if (!FILL) ct_len[j] = o;
+}
 
-int ttdft_assemble_hloc(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz, int32_t rvx,
-                        int32_t rvy, int32_t rvz, const double *X, const double *DX,
-                        const double *Y, const double *DY, const double *Z, const double *DZ,
-                        const double *PX, const double *PY, const double *PZ, const double *G,
-                        double truncation, int64_t *nnz, int64_t *rowptr, int32_t *col,
-                        double *val) {
-    clear_error();
-    TT_CHECK(ctx, TTDFT_EBADARG, "ctx is NULL");
-    TT_CHECK(rx > 0 && ry > 0 && rz > 0 && rvx > 0 && rvy > 0 && rvz > 0, TTDFT_EBADARG,
-             "assemble_hloc: ranks must be > 0");
-    TT_CHECK(X && DX && Y && DY && Z && DZ && PX && PY && PZ && G && nnz, TTDFT_EBADARG,
-             "assemble_hloc: NULL input");
-    TT_CHECK(truncation >= 0.0, TTDFT_EBADARG, "assemble_hloc: truncation < 0");
-    TT_CHECK(rx <= 65535 && (int64_t)rx * ry * rz < ((int64_t)1 << 31), TTDFT_EBADARG,
-             "assemble_hloc: rx = %d or n too large", rx);
-    TT_CHECK((int64_t)rx * rx * ry * ry < ((int64_t)1 << 31), TTDFT_EBADARG,
+__global__ __launch_bounds__(256) void to_f32_kernel(const double *__restrict__ a,
+                                                     float *__restrict__ b, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) b[i] = (float)a[i];
+}
+
+// ---- device pipelines -------------------------------------------------------
+
+struct HlocIn {
+    int32_t rx, ry, rz, rvx, rvy, rvz;
+    const double *X, *DX, *Y, *DY, *Z, *DZ, *PX, *PY, *PZ, *G;
+    double trunc;
+};
+
+// CSR on the device, with a host copy of its row pointer
+struct DevCsr {
+    DevBuf rowptr, col, val;
+    std::vector<int64_t> rp;
+    int64_t nnz = 0;
+};
+
+int check_hloc(const HlocIn &a) {
+    TT_CHECK(a.rx > 0 && a.ry > 0 && a.rz > 0 && a.rvx > 0 && a.rvy > 0 && a.rvz > 0,
+             TTDFT_EBADARG, "assemble_hloc: ranks must be > 0");
+    TT_CHECK(a.X && a.DX && a.Y && a.DY && a.Z && a.DZ && a.PX && a.PY && a.PZ && a.G,
+             TTDFT_EBADARG, "assemble_hloc: NULL input");
+    TT_CHECK(a.trunc >= 0.0, TTDFT_EBADARG, "assemble_hloc: truncation < 0");
+    TT_CHECK(a.rx <= 65535 && (int64_t)a.rx * a.ry * a.rz < ((int64_t)1 << 31), TTDFT_EBADARG,
+             "assemble_hloc: rx = %d or n too large", a.rx);
+    TT_CHECK((int64_t)a.rx * a.rx * a.ry * a.ry < ((int64_t)1 << 31), TTDFT_EBADARG,
              "assemble_hloc: rx^2 ry^2 exceeds the 32-bit GEMM dimension");
+    TT_CHECK(sizeof(int32_t) * ((size_t)a.ry * a.rz + 256) <= 64 * 1024, TTDFT_EBADARG,
+             "assemble_hloc: ry*rz = %d too large", a.ry * a.rz);
+    return TTDFT_OK;
+}
+
+// H_loc rows.  Always: the row pointer (out.rp, out.nnz).  fill: the device
+// CSR (out.rowptr / col / val) and the Kronecker half-bandwidths of the kept
+// entries (ext, -1 when there are none).
+int hloc_run(ttdft_ctx ctx, const HlocIn &a, bool fill, DevCsr &out, int ext[3]) {
+    const int32_t rx = a.rx, ry = a.ry, rz = a.rz, rvx = a.rvx, rvy = a.rvy, rvz = a.rvz;
     const int64_t n = (int64_t)rx * ry * rz;
     const int64_t rx2 = (int64_t)rx * rx, ry2 = (int64_t)ry * ry, rz2 = (int64_t)rz * rz;
-    const int64_t ngroups = rx * ry2;  // g = xI + rx (y + ry yI)
+    const int64_t ngroups = rx * ry2;   // g = xI + rx (y + ry yI)
     const int64_t ntup = rx * ngroups;  // rx^2 ry^2
-    // inputs
     DevBuf dX, dDX, dY, dDY, dZ, dDZ, dPX, dPY, dPZ, dG;
-    TT_TRY(upload_to(ctx, dX, X, sizeof(double) * rx2));
-    TT_TRY(upload_to(ctx, dDX, DX, sizeof(double) * rx2));
-    TT_TRY(upload_to(ctx, dY, Y, sizeof(double) * ry2));
-    TT_TRY(upload_to(ctx, dDY, DY, sizeof(double) * ry2));
-    TT_TRY(upload_to(ctx, dZ, Z, sizeof(double) * rz2));
-    TT_TRY(upload_to(ctx, dDZ, DZ, sizeof(double) * rz2));
-    TT_TRY(upload_to(ctx, dPX, PX, sizeof(double) * rvx * rx2));
-    TT_TRY(upload_to(ctx, dPY, PY, sizeof(double) * rvy * ry2));
-    TT_TRY(upload_to(ctx, dPZ, PZ, sizeof(double) * rvz * rz2));
-    TT_TRY(upload_to(ctx, dG, G, sizeof(double) * rvx * rvy * rvz));
-    // A = PX G, B_r = A_r PY^T
-    DevBuf dA, dB;
-    TT_TRY(dA.alloc(sizeof(double) * rx2 * rvy * rvz));
-    TT_TRY(dB.alloc(sizeof(double) * rvz * ntup));
-    const double one = 1.0, zero = 0.0;
-    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none,
-                          (rocblas_int)rx2, rvy * rvz, rvx, &one, dPX.as<double>(),
-                          (rocblas_int)rx2, dG.as<double>(), rvx, &zero, dA.as<double>(),
-                          (rocblas_int)rx2));
-    TT_BLAS(rocblas_dgemm_strided_batched(
-        ctx->blas, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)rx2,
-        (rocblas_int)ry2, rvy, &one, dA.as<double>(), (rocblas_int)rx2, rx2 * rvy,
-        dPY.as<double>(), (rocblas_int)ry2, 0, &zero, dB.as<double>(), (rocblas_int)rx2, ntup,
-        rvz));
-    // chunks of groups: Cc holds rz^2 x (rx * gc) doubles (about 1 GB)
-    int64_t gc = ((int64_t)1 << 27) / std::max<int64_t>(1, rz2 * rx);
-    gc = std::max<int64_t>(1, std::min<int64_t>(gc, ngroups));
-    gc = env_cap("TTDFT_ASSEMBLE_GROUPS", gc);  // groups (xI, y, yI) per chunk
-    DevBuf dC, dcnt, drel, drowlen, dflag;
-    TT_TRY(dflag.alloc(sizeof(int32_t)));
-    TT_HIP(hipMemsetAsync(dflag.p, 0, sizeof(int32_t), ctx->stream));
+    TT_TRY(upload_to(ctx, dX, a.X, sizeof(double) * rx2));
+    TT_TRY(upload_to(ctx, dDX, a.DX, sizeof(double) * rx2));
+    TT_TRY(upload_to(ctx, dY, a.Y, sizeof(double) * ry2));
+    TT_TRY(upload_to(ctx, dDY, a.DY, sizeof(double) * ry2));
+    TT_TRY(upload_to(ctx, dZ, a.Z, sizeof(double) * rz2));
+    TT_TRY(upload_to(ctx, dDZ, a.DZ, sizeof(double) * rz2));
+    TT_TRY(upload_to(ctx, dPX, a.PX, sizeof(double) * rvx * rx2));
+    TT_TRY(upload_to(ctx, dPY, a.PY, sizeof(double) * rvy * ry2));
+    TT_TRY(upload_to(ctx, dPZ, a.PZ, sizeof(double) * rvz * rz2));
+    TT_TRY(upload_to(ctx, dG, a.G, sizeof(double) * rvx * rvy * rvz));
     // deterministic products: the count and write passes must see equal values
     rocblas_atomics_mode atomics = rocblas_atomics_not_allowed;
     TT_BLAS(rocblas_get_atomics_mode(ctx->blas, &atomics));
@@ -311,6 +386,19 @@ int ttdft_assemble_hloc(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz, int32
         rocblas_atomics_mode m;
         ~AtomicsRestore() { rocblas_set_atomics_mode(h, m); }
     } restore{ctx->blas, atomics};
+    // A = PX G, B_r = A_r PY^T
+    DevBuf dA, dB;
+    TT_TRY(dA.alloc(sizeof(double) * rx2 * rvy * rvz));
+    TT_TRY(dB.alloc(sizeof(double) * rvz * ntup));
+    const double one = 1.0, zero = 0.0;
+    // chunks of groups: Cc holds rz^2 x (rx * gc) doubles (about 1 GB)
+    int64_t gc = ((int64_t)1 << 27) / std::max<int64_t>(1, rz2 * rx);
+    gc = std::max<int64_t>(1, std::min<int64_t>(gc, ngroups));
+    gc = env_cap("TTDFT_ASSEMBLE_GROUPS", gc);  // groups (xI, y, yI) per chunk
+    DevBuf dC, dcnt, drel, drowlen, dflag;
+    TT_TRY(dflag.alloc(sizeof(int32_t) * 4));
+    const int32_t flag0[4] = {0, -1, -1, -1};  // mismatch, half-bandwidths x, y, z
+    TT_HIP(hipMemcpyAsync(dflag.p, flag0, sizeof(flag0), hipMemcpyHostToDevice, ctx->stream));
     TT_TRY(dC.alloc(sizeof(double) * rz2 * rx * gc));
     TT_TRY(dcnt.alloc(sizeof(uint16_t) * n * ry * rz));
     TT_TRY(drel.alloc(sizeof(int32_t) * n * ry * rz));
@@ -323,90 +411,82 @@ int ttdft_assemble_hloc(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz, int32
                                   dPZ.as<double>(), (rocblas_int)rz2, dB.as<double>() + rx * g0,
                                   (rocblas_int)ntup, &zero, dC.as<double>(), (rocblas_int)rz2));
             const unsigned nb = blocks_for(rz2 * ng, 256);
-            if (write)
-                hipLaunchKernelGGL(hloc_compact_kernel<1>, dim3(nb), dim3(256), 0, ctx->stream,
-                                   dC.as<double>(), g0, ng, rx, ry, rz, dX.as<double>(),
-                                   dDX.as<double>(), dY.as<double>(), dDY.as<double>(),
-                                   dZ.as<double>(), dDZ.as<double>(), truncation,
-                                   dcnt.as<uint16_t>(), drel.as<int32_t>(), drowptr, dcol, dval,
-                                   dflag.as<int32_t>());
-            else
-                hipLaunchKernelGGL(hloc_compact_kernel<0>, dim3(nb), dim3(256), 0, ctx->stream,
-                                   dC.as<double>(), g0, ng, rx, ry, rz, dX.as<double>(),
-                                   dDX.as<double>(), dY.as<double>(), dDY.as<double>(),
-                                   dZ.as<double>(), dDZ.as<double>(), truncation,
-                                   dcnt.as<uint16_t>(), drel.as<int32_t>(), drowptr, dcol, dval,
-                                   dflag.as<int32_t>());
+            auto kern = write ? hloc_compact_kernel<1> : hloc_compact_kernel<0>;
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, ctx->stream, dC.as<double>(), g0, ng,
+                               rx, ry, rz, dX.as<double>(), dDX.as<double>(), dY.as<double>(),
+                               dDY.as<double>(), dZ.as<double>(), dDZ.as<double>(), a.trunc,
+                               dcnt.as<uint16_t>(), drel.as<int32_t>(), drowptr, dcol, dval,
+                               dflag.as<int32_t>());
             TT_LAUNCH_CHECK();
         }
         return TTDFT_OK;
     };
     // algorithmic work of one pass: the three products and the compaction
-    // (Veff contraction read once per entry, kinetic term 5 flop per entry)
+    // (kinetic term and sum, 7 flop per entry)
     const double n2 = (double)n * (double)n;
     const double flops = 2.0 * rx2 * rvx * rvy * rvz + 2.0 * ntup * rvy * rvz +
                          2.0 * n2 * rvz + 7.0 * n2;
     PendingTiming tm;
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ASSEMBLY, &tm));
+    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none,
+                          (rocblas_int)rx2, rvy * rvz, rvx, &one, dPX.as<double>(),
+                          (rocblas_int)rx2, dG.as<double>(), rvx, &zero, dA.as<double>(),
+                          (rocblas_int)rx2));
+    TT_BLAS(rocblas_dgemm_strided_batched(
+        ctx->blas, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)rx2,
+        (rocblas_int)ry2, rvy, &one, dA.as<double>(), (rocblas_int)rx2, rx2 * rvy,
+        dPY.as<double>(), (rocblas_int)ry2, 0, &zero, dB.as<double>(), (rocblas_int)rx2, ntup,
+        rvz));
     TT_TRY(run_pass(0, nullptr, nullptr, nullptr));
     const size_t lds = sizeof(int32_t) * ((size_t)ry * rz + 256);
-    TT_CHECK(lds <= 64 * 1024, TTDFT_EBADARG, "assemble_hloc: ry*rz = %d too large", ry * rz);
     hipLaunchKernelGGL(hloc_row_scan_kernel, dim3((unsigned)n), dim3(256), lds, ctx->stream,
                        dcnt.as<uint16_t>(), ry, rz, drel.as<int32_t>(), drowlen.as<int64_t>());
     TT_LAUNCH_CHECK();
-    TT_TRY(timing_end(ctx, &tm, 2.0 * n * ry * rz + 4.0 * n * ry * rz + 8.0 * n, flops));
-    std::vector<int64_t> len(n), rp(n + 1);
+    TT_TRY(timing_end(ctx, &tm, 14.0 * n * ry * rz, flops));
+    std::vector<int64_t> len(n);
+    out.rp.assign(n + 1, 0);
     TT_HIP(hipMemcpyAsync(len.data(), drowlen.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
                           ctx->stream));
     TT_HIP(hipStreamSynchronize(ctx->stream));
-    rp[0] = 0;
-    for (int64_t i = 0; i < n; ++i) rp[i + 1] = rp[i] + len[i];
-    *nnz = rp[n];
-    if (!rowptr) return TTDFT_OK;
-    std::copy(rp.begin(), rp.end(), rowptr);
-    if (!col || !val) return TTDFT_OK;
-    DevBuf drp, dcol, dval;
-    TT_TRY(upload_to(ctx, drp, rp.data(), sizeof(int64_t) * (n + 1)));
-    TT_TRY(dcol.alloc(sizeof(int32_t) * rp[n]));
-    TT_TRY(dval.alloc(sizeof(double) * rp[n]));
+    for (int64_t i = 0; i < n; ++i) out.rp[i + 1] = out.rp[i] + len[i];
+    out.nnz = out.rp[n];
+    if (!fill) return TTDFT_OK;
+    TT_TRY(upload_to(ctx, out.rowptr, out.rp.data(), sizeof(int64_t) * (n + 1)));
+    TT_TRY(out.col.alloc(sizeof(int32_t) * out.nnz));
+    TT_TRY(out.val.alloc(sizeof(double) * out.nnz));
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ASSEMBLY, &tm));
-    TT_TRY(run_pass(1, drp.as<int64_t>(), dcol.as<int32_t>(), dval.as<double>()));
-    TT_TRY(timing_end(ctx, &tm, 12.0 * rp[n] + 8.0 * (n + 1), flops));
-    int32_t mismatch = 0;
-    TT_HIP(hipMemcpyAsync(&mismatch, dflag.p, sizeof(int32_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    TT_HIP(hipMemcpyAsync(col, dcol.p, sizeof(int32_t) * rp[n], hipMemcpyDeviceToHost,
-                          ctx->stream));
-    TT_HIP(hipMemcpyAsync(val, dval.p, sizeof(double) * rp[n], hipMemcpyDeviceToHost,
-                          ctx->stream));
+    TT_TRY(run_pass(1, out.rowptr.as<int64_t>(), out.col.as<int32_t>(), out.val.as<double>()));
+    TT_TRY(timing_end(ctx, &tm, 12.0 * out.nnz + 8.0 * (n + 1), flops));
+    int32_t flags[4];
+    TT_HIP(hipMemcpyAsync(flags, dflag.p, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
     TT_HIP(hipStreamSynchronize(ctx->stream));
-    TT_CHECK(!mismatch, TTDFT_ELIB, "assemble_hloc: count and write passes disagree");
+    TT_CHECK(!flags[0], TTDFT_ELIB, "assemble_hloc: count and write passes disagree");
+    for (int d = 0; d < 3; ++d) ext[d] = flags[1 + d];
     return TTDFT_OK;
 }
 
-int ttdft_assemble_projectors(ttdft_ctx ctx, int32_t p, const int32_t *ranks, int32_t rx,
-                              int32_t ry, int32_t rz, const double *cores, const double *mx,
-                              const double *my, const double *mz, double truncation,
-                              int64_t *nnz, int64_t *rowptr, int32_t *col, double *val) {
-    clear_error();
-    TT_CHECK(ctx, TTDFT_EBADARG, "ctx is NULL");
-    TT_CHECK(p >= 0 && rx > 0 && ry > 0 && rz > 0 && nnz, TTDFT_EBADARG,
+int check_proj(int32_t p, const int32_t *ranks, int32_t rx, int32_t ry, int32_t rz,
+               const double *cores, const double *mx, const double *my, const double *mz,
+               double truncation) {
+    TT_CHECK(p >= 0 && rx > 0 && ry > 0 && rz > 0, TTDFT_EBADARG,
              "assemble_projectors: bad sizes");
     TT_CHECK(p == 0 || (ranks && cores && mx && my && mz), TTDFT_EBADARG,
              "assemble_projectors: NULL input");
     TT_CHECK(truncation >= 0.0, TTDFT_EBADARG, "assemble_projectors: truncation < 0");
-    const int64_t n = (int64_t)rx * ry * rz, nxy = (int64_t)rx * ry;
-    TT_CHECK(n < ((int64_t)1 << 31), TTDFT_EBADARG, "assemble_projectors: n too large");
-    *nnz = 0;
-    if (p == 0) {
-        if (rowptr) rowptr[0] = 0;
-        return TTDFT_OK;
-    }
-    int rmax = 0;
-    for (int i = 0; i < p; ++i) {
+    TT_CHECK((int64_t)rx * ry * rz < ((int64_t)1 << 31), TTDFT_EBADARG,
+             "assemble_projectors: n too large");
+    for (int i = 0; i < p; ++i)
         TT_CHECK(ranks[i] > 0, TTDFT_EBADARG, "assemble_projectors: rank of row %d < 1", i);
-        rmax = std::max(rmax, ranks[i]);
-    }
+    return TTDFT_OK;
+}
+
+// Projector rows (p > 0): row pointer always, device CSR when fill.
+int proj_run(ttdft_ctx ctx, int32_t p, const int32_t *ranks, int32_t rx, int32_t ry, int32_t rz,
+             const double *cores, const double *mx, const double *my, const double *mz,
+             double truncation, bool fill, DevCsr &out) {
+    const int64_t n = (int64_t)rx * ry * rz, nxy = (int64_t)rx * ry;
+    int rmax = 0;
+    for (int i = 0; i < p; ++i) rmax = std::max(rmax, ranks[i]);
     // host: W_i[c][x + rx y] = sum_{a,b} S_i[a,b,c] MX_i[x,a] MY_i[y,b] (small), and MZ
     // padded to rmax terms; zero terms past a row's rank
     std::vector<double> W((size_t)p * rmax * nxy, 0.0), MZ((size_t)p * rmax * rz, 0.0);
@@ -478,29 +558,206 @@ int ttdft_assemble_projectors(ttdft_ctx ctx, int32_t p, const int32_t *ranks, in
     TT_HIP(hipMemcpyAsync(cnt.data(), dcnt.p, sizeof(int32_t) * cnt.size(),
                           hipMemcpyDeviceToHost, ctx->stream));
     TT_HIP(hipStreamSynchronize(ctx->stream));
-    std::vector<int64_t> start(cnt.size()), rp(p + 1, 0);
+    std::vector<int64_t> start(cnt.size());
+    out.rp.assign(p + 1, 0);
     int64_t acc = 0;
     for (int i = 0; i < p; ++i) {
         for (int c = 0; c < nchunk; ++c) {
             start[(size_t)i * nchunk + c] = acc;
             acc += cnt[(size_t)i * nchunk + c];
         }
-        rp[i + 1] = acc;
+        out.rp[i + 1] = acc;
     }
-    *nnz = acc;
-    if (!rowptr) return TTDFT_OK;
-    std::copy(rp.begin(), rp.end(), rowptr);
-    if (!col || !val) return TTDFT_OK;
-    DevBuf dcol, dval;
+    out.nnz = acc;
+    if (!fill) return TTDFT_OK;
     TT_HIP(hipMemcpyAsync(dstart.p, start.data(), sizeof(int64_t) * start.size(),
                           hipMemcpyHostToDevice, ctx->stream));
-    TT_TRY(dcol.alloc(sizeof(int32_t) * acc));
-    TT_TRY(dval.alloc(sizeof(double) * acc));
-    TT_TRY(run_pass(1, dcol.as<int32_t>(), dval.as<double>()));
-    TT_HIP(hipMemcpyAsync(col, dcol.p, sizeof(int32_t) * acc, hipMemcpyDeviceToHost,
-                          ctx->stream));
-    TT_HIP(hipMemcpyAsync(val, dval.p, sizeof(double) * acc, hipMemcpyDeviceToHost, ctx->stream));
+    TT_TRY(upload_to(ctx, out.rowptr, out.rp.data(), sizeof(int64_t) * (p + 1)));
+    TT_TRY(out.col.alloc(sizeof(int32_t) * acc));
+    TT_TRY(out.val.alloc(sizeof(double) * acc));
+    TT_TRY(run_pass(1, out.col.as<int32_t>(), out.val.as<double>()));
+    TT_HIP(hipStreamSynchronize(ctx->stream));  // start[] is read by the copy above
+    return TTDFT_OK;
+}
+
+int download_csr(ttdft_ctx ctx, const DevCsr &c, int64_t rows, int64_t *rowptr, int32_t *col,
+                 double *val) {
+    std::copy(c.rp.begin(), c.rp.begin() + rows + 1, rowptr);
+    if (c.nnz) {
+        TT_HIP(hipMemcpyAsync(col, c.col.p, sizeof(int32_t) * c.nnz, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        TT_HIP(hipMemcpyAsync(val, c.val.p, sizeof(double) * c.nnz, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
     TT_HIP(hipStreamSynchronize(ctx->stream));
+    return TTDFT_OK;
+}
+
+}  // namespace
+}  // namespace ttdft
+
+using namespace ttdft;
+
+extern "C" {
+
+int ttdft_assemble_hloc(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz, int32_t rvx,
+                        int32_t rvy, int32_t rvz, const double *X, const double *DX,
+                        const double *Y, const double *DY, const double *Z, const double *DZ,
+                        const double *PX, const double *PY, const double *PZ, const double *G,
+                        double truncation, int64_t *nnz, int64_t *rowptr, int32_t *col,
+                        double *val) {
+    clear_error();
+    TT_CHECK(ctx, TTDFT_EBADARG, "ctx is NULL");
+    TT_CHECK(nnz, TTDFT_EBADARG, "assemble_hloc: nnz is NULL");
+    const HlocIn a{rx, ry, rz, rvx, rvy, rvz, X, DX, Y, DY, Z, DZ, PX, PY, PZ, G, truncation};
+    TT_TRY(check_hloc(a));
+    TT_HIP(hipSetDevice(ctx->device));
+    const bool fill = rowptr && col && val;
+    DevCsr h;
+    int ext[3];
+    TT_TRY(hloc_run(ctx, a, fill, h, ext));
+    *nnz = h.nnz;
+    if (!rowptr) return TTDFT_OK;
+    if (!fill) {
+        std::copy(h.rp.begin(), h.rp.end(), rowptr);
+        return TTDFT_OK;
+    }
+    return download_csr(ctx, h, (int64_t)rx * ry * rz, rowptr, col, val);
+}
+
+int ttdft_assemble_projectors(ttdft_ctx ctx, int32_t p, const int32_t *ranks, int32_t rx,
+                              int32_t ry, int32_t rz, const double *cores, const double *mx,
+                              const double *my, const double *mz, double truncation,
+                              int64_t *nnz, int64_t *rowptr, int32_t *col, double *val) {
+    clear_error();
+    TT_CHECK(ctx, TTDFT_EBADARG, "ctx is NULL");
+    TT_CHECK(nnz, TTDFT_EBADARG, "assemble_projectors: nnz is NULL");
+    TT_TRY(check_proj(p, ranks, rx, ry, rz, cores, mx, my, mz, truncation));
+    *nnz = 0;
+    if (p == 0) {
+        if (rowptr) rowptr[0] = 0;
+        return TTDFT_OK;
+    }
+    TT_HIP(hipSetDevice(ctx->device));
+    const bool fill = rowptr && col && val;
+    DevCsr c;
+    TT_TRY(proj_run(ctx, p, ranks, rx, ry, rz, cores, mx, my, mz, truncation, fill, c));
+    *nnz = c.nnz;
+    if (!rowptr) return TTDFT_OK;
+    if (!fill) {
+        std::copy(c.rp.begin(), c.rp.end(), rowptr);
+        return TTDFT_OK;
+    }
+    return download_csr(ctx, c, p, rowptr, col, val);
+}
+
+static int ham_assemble(ttdft_ctx ctx, const HlocIn &a, int32_t p, const int32_t *ranks,
+                        const double *cores, const double *mx, const double *my,
+                        const double *mz, double c_truncation, const double *inv_c_lm,
+                        int copies, ttdft_ham_s *H) {
+    const int64_t n = (int64_t)a.rx * a.ry * a.rz;
+    DevCsr h;
+    int ext[3];
+    TT_TRY(hloc_run(ctx, a, true, h, ext));
+    H->n = n;
+    H->nnz = h.nnz;
+    H->copies = copies;
+    for (int64_t i = 0; i < n; ++i)
+        H->max_row_nnz = std::max<int32_t>(H->max_row_nnz, (int32_t)(h.rp[i + 1] - h.rp[i]));
+    H->rowptr = h.rowptr.release<int64_t>();
+    H->col = h.col.release<int32_t>();
+    H->val64 = h.val.release<double>();
+    if (copies & TTDFT_HAM_F32) {
+        TT_HIP(hipMalloc(&H->val32, sizeof(float) * std::max<int64_t>(1, h.nnz)));
+        if (h.nnz) {
+            hipLaunchKernelGGL(to_f32_kernel, dim3(blocks_for(h.nnz, 256)), dim3(256), 0,
+                               ctx->stream, H->val64, H->val32, h.nnz);
+            TT_LAUNCH_CHECK();
+        }
+    }
+    if (ext[0] >= 0) TT_TRY(build_band_device(ctx, H, a.rx, a.ry, a.rz, ext[0], ext[1], ext[2],
+                                              copies));
+    if (!(copies & TTDFT_HAM_F64)) {
+        TT_HIP(hipStreamSynchronize(ctx->stream));
+        TT_HIP(hipFree(H->val64));
+        H->val64 = nullptr;
+    }
+    H->p = p;
+    if (p == 0) return TTDFT_OK;
+    DevCsr c;
+    TT_TRY(proj_run(ctx, p, ranks, a.rx, a.ry, a.rz, cores, mx, my, mz, c_truncation, true, c));
+    H->nnz_c = c.nnz;
+    DevBuf dinv, ctlen;
+    TT_TRY(upload_to(ctx, dinv, inv_c_lm, sizeof(double) * p));
+    const size_t nc = (size_t)std::max<int64_t>(1, c.nnz);
+    if (copies & TTDFT_HAM_F64) {
+        TT_HIP(hipMalloc(&H->c_val64, sizeof(double) * nc));
+        TT_HIP(hipMalloc(&H->ct_val64, sizeof(double) * nc));
+    }
+    if (copies & TTDFT_HAM_F32) {
+        TT_HIP(hipMalloc(&H->c_val32, sizeof(float) * nc));
+        TT_HIP(hipMalloc(&H->ct_val32, sizeof(float) * nc));
+    }
+    TT_HIP(hipMalloc(&H->ct_col, sizeof(int32_t) * nc));
+    // D C (row-scaled) and C^T (unscaled), ProjectHamiltonianSparse.cc:316-325
+    hipLaunchKernelGGL(scale_rows_kernel, dim3((unsigned)p), dim3(256), 0, ctx->stream,
+                       c.rowptr.as<int64_t>(), c.val.as<double>(), dinv.as<double>(),
+                       H->c_val64, H->c_val32);
+    TT_LAUNCH_CHECK();
+    TT_TRY(ctlen.alloc(sizeof(int64_t) * n));
+    const unsigned nbj = blocks_for(n, 256);
+    hipLaunchKernelGGL(transpose_rows_kernel<0>, dim3(nbj), dim3(256), 0, ctx->stream,
+                       c.rowptr.as<int64_t>(), c.col.as<int32_t>(), c.val.as<double>(), p, n,
+                       ctlen.as<int64_t>(), nullptr, nullptr, nullptr, nullptr);
+    TT_LAUNCH_CHECK();
+    std::vector<int64_t> ctp(n + 1, 0);
+    TT_HIP(hipMemcpyAsync(ctp.data() + 1, ctlen.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t j = 0; j < n; ++j) ctp[j + 1] += ctp[j];
+    TT_CHECK(ctp[n] == c.nnz, TTDFT_ELIB, "ham_assemble: projector transpose lost entries");
+    TT_HIP(hipMalloc(&H->ct_rowptr, sizeof(int64_t) * (n + 1)));
+    TT_HIP(hipMemcpyAsync(H->ct_rowptr, ctp.data(), sizeof(int64_t) * (n + 1),
+                          hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(transpose_rows_kernel<1>, dim3(nbj), dim3(256), 0, ctx->stream,
+                       c.rowptr.as<int64_t>(), c.col.as<int32_t>(), c.val.as<double>(), p, n,
+                       nullptr, H->ct_rowptr, H->ct_col, H->ct_val64, H->ct_val32);
+    TT_LAUNCH_CHECK();
+    TT_HIP(hipStreamSynchronize(ctx->stream));  // ctp is read by the copy above
+    H->c_rowptr = c.rowptr.release<int64_t>();
+    H->c_col = c.col.release<int32_t>();
+    return TTDFT_OK;
+}
+
+int ttdft_ham_assemble_tucker(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz, int32_t rvx,
+                              int32_t rvy, int32_t rvz, const double *X, const double *DX,
+                              const double *Y, const double *DY, const double *Z,
+                              const double *DZ, const double *PX, const double *PY,
+                              const double *PZ, const double *G, double truncation, int32_t p,
+                              const int32_t *ranks, const double *cores, const double *mx,
+                              const double *my, const double *mz, double c_truncation,
+                              const double *inv_c_lm, int copies, ttdft_ham *out) {
+    clear_error();
+    TT_CHECK(ctx && out, TTDFT_EBADARG, "ham_assemble_tucker: NULL ctx or out");
+    *out = nullptr;
+    TT_CHECK(copies & (TTDFT_HAM_F64 | TTDFT_HAM_F32), TTDFT_EBADARG,
+             "ham_assemble_tucker: no device copy requested");
+    const HlocIn a{rx, ry, rz, rvx, rvy, rvz, X, DX, Y, DY, Z, DZ, PX, PY, PZ, G, truncation};
+    TT_TRY(check_hloc(a));
+    TT_TRY(check_proj(p, ranks, rx, ry, rz, cores, mx, my, mz, c_truncation));
+    TT_CHECK(p == 0 || inv_c_lm, TTDFT_EBADARG, "ham_assemble_tucker: inv_c_lm is NULL");
+    TT_HIP(hipSetDevice(ctx->device));
+    ttdft_ham H = new ttdft_ham_s();
+    H->ctx = ctx;
+    const int rc = ham_assemble(ctx, a, p, ranks, cores, mx, my, mz, c_truncation, inv_c_lm,
+                                copies, H);
+    if (rc != TTDFT_OK) {
+        const std::string msg = ttdft_last_error();
+        ttdft_ham_free(H);
+        set_error("%s", msg.c_str());
+        return rc;
+    }
+    *out = H;
     return TTDFT_OK;
 }
 

@@ -510,6 +510,80 @@ int launch_band_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t
 // neighbouring line p = (y' - 4h + by) + (2by + 4)(z' - z + bz) and x' =
 // 4g - bx + 4ks + kk; brick row i = (x - 4g) + 4 (y - 4h).  Zero where H_loc
 // has no entry.
+namespace {
+
+// Band layout of H_loc for half-bandwidths (bx, by, bz); false: keep CSR.
+struct BandShape {
+    int ks, gxn, hyn, npy, np;
+    int64_t slots;  // A-fragment elements
+};
+bool band_shape(int32_t rx, int32_t ry, int32_t rz, int bx, int by, int bz, int64_t nnz,
+                BandShape *b) {
+    const int64_t n = (int64_t)rx * ry * rz;
+    const int64_t band_slots = n * (int64_t)(2 * bx + 1) * (2 * by + 1) * (2 * bz + 1);
+    const double fill = band_slots > 0 ? (double)nnz / (double)band_slots : 0.0;
+    if (bx < 1 || bx > 7 || fill < 0.15 || rx < 4) return false;
+    b->ks = (4 + 2 * bx + 3) / 4;
+    b->gxn = (rx + 3) / 4;
+    b->hyn = (ry + 3) / 4;
+    b->npy = 2 * by + 4;
+    b->np = b->npy * (2 * bz + 1);
+    b->slots = (int64_t)rz * b->hyn * b->gxn * b->np * b->ks * 64;
+    return true;
+}
+
+void set_band_fields(ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz, int bx, int by, int bz,
+                     const BandShape &b) {
+    const int nxc = (b.gxn + 15) / 16;
+    H->format = 1;
+    H->rx = rx;
+    H->ry = ry;
+    H->rz = rz;
+    H->bx = bx;
+    H->by = by;
+    H->bz = bz;
+    H->ks = b.ks;
+    H->gxn = b.gxn;
+    H->hyn = b.hyn;
+    H->nxc = nxc;
+    H->gxb = std::max(4, (b.gxn + nxc - 1) / nxc);
+}
+
+// Fragment slot of entry (row r, column j) of H_loc; the row lies in brick
+// (x>>2, y>>2, z), the column in line (y', z') at window offset c.
+__host__ __device__ inline int64_t band_slot(int64_t r, int64_t j, int32_t rx, int32_t ry, int bx,
+                                             int by, int bz, int ks, int gxn, int hyn, int npy,
+                                             int np) {
+    const int x = (int)(r % rx), y = (int)((r / rx) % ry), z = (int)(r / ((int64_t)rx * ry));
+    const int xj = (int)(j % rx), yj = (int)((j / rx) % ry), zj = (int)(j / ((int64_t)rx * ry));
+    const int g = x >> 2, h = y >> 2;
+    const int i16 = (x & 3) + 4 * (y & 3);
+    const int64_t base = (((int64_t)z * hyn + h) * gxn + g) * np;
+    const int p = (yj - 4 * h + by) + npy * (zj - z + bz);
+    const int c = xj - 4 * g + bx;  // in [0, 4 ks)
+    return ((base + p) * ks + (c >> 2)) * 64 + i16 + 16 * (c & 3);
+}
+
+// One wavefront per row of the device CSR; every (row, column) pair of the
+// band has its own slot, so the stores never collide.
+__global__ __launch_bounds__(256) void band_scatter_kernel(
+    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const double *__restrict__ val, int64_t n, int32_t rx, int32_t ry, int bx, int by, int bz,
+    int ks, int gxn, int hyn, int npy, int np, double *__restrict__ mf) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    for (int64_t e = rowptr[r] + (threadIdx.x & 63); e < rowptr[r + 1]; e += 64)
+        mf[band_slot(r, col[e], rx, ry, bx, by, bz, ks, gxn, hyn, npy, np)] = val[e];
+}
+
+__global__ __launch_bounds__(256) void f64_to_f32_kernel(const double *__restrict__ a,
+                                                         float *__restrict__ b, int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) b[i] = (float)a[i];
+}
+
+}  // namespace
+
 int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz,
                const int64_t *rowptr, const int32_t *colind, const double *val, int copies) {
     const int64_t n = (int64_t)rx * ry * rz;
@@ -543,36 +617,18 @@ int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz
     const int bx = *std::max_element(mx.begin(), mx.end());
     const int by = *std::max_element(my.begin(), my.end());
     const int bz = *std::max_element(mz.begin(), mz.end());
-    const int64_t band_slots = n * (int64_t)(2 * bx + 1) * (2 * by + 1) * (2 * bz + 1);
-    const double fill = band_slots > 0 ? (double)rowptr[n] / (double)band_slots : 0.0;
-    if (bx < 1 || bx > 7 || fill < 0.15 || rx < 4) return TTDFT_OK;  // keep CSR
-
-    const int KS = (4 + 2 * bx + 3) / 4;
-    const int gxn = (rx + 3) / 4, hyn = (ry + 3) / 4;
-    const int NPY = 2 * by + 4, NP = NPY * (2 * bz + 1);
-    const int64_t slots = (int64_t)rz * hyn * gxn * NP * KS * 64;
-    std::vector<double> mf((size_t)slots, 0.0);
+    BandShape b;
+    if (!band_shape(rx, ry, rz, bx, by, bz, rowptr[n], &b)) return TTDFT_OK;  // keep CSR
+    std::vector<double> mf((size_t)b.slots, 0.0);
     {
         // rows of one thread fill distinct brick rows: no two threads write a slot
         std::vector<std::thread> th;
         for (unsigned t = 0; t < nt; ++t)
             th.emplace_back([&, t]() {
-                for (int64_t r = n * t / nt; r < n * (t + 1) / nt; ++r) {
-                    const int x = (int)(r % rx), y = (int)((r / rx) % ry),
-                              z = (int)(r / ((int64_t)rx * ry));
-                    const int g = x >> 2, h = y >> 2;
-                    const int i16 = (x & 3) + 4 * (y & 3);
-                    const int64_t base = (((int64_t)z * hyn + h) * gxn + g) * NP;
-                    for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
-                        const int64_t j = colind[e];
-                        const int xj = (int)(j % rx), yj = (int)((j / rx) % ry),
-                                  zj = (int)(j / ((int64_t)rx * ry));
-                        const int p = (yj - 4 * h + by) + NPY * (zj - z + bz);
-                        const int c = xj - 4 * g + bx;  // in [0, 4 KS)
-                        mf[(size_t)(((base + p) * KS + (c >> 2)) * 64 + i16 + 16 * (c & 3))] +=
-                            val[e];
-                    }
-                }
+                for (int64_t r = n * t / nt; r < n * (t + 1) / nt; ++r)
+                    for (int64_t e = rowptr[r]; e < rowptr[r + 1]; ++e)
+                        mf[(size_t)band_slot(r, colind[e], rx, ry, bx, by, bz, b.ks, b.gxn,
+                                             b.hyn, b.npy, b.np)] += val[e];
             });
         for (auto &x : th) x.join();
     }
@@ -590,19 +646,37 @@ int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz
                               hipMemcpyHostToDevice, ctx->stream));
         TT_HIP(hipStreamSynchronize(ctx->stream));
     }
-    const int nxc = (gxn + 15) / 16;
-    H->format = 1;
-    H->rx = rx;
-    H->ry = ry;
-    H->rz = rz;
-    H->bx = bx;
-    H->by = by;
-    H->bz = bz;
-    H->ks = KS;
-    H->gxn = gxn;
-    H->hyn = hyn;
-    H->nxc = nxc;
-    H->gxb = std::max(4, (gxn + nxc - 1) / nxc);
+    set_band_fields(H, rx, ry, rz, bx, by, bz, b);
+    return TTDFT_OK;
+}
+
+int build_band_device(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz, int bx,
+                      int by, int bz, int copies) {
+    const int64_t n = (int64_t)rx * ry * rz;
+    BandShape b;
+    if (!band_shape(rx, ry, rz, bx, by, bz, H->nnz, &b)) return TTDFT_OK;  // keep CSR
+    TT_CHECK(H->val64, TTDFT_EBADARG, "build_band_device: FP64 CSR values missing");
+    double *mf = nullptr;
+    TT_HIP(hipMalloc(&mf, sizeof(double) * b.slots));
+    H->mf64 = mf;  // owned by H from here on
+    TT_HIP(hipMemsetAsync(mf, 0, sizeof(double) * b.slots, ctx->stream));
+    const unsigned nb = (unsigned)((n + 3) / 4);
+    hipLaunchKernelGGL(band_scatter_kernel, dim3(nb), dim3(256), 0, ctx->stream, H->rowptr,
+                       H->col, H->val64, n, rx, ry, bx, by, bz, b.ks, b.gxn, b.hyn, b.npy, b.np,
+                       mf);
+    TT_LAUNCH_CHECK();
+    if (copies & TTDFT_HAM_F32) {
+        TT_HIP(hipMalloc(&H->mf32, sizeof(float) * b.slots));
+        hipLaunchKernelGGL(f64_to_f32_kernel, dim3((unsigned)((b.slots + 255) / 256)),
+                           dim3(256), 0, ctx->stream, mf, H->mf32, b.slots);
+        TT_LAUNCH_CHECK();
+    }
+    if (!(copies & TTDFT_HAM_F64)) {
+        TT_HIP(hipStreamSynchronize(ctx->stream));
+        TT_HIP(hipFree(mf));
+        H->mf64 = nullptr;
+    }
+    set_band_fields(H, rx, ry, rz, bx, by, bz, b);
     return TTDFT_OK;
 }
 

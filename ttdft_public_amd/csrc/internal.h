@@ -136,6 +136,14 @@ int timing_flush(ttdft_ctx ctx);
 int launch_nl_project(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                       int32_t k, int dtype, void *W);
 int64_t nl_workspace(const ttdft_ham_s *H, int32_t k);
+
+// Kronecker band storage of H_loc (format 1) from host CSR, or from the device
+// CSR already in H (H->rowptr / col / val64) with known half-bandwidths;
+// either keeps CSR only when the band is too wide or too sparse.
+int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz,
+               const int64_t *rowptr, const int32_t *colind, const double *val, int copies);
+int build_band_device(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz, int bx,
+                      int by, int bz, int copies);
 // Y = alpha*(H_loc X + C^T W) + beta*X + delta*Z   (Z may alias Y; flags select terms)
 // use_beta / use_delta select whether X / Z are read.
 int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,

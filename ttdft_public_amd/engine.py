@@ -128,6 +128,10 @@ _SIGS = {
     "ttdft_assemble_projectors": (ctypes.c_int, [_vp, _i32, _c_i32p, _i32, _i32, _i32, _c_dp,
                                                  _c_dp, _c_dp, _c_dp, _d, _c_i64p, _c_i64p,
                                                  _c_i32p, _c_dp]),
+    "ttdft_ham_assemble_tucker": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, _i32, _i32]
+                                  + [_c_dp] * 10 + [_d, _i32, _c_i32p, _c_dp, _c_dp, _c_dp,
+                                                    _c_dp, _d, _c_dp, ctypes.c_int,
+                                                    ctypes.POINTER(_vp)]),
     "ttdft_rng_uniform": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
     "ttdft_rng_normal": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
 }
@@ -235,6 +239,30 @@ class SynthHam:
         self.c_col = self.c_col[: self.nnz_c]
         self.c_val = self.c_val[: self.nnz_c]
         self.inv_c_lm = self.inv_c_lm[:p]
+
+
+def _hloc_args(X, DX, Y, DY, Z, DZ, PX, PY, PZ, G):
+    """(arrays to keep alive, C arguments rx..rvz, X..G) of the H_loc assembly."""
+    rx, ry, rz = X.shape[0], Y.shape[0], Z.shape[0]
+    rvx, rvy, rvz = PX.shape[0], PY.shape[0], PZ.shape[0]
+    mats = [np.ascontiguousarray(m, dtype=np.float64) for m in (X, DX, Y, DY, Z, DZ, PX, PY, PZ)]
+    g = np.ascontiguousarray(np.asarray(G, dtype=np.float64).reshape(rvx, rvy, rvz)
+                             .transpose(2, 1, 0)).reshape(-1)  # first index fastest
+    keep = mats + [g]
+    return keep, [rx, ry, rz, rvx, rvy, rvz] + [_dptr(m) for m in keep]
+
+
+def _proj_args(cores, mx, my, mz):
+    """(arrays to keep alive, C arguments ranks, cores, mx, my, mz) of the
+    projector assembly: cores[i] is (r_i, r_i, r_i) indexed [a, b, c] (stored
+    first index fastest), factors column-major."""
+    ranks = np.array([c.shape[0] for c in cores], np.int32)
+    cc = np.concatenate([np.asarray(c, np.float64).transpose(2, 1, 0).reshape(-1)
+                         for c in cores])
+    fx, fy, fz = (np.concatenate([np.asarray(m, np.float64).T.reshape(-1) for m in ms])
+                  for ms in (mx, my, mz))
+    keep = [ranks, cc, fx, fy, fz]
+    return keep, [_i32ptr(ranks), _dptr(cc), _dptr(fx), _dptr(fy), _dptr(fz)]
 
 
 class Context:
@@ -353,14 +381,9 @@ class Context:
         """H_loc rows as (rowptr, col, val) numpy arrays, assembled on the device
         (dft/ProjectHamiltonianSparse.cc:330-575).  X..DZ are r x r with [I][J]
         indexing, PX is (rvx, rx, rx) = MatPotX[p][I][J], G has shape (rvx, rvy, rvz)."""
-        rx, ry, rz = X.shape[0], Y.shape[0], Z.shape[0]
-        rvx, rvy, rvz = PX.shape[0], PY.shape[0], PZ.shape[0]
-        mats = [np.ascontiguousarray(m, dtype=np.float64)
-                for m in (X, DX, Y, DY, Z, DZ, PX, PY, PZ)]
-        g = np.ascontiguousarray(np.asarray(G, dtype=np.float64).reshape(rvx, rvy, rvz)
-                                 .transpose(2, 1, 0)).reshape(-1)  # first index fastest
-        args = [self.h, rx, ry, rz, rvx, rvy, rvz] + [_dptr(m) for m in mats] + [
-            _dptr(g), float(truncation)]
+        keep, hargs = _hloc_args(X, DX, Y, DY, Z, DZ, PX, PY, PZ, G)
+        rx, ry, rz = hargs[:3]
+        args = [self.h] + hargs + [float(truncation)]
         nnz = ctypes.c_int64()
         _check(lib().ttdft_assemble_hloc(*args, ctypes.byref(nnz), None, None, None))
         n = rx * ry * rz
@@ -376,14 +399,9 @@ class Context:
         assembled on the device.  cores[i] has shape (r_i, r_i, r_i) indexed [a, b, c];
         mx[i] is rx x r_i, my[i] ry x r_i, mz[i] rz x r_i."""
         p = len(cores)
-        ranks = np.array([c.shape[0] for c in cores], np.int32)
-        rx, ry, rz = mx[0].shape[0], my[0].shape[0], mz[0].shape[0]
-        cc = np.concatenate([np.asarray(c, np.float64).transpose(2, 1, 0).reshape(-1)
-                             for c in cores])
-        fx, fy, fz = (np.concatenate([np.asarray(m, np.float64).T.reshape(-1) for m in ms])
-                      for ms in (mx, my, mz))  # column-major factors
-        args = [self.h, p, _i32ptr(ranks), rx, ry, rz, _dptr(cc), _dptr(fx), _dptr(fy),
-                _dptr(fz), float(truncation)]
+        keep, pargs = _proj_args(cores, mx, my, mz)
+        args = [self.h, p, pargs[0], mx[0].shape[0], my[0].shape[0], mz[0].shape[0]] + \
+            pargs[1:] + [float(truncation)]
         nnz = ctypes.c_int64()
         _check(lib().ttdft_assemble_projectors(*args, ctypes.byref(nnz), None, None, None))
         rowptr = np.empty(p + 1, np.int64)
@@ -392,6 +410,27 @@ class Context:
         _check(lib().ttdft_assemble_projectors(*args, ctypes.byref(nnz), _i64ptr(rowptr),
                                                _i32ptr(col), _dptr(val)))
         return rowptr, col[: nnz.value], val[: nnz.value]
+
+    def ham_assemble_tucker(self, hloc, truncation, proj=None, c_truncation=0.0,
+                            inv_c_lm=None, copies=HAM_F64):
+        """Device-resident H = H_loc + C^T D C from the assembly inputs:
+        hloc = (X, DX, Y, DY, Z, DZ, PX, PY, PZ, G) as for assemble_hloc,
+        proj = (cores, mx, my, mz) as for assemble_projectors (or None)."""
+        keep, hargs = _hloc_args(*hloc)
+        rx, ry, rz = hargs[:3]
+        if proj:
+            p = len(proj[0])
+            keep2, pargs = _proj_args(*proj)
+            inv = np.ascontiguousarray(inv_c_lm, np.float64)
+            assert inv.shape == (p,)
+            tail = [p] + pargs + [float(c_truncation), _dptr(inv)]
+        else:
+            keep2, inv = None, None
+            tail = [0, None, None, None, None, None, 0.0, None]
+        h = _vp()
+        _check(lib().ttdft_ham_assemble_tucker(self.h, *hargs, float(truncation), *tail, copies,
+                                               ctypes.byref(h)))
+        return Hamiltonian.adopt(self, h, rx * ry * rz)
 
     def random_subspace(self, X, ldx, n, k, seed=20261202, dtype=F64):
         _check(lib().ttdft_random_subspace(self.h, X, ldx, n, k, seed, dtype))
@@ -443,6 +482,15 @@ class Hamiltonian:
                                                  *args, copies, ctypes.byref(h)))
         self.h = h
         self.n = n
+
+    @classmethod
+    def adopt(cls, ctx, handle, n):
+        """Wrap a ttdft_ham created on the device (ttdft_ham_assemble_tucker)."""
+        obj = cls.__new__(cls)
+        obj.ctx = ctx
+        obj.h = handle
+        obj.n = n
+        return obj
 
     def format(self):
         """(0 = CSR | 1 = Kronecker band, (bx, by, bz))"""
