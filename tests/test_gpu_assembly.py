@@ -67,22 +67,39 @@ HLOC_CASES = [
     ((14, 12, 10), (5, 4, 3), 3, 1e-3),
     ((8, 8, 8), (8, 8, 8), 7, 0.0),       # dense factors, nothing truncated
     ((20, 6, 9), (1, 2, 7), 4, 1e-2),     # rank-1 mode, ragged dims
-    ((5, 17, 3), (3, 1, 2), 1, 0.5),      # heavy truncation, short z axis
+    ((5, 17, 3), (3, 1, 2), 1, 0.5),      # heavy truncation, short z axis, ragged last tile
+    ((9, 7, 13), (2, 3, 10), 2, 1e-3),    # rz^2 = 169: partial row tile; 3 product terms
 ]
 
+# "fused": product and compaction in one matrix-core kernel (rx <= 256);
+# "gemm": library products into HBM, then the compaction kernel
+PATHS = ["fused", "gemm"]
 
+
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("dims,rv,band,trunc", HLOC_CASES)
-def test_assemble_hloc_matches_oracle(ctx, dims, rv, band, trunc):
+def test_assemble_hloc_matches_oracle(ctx, monkeypatch, dims, rv, band, trunc, path):
+    monkeypatch.setenv("TTDFT_ASSEMBLE_PATH", path)
     args = hloc_inputs(11, dims, rv, band)
     n = int(np.prod(dims))
-    print(f"assemble_hloc dims={dims} rv={rv} band={band} trunc={trunc}", flush=True)
+    print(f"assemble_hloc dims={dims} rv={rv} band={band} trunc={trunc} path={path}", flush=True)
     got = ctx.assemble_hloc(*args, trunc)
     ref = oracle.assemble_hloc(*args, trunc)
     assert_csr_match(got, ref, n, trunc)
 
 
+def test_assemble_hloc_wide_x_axis(ctx):
+    """rx = 260 > 256 columns per matrix-core tile: the library-product path."""
+    dims, rv = (260, 3, 2), (3, 2, 2)
+    args = hloc_inputs(12, dims, rv, 5)
+    got = ctx.assemble_hloc(*args, 1e-3)
+    ref = oracle.assemble_hloc(*args, 1e-3)
+    assert_csr_match(got, ref, int(np.prod(dims)), 1e-3)
+
+
 def test_assemble_hloc_chunked(ctx, monkeypatch):
     """Seven groups (xI, y, yI) per chunk: the last chunk is ragged."""
+    monkeypatch.setenv("TTDFT_ASSEMBLE_PATH", "gemm")
     monkeypatch.setenv("TTDFT_ASSEMBLE_GROUPS", "7")
     dims, rv = (9, 7, 6), (3, 4, 2)
     args = hloc_inputs(3, dims, rv, 2)
@@ -97,12 +114,15 @@ def test_assemble_hloc_all_truncated(ctx):
     assert len(c) == 0 and np.all(rp == 0)
 
 
-def test_assemble_hloc_symmetric_full_size(ctx):
+@pytest.mark.parametrize("path", PATHS)
+def test_assemble_hloc_symmetric_full_size(ctx, monkeypatch, path):
     """50^3 basis, Veff Tucker ranks 50^3 (the 172-atom workload): with
     symmetric 1-D factors H_loc is symmetric, so the assembled matrix must
     equal its transpose (structure exactly, away from the threshold)."""
+    monkeypatch.setenv("TTDFT_ASSEMBLE_PATH", path)
     dims, rv, trunc = (50, 50, 50), (50, 50, 50), 1e-2
     args = hloc_inputs(2, dims, rv, 3, sym=True)
+    print(f"assemble_hloc 50^3 path={path}", flush=True)
     rp, c, v = ctx.assemble_hloc(*args, trunc)
     n = int(np.prod(dims))
     assert len(c) > 10 * n

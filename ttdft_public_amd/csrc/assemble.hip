@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void hloc_compact_kernel(
         limit = valid ? cnt[key] : 0;
     }
     const int32_t col0 = (int32_t)((int64_t)rx * (y + (int64_t)ry * z));
-    int kept = 0, dxmax = -1;
+    int kept = 0;
     for (int x = 0; x < rx; ++x) {
         // the reference's dyad: 0.5 (DX Y Z + X DY Z + X Y DZ), then + potential
         const double kin = dxr[x] * yv * zv + xr[x] * dyv * zv + xr[x] * yv * dzv;
@@ -137,7 +137,6 @@ __global__ __launch_bounds__(256) void hloc_compact_kernel(
                 val[base + kept] = v;
             }
             ++kept;
-            dxmax = max(dxmax, abs(x - xI));
         }
     }
     if (!WRITE) {
@@ -147,19 +146,6 @@ __global__ __launch_bounds__(256) void hloc_compact_kernel(
     // Both passes evaluate the same products, so the counts agree; a
     // difference would leave a slot unwritten and is reported to the host.
     if (kept != limit) flags[0] = 1;
-    // Kronecker half-bandwidths of the kept entries: wavefront maxima, then
-    // one atomic per wavefront.
-    int e0 = dxmax, e1 = kept ? abs(y - yI) : -1, e2 = kept ? abs(z - zI) : -1;
-    for (int o = 32; o > 0; o >>= 1) {
-        e0 = max(e0, __shfl_xor(e0, o));
-        e1 = max(e1, __shfl_xor(e1, o));
-        e2 = max(e2, __shfl_xor(e2, o));
-    }
-    if ((threadIdx.x & 63) == 0 && e0 >= 0) {
-        atomicMax(flags + 1, e0);
-        atomicMax(flags + 2, e1);
-        atomicMax(flags + 3, e2);
-    }
 }
 
 // Per row: exclusive scan of cnt[row][y][z] in the CSR order (z outer, y
@@ -208,6 +194,234 @@ __global__ __launch_bounds__(256) void hloc_row_scan_kernel(const uint16_t *__re
         const int y = e / rz, z = e - y * rz;
         r[e] = v[z * ry + y];
     }
+}
+
+// ---- fused product + compaction (matrix cores) -------------------------------
+//
+// C[pi][t] = sum_r Ae[pi][r] Be[r][t] with the kinetic term folded in as two
+// more product terms: Ae[pi][rvz] = Z[zI][z], Ae[pi][rvz+1] = DZ[zI][z],
+// Be[rvz][t] = 1/2 (DX[xI][x] Y[yI][y] + X[xI][x] DY[yI][y]),
+// Be[rvz+1][t] = 1/2 X[xI][x] Y[yI][y], zero terms up to a multiple of 4.
+// A workgroup computes a 64 (pi) x 256 (t) tile that holds Gt whole groups
+// g (Gt rx <= 256 columns) on the FP64 16x16x4 MFMA; wave w owns columns
+// [64 w, 64 w + 64), all 64 rows, 16 accumulators.  The tile never leaves
+// the chip: the epilogue thresholds it, turns the keep bits into per-row
+// masks in LDS (wavefront ballots), and from the masks
+//   COUNT: cnt[row][y][z] = kept x of segment (pi, g);
+//   WRITE: each kept entry goes to rowptr[row] + rel[key] + (kept x' < x).
+struct FusedArgs {
+    const double *A;  // Ae, (Mp rows) x (K4 + 8 planes), A[pi + Mp k], rows >= M zero
+    const double *B;  // Be, (K4 + 8 planes) x ldb, B[k ldb + t], ldb >= ntup + 256
+    int64_t ntup, ldb, ngroups, ntiles, nblocks;
+    int32_t M, Mp, K4, rx, ry, rz, Gt, mtiles;
+    double trunc;
+    uint16_t *cnt;
+    const int32_t *rel;
+    const int64_t *rowptr;
+    int32_t *col;
+    double *val;
+    int32_t *flags;  // mismatch, half-bandwidths x, y, z
+};
+
+typedef double fd4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kFusedGt = 16;  // most groups per tile (rx >= 16 uses fewer)
+
+// bits [lo, hi) of a 256-bit row mask held as 16 x 16 bits
+__device__ inline int mask_popc(const uint16_t *m, int lo, int hi) {
+    int c = 0;
+    for (int q = lo >> 4; q <= (hi - 1) >> 4 && lo < hi; ++q) {
+        uint32_t bits = m[q];
+        const int b0 = q * 16;
+        if (lo > b0) bits &= ~((1u << (lo - b0)) - 1u);
+        if (hi < b0 + 16) bits &= (1u << (hi - b0)) - 1u;
+        c += __popc(bits);
+    }
+    return c;
+}
+
+template <int WRITE>
+__global__ __launch_bounds__(256, 2) void hloc_fused_kernel(FusedArgs a) {
+    __shared__ uint16_t mask[64][16];
+    __shared__ int64_t segbase[64 * kFusedGt];
+    __shared__ int32_t segcnt[64 * kFusedGt];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // consecutive logical tiles on one XCD (dispatch is round-robin over 8)
+    const int64_t b = blockIdx.x, per = a.nblocks >> 3;
+    const int64_t L = (b & 7) * per + (b >> 3);
+    const int mt = (int)(L % a.mtiles);
+    const int64_t nt = L / a.mtiles;
+    if (nt >= a.ntiles) return;  // whole workgroup: grid padding
+    const int64_t g0 = nt * a.Gt;
+    const int gcnt = (int)min((int64_t)a.Gt, a.ngroups - g0);
+    const int ncols = gcnt * a.rx;
+    const int64_t t0 = g0 * a.rx;
+    const int pi0 = mt * 64;
+    const int li = lane & 15, lk = lane >> 4;
+    // A fragment of row tile i: rows pi0 + 16 i + li, term 4 ks + lk; B fragment of
+    // column tile j: columns t0 + 64 w + 16 j + li.  Rows past M are zero rows of
+    // Ae, columns past the tile's groups lie in Be (padding included) and are
+    // masked in the epilogue; the prefetch runs at most 8 terms past K4.
+    const double *pa = a.A + pi0 + li + (int64_t)a.Mp * lk;
+    const double *pb = a.B + t0 + 64 * w + li + a.ldb * lk;
+    const int64_t sa = 4 * (int64_t)a.Mp, sb = 4 * a.ldb;
+    const int nks = a.K4 >> 2;
+    fd4_t acc[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[i][j] = fd4_t{0.0, 0.0, 0.0, 0.0};
+    double fa0[4], fb0[4], fa1[4], fb1[4];
+    auto load = [&](double *fa, double *fb, const double *qa, const double *qb) {
+        for (int q = 0; q < 4; ++q) {
+            fa[q] = qa[16 * q];
+            fb[q] = qb[16 * q];
+        }
+    };
+    auto mma = [&](const double *fa, const double *fb) {
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] =
+                    __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    };
+    load(fa0, fb0, pa, pb);
+    load(fa1, fb1, pa + sa, pb + sb);
+    pa += 2 * sa;
+    pb += 2 * sb;
+    int ks = 0;
+    for (; ks + 2 <= nks; ks += 2) {
+        mma(fa0, fb0);
+        load(fa0, fb0, pa, pb);
+        mma(fa1, fb1);
+        load(fa1, fb1, pa + sa, pb + sb);
+        pa += 2 * sa;
+        pb += 2 * sb;
+    }
+    if (ks < nks) mma(fa0, fb0);
+    // ---- epilogue: keep bits -> row masks
+    // lane (li, lk), tile (i, j), element v: row 16 i + lk + 4 v, column 64 w + 16 j + li
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int colj = 64 * w + 16 * j + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = 16 * i + lk + 4 * v;
+                const bool keep =
+                    colj < ncols && pi0 + row < a.M && fabs(acc[i][j][v]) > a.trunc;
+                const uint64_t m = __ballot(keep);
+                if (lane < 4) mask[16 * i + lane + 4 * v][4 * w + j] =
+                                  (uint16_t)(m >> (16 * lane));
+            }
+        }
+    __syncthreads();
+    // ---- segments (row, group): counts, and for WRITE the base offsets
+    const int nseg = 64 * gcnt;
+    for (int s = threadIdx.x; s < nseg; s += 256) {
+        const int r = s % 64, gl = s / 64;
+        const int pi = pi0 + r;
+        if (pi >= a.M) continue;
+        const int kept = mask_popc(mask[r], gl * a.rx, gl * a.rx + a.rx);
+        const int zI = pi / a.rz, z = pi - zI * a.rz;
+        const int g = (int)g0 + gl;  // groups < rx^2 ry^2 < 2^31
+        const int gy = g / a.rx, xI = g - gy * a.rx;
+        const int yI = gy / a.ry, y = gy - yI * a.ry;
+        const int64_t rowid = xI + (int64_t)a.rx * (yI + (int64_t)a.ry * zI);
+        const int64_t key = (rowid * a.ry + y) * a.rz + z;
+        if (!WRITE) {
+            a.cnt[key] = (uint16_t)kept;
+        } else {
+            const int expect = a.cnt[key];
+            if (kept != expect) a.flags[0] = 1;
+            segbase[r * kFusedGt + gl] = a.rowptr[rowid] + a.rel[key];
+            segcnt[r * kFusedGt + gl] = min(kept, expect);
+        }
+    }
+    if (!WRITE) return;
+    __syncthreads();
+    // ---- scatter the kept entries (loops unrolled: acc stays in registers)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int colj = 64 * w + 16 * j + li;
+        const int gl = colj / a.rx, x = colj - gl * a.rx;
+        const int g = (int)g0 + gl;
+        const int gy = g / a.rx;
+        const int yI = gy / a.ry, y = gy - yI * a.ry;
+        const int colx = x + a.rx * y;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * i + lk + 4 * v;
+                const int pi = pi0 + r;
+                const double hv = acc[i][j][v];
+                if (colj < ncols && pi < a.M && fabs(hv) > a.trunc) {
+                    const int rank = mask_popc(mask[r], gl * a.rx, colj);
+                    if (rank < segcnt[r * kFusedGt + gl]) {
+                        const int64_t pos = segbase[r * kFusedGt + gl] + rank;
+                        a.col[pos] = colx + a.rx * a.ry * (pi % a.rz);
+                        a.val[pos] = hv;
+                    }
+                }
+            }
+    }
+}
+
+// Kronecker half-bandwidths of a CSR matrix in the basis x + rx (y + ry z):
+// max |x - x'|, |y - y'|, |z - z'| over its entries.  One wavefront per row,
+// one atomic per workgroup and axis.
+__global__ __launch_bounds__(256) void band_extent_kernel(const int64_t *__restrict__ rowptr,
+                                                          const int32_t *__restrict__ col,
+                                                          int64_t n, int rx, int ry,
+                                                          int32_t *__restrict__ ext) {
+    __shared__ int red[4][3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t r = (int64_t)blockIdx.x * 4 + w;
+    int e0 = -1, e1 = -1, e2 = -1;
+    if (r < n) {
+        const int rxy = rx * ry;
+        const int z = (int)(r / rxy), y = (int)(r % rxy) / rx, x = (int)(r % rx);
+        for (int64_t e = rowptr[r] + lane; e < rowptr[r + 1]; e += 64) {
+            const int j = col[e];
+            const int zj = j / rxy, yj = (j - zj * rxy) / rx, xj = j % rx;
+            e0 = max(e0, abs(xj - x));
+            e1 = max(e1, abs(yj - y));
+            e2 = max(e2, abs(zj - z));
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        e0 = max(e0, __shfl_xor(e0, o));
+        e1 = max(e1, __shfl_xor(e1, o));
+        e2 = max(e2, __shfl_xor(e2, o));
+    }
+    if (lane == 0) {
+        red[w][0] = e0;
+        red[w][1] = e1;
+        red[w][2] = e2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int m = -1;
+        for (int q = 0; q < 4; ++q) m = max(m, red[q][threadIdx.x]);
+        if (m >= 0) atomicMax(ext + threadIdx.x, m);
+    }
+}
+
+// Be kinetic planes (plane stride ldb): t = x + rx g, g = xI + rx (y + ry yI).
+__global__ __launch_bounds__(256) void kinetic_planes_kernel(
+    const double *__restrict__ X, const double *__restrict__ DX, const double *__restrict__ Y,
+    const double *__restrict__ DY, int rx, int ry, int64_t ntup, int64_t ldb,
+    double *__restrict__ planes) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntup) return;
+    const int x = (int)(t % rx);
+    const int64_t g = t / rx;
+    const int xI = (int)(g % rx);
+    const int64_t gy = g / rx;
+    const int y = (int)(gy % ry), yI = (int)(gy / ry);
+    const double xv = X[(int64_t)xI * rx + x], dxv = DX[(int64_t)xI * rx + x];
+    const double yv = Y[(int64_t)yI * ry + y], dyv = DY[(int64_t)yI * ry + y];
+    planes[t] = 0.5 * (dxv * yv + xv * dyv);
+    planes[ldb + t] = 0.5 * (xv * yv);
 }
 
 // Dense projector rows: D[i][x + rx (y + ry z)] = sum_c W[i][c][x + rx y] MZ[i][z + rz c]
@@ -386,10 +600,27 @@ int hloc_run(ttdft_ctx ctx, const HlocIn &a, bool fill, DevCsr &out, int ext[3])
         rocblas_atomics_mode m;
         ~AtomicsRestore() { rocblas_set_atomics_mode(h, m); }
     } restore{ctx->blas, atomics};
-    // A = PX G, B_r = A_r PY^T
-    DevBuf dA, dB;
+    // Matrix-core path (fused product + compaction) for rx <= 256; the
+    // chunked library-product path otherwise, or when TTDFT_ASSEMBLE_PATH=gemm.
+    const char *path_env = std::getenv("TTDFT_ASSEMBLE_PATH");
+    const bool fused = rx <= 256 && !(path_env && std::string(path_env) == "gemm");
+    const int K4 = (rvz + 2 + 3) / 4 * 4;  // Veff terms + 2 kinetic terms, multiple of 4
+    // A = PX G, B_r = A_r PY^T (B_r are the first rvz planes of Be on the fused path)
+    const int64_t mp = (rz2 + 63) / 64 * 64;          // Ae rows, padded to the tile
+    const int64_t ldb = fused ? ntup + 256 : ntup;    // Be plane stride, padded
+    DevBuf dA, dB, dAe;
     TT_TRY(dA.alloc(sizeof(double) * rx2 * rvy * rvz));
-    TT_TRY(dB.alloc(sizeof(double) * rvz * ntup));
+    TT_TRY(dB.alloc(sizeof(double) * (fused ? K4 + 8 : rvz) * ldb));
+    if (fused) {
+        std::vector<double> ae((size_t)mp * (K4 + 8), 0.0);
+        for (int r = 0; r < rvz; ++r)
+            std::copy(a.PZ + rz2 * r, a.PZ + rz2 * (r + 1), ae.begin() + mp * r);
+        std::copy(a.Z, a.Z + rz2, ae.begin() + mp * rvz);
+        std::copy(a.DZ, a.DZ + rz2, ae.begin() + mp * (rvz + 1));
+        TT_TRY(upload_to(ctx, dAe, ae.data(), sizeof(double) * ae.size()));
+        TT_HIP(hipMemsetAsync(dB.p, 0, sizeof(double) * (K4 + 8) * ldb, ctx->stream));
+        TT_HIP(hipStreamSynchronize(ctx->stream));  // ae goes out of scope
+    }
     const double one = 1.0, zero = 0.0;
     // chunks of groups: Cc holds rz^2 x (rx * gc) doubles (about 1 GB)
     int64_t gc = ((int64_t)1 << 27) / std::max<int64_t>(1, rz2 * rx);
@@ -399,17 +630,52 @@ int hloc_run(ttdft_ctx ctx, const HlocIn &a, bool fill, DevCsr &out, int ext[3])
     TT_TRY(dflag.alloc(sizeof(int32_t) * 4));
     const int32_t flag0[4] = {0, -1, -1, -1};  // mismatch, half-bandwidths x, y, z
     TT_HIP(hipMemcpyAsync(dflag.p, flag0, sizeof(flag0), hipMemcpyHostToDevice, ctx->stream));
-    TT_TRY(dC.alloc(sizeof(double) * rz2 * rx * gc));
+    if (!fused) TT_TRY(dC.alloc(sizeof(double) * rz2 * rx * gc));
     TT_TRY(dcnt.alloc(sizeof(uint16_t) * n * ry * rz));
     TT_TRY(drel.alloc(sizeof(int32_t) * n * ry * rz));
     TT_TRY(drowlen.alloc(sizeof(int64_t) * n));
+    FusedArgs fa{};
+    if (fused) {
+        fa.A = dAe.as<double>();
+        fa.B = dB.as<double>();
+        fa.ntup = ntup;
+        fa.ldb = ldb;
+        fa.ngroups = ngroups;
+        fa.M = (int32_t)rz2;
+        fa.Mp = (int32_t)mp;
+        fa.K4 = K4;
+        fa.rx = rx;
+        fa.ry = ry;
+        fa.rz = rz;
+        fa.Gt = std::min(kFusedGt, 256 / rx);
+        fa.mtiles = (int32_t)((rz2 + 63) / 64);
+        fa.ntiles = (ngroups + fa.Gt - 1) / fa.Gt;
+        fa.nblocks = (fa.mtiles * fa.ntiles + 7) / 8 * 8;
+        fa.trunc = a.trunc;
+        fa.cnt = dcnt.as<uint16_t>();
+        fa.rel = drel.as<int32_t>();
+        fa.flags = dflag.as<int32_t>();
+        TT_CHECK(fa.nblocks < ((int64_t)1 << 31), TTDFT_EBADARG,
+                 "assemble_hloc: %lld workgroups exceed the grid limit",
+                 (long long)fa.nblocks);
+    }
     auto run_pass = [&](int write, const int64_t *drowptr, int32_t *dcol, double *dval) -> int {
+        if (fused) {
+            FusedArgs f = fa;
+            f.rowptr = drowptr;
+            f.col = dcol;
+            f.val = dval;
+            auto kern = write ? hloc_fused_kernel<1> : hloc_fused_kernel<0>;
+            hipLaunchKernelGGL(kern, dim3((unsigned)f.nblocks), dim3(256), 0, ctx->stream, f);
+            TT_LAUNCH_CHECK();
+            return TTDFT_OK;
+        }
         for (int64_t g0 = 0; g0 < ngroups; g0 += gc) {
             const int64_t ng = std::min<int64_t>(gc, ngroups - g0);
             TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
                                   (rocblas_int)rz2, (rocblas_int)(rx * ng), rvz, &one,
                                   dPZ.as<double>(), (rocblas_int)rz2, dB.as<double>() + rx * g0,
-                                  (rocblas_int)ntup, &zero, dC.as<double>(), (rocblas_int)rz2));
+                                  (rocblas_int)ldb, &zero, dC.as<double>(), (rocblas_int)rz2));
             const unsigned nb = blocks_for(rz2 * ng, 256);
             auto kern = write ? hloc_compact_kernel<1> : hloc_compact_kernel<0>;
             hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, ctx->stream, dC.as<double>(), g0, ng,
@@ -425,7 +691,7 @@ int hloc_run(ttdft_ctx ctx, const HlocIn &a, bool fill, DevCsr &out, int ext[3])
     // (kinetic term and sum, 7 flop per entry)
     const double n2 = (double)n * (double)n;
     const double flops = 2.0 * rx2 * rvx * rvy * rvz + 2.0 * ntup * rvy * rvz +
-                         2.0 * n2 * rvz + 7.0 * n2;
+                         2.0 * n2 * rvz + 7.0 * n2;  // same count for both paths
     PendingTiming tm;
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ASSEMBLY, &tm));
     TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none,
@@ -435,8 +701,15 @@ int hloc_run(ttdft_ctx ctx, const HlocIn &a, bool fill, DevCsr &out, int ext[3])
     TT_BLAS(rocblas_dgemm_strided_batched(
         ctx->blas, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)rx2,
         (rocblas_int)ry2, rvy, &one, dA.as<double>(), (rocblas_int)rx2, rx2 * rvy,
-        dPY.as<double>(), (rocblas_int)ry2, 0, &zero, dB.as<double>(), (rocblas_int)rx2, ntup,
+        dPY.as<double>(), (rocblas_int)ry2, 0, &zero, dB.as<double>(), (rocblas_int)rx2, ldb,
         rvz));
+    if (fused) {
+        hipLaunchKernelGGL(kinetic_planes_kernel, dim3(blocks_for(ntup, 256)), dim3(256), 0,
+                           ctx->stream, dX.as<double>(), dDX.as<double>(), dY.as<double>(),
+                           dDY.as<double>(), rx, ry, ntup, ldb,
+                           dB.as<double>() + (int64_t)rvz * ldb);
+        TT_LAUNCH_CHECK();
+    }
     TT_TRY(run_pass(0, nullptr, nullptr, nullptr));
     const size_t lds = sizeof(int32_t) * ((size_t)ry * rz + 256);
     hipLaunchKernelGGL(hloc_row_scan_kernel, dim3((unsigned)n), dim3(256), lds, ctx->stream,
@@ -457,6 +730,10 @@ int hloc_run(ttdft_ctx ctx, const HlocIn &a, bool fill, DevCsr &out, int ext[3])
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ASSEMBLY, &tm));
     TT_TRY(run_pass(1, out.rowptr.as<int64_t>(), out.col.as<int32_t>(), out.val.as<double>()));
     TT_TRY(timing_end(ctx, &tm, 12.0 * out.nnz + 8.0 * (n + 1), flops));
+    hipLaunchKernelGGL(band_extent_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0,
+                       ctx->stream, out.rowptr.as<int64_t>(), out.col.as<int32_t>(), n, rx, ry,
+                       dflag.as<int32_t>() + 1);
+    TT_LAUNCH_CHECK();
     int32_t flags[4];
     TT_HIP(hipMemcpyAsync(flags, dflag.p, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
     TT_HIP(hipStreamSynchronize(ctx->stream));
