@@ -43,17 +43,15 @@ def test_driver_fails_cleanly_without_device():
     assert "ttdft status" in r.stderr
 
 
-@pytest.mark.gpu
-def test_driver_matches_oracle_chfsi():
-    dims, bands, p, k, m, nelec, nscf = (8, 8, 8), (2, 2, 3), 5, 16, 12, 19, 5
+def run_driver_against_oracle(args, H, n, k, m, nelec, nscf):
+    """Run the driver, then the oracle's ChFSI steps on H from the same start
+    block, and compare each SCF iteration's Ritz values and Fermi level."""
     r = subprocess.run([DRIVER, "--scf", str(nscf), "--k", str(k), "--m", str(m), "--nelec",
-                        str(nelec)], capture_output=True, text=True, timeout=120)
+                        str(nelec)] + args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = parse_driver(r.stdout)
     assert len(got) == nscf
-    s = E.SynthHam(*dims, *bands, p)
-    H = oracle.Ham.from_synth(s)
-    X = oracle.orth(oracle.rng_normal(5, s.n * k).reshape(s.n, k))
+    X = oracle.orth(oracle.rng_normal(5, n * k).reshape(n, k))
     ev = np.zeros(k)
     st = oracle.new_state()
     occ_index = min(int(nelec / 2.0 + 10), k - 1)
@@ -65,3 +63,52 @@ def test_driver_matches_oracle_chfsi():
         assert np.abs(evg[: nocc + 1] - ev[: nocc + 1]).max() < 1e-9, scf
         assert np.abs(evg - ev).max() < 1e-6, scf
         assert abs(efg - oracle.fermi_energy(ev, nelec)) < 1e-9
+    return r.stdout
+
+
+@pytest.mark.gpu
+def test_driver_matches_oracle_chfsi():
+    dims, bands, p, k, m, nelec, nscf = (8, 8, 8), (2, 2, 3), 5, 16, 12, 19, 5
+    s = E.SynthHam(*dims, *bands, p)
+    run_driver_against_oracle([], oracle.Ham.from_synth(s), s.n, k, m, nelec, nscf)
+
+
+def write_assembly_file(path, hl, proj, inv, trunc, c_trunc):
+    """The driver's --assembly input (layout in chfsi_driver.cc)."""
+    X, DX, Y, DY, Z, DZ, PX, PY, PZ, G = hl
+    cores, mx, my, mz = proj
+    with open(path, "wb") as f:
+        np.array([X.shape[0], Y.shape[0], Z.shape[0], PX.shape[0], PY.shape[0], PZ.shape[0],
+                  len(cores), 0], np.int32).tofile(f)
+        np.array([c.shape[0] for c in cores], np.int32).tofile(f)
+        np.array([trunc, c_trunc], np.float64).tofile(f)
+        for a in (X, DX, Y, DY, Z, DZ, PX, PY, PZ):
+            np.ascontiguousarray(a, np.float64).tofile(f)
+        np.ascontiguousarray(np.transpose(G, (2, 1, 0))).tofile(f)  # first index fastest
+        np.concatenate([np.transpose(c, (2, 1, 0)).reshape(-1) for c in cores]).tofile(f)
+        for ms in (mx, my, mz):
+            np.concatenate([m.T.reshape(-1) for m in ms]).tofile(f)
+        np.asarray(inv, np.float64).tofile(f)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("device", [False, True])
+def test_driver_on_assembled_hamiltonian(tmp_path, device):
+    """ChFSI on H built by ProjectHamiltonianSparse::Create_Hloc / Create_Cnloc
+    (host CSR) or AX::assemble_system (device memory), against the oracle's
+    ChFSI on the oracle's own assembly of the same inputs."""
+    from test_gpu_assembly import hloc_inputs, proj_inputs
+
+    dims, rv, k, m, nelec, nscf = (9, 8, 7), (3, 4, 2), 16, 12, 19, 4
+    hl = hloc_inputs(31, dims, rv, 2, sym=True)
+    proj = proj_inputs(32, dims, [2, 3, 1])
+    inv = np.array([0.8, -0.4, 1.3])
+    path = str(tmp_path / "assembly.bin")
+    write_assembly_file(path, hl, proj, inv, 1e-6, 1e-8)
+    rp, c, v = oracle.assemble_hloc(*hl, 1e-6)
+    crp, cc, cv = oracle.assemble_projectors(*proj, 1e-8)
+    n = int(np.prod(dims))
+    H = oracle.Ham(n, rp, c, v, crp, cc, cv, inv)
+    args = ["--assembly", path] + (["--device-assembly"] if device else [])
+    out = run_driver_against_oracle(args, H, n, k, m, nelec, nscf)
+    assert ("assembled H on the device" in out) == device

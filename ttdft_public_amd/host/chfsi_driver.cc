@@ -6,6 +6,15 @@
 //
 //   chfsi_driver [--dims RX RY RZ] [--bands BX BY BZ] [--proj P] [--k K]
 //                [--m M] [--nelec NE] [--scf S] [--h H] [--seed S] [--f32]
+//                [--assembly FILE [--device-assembly]]
+//
+// --assembly FILE takes H from assembly inputs instead of the synthetic
+// operator: Create_Hloc + Create_Cnloc into host CSR, then setup_system; with
+// --device-assembly, AX::assemble_system builds it in device memory.  FILE is
+// raw little-endian: int32 rx ry rz rvx rvy rvz p 0, int32 ranks[p], float64
+// truncation, c_truncation, then float64 X DX Y DY Z DZ (r x r each),
+// PotX PotY PotZ (rv x r x r), core (first index fastest), projector cores,
+// factors X, Y, Z (column-major, concatenated), inv_C_lm[p].
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -16,11 +25,77 @@
 
 using namespace ttdft_host;
 
+namespace {
+
+struct AssemblyInputs {
+    int32_t r[3] = {0, 0, 0};
+    HlocFactors hf;
+    ProjectorFactors pf;
+    double truncation = 0.0, c_truncation = 0.0;
+};
+
+template <typename T>
+void read_into(FILE *f, std::vector<T> &v, size_t count) {
+    v.resize(count);
+    if (count && fread(v.data(), sizeof(T), count, f) != count)
+        throw Error(TTDFT_EBADARG, "assembly file: truncated");
+}
+
+AssemblyInputs read_assembly(const char *path) {
+    FILE *f = fopen(path, "rb");
+    if (!f) throw Error(TTDFT_EBADARG, std::string("cannot open ") + path);
+    AssemblyInputs a;
+    std::vector<int32_t> hdr;
+    std::vector<double> tr;
+    try {
+        read_into(f, hdr, 8);
+        for (int d = 0; d < 3; ++d) a.r[d] = hdr[d];
+        a.hf.rankVeffX = hdr[3];
+        a.hf.rankVeffY = hdr[4];
+        a.hf.rankVeffZ = hdr[5];
+        const int p = hdr[6];
+        read_into(f, a.pf.ranks, p);
+        read_into(f, tr, 2);
+        a.truncation = tr[0];
+        a.c_truncation = tr[1];
+        const size_t rx = a.r[0], ry = a.r[1], rz = a.r[2];
+        read_into(f, a.hf.MatX, rx * rx);
+        read_into(f, a.hf.MatDX, rx * rx);
+        read_into(f, a.hf.MatY, ry * ry);
+        read_into(f, a.hf.MatDY, ry * ry);
+        read_into(f, a.hf.MatZ, rz * rz);
+        read_into(f, a.hf.MatDZ, rz * rz);
+        read_into(f, a.hf.MatPotX, (size_t)hdr[3] * rx * rx);
+        read_into(f, a.hf.MatPotY, (size_t)hdr[4] * ry * ry);
+        read_into(f, a.hf.MatPotZ, (size_t)hdr[5] * rz * rz);
+        read_into(f, a.hf.coreVeff, (size_t)hdr[3] * hdr[4] * hdr[5]);
+        size_t nc = 0, nr = 0;
+        for (int32_t q : a.pf.ranks) {
+            nc += (size_t)q * q * q;
+            nr += q;
+        }
+        read_into(f, a.pf.cores, nc);
+        read_into(f, a.pf.factorX, rx * nr);
+        read_into(f, a.pf.factorY, ry * nr);
+        read_into(f, a.pf.factorZ, rz * nr);
+        read_into(f, a.pf.inv_C_lm, p);
+    } catch (...) {
+        fclose(f);
+        throw;
+    }
+    fclose(f);
+    return a;
+}
+
+}  // namespace
+
 int main(int argc, char **argv) {
     int dims[3] = {8, 8, 8}, bands[3] = {2, 2, 3};
     int nproj = 5, k = 16, m = 12, nelec = 19, nscf = 6, f32 = 0;
     double h = 2.0;
     unsigned long long seed = 20261202ULL;
+    const char *assembly = nullptr;
+    int device_assembly = 0;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--dims") && i + 3 < argc) {
             for (int d = 0; d < 3; ++d) dims[d] = atoi(argv[++i]);
@@ -42,39 +117,57 @@ int main(int argc, char **argv) {
             seed = strtoull(argv[++i], nullptr, 10);
         } else if (!strcmp(argv[i], "--f32")) {
             f32 = 1;
+        } else if (!strcmp(argv[i], "--assembly") && i + 1 < argc) {
+            assembly = argv[++i];
+        } else if (!strcmp(argv[i], "--device-assembly")) {
+            device_assembly = 1;
         } else {
             fprintf(stderr, "unknown argument %s\n", argv[i]);
             return 2;
         }
     }
     try {
-        // ---- synthetic projected Hamiltonian (host CSR) ----
-        ttdft_synth_params prm{dims[0], dims[1], dims[2], bands[0], bands[1], bands[2],
-                               nproj,   h,       1e-6,    seed};
-        int64_t nnz = 0, nnz_c = 0;
-        check(ttdft_synth_hamiltonian(&prm, &nnz, &nnz_c, nullptr, nullptr, nullptr, nullptr,
-                                      nullptr, nullptr, nullptr));
-        ProjectHamiltonianSparse H;
-        const int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
-        H.H_loc.rows = H.H_loc.cols = n;
-        H.H_loc.rowptr.resize(n + 1);
-        H.H_loc.col.resize(nnz);
-        H.H_loc.val.resize(nnz);
-        H.C_nloc.rows = nproj;
-        H.C_nloc.cols = n;
-        H.C_nloc.rowptr.resize(nproj + 1);
-        H.C_nloc.col.resize(nnz_c);
-        H.C_nloc.val.resize(nnz_c);
-        H.inv_C_lm.resize(nproj);
-        check(ttdft_synth_hamiltonian(&prm, &nnz, &nnz_c, H.H_loc.rowptr.data(),
-                                      H.H_loc.col.data(), H.H_loc.val.data(),
-                                      H.C_nloc.rowptr.data(), H.C_nloc.col.data(),
-                                      H.C_nloc.val.data(), H.inv_C_lm.data()));
-        H.rankTuckerBasisX = dims[0];
-        H.rankTuckerBasisY = dims[1];
-        H.rankTuckerBasisZ = dims[2];
-
         DeviceContext ctx(0);
+        ProjectHamiltonianSparse H;
+        AssemblyInputs asmin;
+        if (assembly) {
+            // ---- H from assembly inputs (Create_Hloc / Create_Cnloc) ----
+            asmin = read_assembly(assembly);
+            for (int d = 0; d < 3; ++d) dims[d] = asmin.r[d];
+            if (!device_assembly) {
+                H.Create_Hloc(ctx, dims[0], dims[1], dims[2], asmin.hf, asmin.truncation);
+                H.Create_Cnloc(ctx, asmin.pf, asmin.c_truncation);
+                printf("assembled H: nnz=%lld nnz_c=%lld\n", (long long)H.H_loc.nnz(),
+                       (long long)H.C_nloc.nnz());
+            }
+        } else {
+            // ---- synthetic projected Hamiltonian (host CSR) ----
+            ttdft_synth_params prm{dims[0], dims[1], dims[2], bands[0], bands[1], bands[2],
+                                   nproj,   h,       1e-6,    seed};
+            int64_t nnz = 0, nnz_c = 0;
+            check(ttdft_synth_hamiltonian(&prm, &nnz, &nnz_c, nullptr, nullptr, nullptr,
+                                          nullptr, nullptr, nullptr, nullptr));
+            const int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
+            H.H_loc.rows = H.H_loc.cols = n;
+            H.H_loc.rowptr.resize(n + 1);
+            H.H_loc.col.resize(nnz);
+            H.H_loc.val.resize(nnz);
+            H.C_nloc.rows = nproj;
+            H.C_nloc.cols = n;
+            H.C_nloc.rowptr.resize(nproj + 1);
+            H.C_nloc.col.resize(nnz_c);
+            H.C_nloc.val.resize(nnz_c);
+            H.inv_C_lm.resize(nproj);
+            check(ttdft_synth_hamiltonian(&prm, &nnz, &nnz_c, H.H_loc.rowptr.data(),
+                                          H.H_loc.col.data(), H.H_loc.val.data(),
+                                          H.C_nloc.rowptr.data(), H.C_nloc.col.data(),
+                                          H.C_nloc.val.data(), H.inv_C_lm.data()));
+            H.rankTuckerBasisX = dims[0];
+            H.rankTuckerBasisY = dims[1];
+            H.rankTuckerBasisZ = dims[2];
+        }
+        const int64_t n = (int64_t)dims[0] * dims[1] * dims[2];
+
         const int dtype = f32 ? TTDFT_F32 : TTDFT_F64;
         MultTM multtm(ctx);
         EigenSolver eigenSolver;
@@ -99,7 +192,17 @@ int main(int argc, char **argv) {
         const int maxLanczosIter = 20;
 
         AX ax(1, k);
-        ax.setup_system(ctx, H, dtype == TTDFT_F64 ? TTDFT_HAM_F64 : (TTDFT_HAM_F64 | TTDFT_HAM_F32));
+        const int copies = dtype == TTDFT_F64 ? TTDFT_HAM_F64 : (TTDFT_HAM_F64 | TTDFT_HAM_F32);
+        if (assembly && device_assembly) {
+            ax.assemble_system(ctx, dims[0], dims[1], dims[2], asmin.hf, asmin.truncation,
+                               asmin.pf, asmin.c_truncation, copies);
+            int64_t hn = 0, hnnz = 0;
+            int32_t hp = 0;
+            check(ttdft_ham_info(ax.handle(), &hn, &hnnz, &hp));
+            printf("assembled H on the device: nnz=%lld p=%d\n", (long long)hnnz, hp);
+        } else {
+            ax.setup_system(ctx, H, copies);
+        }
         DenseBlock HP(ctx, n, k, dtype);
         std::vector<double> Q((size_t)k * k);
         double *dQ = nullptr;

@@ -1,6 +1,7 @@
 // Implementation of the reference-shaped C++ host layer over the C ABI.
 #include "ttdft_host.h"
 
+#include <algorithm>
 #include <cstring>
 
 namespace ttdft_host {
@@ -60,6 +61,22 @@ void AX::setup_system(DeviceContext &ctx, const ProjectHamiltonianSparse &H, int
                                p, crp, cc, cv, ic, copies, &ham_));
 }
 
+void AX::assemble_system(DeviceContext &ctx, int32_t rx, int32_t ry, int32_t rz,
+                         const HlocFactors &f, double truncation, const ProjectorFactors &pf,
+                         double c_truncation, int copies) {
+    free_mata_on_device();
+    ctx_ = &ctx;
+    const int32_t p = (int32_t)pf.ranks.size();
+    check(ttdft_ham_assemble_tucker(
+        ctx.handle(), rx, ry, rz, f.rankVeffX, f.rankVeffY, f.rankVeffZ, f.MatX.data(),
+        f.MatDX.data(), f.MatY.data(), f.MatDY.data(), f.MatZ.data(), f.MatDZ.data(),
+        f.MatPotX.data(), f.MatPotY.data(), f.MatPotZ.data(), f.coreVeff.data(), truncation, p,
+        p ? pf.ranks.data() : nullptr, p ? pf.cores.data() : nullptr,
+        p ? pf.factorX.data() : nullptr, p ? pf.factorY.data() : nullptr,
+        p ? pf.factorZ.data() : nullptr, c_truncation, p ? pf.inv_C_lm.data() : nullptr, copies,
+        &ham_));
+}
+
 void AX::perform_ax(DenseBlock &X, DenseBlock &AXout) {
     if (!ham_) throw Error(TTDFT_EBADARG, "perform_ax before setup_system");
     check(ttdft_ham_apply(ctx_->handle(), ham_, X.data(), X.ld(), AXout.data(), AXout.ld(),
@@ -69,6 +86,58 @@ void AX::perform_ax(DenseBlock &X, DenseBlock &AXout) {
 void AX::free_mata_on_device() {
     if (ham_) ttdft_ham_free(ham_);
     ham_ = nullptr;
+}
+
+// ---- ProjectHamiltonianSparse ------------------------------------------------------
+void ProjectHamiltonianSparse::Create_Hloc(DeviceContext &ctx, int32_t rx, int32_t ry, int32_t rz,
+                                           const HlocFactors &f, double truncation) {
+    const int64_t n = (int64_t)rx * ry * rz;
+    auto call = [&](int64_t *nnz, int64_t *rp, int32_t *c, double *v) {
+        check(ttdft_assemble_hloc(ctx.handle(), rx, ry, rz, f.rankVeffX, f.rankVeffY,
+                                  f.rankVeffZ, f.MatX.data(), f.MatDX.data(), f.MatY.data(),
+                                  f.MatDY.data(), f.MatZ.data(), f.MatDZ.data(),
+                                  f.MatPotX.data(), f.MatPotY.data(), f.MatPotZ.data(),
+                                  f.coreVeff.data(), truncation, nnz, rp, c, v));
+    };
+    int64_t nnz = 0;
+    call(&nnz, nullptr, nullptr, nullptr);  // Compute_nnz_loc
+    H_loc.rows = H_loc.cols = n;
+    H_loc.rowptr.assign(n + 1, 0);
+    H_loc.col.resize(std::max<int64_t>(nnz, 1));
+    H_loc.val.resize(std::max<int64_t>(nnz, 1));
+    call(&nnz, H_loc.rowptr.data(), H_loc.col.data(), H_loc.val.data());
+    H_loc.col.resize(nnz);
+    H_loc.val.resize(nnz);
+    rankTuckerBasisX = rx;
+    rankTuckerBasisY = ry;
+    rankTuckerBasisZ = rz;
+}
+
+void ProjectHamiltonianSparse::Create_Cnloc(DeviceContext &ctx, const ProjectorFactors &f,
+                                            double truncation) {
+    const int32_t p = (int32_t)f.ranks.size();
+    if (rankTuckerBasisX <= 0) throw Error(TTDFT_EBADARG, "Create_Cnloc before Create_Hloc");
+    if ((int32_t)f.inv_C_lm.size() != p) throw Error(TTDFT_EBADARG, "Create_Cnloc: inv_C_lm size");
+    auto call = [&](int64_t *nnz, int64_t *rp, int32_t *c, double *v) {
+        check(ttdft_assemble_projectors(ctx.handle(), p, p ? f.ranks.data() : nullptr,
+                                        rankTuckerBasisX, rankTuckerBasisY, rankTuckerBasisZ,
+                                        p ? f.cores.data() : nullptr,
+                                        p ? f.factorX.data() : nullptr,
+                                        p ? f.factorY.data() : nullptr,
+                                        p ? f.factorZ.data() : nullptr, truncation, nnz, rp, c,
+                                        v));
+    };
+    int64_t nnz = 0;
+    call(&nnz, nullptr, nullptr, nullptr);
+    C_nloc.rows = p;
+    C_nloc.cols = (int64_t)rankTuckerBasisX * rankTuckerBasisY * rankTuckerBasisZ;
+    C_nloc.rowptr.assign(p + 1, 0);
+    C_nloc.col.resize(std::max<int64_t>(nnz, 1));
+    C_nloc.val.resize(std::max<int64_t>(nnz, 1));
+    call(&nnz, C_nloc.rowptr.data(), C_nloc.col.data(), C_nloc.val.data());
+    C_nloc.col.resize(nnz);
+    C_nloc.val.resize(nnz);
+    inv_C_lm = f.inv_C_lm;
 }
 
 // ---- ChebyShevFilter -------------------------------------------------------------

@@ -81,11 +81,40 @@ private:
     void *ptr_ = nullptr;
 };
 
+// What Create_Hloc builds each row from (dft/ProjectHamiltonianSparse.cc:330-575):
+// the 1-D overlap and kinetic matrices of the Tucker basis (MatX[I][J],
+// MatDX, ... row-major r x r), the 1-D potential matrices MatPotX[p][I][J]
+// (rankVeffX x rx x rx), and the Veff Tucker core, first index fastest.
+// The quadrature that forms them from the FEM basis (:577-775) stays with the
+// caller.
+struct HlocFactors {
+    int32_t rankVeffX = 0, rankVeffY = 0, rankVeffZ = 0;
+    std::vector<double> MatX, MatDX, MatY, MatDY, MatZ, MatDZ;
+    std::vector<double> MatPotX, MatPotY, MatPotZ;
+    std::vector<double> coreVeff;
+};
+
+// What Create_Cnloc builds each projector row from (:196-278): per row a
+// Tucker core (r^3, first index fastest) and column-major factors (rx x r,
+// ry x r, rz x r), concatenated over the rows, and the row's 1/C_lm.
+struct ProjectorFactors {
+    std::vector<int32_t> ranks;
+    std::vector<double> cores, factorX, factorY, factorZ;
+    std::vector<double> inv_C_lm;
+};
+
 struct ProjectHamiltonianSparse {
     CsrMatrix H_loc;                // n x n, symmetric
     CsrMatrix C_nloc;               // p x n projector rows, not row-scaled
     std::vector<double> inv_C_lm;   // p
     int32_t rankTuckerBasisX = 0, rankTuckerBasisY = 0, rankTuckerBasisZ = 0;
+
+    // dft/ProjectHamiltonianSparse.cc:330-575 (with Compute_nnz_loc, :1223-1592):
+    // H_loc assembled on the device, entries |h| > truncation, into host CSR
+    void Create_Hloc(DeviceContext &ctx, int32_t rankTuckerBasisX, int32_t rankTuckerBasisY,
+                     int32_t rankTuckerBasisZ, const HlocFactors &f, double truncation);
+    // :83-328: projector rows assembled on the device into host CSR (unscaled)
+    void Create_Cnloc(DeviceContext &ctx, const ProjectorFactors &f, double truncation);
 };
 
 // eigensolver/Mult.h:30-114
@@ -99,6 +128,12 @@ public:
     // perform_ax applies the whole operator the filter needs.
     void setup_system(DeviceContext &ctx, const ProjectHamiltonianSparse &H,
                       int copies = TTDFT_HAM_F64);
+    // Create_Hloc + Create_Cnloc + setup_system in one step, with H built in
+    // device memory (no host CSR, no upload)
+    void assemble_system(DeviceContext &ctx, int32_t rankTuckerBasisX, int32_t rankTuckerBasisY,
+                         int32_t rankTuckerBasisZ, const HlocFactors &hf, double truncation,
+                         const ProjectorFactors &pf, double c_truncation,
+                         int copies = TTDFT_HAM_F64);
     // AX = H X (written, unlike the published perform_ax, MultCuda.cu:719-820)
     void perform_ax(DenseBlock &X, DenseBlock &AXout);
     void free_mata_on_device();
