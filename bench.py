@@ -163,6 +163,74 @@ def assembly_cpu_sample(hl, rows=120):
     return rows / dt, dt
 
 
+MIX_NODES = 301      # FEM nodes per axis: 60 elements of order 5
+MIX_HISTORY = 10     # the example deck's history and alpha
+MIX_ALPHA = 0.5
+
+
+def bench_mixing(ctx, E, reps=3):
+    """Anderson mixing with its history in HBM: 600^3 quadrature-grid and 301^3
+    nodal densities, 10 history entries (synthetic seeded fields)."""
+    ng, nn = QUAD ** 3, MIX_NODES ** 3
+    dj, dw = E.DeviceArray(ctx, (ng,)), E.DeviceArray(ctx, (ng,))
+    ctx.fill_uniform(dj.ptr, ng, 1, 0.5, 2.0)
+    ctx.fill_uniform(dw.ptr, ng, 2, 0.1, 1.0)
+    mix = E.Mixer(ctx, ng, nn, dj.ptr, dw.ptr, MIX_ALPHA, MIX_HISTORY)
+    bufs = [E.DeviceArray(ctx, (n,)) for n in (nn, ng, nn, ng)]  # nodal/grid in, nodal/grid out
+    for k in range(MIX_HISTORY):
+        for q, (b, n) in enumerate(zip(bufs, (nn, ng, nn, ng))):
+            ctx.fill_uniform(b.ptr, n, 100 + 4 * k + q, 0.5, 1.5)
+        mix.update(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, k)
+    mix.compute_rho_in(MIX_HISTORY, bufs[0].ptr, bufs[1].ptr)  # warm-up
+    ctx.synchronize()
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    t = time.perf_counter()
+    for _ in range(reps):
+        mix.compute_rho_in(MIX_HISTORY, bufs[0].ptr, bufs[1].ptr)
+    ctx.synchronize()
+    wall = (time.perf_counter() - t) / reps
+    st = ctx.kernel_stats(E.KSTAT_MIXING)
+    ctx.set_profiling(False)
+    mix.free()
+    for b in bufs + [dj, dw]:
+        b.free()
+    dev_ms = st["ms"] / reps
+    gbs = st["bytes"] / (st["ms"] * 1e-3) / 1e9
+    return {
+        "workload": "Anderson mixing step, history 10 in HBM: 600^3 grid + 301^3 nodal "
+                    "densities, alpha 0.5 (synthetic seeded fields)",
+        "ms_per_mixing": round(1e3 * wall, 2),
+        "device_ms": round(dev_ms, 2),
+        "grid_points_per_s": round(ng / wall, 1),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "mix_gram_kernel (FP64 MFMA Gram of the differences) + mix_combine_kernel",
+            "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bytes_per_mixing": st["bytes"] / reps,
+        },
+    }
+
+
+def mixing_cpu_sample(n=8_000_000):
+    """Oracle restatement of AndersonMixing (one core) on n grid points with
+    the same history length."""
+    import oracle
+
+    rng = np.random.default_rng(5)
+    rin = rng.uniform(0.5, 1.5, (MIX_HISTORY, n))
+    rout = rng.uniform(0.5, 1.5, (MIX_HISTORY, n))
+    jac, wts = rng.uniform(0.5, 2.0, n), rng.uniform(0.1, 1.0, n)
+    t0 = time.perf_counter()
+    c = oracle.anderson_constants(rin, rout, jac, wts)
+    oracle.anderson_mix(rin, rout, c, MIX_ALPHA)
+    dt = time.perf_counter() - t0
+    return n / dt, dt
+
+
 def profiled_traffic(kernel_tag):
     """HBM bytes per launch of the FP64 band SpMM from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a block copy)."""
@@ -187,10 +255,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-density", action="store_true")
-    ap.add_argument("--mode", default="step", choices=["step", "spmm", "assembly"],
+    ap.add_argument("--mode", default="step", choices=["step", "spmm", "assembly", "mixing"],
                     help="spmm: only 3 FP64 H applications and 3 copies of one 125000 x 300 "
                          "FP64 block (copy = calibration of rocprofv3 byte counters); "
-                         "assembly: only the device-resident H assembly (3 runs)")
+                         "assembly: only the device-resident H assembly (3 runs); "
+                         "mixing: only the Anderson mixing step (4 runs)")
+    ap.add_argument("--no-mixing", action="store_true",
+                    help="skip the Anderson mixing measurement")
     ap.add_argument("--no-assembly", action="store_true",
                     help="skip the H assembly measurement")
     args = ap.parse_args()
@@ -205,6 +276,10 @@ def main():
         ctx = E.Context(0)
         _, res = bench_assembly(ctx, E, reps=2)
         log(json.dumps(res))
+        return
+    if args.mode == "mixing":
+        ctx = E.Context(0)
+        log(json.dumps(bench_mixing(ctx, E)))
         return
 
     t0 = time.perf_counter()
@@ -369,6 +444,21 @@ def main():
                           f"(dense dyad + three mode products), {dt:.1f} s",
             }
         result["assembly"] = asm
+    if not args.no_mixing:
+        if dRho is not None:
+            dRho.free()
+        mixres = bench_mixing(ctx, E)
+        if not args.no_cpu:
+            rate, dt = mixing_cpu_sample()
+            mixres["cpu_baseline"] = {
+                "value": round(rate, 1),
+                "unit": "grid points/s",
+                "cores": 1,
+                "kind": "port",
+                "sample": f"8e6 grid points, history 10, by the C oracle's restatement of "
+                          f"computeMixingConstants + computeRhoIn (grid only), {dt:.1f} s",
+            }
+        result["mixing"] = mixres
     ctx.synchronize()
     print(json.dumps(result))
 

@@ -71,11 +71,13 @@ enum {
     TTDFT_KSTAT_ROTATE = 3,  /* X Q                                       */
     TTDFT_KSTAT_DENSITY = 4, /* fused density contraction                 */
     TTDFT_KSTAT_ASSEMBLY = 5, /* H_loc assembly (device part of each pass) */
-    TTDFT_KSTAT_COUNT = 6
+    TTDFT_KSTAT_MIXING = 6,  /* Anderson mixing: Gram pass and combination */
+    TTDFT_KSTAT_COUNT = 7
 };
 
 typedef struct ttdft_ctx_s *ttdft_ctx;
 typedef struct ttdft_ham_s *ttdft_ham;
+typedef struct ttdft_mixer_s *ttdft_mixer;
 
 /* ---- errors ---------------------------------------------------------------- */
 /* Message of the last failure on this thread ("" if none). */
@@ -261,6 +263,10 @@ int ttdft_grid_residual(ttdft_ctx ctx, const double *d_f, const double *d_g,
                         const double *dwy, const double *dwz, double *result);
 /* f *= s in place (renormalisation of main.cc:674-680). */
 int ttdft_grid_scale(ttdft_ctx ctx, double *d_f, int64_t count, double s);
+/* f[i] = lo + (hi - lo) u_i, u_i uniform in [0, 1) from a hash of (seed, i):
+ * synthetic device fields for benchmarks and tests (no reference counterpart). */
+int ttdft_fill_uniform(ttdft_ctx ctx, double *d_f, int64_t count, uint64_t seed, double lo,
+                       double hi);
 
 /* ---- the ChFSI driver block --------------------------------------------------- */
 /* State main.cc carries between SCF iterations (lines 730-734, 685-688). */
@@ -365,6 +371,33 @@ int ttdft_ham_assemble_tucker(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz,
                               const int32_t *ranks, const double *cores, const double *mx,
                               const double *my, const double *mz, double c_truncation,
                               const double *inv_c_lm, int copies, ttdft_ham *out);
+
+/* ---- Anderson density mixing (history in device memory) --------------------
+ * One mixer = the AndersonMixing object of main.cc:698-707: densities on the
+ * quadrature grid (n_grid values) and at the FEM nodes (n_nodal, may be 0),
+ * device arrays.  The history stays in device memory (no files); restart from
+ * files is not provided.  jacobian and weights are grid fields (jacob3DMat,
+ * weight3DMat) copied at creation; max_history <= 64.
+ * replaces AndersonMixing::AndersonMixing (dft/AndersonMixing.cc:51-147) */
+int ttdft_mixer_create(ttdft_ctx ctx, int64_t n_grid, int64_t n_nodal, const double *d_jacob,
+                       const double *d_weight, double alpha, int32_t max_history,
+                       ttdft_mixer *out);
+int ttdft_mixer_destroy(ttdft_mixer m);
+/* replaces AndersonMixing::updateRho (:376-443): appends copies of the four
+ * densities, dropping the oldest entry beyond max_history. */
+int ttdft_mixer_update(ttdft_mixer m, const double *d_rho_nodal_in, const double *d_rho_grid_in,
+                       const double *d_rho_nodal_out, const double *d_rho_grid_out,
+                       int32_t scf_iter);
+/* replaces AndersonMixing::computeRhoIn(scfIter, rhoNodalIn, rhoGridIn)
+ * (:325-374, with computeMixingConstants :150-262 and the mix :264-323):
+ * overwrites both densities in place. */
+int ttdft_mixer_compute_rho_in(ttdft_mixer m, int32_t scf_iter, double *d_rho_nodal_in,
+                               double *d_rho_grid_in);
+/* The constants of the last Anderson step (count = history size - 1). */
+int ttdft_mixer_constants(ttdft_mixer m, double *constants, int32_t *count);
+/* replaces AndersonMixing::clearHistory (:445-461) */
+int ttdft_mixer_clear_history(ttdft_mixer m, int32_t scf_iter);
+int ttdft_mixer_history_size(ttdft_mixer m, int32_t *size);
 
 /* Host random streams (same generator the engine uses internally). */
 int ttdft_rng_uniform(uint64_t seed, int64_t count, double *out);

@@ -124,6 +124,12 @@ def lib():
                                               ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double,
                                               _i64p, _i32p, _dp]
         L.orc_assemble_projectors.restype = ctypes.c_int64
+        L.orc_anderson_constants.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp,
+                                             _dp]
+        L.orc_anderson_constants.restype = ctypes.c_int
+        L.orc_anderson_mix.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp,
+                                       ctypes.c_double, _dp]
+        L.orc_anderson_mix.restype = None
         L.orc_rng_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
         L.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
         _lib = L
@@ -425,3 +431,31 @@ def assemble_projectors(cores, mx, my, mz, truncation):
     val = np.zeros(max(nnz, 1))
     L.orc_assemble_projectors(*args, _i64(rowptr), _i32(col), _d(val))
     return rowptr, col[:nnz], val[:nnz]
+
+
+def anderson_constants(rho_in, rho_out, jacob, weight):
+    """Mixing constants of AndersonMixing.cc:150-262.  rho_in / rho_out are
+    (h, n) arrays, oldest history entry first."""
+    rin = np.ascontiguousarray(rho_in, dtype=np.float64)
+    rout = np.ascontiguousarray(rho_out, dtype=np.float64)
+    h, n = rin.shape
+    j = np.ascontiguousarray(jacob, dtype=np.float64)
+    w = np.ascontiguousarray(weight, dtype=np.float64)
+    c = np.zeros(max(h - 1, 1))
+    info = lib().orc_anderson_constants(h, n, _d(rin), _d(rout), _d(j), _d(w), _d(c))
+    if info != 0:
+        raise np.linalg.LinAlgError(f"singular mixing system (pivot {info})")
+    return c[: h - 1]
+
+
+def anderson_mix(rho_in, rho_out, constants, alpha):
+    """Mixed density of AndersonMixing.cc:264-323 for (h, n) histories."""
+    rin = np.ascontiguousarray(rho_in, dtype=np.float64)
+    rout = np.ascontiguousarray(rho_out, dtype=np.float64)
+    h, n = rin.shape
+    c = np.ascontiguousarray(constants, dtype=np.float64)
+    if c.size == 0:
+        c = np.zeros(1)
+    rho = np.empty(n)
+    lib().orc_anderson_mix(h, n, _d(rin), _d(rout), _d(c), float(alpha), _d(rho))
+    return rho

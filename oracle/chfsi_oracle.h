@@ -192,6 +192,18 @@ int64_t orc_assemble_projectors(int p, const int32_t *ranks, int rx, int ry, int
                                 const double *mz, double truncation, int64_t *rowptr,
                                 int32_t *col, double *val);
 
+/* ---- Anderson mixing (mixing_oracle.c) ---------------------------------------
+ * History entries oldest first: in[j*n + i], out[j*n + i], j < h.
+ * orc_anderson_constants: the h-1 constants of AndersonMixing.cc:150-262
+ * (returns the LU info, 0 on success).  orc_anderson_mix: the mixed density
+ * of :264-323.  Pinned: closed form at h = 2, the normal equations of the
+ * weighted least-squares problem, and the exact fixed point of an affine map
+ * (test_oracle_mixing.py). */
+int orc_anderson_constants(int h, int64_t n, const double *in, const double *out,
+                           const double *jacob, const double *weight, double *constants);
+void orc_anderson_mix(int h, int64_t n, const double *in, const double *out,
+                      const double *constants, double alpha, double *rho);
+
 #ifdef __cplusplus
 }
 #endif

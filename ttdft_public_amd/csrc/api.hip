@@ -28,6 +28,7 @@ int grid_reduce(ttdft_ctx ctx, const double *f, const double *g, int32_t px, int
 int dev_dot(ttdft_ctx ctx, const double *a, const double *b, int64_t n, double *result);
 int dev_axpby(ttdft_ctx ctx, double *y, double a, const double *x, double b, int64_t n);
 int dev_scale(ttdft_ctx ctx, double *f, int64_t n, double s);
+int dev_fill_uniform(ttdft_ctx ctx, double *f, int64_t n, uint64_t seed, double lo, double hi);
 int launch_cm_to_bm(ttdft_ctx ctx, const double *src, int64_t lda, int64_t n, int32_t k,
                     void *dst, int64_t ldd, int dtype);
 int launch_bm_to_cm(ttdft_ctx ctx, const void *src, int64_t lds, int dtype, int64_t n,
@@ -877,6 +878,14 @@ int ttdft_grid_residual(ttdft_ctx ctx, const double *d_f, const double *d_g, int
     TT_CHECK(ctx && d_f && d_g && dwx && dwy && dwz && result, TTDFT_EBADARG,
              "residual: bad arguments");
     return grid_reduce(ctx, d_f, d_g, px, py, pz, dwx, dwy, dwz, 1, result);
+}
+
+int ttdft_fill_uniform(ttdft_ctx ctx, double *d_f, int64_t count, uint64_t seed, double lo,
+                       double hi) {
+    clear_error();
+    TT_CHECK(ctx && (d_f || count == 0) && count >= 0, TTDFT_EBADARG, "fill_uniform: bad arguments");
+    if (count == 0) return TTDFT_OK;
+    return dev_fill_uniform(ctx, d_f, count, seed, lo, hi);
 }
 
 int ttdft_grid_scale(ttdft_ctx ctx, double *d_f, int64_t count, double s) {

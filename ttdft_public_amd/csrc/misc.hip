@@ -220,6 +220,30 @@ int dev_axpby(ttdft_ctx ctx, double *y, double a, const double *x, double b, int
     return TTDFT_OK;
 }
 
+// f[i] = lo + (hi - lo) u_i, u_i in [0, 1) from a splitmix64 hash of (seed, i):
+// the same numbers for any grid shape or launch.
+__global__ __launch_bounds__(256) void fill_uniform_kernel(double *__restrict__ f, int64_t n,
+                                                           uint64_t seed, double lo,
+                                                           double hi) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed + 0x9e3779b97f4a7c15ull * (uint64_t)(i + 1);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        f[i] = lo + (hi - lo) * ((double)(z >> 11) * 0x1.0p-53);
+    }
+}
+
+int dev_fill_uniform(ttdft_ctx ctx, double *f, int64_t n, uint64_t seed, double lo, double hi) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, f,
+                       n, seed, lo, hi);
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
+}
+
 int dev_scale(ttdft_ctx ctx, double *f, int64_t n, double s) {
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;

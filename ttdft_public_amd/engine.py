@@ -23,7 +23,8 @@ F64, F32 = 0, 1
 GEMM_F64, GEMM_BF16X3 = 0, 1
 DEGREE_REFERENCE, DEGREE_EXACT = 0, 1
 HAM_F64, HAM_F32 = 1, 2
-KSTAT_SPMM, KSTAT_NLPROJ, KSTAT_GRAM, KSTAT_ROTATE, KSTAT_DENSITY, KSTAT_ASSEMBLY = range(6)
+KSTAT_SPMM, KSTAT_NLPROJ, KSTAT_GRAM, KSTAT_ROTATE, KSTAT_DENSITY, KSTAT_ASSEMBLY, KSTAT_MIXING = \
+    range(7)
 
 _c_dp = ctypes.POINTER(ctypes.c_double)
 _c_i64p = ctypes.POINTER(ctypes.c_int64)
@@ -114,6 +115,7 @@ _SIGS = {
     "ttdft_grid_residual": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp,
                                            _c_dp]),
     "ttdft_grid_scale": (ctypes.c_int, [_vp, _vp, _i64, _d]),
+    "ttdft_fill_uniform": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_uint64, _d, _d]),
     "ttdft_chfsi_state_init": (ctypes.c_int, [ctypes.POINTER(ChfsiState)]),
     "ttdft_chfsi_step": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i32,
                                         _i32, _i32, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
@@ -132,6 +134,14 @@ _SIGS = {
                                   + [_c_dp] * 10 + [_d, _i32, _c_i32p, _c_dp, _c_dp, _c_dp,
                                                     _c_dp, _d, _c_dp, ctypes.c_int,
                                                     ctypes.POINTER(_vp)]),
+    "ttdft_mixer_create": (ctypes.c_int, [_vp, _i64, _i64, _vp, _vp, _d, _i32,
+                                          ctypes.POINTER(_vp)]),
+    "ttdft_mixer_destroy": (ctypes.c_int, [_vp]),
+    "ttdft_mixer_update": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32]),
+    "ttdft_mixer_compute_rho_in": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
+    "ttdft_mixer_constants": (ctypes.c_int, [_vp, _c_dp, _c_i32p]),
+    "ttdft_mixer_clear_history": (ctypes.c_int, [_vp, _i32]),
+    "ttdft_mixer_history_size": (ctypes.c_int, [_vp, _c_i32p]),
     "ttdft_rng_uniform": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
     "ttdft_rng_normal": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
 }
@@ -365,6 +375,9 @@ class Context:
                                          dwz, ctypes.byref(out)))
         return out.value
 
+    def fill_uniform(self, d_f, count, seed, lo=0.0, hi=1.0):
+        _check(lib().ttdft_fill_uniform(self.h, d_f, count, seed, float(lo), float(hi)))
+
     def grid_scale(self, d_f, count, s):
         _check(lib().ttdft_grid_scale(self.h, d_f, count, s))
 
@@ -551,6 +564,51 @@ class DeviceArray:
         if self.ptr:
             _check(lib().ttdft_free(self.ctx.h, self.ptr))
             self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Mixer:
+    """Anderson density mixing with the history in device memory (ttdft_mixer,
+    the AndersonMixing object of main.cc:698-707).  Densities are device
+    pointers: grid (n_grid values) and nodal (n_nodal, may be 0)."""
+
+    def __init__(self, ctx, n_grid, n_nodal, d_jacob, d_weight, alpha, max_history):
+        h = _vp()
+        _check(lib().ttdft_mixer_create(ctx.h, n_grid, n_nodal, d_jacob, d_weight, float(alpha),
+                                        max_history, ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx
+
+    def update(self, nodal_in, grid_in, nodal_out, grid_out, scf_iter):
+        _check(lib().ttdft_mixer_update(self.h, nodal_in, grid_in, nodal_out, grid_out, scf_iter))
+
+    def compute_rho_in(self, scf_iter, nodal_in, grid_in):
+        _check(lib().ttdft_mixer_compute_rho_in(self.h, scf_iter, nodal_in, grid_in))
+
+    def constants(self):
+        n = ctypes.c_int32()
+        _check(lib().ttdft_mixer_constants(self.h, None, ctypes.byref(n)))
+        c = np.zeros(max(n.value, 1))
+        _check(lib().ttdft_mixer_constants(self.h, _dptr(c), ctypes.byref(n)))
+        return c[: n.value]
+
+    def clear_history(self, scf_iter):
+        _check(lib().ttdft_mixer_clear_history(self.h, scf_iter))
+
+    def history_size(self):
+        n = ctypes.c_int32()
+        _check(lib().ttdft_mixer_history_size(self.h, ctypes.byref(n)))
+        return n.value
+
+    def free(self):
+        if self.h:
+            _check(lib().ttdft_mixer_destroy(self.h))
+            self.h = None
 
     def __del__(self):
         try:
