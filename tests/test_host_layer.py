@@ -112,3 +112,45 @@ def test_driver_on_assembled_hamiltonian(tmp_path, device):
     args = ["--assembly", path] + (["--device-assembly"] if device else [])
     out = run_driver_against_oracle(args, H, n, k, m, nelec, nscf)
     assert ("assembled H on the device" in out) == device
+
+
+@pytest.mark.gpu
+def test_driver_anderson_mixing_matches_oracle(tmp_path):
+    """ttdft_host::AndersonMixing over an 8-step density sequence with history
+    4 (entries dropped), against the oracle's mixing of the same sequence."""
+    rng = np.random.default_rng(21)
+    ng, nn, nsteps, hmax, alpha = 5000, 700, 8, 4, 0.5
+    jac, wts = rng.uniform(0.5, 2.0, ng), rng.uniform(0.1, 1.0, ng)
+    seq = [(rng.uniform(-0.2, 1.5, nn), rng.uniform(-0.2, 1.5, ng),
+            rng.uniform(0.5, 1.5, nn), rng.uniform(0.5, 1.5, ng)) for _ in range(nsteps)]
+    inp, outp = str(tmp_path / "mix_in.bin"), str(tmp_path / "mix_out.bin")
+    with open(inp, "wb") as f:
+        np.array([ng, nn], np.int64).tofile(f)
+        np.array([nsteps, hmax], np.int32).tofile(f)
+        np.array([alpha]).tofile(f)
+        jac.tofile(f)
+        wts.tofile(f)
+        for step in seq:
+            for a in step:
+                a.tofile(f)
+    r = subprocess.run([DRIVER, "--mix", inp, outp], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(outp).reshape(nsteps, nn + ng)
+    hist = []
+    for s, (ni, gi, no, go) in enumerate(seq):
+        if s == 0:
+            en, eg = np.abs(ni), np.abs(gi)
+        elif s == 1:
+            en = alpha * hist[-1][2] + (1 - alpha) * hist[-1][0]
+            eg = alpha * hist[-1][3] + (1 - alpha) * hist[-1][1]
+        else:
+            c = oracle.anderson_constants(np.stack([e[1] for e in hist]),
+                                          np.stack([e[3] for e in hist]), jac, wts)
+            en = oracle.anderson_mix(np.stack([e[0] for e in hist]),
+                                     np.stack([e[2] for e in hist]), c, alpha)
+            eg = oracle.anderson_mix(np.stack([e[1] for e in hist]),
+                                     np.stack([e[3] for e in hist]), c, alpha)
+        assert np.abs(got[s, :nn] - en).max() <= 1e-11 * np.abs(en).max(), s
+        assert np.abs(got[s, nn:] - eg).max() <= 1e-11 * np.abs(eg).max(), s
+        hist.append((got[s, :nn], got[s, nn:], no, go))
+        hist = hist[-hmax:]

@@ -8,6 +8,15 @@
 //                [--m M] [--nelec NE] [--scf S] [--h H] [--seed S] [--f32]
 //                [--assembly FILE [--device-assembly]]
 //
+//   chfsi_driver --mix IN OUT
+//
+// --mix runs AndersonMixing (main.cc:698-707, 862-864, 1437-1442) over a
+// density sequence and exits.  IN: int64 n_grid n_nodal, int32 nsteps
+// max_history, float64 alpha, jacobian[n_grid], weight[n_grid], then per step
+// rho_nodal_in rho_grid_in rho_nodal_out rho_grid_out.  Step s sets rho_in by
+// computeRhoIn(s) from the given rho_in, then pushes (mixed rho_in, rho_out)
+// with updateRho.  OUT: per step the mixed rho_nodal_in, rho_grid_in.
+//
 // --assembly FILE takes H from assembly inputs instead of the synthetic
 // operator: Create_Hloc + Create_Cnloc into host CSR, then setup_system; with
 // --device-assembly, AX::assemble_system builds it in device memory.  FILE is
@@ -15,6 +24,7 @@
 // truncation, c_truncation, then float64 X DX Y DY Z DZ (r x r each),
 // PotX PotY PotZ (rv x r x r), core (first index fastest), projector cores,
 // factors X, Y, Z (column-major, concatenated), inv_C_lm[p].
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -87,6 +97,67 @@ AssemblyInputs read_assembly(const char *path) {
     return a;
 }
 
+int run_mixing(const char *in_path, const char *out_path) {
+    FILE *f = fopen(in_path, "rb");
+    if (!f) throw Error(TTDFT_EBADARG, std::string("cannot open ") + in_path);
+    FILE *o = nullptr;
+    try {
+        std::vector<int64_t> sizes;
+        std::vector<int32_t> steps;
+        std::vector<double> alpha, jac, wts;
+        read_into(f, sizes, 2);
+        read_into(f, steps, 2);
+        read_into(f, alpha, 1);
+        const int64_t ng = sizes[0], nn = sizes[1];
+        read_into(f, jac, ng);
+        read_into(f, wts, ng);
+        DeviceContext ctx(0);
+        double *d[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        const int64_t len[6] = {ng, ng, nn, nn, ng, nn};  // jac wts nodal_in nodal_out grid_in grid_out
+        for (int q = 0; q < 6; ++q)
+            check(ttdft_malloc(ctx.handle(), sizeof(double) * std::max<int64_t>(1, len[q]),
+                               (void **)&d[q]));
+        double *dgrid_out = nullptr;
+        check(ttdft_malloc(ctx.handle(), sizeof(double) * ng, (void **)&dgrid_out));
+        check(ttdft_memcpy_h2d(ctx.handle(), d[0], jac.data(), sizeof(double) * ng));
+        check(ttdft_memcpy_h2d(ctx.handle(), d[1], wts.data(), sizeof(double) * ng));
+        o = fopen(out_path, "wb");
+        if (!o) throw Error(TTDFT_EBADARG, std::string("cannot open ") + out_path);
+        {
+            AndersonMixing mixer(ctx, ng, nn, d[0], d[1], alpha[0], steps[1]);
+            std::vector<double> ni, gi, no, go;
+            for (int s = 0; s < steps[0]; ++s) {
+                read_into(f, ni, nn);
+                read_into(f, gi, ng);
+                read_into(f, no, nn);
+                read_into(f, go, ng);
+                if (nn) check(ttdft_memcpy_h2d(ctx.handle(), d[2], ni.data(), sizeof(double) * nn));
+                check(ttdft_memcpy_h2d(ctx.handle(), d[4], gi.data(), sizeof(double) * ng));
+                mixer.computeRhoIn(s, nn ? d[2] : nullptr, d[4]);
+                if (nn) check(ttdft_memcpy_d2h(ctx.handle(), ni.data(), d[2], sizeof(double) * nn));
+                check(ttdft_memcpy_d2h(ctx.handle(), gi.data(), d[4], sizeof(double) * ng));
+                if (fwrite(ni.data(), sizeof(double), nn, o) != (size_t)nn ||
+                    fwrite(gi.data(), sizeof(double), ng, o) != (size_t)ng)
+                    throw Error(TTDFT_EBADARG, "mix output: write failed");
+                if (nn) check(ttdft_memcpy_h2d(ctx.handle(), d[3], no.data(), sizeof(double) * nn));
+                check(ttdft_memcpy_h2d(ctx.handle(), dgrid_out, go.data(), sizeof(double) * ng));
+                mixer.updateRho(nn ? d[2] : nullptr, d[4], nn ? d[3] : nullptr, dgrid_out, s);
+                const std::vector<double> c = mixer.mixingConstants();
+                printf("step %d: %zu mixing constants\n", s, c.size());
+            }
+        }
+        for (int q = 0; q < 6; ++q) ttdft_free(ctx.handle(), d[q]);
+        ttdft_free(ctx.handle(), dgrid_out);
+    } catch (...) {
+        fclose(f);
+        if (o) fclose(o);
+        throw;
+    }
+    fclose(f);
+    fclose(o);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -94,7 +165,7 @@ int main(int argc, char **argv) {
     int nproj = 5, k = 16, m = 12, nelec = 19, nscf = 6, f32 = 0;
     double h = 2.0;
     unsigned long long seed = 20261202ULL;
-    const char *assembly = nullptr;
+    const char *assembly = nullptr, *mix_in = nullptr, *mix_out = nullptr;
     int device_assembly = 0;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--dims") && i + 3 < argc) {
@@ -121,12 +192,16 @@ int main(int argc, char **argv) {
             assembly = argv[++i];
         } else if (!strcmp(argv[i], "--device-assembly")) {
             device_assembly = 1;
+        } else if (!strcmp(argv[i], "--mix") && i + 2 < argc) {
+            mix_in = argv[++i];
+            mix_out = argv[++i];
         } else {
             fprintf(stderr, "unknown argument %s\n", argv[i]);
             return 2;
         }
     }
     try {
+        if (mix_in) return run_mixing(mix_in, mix_out);
         DeviceContext ctx(0);
         ProjectHamiltonianSparse H;
         AssemblyInputs asmin;

@@ -140,6 +140,31 @@ void ProjectHamiltonianSparse::Create_Cnloc(DeviceContext &ctx, const ProjectorF
     inv_C_lm = f.inv_C_lm;
 }
 
+// ---- AndersonMixing -----------------------------------------------------------------
+AndersonMixing::AndersonMixing(DeviceContext &ctx, int64_t n_grid, int64_t n_nodal,
+                               const double *d_jacob, const double *d_weight, double alpha,
+                               int maxIterHistory) {
+    check(ttdft_mixer_create(ctx.handle(), n_grid, n_nodal, d_jacob, d_weight, alpha,
+                             maxIterHistory, &m_));
+}
+AndersonMixing::~AndersonMixing() { ttdft_mixer_destroy(m_); }
+void AndersonMixing::computeRhoIn(int scfIter, double *rhoNodalIn, double *rhoGridIn) {
+    check(ttdft_mixer_compute_rho_in(m_, scfIter, rhoNodalIn, rhoGridIn));
+}
+void AndersonMixing::updateRho(const double *rhoNodalIn, const double *rhoGridIn,
+                               const double *rhoNodalOut, const double *rhoGridOut,
+                               int scfIter) {
+    check(ttdft_mixer_update(m_, rhoNodalIn, rhoGridIn, rhoNodalOut, rhoGridOut, scfIter));
+}
+void AndersonMixing::clearHistory(int scf_iter) { check(ttdft_mixer_clear_history(m_, scf_iter)); }
+std::vector<double> AndersonMixing::mixingConstants() const {
+    int32_t n = 0;
+    check(ttdft_mixer_constants(m_, nullptr, &n));
+    std::vector<double> c(n);
+    check(ttdft_mixer_constants(m_, c.data(), &n));
+    return c;
+}
+
 // ---- ChebyShevFilter -------------------------------------------------------------
 void ChebyShevFilter::computeFilteredSubspace(ProjectHamiltonianSparse &H, AX &ax, DenseBlock &X,
                                               int basisSize, int m, double a, double b,

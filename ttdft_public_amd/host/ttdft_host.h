@@ -147,6 +147,26 @@ private:
     ttdft_ham ham_ = nullptr;
 };
 
+// dft/AndersonMixing.h:29-92: Anderson mixing of the nodal and quadrature-grid
+// densities, history in device memory.  Densities are device arrays
+// (n_nodal and n_grid doubles); jacobian and weights are device grid fields.
+class AndersonMixing {
+public:
+    AndersonMixing(DeviceContext &ctx, int64_t n_grid, int64_t n_nodal, const double *d_jacob,
+                   const double *d_weight, double alpha, int maxIterHistory = 40);
+    ~AndersonMixing();
+    AndersonMixing(const AndersonMixing &) = delete;
+    AndersonMixing &operator=(const AndersonMixing &) = delete;
+    void computeRhoIn(int scfIter, double *rhoNodalIn, double *rhoGridIn);
+    void updateRho(const double *rhoNodalIn, const double *rhoGridIn, const double *rhoNodalOut,
+                   const double *rhoGridOut, int scfIter);
+    void clearHistory(int scf_iter);
+    std::vector<double> mixingConstants() const;
+
+private:
+    ttdft_mixer m_ = nullptr;
+};
+
 // eigensolver/ChebyShevFilter.h:29-40
 class ChebyShevFilter {
 public:
