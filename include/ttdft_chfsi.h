@@ -72,7 +72,9 @@ enum {
     TTDFT_KSTAT_DENSITY = 4, /* fused density contraction                 */
     TTDFT_KSTAT_ASSEMBLY = 5, /* H_loc assembly (device part of each pass) */
     TTDFT_KSTAT_MIXING = 6,  /* Anderson mixing: Gram pass and combination */
-    TTDFT_KSTAT_COUNT = 7
+    TTDFT_KSTAT_STHOSVD = 7, /* Tucker recompression of a 3-D field        */
+    TTDFT_KSTAT_HARTREE = 8, /* Hartree potential by kernel expansion      */
+    TTDFT_KSTAT_COUNT = 9
 };
 
 typedef struct ttdft_ctx_s *ttdft_ctx;
@@ -398,6 +400,34 @@ int ttdft_mixer_constants(ttdft_mixer m, double *constants, int32_t *count);
 /* replaces AndersonMixing::clearHistory (:445-461) */
 int ttdft_mixer_clear_history(ttdft_mixer m, int32_t scf_iter);
 int ttdft_mixer_history_size(ttdft_mixer m, int32_t *size);
+
+/* ---- Tucker recompression and Hartree potential ------------------------------
+ * ttdft_sthosvd: fixed-rank sequentially truncated HOSVD of a device tensor T
+ * (n0 x n1 x n2, first index fastest), modes 0, 1, 2 in order, the algorithm
+ * of TuckerMPI::STHOSVD that tensor/TensorUtils.cc:245-336 calls on the nodal
+ * density.  Outputs (device): core r0 x r1 x r2 (first index fastest),
+ * factors U_n (n_n x r_n, column-major; the largest |entry| of each column is
+ * positive).  evals (host, r0 + r1 + r2, may be NULL): the kept Gram
+ * eigenvalues per mode, descending.  The interpolation of the factors from
+ * FEM nodes to quadrature points (TensorUtils.cc:282-318) is the caller's.
+ * ttdft_hartree_expansion replaces KSDFTPotential::computeHartreePotNew
+ * (dft/KSDFTPotential.cc:269-340, with :440-668):
+ *   V (+)= sum_i omega_i  core x_0 F0_i x_1 F1_i x_2 F2_i,
+ *   F_d,i[q][a] = sum_e exp(-alpha_i (xq_d[q] - xe_d[e])^2) U_d[e][a] jw_d[e]
+ * with U_d (ne_d x r_d, column-major) at the electrostatic quadrature points
+ * xe_d (jacobian x weight jw_d), targets xq_d (p_d points), V p0 x p1 x p2
+ * first index fastest.  alpha / omega are host arrays used as given (the
+ * reference scales them by 1/A^2 and 1/A after reading, :281-284).  add = 0
+ * overwrites V (INSERT_VALUES), 1 accumulates.  All other arrays are device
+ * memory; the [3] arrays are host arrays of per-axis values or pointers. */
+int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int32_t n2,
+                  int32_t r0, int32_t r1, int32_t r2, double *d_core, double *d_U0, double *d_U1,
+                  double *d_U2, double *evals);
+int ttdft_hartree_expansion(ttdft_ctx ctx, const int32_t r[3], const double *d_core,
+                            const double *const d_U[3], const int32_t ne[3],
+                            const double *const d_xe[3], const double *const d_jw[3],
+                            const int32_t p[3], const double *const d_xq[3], int32_t nterms,
+                            const double *alpha, const double *omega, double *d_V, int add);
 
 /* Host random streams (same generator the engine uses internally). */
 int ttdft_rng_uniform(uint64_t seed, int64_t count, double *out);

@@ -130,6 +130,14 @@ def lib():
         L.orc_anderson_mix.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp,
                                        ctypes.c_double, _dp]
         L.orc_anderson_mix.restype = None
+        L.orc_sthosvd.argtypes = [ctypes.c_int] * 6 + [_dp] * 6
+        L.orc_sthosvd.restype = ctypes.c_int
+        _dpa = ctypes.POINTER(_dp)
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_hartree_expansion.argtypes = [ctypes.c_int] * 3 + [_dp, _dpa, _ip, _dpa, _dpa, _ip,
+                                                                 _dpa, ctypes.c_int, _dp, _dp,
+                                                                 _dp, ctypes.c_int]
+        L.orc_hartree_expansion.restype = None
         L.orc_rng_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
         L.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
         _lib = L
@@ -459,3 +467,51 @@ def anderson_mix(rho_in, rho_out, constants, alpha):
     rho = np.empty(n)
     lib().orc_anderson_mix(h, n, _d(rin), _d(rout), _d(c), float(alpha), _d(rho))
     return rho
+
+
+def sthosvd(T, ranks):
+    """Fixed-rank STHOSVD of a 3-D array T (indexed [i0, i1, i2]): returns
+    (core [a, b, c], [U0, U1, U2] with U_n of shape (n_n, r_n), eigenvalues
+    per mode)."""
+    T = np.asarray(T, dtype=np.float64)
+    n0, n1, n2 = T.shape
+    r0, r1, r2 = ranks
+    tf = np.ascontiguousarray(T.transpose(2, 1, 0)).reshape(-1)  # first index fastest
+    core = np.zeros(r0 * r1 * r2)
+    U = [np.zeros(n * r) for n, r in zip(T.shape, ranks)]
+    ev = np.zeros(r0 + r1 + r2)
+    rc = lib().orc_sthosvd(n0, n1, n2, r0, r1, r2, _d(tf), _d(core), _d(U[0]), _d(U[1]),
+                           _d(U[2]), _d(ev))
+    if rc != 0:
+        raise RuntimeError("orc_sthosvd: eigen-solver did not converge")
+    G = core.reshape(r2, r1, r0).transpose(2, 1, 0)
+    Us = [u.reshape(r, n).T for u, n, r in zip(U, T.shape, ranks)]
+    return G, Us, [ev[:r0], ev[r0:r0 + r1], ev[r0 + r1:]]
+
+
+def hartree_expansion(G, U, xe, jw, xq, alpha, omega, V=None):
+    """Kernel-expansion Hartree potential (KSDFTPotential.cc:269-340,440-668).
+    G: core [a, b, c]; U[d]: (ne_d, r_d) factor values at the electrostatic
+    quadrature points xe[d] with jacobian x weight jw[d]; xq[d]: target points.
+    Returns V indexed [x, y, z] (added to V when given)."""
+    G = np.asarray(G, dtype=np.float64)
+    r0, r1, r2 = G.shape
+    gf = np.ascontiguousarray(G.transpose(2, 1, 0)).reshape(-1)
+    keep = []
+    def arr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _d(a)
+    Up = (_dp * 3)(*[arr(np.asarray(u).T.reshape(-1)) for u in U])  # column-major
+    xep = (_dp * 3)(*[arr(x) for x in xe])
+    jwp = (_dp * 3)(*[arr(x) for x in jw])
+    xqp = (_dp * 3)(*[arr(x) for x in xq])
+    ne = (ctypes.c_int * 3)(*[len(x) for x in xe])
+    p = (ctypes.c_int * 3)(*[len(x) for x in xq])
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    w = np.ascontiguousarray(omega, dtype=np.float64)
+    out = np.zeros(p[0] * p[1] * p[2]) if V is None else \
+        np.ascontiguousarray(np.asarray(V, np.float64).transpose(2, 1, 0)).reshape(-1)
+    lib().orc_hartree_expansion(r0, r1, r2, _d(gf), Up, ne, xep, jwp, p, xqp, len(a), _d(a),
+                                _d(w), _d(out), 0 if V is None else 1)
+    return out.reshape(p[2], p[1], p[0]).transpose(2, 1, 0)

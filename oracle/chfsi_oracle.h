@@ -204,6 +204,21 @@ int orc_anderson_constants(int h, int64_t n, const double *in, const double *out
 void orc_anderson_mix(int h, int64_t n, const double *in, const double *out,
                       const double *constants, double alpha, double *rho);
 
+/* ---- Tucker recompression and Hartree potential (tucker_oracle.c) -----------
+ * orc_sthosvd: fixed-rank STHOSVD (TuckerMPI's algorithm), modes 0, 1, 2;
+ * tensors first index fastest, factors column-major, largest |entry| of each
+ * factor column positive.  orc_hartree_expansion: the kernel-expansion
+ * Hartree potential of KSDFTPotential.cc:269-340,440-668.  Pinned: exact
+ * recovery of multilinear-rank tensors, numpy SVD of the unfoldings, core
+ * all-orthogonality; per-term Gaussian closed forms and the Coulomb potential
+ * of a Gaussian charge (test_oracle_tucker.py). */
+int orc_sthosvd(int n0, int n1, int n2, int r0, int r1, int r2, const double *T, double *core,
+                double *U0, double *U1, double *U2, double *evals);
+void orc_hartree_expansion(int r0, int r1, int r2, const double *G, const double *const U[3],
+                           const int ne[3], const double *const xe[3], const double *const jw[3],
+                           const int p[3], const double *const xq[3], int nterms,
+                           const double *alpha, const double *omega, double *V, int add);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,8 +23,8 @@ F64, F32 = 0, 1
 GEMM_F64, GEMM_BF16X3 = 0, 1
 DEGREE_REFERENCE, DEGREE_EXACT = 0, 1
 HAM_F64, HAM_F32 = 1, 2
-KSTAT_SPMM, KSTAT_NLPROJ, KSTAT_GRAM, KSTAT_ROTATE, KSTAT_DENSITY, KSTAT_ASSEMBLY, KSTAT_MIXING = \
-    range(7)
+(KSTAT_SPMM, KSTAT_NLPROJ, KSTAT_GRAM, KSTAT_ROTATE, KSTAT_DENSITY, KSTAT_ASSEMBLY, KSTAT_MIXING,
+ KSTAT_STHOSVD, KSTAT_HARTREE) = range(9)
 
 _c_dp = ctypes.POINTER(ctypes.c_double)
 _c_i64p = ctypes.POINTER(ctypes.c_int64)
@@ -142,6 +142,12 @@ _SIGS = {
     "ttdft_mixer_constants": (ctypes.c_int, [_vp, _c_dp, _c_i32p]),
     "ttdft_mixer_clear_history": (ctypes.c_int, [_vp, _i32]),
     "ttdft_mixer_history_size": (ctypes.c_int, [_vp, _c_i32p]),
+    "ttdft_sthosvd": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp,
+                                     _vp, _vp, _c_dp]),
+    "ttdft_hartree_expansion": (ctypes.c_int, [_vp, _c_i32p, _vp, ctypes.POINTER(_vp), _c_i32p,
+                                               ctypes.POINTER(_vp), ctypes.POINTER(_vp), _c_i32p,
+                                               ctypes.POINTER(_vp), _i32, _c_dp, _c_dp, _vp,
+                                               ctypes.c_int]),
     "ttdft_rng_uniform": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
     "ttdft_rng_normal": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
 }
@@ -374,6 +380,29 @@ class Context:
         _check(lib().ttdft_grid_residual(self.h, d_f, d_g, pdims[0], pdims[1], pdims[2], dwx, dwy,
                                          dwz, ctypes.byref(out)))
         return out.value
+
+    def sthosvd(self, d_T, dims, ranks, d_core, d_U):
+        """Fixed-rank STHOSVD of a device tensor (first index fastest) into the
+        device core and factors d_U[0..2]; returns the kept Gram eigenvalues."""
+        ev = np.zeros(sum(ranks))
+        _check(lib().ttdft_sthosvd(self.h, d_T, *dims, *ranks, d_core, d_U[0], d_U[1], d_U[2],
+                                   _dptr(ev)))
+        return ev
+
+    def hartree_expansion(self, ranks, d_core, d_U, d_xe, d_jw, ne, d_xq, p, alpha, omega, d_V,
+                          add=False):
+        """Kernel-expansion Hartree potential into the device grid d_V."""
+        r = (ctypes.c_int32 * 3)(*ranks)
+        nes = (ctypes.c_int32 * 3)(*ne)
+        ps = (ctypes.c_int32 * 3)(*p)
+        U = (_vp * 3)(*d_U)
+        xe = (_vp * 3)(*d_xe)
+        jw = (_vp * 3)(*d_jw)
+        xq = (_vp * 3)(*d_xq)
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        w = np.ascontiguousarray(omega, dtype=np.float64)
+        _check(lib().ttdft_hartree_expansion(self.h, r, d_core, U, nes, xe, jw, ps, xq, len(a),
+                                             _dptr(a), _dptr(w), d_V, 1 if add else 0))
 
     def fill_uniform(self, d_f, count, seed, lo=0.0, hi=1.0):
         _check(lib().ttdft_fill_uniform(self.h, d_f, count, seed, float(lo), float(hi)))
