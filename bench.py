@@ -231,6 +231,97 @@ def mixing_cpu_sample(n=8_000_000):
     return n / dt, dt
 
 
+RHO_RANK = 40        # Tucker ranks of the density (example deck: decomposition rank rho 40)
+ELECTRO_PTS = 840    # electrostatic quadrature points per axis: 60 elements x 14 points
+N_TERMS = 35         # exponential-sum terms of the Coulomb kernel
+
+
+def hartree_inputs():
+    """Expansion coefficients (the example's 35-term fit of 1/r, exponents
+    alphak35 / A^2, weights omegak35 / A) and Gauss points: 600 targets and
+    840 electrostatic points per axis on [-40, 40] bohr."""
+    golden = os.path.join(ROOT, "tests", "golden", "kernel_expansion_k35.json")
+    d = json.load(open(golden))
+    alpha = np.array(d["alphak35_1e8"]) / d["Asquare"]
+    omega = np.array(d["omegak35_1e8"]) / np.sqrt(d["Asquare"])
+    pts, _, _ = basis_tables(DIMS[0], QUAD, 3)
+    x, w = np.polynomial.legendre.leggauss(14)
+    edges = np.linspace(-40.0, 40.0, ELECTRO_PTS // 14 + 1)
+    xe = np.concatenate([0.5 * (b - a) * x + 0.5 * (b + a) for a, b in zip(edges[:-1], edges[1:])])
+    jw = np.concatenate([0.5 * (b - a) * w for a, b in zip(edges[:-1], edges[1:])])
+    return alpha, omega, pts, xe, jw
+
+
+def bench_hartree(ctx, E, reps=3):
+    """STHOSVD of the 301^3 nodal density to ranks 40^3, then the Hartree
+    potential on the 600^3 grid from 35 expansion terms (synthetic fields)."""
+    alpha, omega, pts, xe, jw = hartree_inputs()
+    nn, r = MIX_NODES, RHO_RANK
+    dT = E.DeviceArray(ctx, (nn ** 3,))
+    ctx.fill_uniform(dT.ptr, nn ** 3, 7, 0.0, 1.0)
+    dG = E.DeviceArray(ctx, (r ** 3,))
+    dUn = [E.DeviceArray(ctx, (nn * r,)) for _ in range(3)]
+    dUe = [E.DeviceArray(ctx, (ELECTRO_PTS * r,)) for _ in range(3)]
+    for q, u in enumerate(dUe):  # factors at the electrostatic points (FEM interpolation skipped)
+        ctx.fill_uniform(u.ptr, ELECTRO_PTS * r, 20 + q, -1.0, 1.0)
+    dxe, djw, dxq = (E.DeviceArray.from_host(ctx, a) for a in (xe, jw, pts))
+    dV = E.DeviceArray(ctx, (QUAD ** 3,))
+
+    def once():
+        ctx.sthosvd(dT.ptr, (nn,) * 3, (r,) * 3, dG.ptr, [u.ptr for u in dUn])
+        ctx.hartree_expansion((r,) * 3, dG.ptr, [u.ptr for u in dUe], [dxe.ptr] * 3,
+                              [djw.ptr] * 3, (ELECTRO_PTS,) * 3, [dxq.ptr] * 3, (QUAD,) * 3,
+                              alpha, omega, dV.ptr)
+
+    once()  # warm-up
+    ctx.synchronize()
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    t = time.perf_counter()
+    for _ in range(reps):
+        once()
+    ctx.synchronize()
+    wall = (time.perf_counter() - t) / reps
+    sv, ha = ctx.kernel_stats(E.KSTAT_STHOSVD), ctx.kernel_stats(E.KSTAT_HARTREE)
+    ctx.set_profiling(False)
+    for b in [dT, dG, dV, dxe, djw, dxq] + dUn + dUe:
+        b.free()
+    tf = ha["flops"] / (ha["ms"] * 1e-3) / 1e12
+    return {
+        "workload": "STHOSVD of the 301^3 nodal density to ranks 40^3 + Hartree potential on "
+                    "600^3 points from 35 kernel-expansion terms, 840 electrostatic points "
+                    "per axis (synthetic fields)",
+        "ms_per_potential": round(1e3 * wall, 2),
+        "sthosvd_ms": round(sv["ms"] / reps, 2),
+        "hartree_ms": round(ha["ms"] / reps, 2),
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "hartree expansion (its last product: depth 35 x 40, rocBLAS FP64 MFMA)",
+            "achieved": round(tf, 2),
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+            "flops_per_potential": ha["flops"] / reps,
+        },
+    }
+
+
+def hartree_cpu_sample(p=40):
+    """Oracle restatement of computeHartreePotNew (one core) on a p^3 target
+    grid with the workload's ranks, terms and electrostatic points."""
+    import oracle
+
+    alpha, omega, _, xe, jw = hartree_inputs()
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((RHO_RANK,) * 3)
+    U = [rng.uniform(-1, 1, (ELECTRO_PTS, RHO_RANK)) for _ in range(3)]
+    xq = [np.linspace(-38.0, 38.0, p)] * 3
+    t0 = time.perf_counter()
+    oracle.hartree_expansion(G, U, [xe] * 3, [jw] * 3, xq, alpha, omega)
+    dt = time.perf_counter() - t0
+    return p ** 3 / dt, dt
+
+
 def profiled_traffic(kernel_tag):
     """HBM bytes per launch of the FP64 band SpMM from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a block copy)."""
@@ -255,11 +346,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-density", action="store_true")
-    ap.add_argument("--mode", default="step", choices=["step", "spmm", "assembly", "mixing"],
+    ap.add_argument("--mode", default="step",
+                    choices=["step", "spmm", "assembly", "mixing", "hartree"],
                     help="spmm: only 3 FP64 H applications and 3 copies of one 125000 x 300 "
                          "FP64 block (copy = calibration of rocprofv3 byte counters); "
                          "assembly: only the device-resident H assembly (3 runs); "
-                         "mixing: only the Anderson mixing step (4 runs)")
+                         "mixing: only the Anderson mixing step (4 runs); "
+                         "hartree: only STHOSVD + Hartree potential (4 runs)")
+    ap.add_argument("--no-hartree", action="store_true",
+                    help="skip the STHOSVD + Hartree measurement")
     ap.add_argument("--no-mixing", action="store_true",
                     help="skip the Anderson mixing measurement")
     ap.add_argument("--no-assembly", action="store_true",
@@ -280,6 +375,10 @@ def main():
     if args.mode == "mixing":
         ctx = E.Context(0)
         log(json.dumps(bench_mixing(ctx, E)))
+        return
+    if args.mode == "hartree":
+        ctx = E.Context(0)
+        log(json.dumps(bench_hartree(ctx, E)))
         return
 
     t0 = time.perf_counter()
@@ -459,6 +558,21 @@ def main():
                           f"computeMixingConstants + computeRhoIn (grid only), {dt:.1f} s",
             }
         result["mixing"] = mixres
+    if not args.no_hartree:
+        hres = bench_hartree(ctx, E)
+        if not args.no_cpu:
+            rate, dt = hartree_cpu_sample()
+            hres["cpu_baseline"] = {
+                "value": round(rate, 1),
+                "unit": "grid points/s",
+                "cores": 1,
+                "kind": "port",
+                "sample": f"40^3 targets, ranks 40^3, 35 terms, 840 electrostatic points per "
+                          f"axis, by the C oracle's restatement of computeHartreePotNew, "
+                          f"{dt:.1f} s (the device figure is 600^3 / ms_per_potential)",
+            }
+        hres["grid_points_per_s"] = round(QUAD ** 3 / (hres["ms_per_potential"] * 1e-3), 1)
+        result["hartree"] = hres
     ctx.synchronize()
     print(json.dumps(result))
 
