@@ -127,6 +127,28 @@ def test_band_spmm_matches_oracle(ctx, dims, bands, k, dtype):
     assert rel(dY.to_host().astype(np.float64), ref) < (1e-13 if dtype == E.F64 else 1e-5)
 
 
+@pytest.mark.parametrize("dims,bands", [((20, 18, 16), (6, 6, 7)), ((13, 11, 9), (7, 3, 2)),
+                                         ((9, 10, 7), (1, 2, 2))])
+@pytest.mark.parametrize("k", [257, 300, 400])
+def test_band_spmm_state_groups_in_one_workgroup(ctx, monkeypatch, dims, bands, k):
+    """FP64 with more than 256 states runs all state groups of a row block in
+    one workgroup (x-chunks of 3 bricks, the last one ragged); it must give
+    the same bits as one workgroup per state group, and match the oracle."""
+    s = E.SynthHam(*dims, *bands, 4, seed=sum(dims) + k)
+    H = oracle.Ham.from_synth(s)
+    ham = ctx.ham_upload(s, E.HAM_F64, tucker=True)
+    assert ham.format()[0] == 1
+    X = np.random.default_rng(k).standard_normal((s.n, k))
+    dX, dY = dev(ctx, X), E.DeviceArray(ctx, (s.n, k))
+    out = {}
+    for co in ("1", "0"):
+        monkeypatch.setenv("TTDFT_BRICK_COGROUPS", co)
+        ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, E.F64)
+        out[co] = dY.to_host()
+    assert np.array_equal(out["1"], out["0"])
+    assert rel(out["1"], oracle.ham_apply(H, X)) < 1e-13
+
+
 def test_band_filter_with_holes_matches_oracle(ctx):
     """Truncation leaves holes inside the band: stored as zeros, same result."""
     s = E.SynthHam(10, 10, 10, 4, 4, 4, 6, truncation=2e-5, seed=3)

@@ -23,6 +23,7 @@
 // is round-robin over the 8 XCDs), keeping neighbouring lines' X in one L2.
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -175,25 +176,36 @@ __device__ __forceinline__ void lds_read_n(const T *p, T (&v)[N]) {
 // KS: 4-column MFMA steps covering the x' window (4 KS >= 4 + 2 bx).
 // NST: 16-state tiles per workgroup.  DB: bytes per lane of one global->LDS
 // load (16 when a row of X is 16-byte aligned, else 4).
-template <typename T, int KS, int NST, int DB, int DEPTH>
+// GXM: most x-bricks per workgroup (16, or 8 for half the LDS and two
+// workgroups per CU).  G: state groups per workgroup.  G = 1: one wave per
+// x-brick, S = 16 NST states.  G > 1: G waves per x-brick, one per group of S
+// states, all G S states of a line staged together, so the G waves of a brick
+// load the same A fragments at the same line step (the fragments come from
+// HBM once instead of once per state group); workgroups then hold exactly GXM
+// x-bricks.
+template <typename T, int KS, int NST, int DB, int DEPTH, int GXM, int G>
 __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     typedef typename Acc4<T>::type acc_t;
-    constexpr int S = 16 * NST;
+    constexpr int S = 16 * NST;                  // states per wave
+    constexpr int ST = G * S;                    // states per staged row
     constexpr int ES = sizeof(T);
     constexpr int CH = 64 * DB;                  // bytes one wave loads per instruction
-    constexpr int TRMAX = 4 * 15 + 4 * KS;       // staged rows at 16 x-bricks
-    constexpr int BUF = (TRMAX * S * ES + CH - 1) / CH * CH;
-    constexpr int NQ = DB == 16 ? 4 : 16;        // loads per wave per line (fixed count)
+    constexpr int TRMAX = 4 * (GXM - 1) + 4 * KS;  // staged rows at GXM x-bricks
+    constexpr int BUF = (TRMAX * ST * ES + CH - 1) / CH * CH;
+    // loads per wave per line (a fixed count): G = 1 keeps at least 4 waves
+    // and 4 + KS - 1 rows per wave; G > 1 runs G GXM waves
+    constexpr int NQ = G == 1 ? (DB == 16 ? 4 : 16) : (BUF / CH + G * GXM - 1) / (G * GXM);
     constexpr int VE = DB >= ES ? DB / ES : 1;   // elements per lane-load
     constexpr int NB = DEPTH + 1;  // LDS buffers: lines i .. i+DEPTH
-    static_assert(DEPTH == 2 || DEPTH == 3, "2 or 3 lines in flight");
+    static_assert(DEPTH >= 1 && DEPTH <= 3, "1 to 3 lines in flight");
+    static_assert((4 * (KS - 1) * ST + 16 * (NST - 1)) * ES < 65536, "ds offset field");
     __shared__ __attribute__((aligned(16))) unsigned char lb0[BUF];
     __shared__ __attribute__((aligned(16))) unsigned char lb1[BUF];
-    __shared__ __attribute__((aligned(16))) unsigned char lb2[BUF];
+    __shared__ __attribute__((aligned(16))) unsigned char lb2[NB >= 3 ? BUF : 16];
     __shared__ __attribute__((aligned(16))) unsigned char lb3[NB == 4 ? BUF : 16];
-    const int nw = a.gxb + a.nsplit;
+    const int nw = G == 1 ? a.gxb + a.nsplit : G * a.gxb;
     const int TR = 4 * (a.gxb - 1) + 4 * KS;  // staged x' rows
-    const int nch = (TR * S * ES + CH - 1) / CH;
+    const int nch = (TR * ST * ES + CH - 1) / CH;
     const int lane = threadIdx.x & 63;
     const int wl = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int li = lane & 15, lk = lane >> 4;
@@ -209,15 +221,18 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     t /= a.nxc;
     const int h = (int)(t % a.hyn);
     const int z = (int)(t / a.hyn);
-    const int s0 = sg * S;
+    const int s0 = sg * ST;
     // Waves are spread round-robin over the 4 SIMDs and the MFMA work of a
     // line ends at a barrier, so the SIMD with the most tile work sets the
     // pace.  With gxb = 1 or 2 (mod 4) the state tiles of the last one or two
     // x-bricks are split over two waves each (mode 1: lower tiles, mode 2:
     // upper tiles), which evens out the MFMA work per SIMD.
     constexpr int NH = NST - NST / 2;  // tiles of the lower half
-    int bl = wl, mode = 0;
-    if (wl >= a.gxb - a.nsplit) {
+    int bl = wl, mode = 0, sgw = 0;
+    if (G > 1) {
+        bl = wl / G;
+        sgw = wl - bl * G;
+    } else if (wl >= a.gxb - a.nsplit) {
         const int q = wl - (a.gxb - a.nsplit);
         bl = a.gxb - a.nsplit + (q >> 1);
         mode = 1 + (q & 1);
@@ -256,7 +271,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
         q = q < nch ? q : nch - 1;
         const int off = q * CH + lane * DB;  // byte offset in the tile
         const int e = off / ES;
-        const int row = e / S, s = e - row * S;
+        const int row = e / ST, s = e - row * ST;
         const int xp = xw0 + row;
         const int xpc = xp < 0 ? 0 : (xp >= a.rx ? a.rx - 1 : xp);
         int st = s0 + s;
@@ -306,16 +321,16 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
         constexpr int NT = HI - LO;
         switch (ks) {
         case 1:
-            if constexpr (KS > 1) lds_read_n<T, NT, (4 * 1 * S + 16 * LO) * ES, 16 * ES>(Xb, b[1 < KS ? 1 : 0]);
+            if constexpr (KS > 1) lds_read_n<T, NT, (4 * 1 * ST + 16 * LO) * ES, 16 * ES>(Xb, b[1 < KS ? 1 : 0]);
             break;
         case 2:
-            if constexpr (KS > 2) lds_read_n<T, NT, (4 * 2 * S + 16 * LO) * ES, 16 * ES>(Xb, b[2 < KS ? 2 : 0]);
+            if constexpr (KS > 2) lds_read_n<T, NT, (4 * 2 * ST + 16 * LO) * ES, 16 * ES>(Xb, b[2 < KS ? 2 : 0]);
             break;
         case 3:
-            if constexpr (KS > 3) lds_read_n<T, NT, (4 * 3 * S + 16 * LO) * ES, 16 * ES>(Xb, b[3 < KS ? 3 : 0]);
+            if constexpr (KS > 3) lds_read_n<T, NT, (4 * 3 * ST + 16 * LO) * ES, 16 * ES>(Xb, b[3 < KS ? 3 : 0]);
             break;
         case 4:
-            if constexpr (KS > 4) lds_read_n<T, NT, (4 * 4 * S + 16 * LO) * ES, 16 * ES>(Xb, b[4 < KS ? 4 : 0]);
+            if constexpr (KS > 4) lds_read_n<T, NT, (4 * 4 * ST + 16 * LO) * ES, 16 * ES>(Xb, b[4 < KS ? 4 : 0]);
             break;
         default:
             break;
@@ -339,7 +354,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
         }
     };
     auto compute = [&](const unsigned char *buf, const T *av) {
-        const T *Xb = reinterpret_cast<const T *>(buf) + (4 * bl + lk) * S + li;
+        const T *Xb = reinterpret_cast<const T *>(buf) + (4 * bl + lk) * ST + sgw * S + li;
         if (mode == 0)
             mfma_tiles(Xb, av, std::integral_constant<int, 0>(), std::integral_constant<int, NST>());
         else if (mode == 1)
@@ -368,11 +383,14 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
             return lb3;
     };
     constexpr int WAIT = (DEPTH - 1) * (NQ + KS);
+    constexpr int PERIOD = NB;
     T av[NB][KS];
     fetch(0, lb0);
     load_a(0, av[0]);
-    fetch(1, lb1);
-    load_a(1, av[1]);
+    if constexpr (DEPTH >= 2) {
+        fetch(1, lb1);
+        load_a(1, av[1]);
+    }
     if constexpr (DEPTH == 3) {
         fetch(2, lb2);
         load_a(2, av[2]);
@@ -387,13 +405,15 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
         TT_WAIT_VM_LDS(WAIT);
         line_barrier();
     };
-    for (int base = 0; base < nlines; base += NB) {
+    for (int base = 0; base < nlines; base += PERIOD) {
         step(base, std::integral_constant<int, 0>());
         if (base + 1 >= nlines) break;
         step(base + 1, std::integral_constant<int, 1>());
-        if (base + 2 >= nlines) break;
-        step(base + 2, std::integral_constant<int, 2>());
-        if constexpr (NB == 4) {
+        if constexpr (PERIOD >= 3) {
+            if (base + 2 >= nlines) break;
+            step(base + 2, std::integral_constant<int, 2>());
+        }
+        if constexpr (PERIOD >= 4) {
             if (base + 3 >= nlines) break;
             step(base + 3, std::integral_constant<int, 3>());
         }
@@ -407,7 +427,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
 #pragma unroll
     for (int st = 0; st < NST; ++st) {
         if ((mode == 1 && st >= NH) || (mode == 2 && st < NH)) continue;
-        const int s = s0 + 16 * st + li;
+        const int s = s0 + sgw * S + 16 * st + li;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = sizeof(T) == 8 ? (lk + 4 * v) : (4 * lk + v);
@@ -429,47 +449,99 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     }
 }
 
+// Workgroup shapes with all state groups co-resident (G > 1): FP64 3 x-bricks
+// x 5 groups of 64 states, one line in flight (a 320-state stage is 61-72 KB);
+// FP32 4 x-bricks x 4 groups of 80 states, two lines in flight.
+template <typename T, int V>
+struct CoShape;
+template <>
+struct CoShape<double, 0> {
+    static constexpr int GXM = 3, G = 5, DEPTH = 1;
+};
+template <int V>
+struct CoShape<float, V> {
+    static constexpr int GXM = 4, G = 4, DEPTH = 2;
+};
+
 template <typename T, int KS, int DB>
-int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a) {
+int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
     constexpr int NST = kStateTiles<T>();
+    const int64_t blocks = (a.nblocks + 7) / 8 * 8;
+    if (cogroups == 1) {
+        typedef CoShape<T, 0> C;
+        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, C::DEPTH, C::GXM, C::G>),
+                           dim3((unsigned)blocks), dim3(64 * C::G * C::GXM), 0, ctx->stream, a);
+        TT_LAUNCH_CHECK();
+        return TTDFT_OK;
+    }
     // Two lines in flight.  (Three fit in LDS for FP64 but measured slower
     // on the 172-atom workload: 8.2 ms against 7.7 ms per H application.)
     constexpr int DEPTH = 2;
     const int nthr = 64 * (a.gxb + a.nsplit);
-    const int64_t blocks = (a.nblocks + 7) / 8 * 8;
-    hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH>), dim3((unsigned)blocks),
-                       dim3(nthr), 0, ctx->stream, a);
+    if (a.gxb <= 8)
+        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 8, 1>),
+                           dim3((unsigned)blocks), dim3(nthr), 0, ctx->stream, a);
+    else
+        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 16, 1>),
+                           dim3((unsigned)blocks), dim3(nthr), 0, ctx->stream, a);
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
 
 template <typename T, int KS>
-int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a) {
-    TT_CHECK(a.gxb >= 1 && a.gxb + a.nsplit <= 16, TTDFT_EBADARG, "band SpMM: %d x-bricks per chunk",
-             a.gxb);
+int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
+    TT_CHECK(a.gxb >= 1 && (cogroups ? CoShape<T, 0>::G * a.gxb : a.gxb + a.nsplit) <= 16,
+             TTDFT_EBADARG, "band SpMM: %d x-bricks per chunk", a.gxb);
     TT_CHECK((int64_t)a.rx * a.ldx * (int64_t)sizeof(T) < ((int64_t)1 << 31), TTDFT_EBADARG,
              "band SpMM: one x-line of X exceeds 2 GiB");
     // 16-byte loads need every X row 16-byte aligned
     const bool wide = (a.ldx * (int64_t)sizeof(T)) % 16 == 0 &&
                       (reinterpret_cast<uintptr_t>(a.X) & 15) == 0;
-    if (wide) return launch_db<T, KS, 16>(ctx, a);
-    return launch_db<T, KS, 4>(ctx, a);
+    if (wide) return launch_db<T, KS, 16>(ctx, a, cogroups);
+    return launch_db<T, KS, 4>(ctx, a, cogroups);
 }
 
 template <typename T>
 int launch_brick(ttdft_ctx ctx, BrickArgs<T> a, int ks) {
-    a.nsg = (a.k + 16 * kStateTiles<T>() - 1) / (16 * kStateTiles<T>());
-    a.nsplit = a.gxb % 4 == 1 ? 1 : (a.gxb % 4 == 2 ? 2 : 0);
+    // x-bricks per workgroup: at most TTDFT_BRICK_CHUNK (16 by default; 8 halves
+    // the LDS stage so two workgroups share a CU)
+    int chunk = sizeof(T) == 8 ? 16 : 8;  // FP32: two workgroups per CU measured faster
+    if (const char *env = std::getenv("TTDFT_BRICK_CHUNK")) {
+        const int v = std::atoi(env);
+        if (v == 8 || v == 16) chunk = v;
+    }
+    // All state groups of a row block in one workgroup when there are more
+    // than G - 1 full groups: FP64 only (FP32 measured slower that way).
+    // TTDFT_BRICK_COGROUPS=0 turns it off, 1 on.
+    const int S = 16 * kStateTiles<T>();
+    int cogroups = (sizeof(T) == 8 && a.k > (CoShape<T, 0>::G - 1) * S) ? 1 : 0;
+    if (const char *env = std::getenv("TTDFT_BRICK_COGROUPS")) {
+        const int v = std::atoi(env);
+        if (a.k > (CoShape<T, 0>::G - 1) * S && (v == 0 || v == 1)) cogroups = v;
+    }
+    if (cogroups) {
+        a.gxb = CoShape<T, 0>::GXM;
+        a.nxc = (a.gxn + a.gxb - 1) / a.gxb;
+        a.nsplit = 0;
+        a.nsg = (a.k + CoShape<T, 0>::G * S - 1) / (CoShape<T, 0>::G * S);
+    } else {
+        a.nxc = (a.gxn + chunk - 1) / chunk;
+        a.gxb = std::max(4, (a.gxn + a.nxc - 1) / a.nxc);
+        if (a.gxb > chunk) a.gxb = chunk;
+        a.nxc = (a.gxn + a.gxb - 1) / a.gxb;
+        a.nsg = (a.k + S - 1) / S;
+        a.nsplit = a.gxb % 4 == 1 ? 1 : (a.gxb % 4 == 2 ? 2 : 0);
+    }
     a.nblocks = (int64_t)a.nsg * a.nxc * a.hyn * a.rz;
     switch (ks) {
     case 2:
-        return launch_ks<T, 2>(ctx, a);
+        return launch_ks<T, 2>(ctx, a, cogroups);
     case 3:
-        return launch_ks<T, 3>(ctx, a);
+        return launch_ks<T, 3>(ctx, a, cogroups);
     case 4:
-        return launch_ks<T, 4>(ctx, a);
+        return launch_ks<T, 4>(ctx, a, cogroups);
     case 5:
-        return launch_ks<T, 5>(ctx, a);
+        return launch_ks<T, 5>(ctx, a, cogroups);
     default:
         break;
     }
