@@ -429,6 +429,66 @@ int ttdft_hartree_expansion(ttdft_ctx ctx, const int32_t r[3], const double *d_c
                             const int32_t p[3], const double *const d_xq[3], int32_t nterms,
                             const double *alpha, const double *omega, double *d_V, int add);
 
+/* ---- one-dimensional finite elements and input files -------------------------- */
+/* Mesh types of FEM::generateNodes (fem/FEM.cc:166-375).  "adaptive2"
+ * (:219-254) is not offered: its construction writes and reads one entry past
+ * the end of its own element array. */
+enum {
+    TTDFT_MESH_UNIFORM = 0,
+    TTDFT_MESH_ADAPTIVE = 1,
+    TTDFT_MESH_MANUAL1 = 3,
+    TTDFT_MESH_HARTREE_OUTER = 4
+};
+/* n-point Gauss-Legendre rule on [-1, 1], points ascending (any n in 1..256). */
+int ttdft_fem_gauss_legendre(int32_t n, double *xi, double *w);
+/* The "<n>pt" rule of QuadratureRule::QuadratureRule (fem/QuadratureRule.cc:
+ * 25-560), 2 <= n <= 20, with the points in the order its tables list them. */
+int ttdft_fem_quadrature_rule(int32_t n, double *xi, double *w);
+/* Global node coordinates of nel elements of npe nodes, nel (npe - 1) + 1 of
+ * them (FEM::generateNumberNodes and FEM::generateNodes, fem/FEM.cc:159-375).
+ * inner_size, inner_h and coarsening are used by the non-uniform types only;
+ * the caller adjusts start and end as main.cc:128-151 does. */
+int ttdft_fem_mesh(int32_t type, int32_t nel, int32_t npe, double start, double end,
+                   double inner_size, double inner_h, double coarsening, double *nodes);
+/* Values and xi-derivatives at the quadrature points of nfun nodal fields
+ * (d_nodal: nfun x nnodes, one field per row; d_val, d_dval: nfun x (nel nq),
+ * device), plus the per-point weights jac w and w / jac and the positions
+ * (host, nel nq each, any may be NULL).  xi, w: the rule (host); nodes: the
+ * mesh (host).  replaces FEM::computeFieldAndDiffFieldAtAllQuadPoints
+ * (fem/FEM.cc:897-946), FEM::generateShapeFunction / ...Gradient (:418-528)
+ * and FEM::generateQuadPointData (:552-571). */
+int ttdft_fem_interpolate(ttdft_ctx ctx, int32_t nel, int32_t npe, int32_t nq, const double *xi,
+                          const double *w, const double *nodes, int32_t nfun,
+                          const double *d_nodal, double *d_val, double *d_dval, double *jw,
+                          double *inv_jw, double *xq);
+/* One axis of the projected Hamiltonian's 1-D matrices from r basis functions
+ * at nq quadrature points (d_B, d_DB: r x nq row-major = one function per row;
+ * d_jw, d_inv_jw: nq; d_U: nq x rv column-major Veff factor; all device):
+ *   MatX[I][J]      = sum_q B_I B_J jw
+ *   MatDX[I][J]     = sum_q B'_I B'_J inv_jw
+ *   MatPot[p][I][J] = sum_q U_p B_I B_J jw       (r x r each, row-major)
+ * exactly symmetric (upper triangle mirrored); NULL outputs are skipped.
+ * replaces ProjectHamiltonianSparse::computeOverlapKineticTuckerBasis and
+ * computeOverlapPotentialTuckerBasis (dft/ProjectHamiltonianSparse.cc:577-722). */
+int ttdft_tucker_basis_matrices(ttdft_ctx ctx, int32_t nq, int32_t r, const double *d_B,
+                                const double *d_DB, const double *d_jw, const double *d_inv_jw,
+                                int32_t rv, const double *d_U, double *d_MatX, double *d_MatDX,
+                                double *d_MatPot);
+/* Basis functions at the interior nodes, one per line, padded with a zero at
+ * both ends: nfun x nnodes row-major (values may be NULL to get the sizes).
+ * replaces BasisFunctionReader::copy_basis_function_from_file
+ * (basis/BasisFunctionReader.h:35-52). */
+int ttdft_read_basis_file(const char *path, int32_t *nfun, int32_t *nnodes, double *values);
+/* Atom file: "ntype natoms nelectrons", the lmax of each type, then one line
+ * "Z, x, y, z, _, Zatomic, type" per atom.  atoms: natoms x 7, grouped by
+ * type in type order as all_nuclei (atoms/AtomInformation.cc:28-88); lmax:
+ * ntype entries; either may be NULL. */
+int ttdft_read_atom_file(const char *path, int32_t *ntype, int32_t *natoms, int32_t *nelec,
+                         int32_t *lmax, double *atoms);
+/* Whitespace-separated numeric table (pseudopotential tables nlpV_*,
+ * nlpWaveFun_*, Density_*, locPotential_*): nrows x ncols row-major. */
+int ttdft_read_table(const char *path, int32_t *nrows, int32_t *ncols, double *values);
+
 /* Host random streams (same generator the engine uses internally). */
 int ttdft_rng_uniform(uint64_t seed, int64_t count, double *out);
 int ttdft_rng_normal(uint64_t seed, int64_t count, double *out);

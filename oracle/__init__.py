@@ -138,6 +138,15 @@ def lib():
                                                                  _dpa, ctypes.c_int, _dp, _dp,
                                                                  _dp, ctypes.c_int]
         L.orc_hartree_expansion.restype = None
+        L.orc_fem_mesh.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 5 + [_dp]
+        L.orc_fem_mesh.restype = ctypes.c_int64
+        L.orc_fem_shape.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_fem_shape.restype = None
+        L.orc_fem_interpolate.argtypes = [ctypes.c_int] * 3 + [_dp] * 3 + [ctypes.c_int] + [_dp] * 6
+        L.orc_fem_interpolate.restype = None
+        L.orc_tucker_basis_matrices.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 4 + [
+            ctypes.c_int] + [_dp] * 4
+        L.orc_tucker_basis_matrices.restype = None
         L.orc_rng_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
         L.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
         _lib = L
@@ -515,3 +524,66 @@ def hartree_expansion(G, U, xe, jw, xq, alpha, omega, V=None):
     lib().orc_hartree_expansion(r0, r1, r2, _d(gf), Up, ne, xep, jwp, p, xqp, len(a), _d(a),
                                 _d(w), _d(out), 0 if V is None else 1)
     return out.reshape(p[2], p[1], p[0]).transpose(2, 1, 0)
+
+
+MESH_TYPES = {"uniform": 0, "adaptive": 1, "manual1": 3, "hartree_outer": 4}
+
+
+def fem_mesh(mesh_type, nel, npe, start, end, inner_size=0.0, inner_h=0.0, coarsening=0.0):
+    """Global node coordinates (fem/FEM.cc:159-375)."""
+    nodes = np.zeros(nel * (npe - 1) + 1)
+    n = lib().orc_fem_mesh(MESH_TYPES[mesh_type], nel, npe, start, end, inner_size, inner_h,
+                           coarsening, _d(nodes))
+    if n < 0:
+        raise ValueError(mesh_type)
+    return nodes
+
+
+def fem_shape(npe, xi):
+    """Shape functions and their xi-derivatives at the points xi: arrays
+    (npe, len(xi)) (fem/FEM.cc:418-528)."""
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    S = np.zeros((npe, len(xi)))
+    D = np.zeros((npe, len(xi)))
+    lib().orc_fem_shape(npe, len(xi), _d(xi), _d(S), _d(D))
+    return S, D
+
+
+def fem_interpolate(nel, npe, xi, w, nodes, nodal=None):
+    """Values and xi-derivatives of the nodal fields (rows of nodal) at the
+    quadrature points, with jw, inv_jw and the point positions
+    (fem/FEM.cc:552-571, 897-946).  Returns (val, dval, jw, inv_jw, xq)."""
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    nq = len(xi)
+    npts = nel * nq
+    jw, ijw, xq = np.zeros(npts), np.zeros(npts), np.zeros(npts)
+    if nodal is None:
+        nodal = np.zeros((0, len(nodes)))
+    nodal = np.ascontiguousarray(np.atleast_2d(nodal), dtype=np.float64)
+    nf = nodal.shape[0]
+    val = np.zeros((max(nf, 1), npts))
+    dval = np.zeros((max(nf, 1), npts))
+    lib().orc_fem_interpolate(nel, npe, nq, _d(xi), _d(w), _d(nodes), nf, _d(nodal), _d(val),
+                              _d(dval), _d(jw), _d(ijw), _d(xq))
+    return val[:nf], dval[:nf], jw, ijw, xq
+
+
+def tucker_basis_matrices(B, DB, jw, inv_jw, U=None):
+    """MatX, MatDX and MatPot (rv, r, r) of one axis
+    (dft/ProjectHamiltonianSparse.cc:577-722).  B, DB: (r, nq) basis values
+    and xi-derivatives at the quadrature points; U: (nq, rv) or None."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    DB = np.ascontiguousarray(DB, dtype=np.float64)
+    r, nq = B.shape
+    jw = np.ascontiguousarray(jw, dtype=np.float64)
+    inv_jw = np.ascontiguousarray(inv_jw, dtype=np.float64)
+    MX, MDX = np.zeros((r, r)), np.zeros((r, r))
+    rv = 0 if U is None else U.shape[1]
+    Uc = np.ascontiguousarray(np.asarray(U if U is not None else np.zeros((nq, 1)),
+                                         dtype=np.float64).T)  # column-major nq x rv
+    MP = np.zeros((max(rv, 1), r, r))
+    lib().orc_tucker_basis_matrices(nq, r, _d(B), _d(DB), _d(jw), _d(inv_jw), rv, _d(Uc), _d(MX),
+                                    _d(MDX), _d(MP) if rv else None)
+    return MX, MDX, (MP if rv else None)

@@ -219,6 +219,29 @@ void orc_hartree_expansion(int r0, int r1, int r2, const double *G, const double
                            const int p[3], const double *const xq[3], int nterms,
                            const double *alpha, const double *omega, double *V, int add);
 
+/* ---- one-dimensional finite elements (fem_oracle.c) -------------------------
+ * orc_fem_mesh: FEM::generateNodes for the uniform, adaptive, manual1 and
+ * hartree_outer meshes.  orc_fem_shape / orc_fem_interpolate: equispaced
+ * Lagrange shape functions (values in product form, derivatives from monomial
+ * coefficients as the reference forms them) and the per-element interpolation
+ * with the per-point weights.  orc_tucker_basis_matrices: MatX, MatDX and
+ * MatPot of ProjectHamiltonianSparse.cc:577-722.  Pinned: numpy's uniform
+ * linspace and the closed forms of the graded meshes (end points, growth
+ * ratios); partition of unity, Kronecker property and exact reproduction of
+ * polynomials with their derivatives; exact integration of polynomials of
+ * degree 2 nq - 1; the example's basis functions orthonormal to 3.4e-5 and
+ * their kinetic and potential matrices against numpy's Lagrange route
+ * (test_oracle_fem.py). */
+int64_t orc_fem_mesh(int type, int nel, int npe, double start, double end, double inner_size,
+                     double inner_h, double coarsening, double *nodes);
+void orc_fem_shape(int npe, int nq, const double *xi, double *S, double *D);
+void orc_fem_interpolate(int nel, int npe, int nq, const double *xi, const double *w,
+                         const double *nodes, int nfun, const double *nodal, double *val,
+                         double *dval, double *jw, double *inv_jw, double *xq);
+void orc_tucker_basis_matrices(int nq, int r, const double *B, const double *DB,
+                               const double *jw, const double *inv_jw, int rv, const double *U,
+                               double *MatX, double *MatDX, double *MatPot);
+
 #ifdef __cplusplus
 }
 #endif

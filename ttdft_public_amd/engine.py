@@ -148,6 +148,17 @@ _SIGS = {
                                                ctypes.POINTER(_vp), ctypes.POINTER(_vp), _c_i32p,
                                                ctypes.POINTER(_vp), _i32, _c_dp, _c_dp, _vp,
                                                ctypes.c_int]),
+    "ttdft_fem_gauss_legendre": (ctypes.c_int, [_i32, _c_dp, _c_dp]),
+    "ttdft_fem_quadrature_rule": (ctypes.c_int, [_i32, _c_dp, _c_dp]),
+    "ttdft_fem_mesh": (ctypes.c_int, [_i32, _i32, _i32, _d, _d, _d, _d, _d, _c_dp]),
+    "ttdft_fem_interpolate": (ctypes.c_int, [_vp, _i32, _i32, _i32, _c_dp, _c_dp, _c_dp, _i32,
+                                             _vp, _vp, _vp, _c_dp, _c_dp, _c_dp]),
+    "ttdft_tucker_basis_matrices": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _i32,
+                                                   _vp, _vp, _vp, _vp]),
+    "ttdft_read_basis_file": (ctypes.c_int, [ctypes.c_char_p, _c_i32p, _c_i32p, _c_dp]),
+    "ttdft_read_atom_file": (ctypes.c_int, [ctypes.c_char_p, _c_i32p, _c_i32p, _c_i32p, _c_i32p,
+                                            _c_dp]),
+    "ttdft_read_table": (ctypes.c_int, [ctypes.c_char_p, _c_i32p, _c_i32p, _c_dp]),
     "ttdft_rng_uniform": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
     "ttdft_rng_normal": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
 }
@@ -205,6 +216,69 @@ def rng_uniform(seed, count):
 def rng_normal(seed, count):
     out = np.empty(count)
     _check(lib().ttdft_rng_normal(seed, count, _dptr(out)))
+    return out
+
+
+MESH_UNIFORM, MESH_ADAPTIVE, MESH_MANUAL1, MESH_HARTREE_OUTER = 0, 1, 3, 4
+MESH_TYPES = {"uniform": MESH_UNIFORM, "adaptive": MESH_ADAPTIVE, "manual1": MESH_MANUAL1,
+              "hartree_outer": MESH_HARTREE_OUTER}
+
+
+def gauss_legendre(n):
+    """n-point Gauss-Legendre rule on [-1, 1], points ascending."""
+    xi, w = np.empty(n), np.empty(n)
+    _check(lib().ttdft_fem_gauss_legendre(n, _dptr(xi), _dptr(w)))
+    return xi, w
+
+
+def quadrature_rule(name):
+    """The reference's "<n>pt" rule (fem/QuadratureRule.cc) in its point
+    order: name is "10pt" or the number of points."""
+    n = int(str(name).rstrip("pt"))
+    xi, w = np.empty(n), np.empty(n)
+    _check(lib().ttdft_fem_quadrature_rule(n, _dptr(xi), _dptr(w)))
+    return xi, w
+
+
+def fem_mesh(mesh_type, nel, npe, start, end, inner_size=0.0, inner_h=0.0, coarsening=0.0):
+    """Global node coordinates of FEM::generateNodes (fem/FEM.cc:159-375)."""
+    t = MESH_TYPES[mesh_type] if isinstance(mesh_type, str) else int(mesh_type)
+    nodes = np.empty(nel * (npe - 1) + 1)
+    _check(lib().ttdft_fem_mesh(t, nel, npe, start, end, inner_size, inner_h, coarsening,
+                                _dptr(nodes)))
+    return nodes
+
+
+def read_basis_file(path):
+    """Basis functions of a file, zero padded at both ends: (nfun, nnodes)."""
+    nf, nn = ctypes.c_int32(), ctypes.c_int32()
+    p = os.fsencode(path)
+    _check(lib().ttdft_read_basis_file(p, ctypes.byref(nf), ctypes.byref(nn), None))
+    out = np.empty((nf.value, nn.value))
+    _check(lib().ttdft_read_basis_file(p, ctypes.byref(nf), ctypes.byref(nn), _dptr(out)))
+    return out
+
+
+def read_atom_file(path):
+    """(number of electrons, lmax per type, atoms (natoms, 7) grouped by type)."""
+    nt, na, ne = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    p = os.fsencode(path)
+    _check(lib().ttdft_read_atom_file(p, ctypes.byref(nt), ctypes.byref(na), ctypes.byref(ne),
+                                      None, None))
+    lmax = np.empty(nt.value, dtype=np.int32)
+    atoms = np.empty((na.value, 7))
+    _check(lib().ttdft_read_atom_file(p, ctypes.byref(nt), ctypes.byref(na), ctypes.byref(ne),
+                                      _i32ptr(lmax), _dptr(atoms)))
+    return ne.value, lmax, atoms
+
+
+def read_table(path):
+    """Whitespace-separated numeric table: (nrows, ncols)."""
+    nr, nc = ctypes.c_int32(), ctypes.c_int32()
+    p = os.fsencode(path)
+    _check(lib().ttdft_read_table(p, ctypes.byref(nr), ctypes.byref(nc), None))
+    out = np.empty((nr.value, nc.value))
+    _check(lib().ttdft_read_table(p, ctypes.byref(nr), ctypes.byref(nc), _dptr(out)))
     return out
 
 
@@ -403,6 +477,27 @@ class Context:
         w = np.ascontiguousarray(omega, dtype=np.float64)
         _check(lib().ttdft_hartree_expansion(self.h, r, d_core, U, nes, xe, jw, ps, xq, len(a),
                                              _dptr(a), _dptr(w), d_V, 1 if add else 0))
+
+    def fem_interpolate(self, nel, npe, xi, w, nodes, nfun=0, d_nodal=None, d_val=None,
+                        d_dval=None):
+        """Values and xi-derivatives at the quadrature points of nfun device
+        nodal fields (rows of d_nodal) into d_val / d_dval; returns the host
+        arrays (jw, inv_jw, positions) of the nel len(xi) points."""
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        npts = nel * len(xi)
+        jw, ijw, xq = np.empty(npts), np.empty(npts), np.empty(npts)
+        _check(lib().ttdft_fem_interpolate(self.h, nel, npe, len(xi), _dptr(xi), _dptr(w),
+                                           _dptr(nodes), nfun, d_nodal, d_val, d_dval,
+                                           _dptr(jw), _dptr(ijw), _dptr(xq)))
+        return jw, ijw, xq
+
+    def tucker_basis_matrices(self, nq, r, d_B, d_DB, d_jw, d_inv_jw, rv=0, d_U=None,
+                              d_MatX=None, d_MatDX=None, d_MatPot=None):
+        """MatX, MatDX, MatPot of one axis into device buffers (r x r each)."""
+        _check(lib().ttdft_tucker_basis_matrices(self.h, nq, r, d_B, d_DB, d_jw, d_inv_jw, rv,
+                                                 d_U, d_MatX, d_MatDX, d_MatPot))
 
     def fill_uniform(self, d_f, count, seed, lo=0.0, hi=1.0):
         _check(lib().ttdft_fill_uniform(self.h, d_f, count, seed, float(lo), float(hi)))
