@@ -489,6 +489,54 @@ int ttdft_read_atom_file(const char *path, int32_t *ntype, int32_t *natoms, int3
  * nlpWaveFun_*, Density_*, locPotential_*): nrows x ncols row-major. */
 int ttdft_read_table(const char *path, int32_t *nrows, int32_t *ncols, double *values);
 
+/* ---- effective potential and grid utilities of the SCF loop ------------------ */
+enum {
+    TTDFT_LDA_PZ81 = 0,          /* Slater exchange + Perdew-Zunger 1981 correlation   */
+    TTDFT_LDA_PZ81_REFERENCE = 1 /* the same, with the reference's 1/33 in place of 1/3
+                                    in the high-density correlation potential
+                                    (dft/KSDFTPotential.cc:262)                       */
+};
+/* V_xc(rho) (d_vxc, written or added to) and e_xc(rho), the energy per
+ * particle (d_exc, written; may be NULL), pointwise; rho <= 1e-30 gives 0.
+ * replaces KSDFTPotential::computeLDAExchangePot and computeLDACorrelationPot
+ * (dft/KSDFTPotential.cc:127-267). */
+int ttdft_lda_xc(ttdft_ctx ctx, const double *d_rho, int64_t n, double *d_vxc, double *d_exc,
+                 int add, int variant);
+/* V(q) = sum_a -Z_a erf(|q - R_a| / rc) / |q - R_a| on the px x py x pz tensor
+ * grid of 1-D coordinates (device); atoms: host natoms x 4 (x, y, z, Z).  The
+ * local pseudopotential of the synthetic clusters; it takes the place of
+ * KSDFTPotential::computeEvanescentPSPOnGrid (dft/KSDFTPotential.cc:342-392). */
+int ttdft_local_pseudopotential(ttdft_ctx ctx, int32_t natoms, const double *atoms, double rc,
+                                int32_t px, int32_t py, int32_t pz, const double *d_gx,
+                                const double *d_gy, const double *d_gz, double *d_V, int add);
+/* w[q] = wx[qx] wy[qy] wz[qz] (the 3-D jacobian x weight tensor of main.cc:698-707). */
+int ttdft_grid_weights(ttdft_ctx ctx, int32_t px, int32_t py, int32_t pz, const double *d_wx,
+                       const double *d_wy, const double *d_wz, double *d_w);
+/* sum f g wx wy wz (energy integrals of KSDFTEnergyFunctional, dft/
+ * KSDFTEnergyFunctional.cc:165-192, for a product of two fields). */
+int ttdft_grid_dot(ttdft_ctx ctx, const double *d_f, const double *d_g, int32_t px, int32_t py,
+                   int32_t pz, const double *d_wx, const double *d_wy, const double *d_wz,
+                   double *result);
+/* Trilinear interpolation of a uniform-grid field (n[0] x n[1] x n[2], x
+ * fastest, point i at origin + i h) onto a tensor grid; points outside the
+ * uniform box get 0.  add = 1 accumulates.  For the learned initial density
+ * (no reference counterpart). */
+int ttdft_grid_resample(ttdft_ctx ctx, const double *d_src, const int32_t n[3],
+                        const double origin[3], const double h[3], int32_t px, int32_t py,
+                        int32_t pz, const double *d_gx, const double *d_gy, const double *d_gz,
+                        double *d_dst, int add);
+/* out = sum_a amp_a exp(-|r - R_a|^2 / (2 sigma^2)) on a uniform grid; atoms:
+ * host natoms x 4 (x, y, z, amp).  The network input of the learned initial
+ * density (no reference counterpart). */
+int ttdft_gaussian_splat(ttdft_ctx ctx, int32_t natoms, const double *atoms, double sigma,
+                         const int32_t n[3], const double origin[3], const double h[3],
+                         double *d_out);
+/* y = a y + b x in place (x may be NULL when b = 0). */
+int ttdft_grid_axpby(ttdft_ctx ctx, double *d_y, double a, const double *d_x, double b,
+                     int64_t n);
+/* f = max(f, lo) in place. */
+int ttdft_grid_clamp(ttdft_ctx ctx, double *d_f, int64_t n, double lo);
+
 /* Host random streams (same generator the engine uses internally). */
 int ttdft_rng_uniform(uint64_t seed, int64_t count, double *out);
 int ttdft_rng_normal(uint64_t seed, int64_t count, double *out);

@@ -74,8 +74,12 @@ __global__ __launch_bounds__(256) void grid_reduce_kernel(const double *__restri
         const int qy = (int)(rest % py);
         const int qz = (int)(rest / py);
         double v = f[idx];
-        if (g) v = v - g[idx];
-        if (squared) v = v * v;
+        if (squared == 2) {
+            v = v * g[idx];
+        } else {
+            if (g) v = v - g[idx];
+            if (squared) v = v * v;
+        }
         s += v * (wx[qx] * wy[qy] * wz[qz]);
     }
     const double t = block_sum(s);

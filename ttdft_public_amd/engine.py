@@ -159,6 +159,16 @@ _SIGS = {
     "ttdft_read_atom_file": (ctypes.c_int, [ctypes.c_char_p, _c_i32p, _c_i32p, _c_i32p, _c_i32p,
                                             _c_dp]),
     "ttdft_read_table": (ctypes.c_int, [ctypes.c_char_p, _c_i32p, _c_i32p, _c_dp]),
+    "ttdft_lda_xc": (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "ttdft_local_pseudopotential": (ctypes.c_int, [_vp, _i32, _c_dp, _d, _i32, _i32, _i32, _vp,
+                                                   _vp, _vp, _vp, ctypes.c_int]),
+    "ttdft_grid_weights": (ctypes.c_int, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
+    "ttdft_grid_dot": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _c_dp]),
+    "ttdft_grid_resample": (ctypes.c_int, [_vp, _vp, _c_i32p, _c_dp, _c_dp, _i32, _i32, _i32,
+                                           _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "ttdft_gaussian_splat": (ctypes.c_int, [_vp, _i32, _c_dp, _d, _c_i32p, _c_dp, _c_dp, _vp]),
+    "ttdft_grid_clamp": (ctypes.c_int, [_vp, _vp, _i64, _d]),
+    "ttdft_grid_axpby": (ctypes.c_int, [_vp, _vp, _d, _vp, _d, _i64]),
     "ttdft_rng_uniform": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
     "ttdft_rng_normal": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
 }
@@ -220,6 +230,7 @@ def rng_normal(seed, count):
 
 
 MESH_UNIFORM, MESH_ADAPTIVE, MESH_MANUAL1, MESH_HARTREE_OUTER = 0, 1, 3, 4
+LDA_PZ81, LDA_PZ81_REFERENCE = 0, 1
 MESH_TYPES = {"uniform": MESH_UNIFORM, "adaptive": MESH_ADAPTIVE, "manual1": MESH_MANUAL1,
               "hartree_outer": MESH_HARTREE_OUTER}
 
@@ -498,6 +509,49 @@ class Context:
         """MatX, MatDX, MatPot of one axis into device buffers (r x r each)."""
         _check(lib().ttdft_tucker_basis_matrices(self.h, nq, r, d_B, d_DB, d_jw, d_inv_jw, rv,
                                                  d_U, d_MatX, d_MatDX, d_MatPot))
+
+    def lda_xc(self, d_rho, count, d_vxc=None, d_exc=None, add=False, variant=LDA_PZ81):
+        _check(lib().ttdft_lda_xc(self.h, d_rho, count, d_vxc, d_exc, 1 if add else 0, variant))
+
+    def local_pseudopotential(self, atoms, rc, pdims, d_gx, d_gy, d_gz, d_V, add=False):
+        """atoms: (natoms, 4) host array of x, y, z, Z."""
+        a = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1, 4)
+        _check(lib().ttdft_local_pseudopotential(self.h, a.shape[0], _dptr(a), rc, *pdims, d_gx,
+                                                 d_gy, d_gz, d_V, 1 if add else 0))
+
+    def grid_weights(self, pdims, d_wx, d_wy, d_wz, d_w):
+        _check(lib().ttdft_grid_weights(self.h, *pdims, d_wx, d_wy, d_wz, d_w))
+
+    def grid_dot(self, d_f, d_g, pdims, dwx, dwy, dwz):
+        out = ctypes.c_double()
+        _check(lib().ttdft_grid_dot(self.h, d_f, d_g, *pdims, dwx, dwy, dwz, ctypes.byref(out)))
+        return out.value
+
+    def grid_resample(self, d_src, n, origin, h, pdims, d_gx, d_gy, d_gz, d_dst, add=False):
+        nn = (ctypes.c_int32 * 3)(*n)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        hh = np.ascontiguousarray(h, dtype=np.float64)
+        _check(lib().ttdft_grid_resample(self.h, d_src, nn, _dptr(o), _dptr(hh), *pdims, d_gx,
+                                         d_gy, d_gz, d_dst, 1 if add else 0))
+
+    def gaussian_splat(self, atoms, sigma, n, origin, h, d_out):
+        """atoms: (natoms, 4) host array of x, y, z, amplitude."""
+        a = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1, 4)
+        nn = (ctypes.c_int32 * 3)(*n)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        hh = np.ascontiguousarray(h, dtype=np.float64)
+        _check(lib().ttdft_gaussian_splat(self.h, a.shape[0], _dptr(a), sigma, nn, _dptr(o),
+                                          _dptr(hh), d_out))
+
+    def grid_axpby(self, d_y, a, d_x, b, count):
+        """y = a y + b x on the device."""
+        _check(lib().ttdft_grid_axpby(self.h, d_y, a, d_x, b, count))
+
+    def copy(self, d_dst, d_src, nbytes):
+        _check(lib().ttdft_memcpy_d2d(self.h, d_dst, d_src, nbytes))
+
+    def grid_clamp(self, d_f, count, lo=0.0):
+        _check(lib().ttdft_grid_clamp(self.h, d_f, count, lo))
 
     def fill_uniform(self, d_f, count, seed, lo=0.0, hi=1.0):
         _check(lib().ttdft_fill_uniform(self.h, d_f, count, seed, float(lo), float(hi)))
