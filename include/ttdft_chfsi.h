@@ -537,6 +537,46 @@ int ttdft_grid_axpby(ttdft_ctx ctx, double *d_y, double a, const double *d_x, do
 /* f = max(f, lo) in place. */
 int ttdft_grid_clamp(ttdft_ctx ctx, double *d_f, int64_t n, double lo);
 
+/* ---- layers of the learned initial-density network ----------------------------
+ * No reference counterpart: the network predicts a correction to the atomic
+ * superposition of InitialGuess::initialize_rho (dft/InitialGuess.cc:683-762).
+ * Tensors: one sample, [C][D][H][W] FP32 on the device (x fastest); dims =
+ * {D, H, W}.  Convolutions: kernel ks = 1 or 3, stride 1, zero padding ks/2,
+ * weights [co][ci][ks^3] (taps z-major), bias [co] (may be NULL).
+ * accumulate = 1 adds into the output instead of writing it. */
+int ttdft_nn_conv3d_forward(ttdft_ctx ctx, const float *d_x, int32_t ci, const int32_t dims[3],
+                            const float *d_w, const float *d_b, int32_t co, int32_t ks,
+                            float *d_y, int accumulate);
+int ttdft_nn_conv3d_backward_data(ttdft_ctx ctx, const float *d_dy, int32_t co,
+                                  const int32_t dims[3], const float *d_w, int32_t ci,
+                                  int32_t ks, float *d_dx, int accumulate);
+/* dW (and db when not NULL) from the layer input and the output gradient. */
+int ttdft_nn_conv3d_backward_weight(ttdft_ctx ctx, const float *d_x, int32_t ci,
+                                    const int32_t dims[3], const float *d_dy, int32_t co,
+                                    int32_t ks, float *d_dw, float *d_db, int accumulate);
+/* y = max(y, slope y) in place; backward: dy *= (y > 0 ? 1 : slope). */
+int ttdft_nn_leaky_relu(ttdft_ctx ctx, float *d_y, int64_t n, float slope);
+int ttdft_nn_leaky_relu_backward(ttdft_ctx ctx, const float *d_y, float *d_dy, int64_t n,
+                                 float slope);
+/* 2x2x2 average pooling and nearest 2x upsampling; dims are the fine grid's. */
+int ttdft_nn_avgpool2(ttdft_ctx ctx, const float *d_fine, int32_t c, const int32_t fine_dims[3],
+                      float *d_coarse);
+int ttdft_nn_avgpool2_backward(ttdft_ctx ctx, const float *d_dcoarse, int32_t c,
+                               const int32_t fine_dims[3], float *d_dfine, int accumulate);
+int ttdft_nn_upsample2(ttdft_ctx ctx, const float *d_coarse, int32_t c,
+                       const int32_t fine_dims[3], float *d_fine);
+int ttdft_nn_upsample2_backward(ttdft_ctx ctx, const float *d_dfine, int32_t c,
+                                const int32_t fine_dims[3], float *d_dcoarse, int accumulate);
+/* loss = mean (y - t)^2 (host); dy = 2 (y - t) / n when not NULL. */
+int ttdft_nn_mse(ttdft_ctx ctx, const float *d_y, const float *d_t, int64_t n, float *d_dy,
+                 double *loss);
+/* One Adam step (step >= 1 counts from 1) over n parameters. */
+int ttdft_nn_adam(ttdft_ctx ctx, float *d_p, const float *d_g, float *d_m, float *d_v, int64_t n,
+                  float lr, float beta1, float beta2, float eps, int32_t step);
+/* dst = scale src, FP64 -> FP32 and FP32 -> FP64. */
+int ttdft_nn_to_f32(ttdft_ctx ctx, const double *d_src, float *d_dst, int64_t n, double scale);
+int ttdft_nn_to_f64(ttdft_ctx ctx, const float *d_src, double *d_dst, int64_t n, double scale);
+
 /* Host random streams (same generator the engine uses internally). */
 int ttdft_rng_uniform(uint64_t seed, int64_t count, double *out);
 int ttdft_rng_normal(uint64_t seed, int64_t count, double *out);

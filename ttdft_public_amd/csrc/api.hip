@@ -340,6 +340,7 @@ int ttdft_ctx_destroy(ttdft_ctx ctx) {
     }
     timing_flush(ctx);
     for (auto &s : ctx->scratch) s.release();
+    for (auto &s : ctx->nn_scratch) s.release();
     dfree(ctx->d_info);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->own_stream) {

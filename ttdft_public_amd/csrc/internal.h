@@ -87,6 +87,8 @@ struct ttdft_ctx_s {
     // scratch: [0] second X block for the filter / rotation, [1] small k x k
     // work, [2] Gram partials, [3] nonlocal projections, [4..7] misc
     ttdft::Scratch scratch[8];
+    // network-layer work: [0] transformed weights, [1] weight-gradient partials
+    ttdft::Scratch nn_scratch[2];
     int *d_info = nullptr;  // rocSOLVER info
 };
 

@@ -169,6 +169,23 @@ _SIGS = {
     "ttdft_gaussian_splat": (ctypes.c_int, [_vp, _i32, _c_dp, _d, _c_i32p, _c_dp, _c_dp, _vp]),
     "ttdft_grid_clamp": (ctypes.c_int, [_vp, _vp, _i64, _d]),
     "ttdft_grid_axpby": (ctypes.c_int, [_vp, _vp, _d, _vp, _d, _i64]),
+    "ttdft_nn_conv3d_forward": (ctypes.c_int, [_vp, _vp, _i32, _c_i32p, _vp, _vp, _i32, _i32,
+                                               _vp, ctypes.c_int]),
+    "ttdft_nn_conv3d_backward_data": (ctypes.c_int, [_vp, _vp, _i32, _c_i32p, _vp, _i32, _i32,
+                                                     _vp, ctypes.c_int]),
+    "ttdft_nn_conv3d_backward_weight": (ctypes.c_int, [_vp, _vp, _i32, _c_i32p, _vp, _i32, _i32,
+                                                       _vp, _vp, ctypes.c_int]),
+    "ttdft_nn_leaky_relu": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_float]),
+    "ttdft_nn_leaky_relu_backward": (ctypes.c_int, [_vp, _vp, _vp, _i64, ctypes.c_float]),
+    "ttdft_nn_avgpool2": (ctypes.c_int, [_vp, _vp, _i32, _c_i32p, _vp]),
+    "ttdft_nn_avgpool2_backward": (ctypes.c_int, [_vp, _vp, _i32, _c_i32p, _vp, ctypes.c_int]),
+    "ttdft_nn_upsample2": (ctypes.c_int, [_vp, _vp, _i32, _c_i32p, _vp]),
+    "ttdft_nn_upsample2_backward": (ctypes.c_int, [_vp, _vp, _i32, _c_i32p, _vp, ctypes.c_int]),
+    "ttdft_nn_mse": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _c_dp]),
+    "ttdft_nn_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float, _i32]),
+    "ttdft_nn_to_f32": (ctypes.c_int, [_vp, _vp, _vp, _i64, _d]),
+    "ttdft_nn_to_f64": (ctypes.c_int, [_vp, _vp, _vp, _i64, _d]),
     "ttdft_rng_uniform": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
     "ttdft_rng_normal": (ctypes.c_int, [ctypes.c_uint64, _i64, _c_dp]),
 }

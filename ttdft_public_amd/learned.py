@@ -1,0 +1,359 @@
+"""Learned correction to the atomic-superposition initial density.
+
+A four-level 3-D U-Net (widths 16-32-64-128, 1.46 M parameters) maps two
+channels on a uniform n^3 grid over the simulation box -- Gaussian splats of
+the atom positions (sigma 1 bohr) and the renormalised atomic superposition
+-- to the correction rho_converged - rho_superposition.  Every layer runs on
+the device through the engine's hand-written kernels (csrc/unet.hip):
+forward, backward and the Adam update; the training data are densities that
+the SCF loop (scf.py) converges on seeded clusters.
+
+The start handed to the SCF is the superposition on the quadrature grid plus
+the predicted correction (trilinear from the uniform grid), clamped at zero
+and renormalised to the electron count.  The plain superposition stays as the
+fallback: if the first SCF residual from the learned start exceeds the
+superposition's, the loop restarts from the superposition
+(InitialGuess::initialize_rho, dft/InitialGuess.cc:683-762, is the start
+this corrects).
+"""
+import ctypes
+import math
+import os
+import time
+
+import numpy as np
+
+from . import engine as E
+from . import scf as S
+
+SLOPE = 0.1
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                               "unet_initial_density.npz")
+
+
+def layer_specs(cin=2, widths=(16, 32, 64, 128)):
+    """(name, ci, co, ks) of the network's convolutions, in parameter order."""
+    c0, c1, c2, c3 = widths
+    return [("enc0a", cin, c0, 3), ("enc0b", c0, c0, 3),
+            ("enc1a", c0, c1, 3), ("enc1b", c1, c1, 3),
+            ("enc2a", c1, c2, 3), ("enc2b", c2, c2, 3),
+            ("bot_a", c2, c3, 3), ("bot_b", c3, c3, 3),
+            ("dec2a", c3 + c2, c2, 3), ("dec2b", c2, c2, 3),
+            ("dec1a", c2 + c1, c1, 3), ("dec1b", c1, c1, 3),
+            ("dec0a", c1 + c0, c0, 3), ("dec0b", c0, c0, 3),
+            ("head", c0, 1, 1)]
+
+
+def _c(rc):
+    E._check(rc)
+
+
+class UNet:
+    """The network's parameters, Adam state and activations on one device."""
+
+    def __init__(self, ctx, n=64, cin=2, widths=(16, 32, 64, 128), seed=0):
+        assert n % 8 == 0
+        self.ctx, self.n, self.cin, self.widths = ctx, n, cin, tuple(widths)
+        self.specs = layer_specs(cin, widths)
+        self.offsets = {}
+        o = 0
+        for name, ci, co, ks in self.specs:
+            self.offsets[name] = (o, o + co * ci * ks ** 3)
+            o += co * ci * ks ** 3 + co
+        self.nparams = o
+        rng = np.random.default_rng(seed)
+        init = np.zeros(o, np.float32)
+        for name, ci, co, ks in self.specs:
+            wo, _ = self.offsets[name]
+            fan = ci * ks ** 3
+            std = math.sqrt(2.0 / ((1 + SLOPE ** 2) * fan)) if name != "head" else 1e-2 / fan
+            init[wo:wo + co * fan] = rng.standard_normal(co * fan) * std
+        self.params = E.DeviceArray.from_host(ctx, init)
+        zeros = np.zeros(o, np.float32)
+        self.grads = E.DeviceArray.from_host(ctx, zeros)
+        self.m = E.DeviceArray.from_host(ctx, zeros)
+        self.v = E.DeviceArray.from_host(ctx, zeros)
+        self.step_count = 0
+        self.in_scale, self.out_scale = 1.0, 1.0
+        self._alloc()
+
+    # ---- buffers -----------------------------------------------------------------
+    def _alloc(self):
+        n, (c0, c1, c2, c3) = self.n, self.widths
+        P = [n ** 3 // 8 ** l for l in range(4)]
+        self.P = P
+        self.dims = [(n >> l,) * 3 for l in range(4)]
+        f32 = np.float32
+        shapes = {"x": self.cin * P[0], "enc0a": c0 * P[0], "cat0": (c1 + c0) * P[0],
+                  "p0": c0 * P[1], "enc1a": c1 * P[1], "cat1": (c2 + c1) * P[1],
+                  "p1": c1 * P[2], "enc2a": c2 * P[2], "cat2": (c3 + c2) * P[2],
+                  "p2": c2 * P[3], "bot_a": c3 * P[3], "bot_b": c3 * P[3],
+                  "dec2a": c2 * P[2], "dec2b": c2 * P[2], "dec1a": c1 * P[1],
+                  "dec1b": c1 * P[1], "dec0a": c0 * P[0], "dec0b": c0 * P[0], "y": P[0],
+                  "t": P[0]}
+        self.a = {k: E.DeviceArray(self.ctx, (v,), f32) for k, v in shapes.items()}
+        self.g = {k: E.DeviceArray(self.ctx, (v,), f32) for k, v in shapes.items()
+                  if k not in ("x", "t")}
+
+    def _w(self, name):
+        wo, bo = self.offsets[name]
+        return self.params.ptr + 4 * wo, self.params.ptr + 4 * bo
+
+    def _gw(self, name):
+        wo, bo = self.offsets[name]
+        return self.grads.ptr + 4 * wo, self.grads.ptr + 4 * bo
+
+    @staticmethod
+    def _sub(buf, channel_offset, P):
+        return buf.ptr + 4 * channel_offset * P
+
+    def _dims(self, level):
+        return (ctypes.c_int32 * 3)(*self.dims[level])
+
+    # ---- layers --------------------------------------------------------------------
+    def _conv(self, name, x_ptr, y_ptr, level, act=True):
+        _, ci, co, ks = next(s for s in self.specs if s[0] == name)
+        w, b = self._w(name)
+        _c(E.lib().ttdft_nn_conv3d_forward(self.ctx.h, x_ptr, ci, self._dims(level), w, b, co, ks,
+                                           y_ptr, 0))
+        if act:
+            _c(E.lib().ttdft_nn_leaky_relu(self.ctx.h, y_ptr, co * self.P[level], SLOPE))
+
+    def _conv_back(self, name, x_ptr, y_ptr, gy_ptr, gx_ptr, level, act=True):
+        """Gradient of one convolution (+ activation): parameter gradients are
+        written (added when self._acc), the input gradient written to gx_ptr
+        (if not None)."""
+        _, ci, co, ks = next(s for s in self.specs if s[0] == name)
+        if act:
+            _c(E.lib().ttdft_nn_leaky_relu_backward(self.ctx.h, y_ptr, gy_ptr, co * self.P[level],
+                                                    SLOPE))
+        gw, gb = self._gw(name)
+        d = self._dims(level)
+        _c(E.lib().ttdft_nn_conv3d_backward_weight(self.ctx.h, x_ptr, ci, d, gy_ptr, co, ks, gw, gb,
+                                                   self._acc))
+        if gx_ptr is not None:
+            w, _ = self._w(name)
+            _c(E.lib().ttdft_nn_conv3d_backward_data(self.ctx.h, gy_ptr, co, d, w, ci, ks, gx_ptr,
+                                                     0))
+
+    def forward(self):
+        """y = net(x) on the buffers a["x"] -> a["y"]."""
+        a, P, (c0, c1, c2, c3) = self.a, self.P, self.widths
+        L, h = E.lib(), self.ctx.h
+        s0, s1, s2 = (self._sub(a["cat0"], c1, P[0]), self._sub(a["cat1"], c2, P[1]),
+                      self._sub(a["cat2"], c3, P[2]))
+        self._conv("enc0a", a["x"].ptr, a["enc0a"].ptr, 0)
+        self._conv("enc0b", a["enc0a"].ptr, s0, 0)
+        _c(L.ttdft_nn_avgpool2(h, s0, c0, self._dims(0), a["p0"].ptr))
+        self._conv("enc1a", a["p0"].ptr, a["enc1a"].ptr, 1)
+        self._conv("enc1b", a["enc1a"].ptr, s1, 1)
+        _c(L.ttdft_nn_avgpool2(h, s1, c1, self._dims(1), a["p1"].ptr))
+        self._conv("enc2a", a["p1"].ptr, a["enc2a"].ptr, 2)
+        self._conv("enc2b", a["enc2a"].ptr, s2, 2)
+        _c(L.ttdft_nn_avgpool2(h, s2, c2, self._dims(2), a["p2"].ptr))
+        self._conv("bot_a", a["p2"].ptr, a["bot_a"].ptr, 3)
+        self._conv("bot_b", a["bot_a"].ptr, a["bot_b"].ptr, 3)
+        _c(L.ttdft_nn_upsample2(h, a["bot_b"].ptr, c3, self._dims(2), a["cat2"].ptr))
+        self._conv("dec2a", a["cat2"].ptr, a["dec2a"].ptr, 2)
+        self._conv("dec2b", a["dec2a"].ptr, a["dec2b"].ptr, 2)
+        _c(L.ttdft_nn_upsample2(h, a["dec2b"].ptr, c2, self._dims(1), a["cat1"].ptr))
+        self._conv("dec1a", a["cat1"].ptr, a["dec1a"].ptr, 1)
+        self._conv("dec1b", a["dec1a"].ptr, a["dec1b"].ptr, 1)
+        _c(L.ttdft_nn_upsample2(h, a["dec1b"].ptr, c1, self._dims(0), a["cat0"].ptr))
+        self._conv("dec0a", a["cat0"].ptr, a["dec0a"].ptr, 0)
+        self._conv("dec0b", a["dec0a"].ptr, a["dec0b"].ptr, 0)
+        self._conv("head", a["dec0b"].ptr, a["y"].ptr, 0, act=False)
+
+    def backward(self, accumulate=False):
+        """Parameter gradients for the output gradient g["y"], written or (with
+        accumulate, over the samples of a batch) added to self.grads."""
+        self._acc = 1 if accumulate else 0
+        a, g, P, (c0, c1, c2, c3) = self.a, self.g, self.P, self.widths
+        L, h = E.lib(), self.ctx.h
+        s0, s1, s2 = (self._sub(a["cat0"], c1, P[0]), self._sub(a["cat1"], c2, P[1]),
+                      self._sub(a["cat2"], c3, P[2]))
+        gs0, gs1, gs2 = (self._sub(g["cat0"], c1, P[0]), self._sub(g["cat1"], c2, P[1]),
+                         self._sub(g["cat2"], c3, P[2]))
+        cb = self._conv_back
+        cb("head", a["dec0b"].ptr, None, g["y"].ptr, g["dec0b"].ptr, 0, act=False)
+        cb("dec0b", a["dec0a"].ptr, a["dec0b"].ptr, g["dec0b"].ptr, g["dec0a"].ptr, 0)
+        cb("dec0a", a["cat0"].ptr, a["dec0a"].ptr, g["dec0a"].ptr, g["cat0"].ptr, 0)
+        _c(L.ttdft_nn_upsample2_backward(h, g["cat0"].ptr, c1, self._dims(0), g["dec1b"].ptr, 0))
+        cb("dec1b", a["dec1a"].ptr, a["dec1b"].ptr, g["dec1b"].ptr, g["dec1a"].ptr, 1)
+        cb("dec1a", a["cat1"].ptr, a["dec1a"].ptr, g["dec1a"].ptr, g["cat1"].ptr, 1)
+        _c(L.ttdft_nn_upsample2_backward(h, g["cat1"].ptr, c2, self._dims(1), g["dec2b"].ptr, 0))
+        cb("dec2b", a["dec2a"].ptr, a["dec2b"].ptr, g["dec2b"].ptr, g["dec2a"].ptr, 2)
+        cb("dec2a", a["cat2"].ptr, a["dec2a"].ptr, g["dec2a"].ptr, g["cat2"].ptr, 2)
+        _c(L.ttdft_nn_upsample2_backward(h, g["cat2"].ptr, c3, self._dims(2), g["bot_b"].ptr, 0))
+        cb("bot_b", a["bot_a"].ptr, a["bot_b"].ptr, g["bot_b"].ptr, g["bot_a"].ptr, 3)
+        cb("bot_a", a["p2"].ptr, a["bot_a"].ptr, g["bot_a"].ptr, g["p2"].ptr, 3)
+        _c(L.ttdft_nn_avgpool2_backward(h, g["p2"].ptr, c2, self._dims(2), gs2, 1))
+        cb("enc2b", a["enc2a"].ptr, s2, gs2, g["enc2a"].ptr, 2)
+        cb("enc2a", a["p1"].ptr, a["enc2a"].ptr, g["enc2a"].ptr, g["p1"].ptr, 2)
+        _c(L.ttdft_nn_avgpool2_backward(h, g["p1"].ptr, c1, self._dims(1), gs1, 1))
+        cb("enc1b", a["enc1a"].ptr, s1, gs1, g["enc1a"].ptr, 1)
+        cb("enc1a", a["p0"].ptr, a["enc1a"].ptr, g["enc1a"].ptr, g["p0"].ptr, 1)
+        _c(L.ttdft_nn_avgpool2_backward(h, g["p0"].ptr, c0, self._dims(0), gs0, 1))
+        cb("enc0b", a["enc0a"].ptr, s0, gs0, g["enc0a"].ptr, 0)
+        cb("enc0a", a["x"].ptr, a["enc0a"].ptr, g["enc0a"].ptr, None, 0)
+
+    def loss_and_grad(self):
+        """MSE of a["y"] against a["t"]; writes g["y"]."""
+        loss = ctypes.c_double()
+        _c(E.lib().ttdft_nn_mse(self.ctx.h, self.a["y"].ptr, self.a["t"].ptr, self.P[0],
+                                self.g["y"].ptr, ctypes.byref(loss)))
+        return loss.value
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.step_count += 1
+        _c(E.lib().ttdft_nn_adam(self.ctx.h, self.params.ptr, self.grads.ptr, self.m.ptr,
+                                 self.v.ptr, self.nparams, lr, beta1, beta2, eps,
+                                 self.step_count))
+
+    # ---- persistence ---------------------------------------------------------------
+    def get_params(self):
+        return self.params.to_host()
+
+    def set_params(self, p):
+        self.params.copy_from(np.asarray(p, np.float32))
+
+    def save(self, path, **meta):
+        np.savez_compressed(path, params=self.get_params().astype(np.float32),
+                            widths=np.array(self.widths), cin=self.cin, n=self.n,
+                            in_scale=self.in_scale, out_scale=self.out_scale, **meta)
+
+    @classmethod
+    def load(cls, ctx, path=DEFAULT_WEIGHTS):
+        d = np.load(path, allow_pickle=False)
+        net = cls(ctx, int(d["n"]), int(d["cin"]), tuple(int(w) for w in d["widths"]))
+        net.set_params(d["params"])
+        net.in_scale, net.out_scale = float(d["in_scale"]), float(d["out_scale"])
+        return net
+
+
+# ---- inputs and targets on the uniform grid --------------------------------------
+def uniform_points(scf, n):
+    """n points per axis spanning the box (point i at -box/2 + i box/(n-1))."""
+    return np.linspace(-scf.cfg.box / 2, scf.cfg.box / 2, n)
+
+
+def network_input(net, scf, d_sup_uniform=None):
+    """Fills net.a["x"]: [splats, superposition * in_scale] on the n^3 grid."""
+    n = net.n
+    pts = uniform_points(scf, n)
+    h = pts[1] - pts[0]
+    d_pts = E.DeviceArray.from_host(scf.ctx, pts)
+    d_splat = E.DeviceArray(scf.ctx, (n ** 3,))
+    atoms = np.column_stack([scf.xyz, np.ones(scf.natoms)])
+    scf.ctx.gaussian_splat(atoms, 1.0, (n, n, n), (pts[0],) * 3, (h,) * 3, d_splat.ptr)
+    if d_sup_uniform is None:
+        d_sup_uniform = E.DeviceArray(scf.ctx, (n ** 3,))
+        scf.superposition(d_sup_uniform, points=d_pts, n=n)
+    L, hh = E.lib(), scf.ctx.h
+    _c(L.ttdft_nn_to_f32(hh, d_splat.ptr, net.a["x"].ptr, n ** 3, 1.0))
+    _c(L.ttdft_nn_to_f32(hh, d_sup_uniform.ptr, net.a["x"].ptr + 4 * n ** 3, n ** 3,
+                         net.in_scale))
+    return d_sup_uniform
+
+
+def learned_start(net, scf, d_rho):
+    """The learned initial density on the SCF's quadrature grid (into d_rho)."""
+    n = net.n
+    network_input(net, scf)
+    net.forward()
+    d_corr = E.DeviceArray(scf.ctx, (n ** 3,))
+    _c(E.lib().ttdft_nn_to_f64(scf.ctx.h, net.a["y"].ptr, d_corr.ptr, n ** 3,
+                               1.0 / net.out_scale))
+    scf.superposition(d_rho)
+    pts = uniform_points(scf, n)
+    h = pts[1] - pts[0]
+    scf.ctx.grid_resample(d_corr.ptr, (n, n, n), (pts[0],) * 3, (h,) * 3, scf.p,
+                          *(a.d_xq.ptr for a in scf.axes), d_rho.ptr, add=True)
+    scf.ctx.grid_clamp(d_rho.ptr, scf.ngrid, 0.0)
+    scf.renormalise(d_rho)
+
+
+def run_learned(net, scf, callback=None):
+    """SCF from the learned start with the superposition fallback: the first
+    residual of the superposition start (one SCF iteration, counted in the
+    time) is the bar the learned start's first residual must meet."""
+    t0 = time.perf_counter()
+    cfg1 = S.ScfConfig(**{**scf.cfg.__dict__, "max_iter": 1})
+    probe = S.Scf(scf.ctx, scf.xyz, float(scf.zval[0]), cfg1)
+    r0 = probe.run().residuals[0]
+    d_start = E.DeviceArray(scf.ctx, (scf.ngrid,))
+    learned_start(net, scf, d_start)
+    r = scf.run(d_start, callback, fallback_residual=r0)
+    r.seconds = time.perf_counter() - t0
+    r.iterations += 1  # the probe iteration
+    return r
+
+
+# ---- training data and training --------------------------------------------------
+def random_cluster(rng, nmin=13, nmax=19, jitter=0.3):
+    """An FCC cut of nmin..nmax atoms around a random centre, jittered and
+    rotated at random."""
+    natoms = int(rng.integers(nmin, nmax + 1))
+    base = S.fcc_cluster(200, jitter=0.0)
+    c = rng.uniform(-1.9, 1.9, 3)
+    d = np.linalg.norm(base - c, axis=1)
+    xyz = base[np.argsort(d, kind="stable")[:natoms]]
+    xyz = xyz - xyz.mean(axis=0)
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    return xyz @ q.T + rng.uniform(-jitter, jitter, xyz.shape)
+
+
+def converged_sample(ctx, xyz, cfg, n):
+    """(splat, superposition, target correction) of one cluster on the n^3 grid,
+    float64 host arrays, and the SCF result."""
+    s = S.Scf(ctx, xyz, 3.0, cfg)
+    r = s.run()
+    pts = E.DeviceArray.from_host(ctx, uniform_points(s, n))
+    d_sup = E.DeviceArray(ctx, (n ** 3,))
+    s.superposition(d_sup, points=pts, n=n)
+    d_conv = s.density_on_uniform(n)
+    sup, conv = d_sup.to_host(), d_conv.to_host()
+    return sup, conv - sup, r
+
+
+def train(ctx, samples, n=64, epochs=40, lr=1e-3, seed=0, log=print):
+    """samples: list of (xyz, superposition, correction) on the n^3 grid.
+    Returns the trained network (scales fitted to the data)."""
+    net = UNet(ctx, n, seed=seed)
+    sup_max = max(float(np.abs(s).max()) for _, s, _ in samples)
+    corr_rms = math.sqrt(np.mean([np.mean(c ** 2) for _, _, c in samples]))
+    net.in_scale, net.out_scale = 1.0 / sup_max, 1.0 / corr_rms
+    dummy_cfg = S.ScfConfig()
+    staged = []
+    for xyz, sup, corr in samples:
+        d_sup = E.DeviceArray.from_host(ctx, sup)
+        d_t = E.DeviceArray.from_host(ctx, corr)
+        staged.append((xyz, d_sup, d_t))
+    rng = np.random.default_rng(seed)
+
+    class _Geo:  # what network_input needs of an Scf for a sample
+        def __init__(self, xyz):
+            self.ctx, self.xyz, self.natoms, self.cfg = ctx, xyz, len(xyz), dummy_cfg
+
+    inputs = []
+    for xyz, d_sup, d_t in staged:
+        network_input(net, _Geo(xyz), d_sup)
+        inputs.append(E.DeviceArray.from_host(ctx, net.a["x"].to_host()))
+    history = []
+    for ep in range(epochs):
+        order = rng.permutation(len(staged))
+        tot = 0.0
+        t0 = time.perf_counter()
+        for i in order:
+            ctx.copy(net.a["x"].ptr, inputs[i].ptr, inputs[i].nbytes)
+            _c(E.lib().ttdft_nn_to_f32(ctx.h, staged[i][2].ptr, net.a["t"].ptr, n ** 3,
+                                       net.out_scale))
+            net.forward()
+            tot += net.loss_and_grad()
+            net.backward()
+            net.adam(lr)
+        ctx.synchronize()
+        history.append(tot / len(staged))
+        log(f"epoch {ep} loss {history[-1]:.5f} ({time.perf_counter() - t0:.2f} s)")
+    net.history = history
+    return net

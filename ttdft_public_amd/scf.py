@@ -78,6 +78,7 @@ class ScfResult:
     evals: np.ndarray
     seconds: float
     phase_seconds: dict = field(default_factory=dict)
+    fell_back: bool = False
 
 
 def coulomb_expansion():
@@ -239,19 +240,31 @@ class Scf:
         self.setup_seconds = time.perf_counter() - t0
 
     # ---- densities ---------------------------------------------------------------
-    def superposition(self, d_rho, width=1.0):
+    def superposition(self, d_rho, width=1.0, points=None, n=None):
         """Atomic superposition start: a Gaussian of the valence charge per atom
         (the radial table is its spline, as InitialGuess reads Density_AT*),
-        renormalised to the electron count (main.cc:674-680)."""
+        renormalised to the electron count on the quadrature grid
+        (main.cc:674-680).  With points (device 1-D coordinates, n per axis)
+        the same renormalised superposition is evaluated there instead.
+        Returns the renormalisation factor."""
         r = np.linspace(0.0, 12.0, 481)
         rho_r = self.zval[0] * (math.pi * width ** 2) ** -1.5 * np.exp(-(r / width) ** 2)
-        self.ctx.initial_density(self.xyz, r, rho_r, *(a.d_xq.ptr for a in self.axes), self.p,
-                                 d_rho.ptr)
-        self.renormalise(d_rho)
+        ax = [a.d_xq.ptr for a in self.axes]
+        if points is None:
+            self.ctx.initial_density(self.xyz, r, rho_r, *ax, self.p, d_rho.ptr)
+            return self.renormalise(d_rho)
+        tmp = E.DeviceArray(self.ctx, (self.ngrid,))
+        self.ctx.initial_density(self.xyz, r, rho_r, *ax, self.p, tmp.ptr)
+        f = self.nelec / self.ctx.grid_integral(tmp.ptr, self.p, *(a.d_jw.ptr for a in self.axes))
+        self.ctx.initial_density(self.xyz, r, rho_r, points.ptr, points.ptr, points.ptr,
+                                 (n, n, n), d_rho.ptr)
+        self.ctx.grid_scale(d_rho.ptr, n ** 3, f)
+        return f
 
     def renormalise(self, d_rho):
         q = self.ctx.grid_integral(d_rho.ptr, self.p, *(a.d_jw.ptr for a in self.axes))
         self.ctx.grid_scale(d_rho.ptr, self.ngrid, self.nelec / q)
+        return self.nelec / q
 
     def integral(self, d_f, d_g=None):
         w = [a.d_jw.ptr for a in self.axes]
@@ -291,7 +304,10 @@ class Scf:
                                             copies)
 
     # ---- the loop ------------------------------------------------------------------
-    def run(self, d_rho_start=None, callback=None):
+    def run(self, d_rho_start=None, callback=None, fallback_residual=None):
+        """SCF from d_rho_start (default: the atomic superposition).  With
+        fallback_residual, a first residual above it restarts the loop from
+        the superposition (the wasted iteration is counted)."""
         ctx, cfg = self.ctx, self.cfg
         n, k, R = self.ngrid, cfg.nstates, cfg.rank
         N = R ** 3
@@ -362,6 +378,14 @@ class Scf:
             phases["mixing"] += time.perf_counter() - t
             if callback:
                 callback(it, res, energy)
+            if it == 0 and fallback_residual is not None and res > fallback_residual:
+                r = self.run(None, callback)
+                r.iterations += 1
+                r.residuals = residuals + r.residuals
+                r.energies = energies + r.energies
+                r.seconds = time.perf_counter() - t_start
+                r.fell_back = True
+                return r
             if res < cfg.tol:
                 converged = True
                 break
