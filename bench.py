@@ -322,6 +322,72 @@ def hartree_cpu_sample(p=40):
     return p ** 3 / dt, dt
 
 
+def bench_scf(ctx, E):
+    """SCF time to 1e-5 Ha/atom on the 13-atom cuboctahedral cluster (nearest
+    neighbours 5.4 bohr, seeded jitter 0.1 bohr, Z = 3, erf local
+    pseudopotential, one s Kleinman-Bylander projector of 1 Ha; 40-bohr box,
+    30 quartic elements per axis graded toward the cluster, Tucker rank 20^3,
+    24 states, degree 20, 500 K, Anderson 0.5 / 10): plain FP64, mixed
+    precision, and mixed precision from the learned initial density.  The bar
+    is the FP64 run's own converged energy (density residual 1e-10); a form's
+    time is the wall time until its energy stays within 1e-5 Ha/atom of it."""
+    from ttdft_public_amd import learned as Ln
+    from ttdft_public_amd import scf as S
+
+    xyz = S.cuboctahedron(5.4, 0.1, seed=20261202)
+    natoms = len(xyz)
+
+    def run(mixed, learned=False):
+        cfg = S.ScfConfig(tol=1e-10, max_iter=80, mixed=mixed)
+        scf = S.Scf(ctx, xyz, 3.0, cfg)
+        stamps = []
+        t0 = time.perf_counter()
+        cb = lambda it, res, e: stamps.append((time.perf_counter() - t0, res, e))
+        if learned:
+            r = Ln.run_learned(Ln.UNet.load(ctx), scf, cb)
+        else:
+            r = scf.run(callback=cb)
+        return r, stamps
+
+    def time_to(stamps, e_ref):
+        ok = [abs(e - e_ref) < 1e-5 * natoms for _, _, e in stamps]
+        for i in range(len(ok)):
+            if all(ok[i:]):
+                return stamps[i][0], i + 1
+        return None, None
+
+    out = {"workload": "SCF of the 13-atom cuboctahedral cluster (39 electrons): 40-bohr box, "
+                       "30 quartic elements per axis graded toward the cluster, 300^3 quadrature "
+                       "grid, Tucker basis 20^3, 24 states, Chebyshev degree 20, 500 K, Anderson "
+                       "0.5 / 10 (seeded geometry, synthetic pseudopotential)"}
+    r64, st64 = run(False)
+    e_ref = r64.energy
+    forms = [("fp64", False, False), ("mixed", True, False)]
+    if os.path.exists(Ln.DEFAULT_WEIGHTS):
+        forms.append(("mixed_learned", True, True))
+    else:
+        out["mixed_learned"] = "network weights not present"
+    for name, mixed, learned in forms:
+        r, st = (r64, st64) if name == "fp64" else run(mixed, learned)
+        t, it = time_to(st, e_ref)
+        out[name] = {"seconds_to_1e-5_Ha_per_atom": None if t is None else round(t, 3),
+                     "iterations_to_1e-5_Ha_per_atom": it,
+                     "iterations_to_residual_1e-10": r.iterations,
+                     "seconds_to_residual_1e-10": round(r.seconds, 3),
+                     "energy_Ha": r.energy,
+                     "energy_diff_per_atom": abs(r.energy - e_ref) / natoms,
+                     "phase_seconds": {k: round(v, 3) for k, v in r.phase_seconds.items()}}
+        if learned:
+            out[name]["fell_back"] = r.fell_back
+        log(f"scf {name}: {out[name]}")
+    if out["fp64"]["seconds_to_1e-5_Ha_per_atom"] and isinstance(out.get("mixed_learned"), dict) \
+            and out["mixed_learned"]["seconds_to_1e-5_Ha_per_atom"]:
+        out["speedup_mixed_learned_vs_fp64"] = round(
+            out["fp64"]["seconds_to_1e-5_Ha_per_atom"]
+            / out["mixed_learned"]["seconds_to_1e-5_Ha_per_atom"], 3)
+    return out
+
+
 def profiled_traffic(kernel_tag):
     """HBM bytes per launch of the FP64 band SpMM from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a block copy)."""
@@ -347,18 +413,20 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-density", action="store_true")
     ap.add_argument("--mode", default="step",
-                    choices=["step", "spmm", "assembly", "mixing", "hartree"],
+                    choices=["step", "spmm", "assembly", "mixing", "hartree", "scf"],
                     help="spmm: only 3 FP64 H applications and 3 copies of one 125000 x 300 "
                          "FP64 block (copy = calibration of rocprofv3 byte counters); "
                          "assembly: only the device-resident H assembly (3 runs); "
                          "mixing: only the Anderson mixing step (4 runs); "
-                         "hartree: only STHOSVD + Hartree potential (4 runs)")
+                         "hartree: only STHOSVD + Hartree potential (4 runs); "
+                         "scf: only the 13-atom SCF (FP64, mixed, mixed + learned start)")
     ap.add_argument("--no-hartree", action="store_true",
                     help="skip the STHOSVD + Hartree measurement")
     ap.add_argument("--no-mixing", action="store_true",
                     help="skip the Anderson mixing measurement")
     ap.add_argument("--no-assembly", action="store_true",
                     help="skip the H assembly measurement")
+    ap.add_argument("--no-scf", action="store_true", help="skip the 13-atom SCF measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU "
@@ -379,6 +447,10 @@ def main():
     if args.mode == "hartree":
         ctx = E.Context(0)
         log(json.dumps(bench_hartree(ctx, E)))
+        return
+    if args.mode == "scf":
+        ctx = E.Context(0)
+        log(json.dumps(bench_scf(ctx, E)))
         return
 
     t0 = time.perf_counter()
@@ -573,6 +645,8 @@ def main():
             }
         hres["grid_points_per_s"] = round(QUAD ** 3 / (hres["ms_per_potential"] * 1e-3), 1)
         result["hartree"] = hres
+    if not args.no_scf:
+        result["scf"] = bench_scf(ctx, E)
     ctx.synchronize()
     print(json.dumps(result))
 

@@ -357,3 +357,104 @@ def train(ctx, samples, n=64, epochs=40, lr=1e-3, seed=0, log=print):
         log(f"epoch {ep} loss {history[-1]:.5f} ({time.perf_counter() - t0:.2f} s)")
     net.history = history
     return net
+
+
+# ---- evaluation and the command line ------------------------------------------------
+def data_config(nelec):
+    """The SCF configuration of the training and evaluation clusters: the
+    13-atom test configuration (40-bohr box, 24 quartic elements per axis,
+    8-point rule, rank 16) with enough states for the electron count."""
+    return S.ScfConfig(nel=24, quad=8, inner_size=12.0, inner_h=1.0, coarsening=1.25, rank=16,
+                       rho_rank=24, veff_rank=24, max_iter=80, tol=1e-8,
+                       nstates=max(24, nelec // 2 + 10))
+
+
+def iterations_to(residuals, tol):
+    for i, r in enumerate(residuals):
+        if r < tol:
+            return i + 1
+    return None
+
+
+def compare_starts(ctx, net, xyz, tol=1e-10):
+    """SCF from the superposition and from the learned start (with fallback) on
+    one cluster: iterations to 1e-6 and to tol, seconds, energies."""
+    nelec = 3 * len(xyz)
+    cfg = data_config(nelec)
+    cfg.tol = tol
+    s0 = S.Scf(ctx, xyz, 3.0, cfg)
+    r0 = s0.run()
+    s1 = S.Scf(ctx, xyz, 3.0, cfg)
+    r1 = run_learned(net, s1)
+    to1 = iterations_to(r1.residuals, 1e-6)
+    to1 = None if to1 is None else to1 + 1  # the fallback probe's iteration
+    return {"natoms": len(xyz),
+            "superposition": {"iterations": r0.iterations, "to_1e-6": iterations_to(r0.residuals, 1e-6),
+                              "first_residual": r0.residuals[0], "seconds": r0.seconds,
+                              "energy": r0.energy, "converged": r0.converged},
+            "learned": {"iterations": r1.iterations, "to_1e-6": to1,
+                        "first_residual": r1.residuals[0], "seconds": r1.seconds,
+                        "energy": r1.energy, "converged": r1.converged,
+                        "fell_back": r1.fell_back},
+            "energy_diff_per_atom": abs(r1.energy - r0.energy) / len(xyz)}
+
+
+def main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="generate SCF densities, train the initial-density "
+                                             "network on the device, evaluate it")
+    ap.add_argument("--train", type=int, default=48, help="training clusters")
+    ap.add_argument("--test", type=int, default=8, help="held-out clusters")
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--n", type=int, default=64, help="uniform grid points per axis")
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--out", required=True, help="weights (.npz)")
+    ap.add_argument("--report", required=True, help="evaluation (.json)")
+    a = ap.parse_args(argv)
+    ctx = E.Context(0)
+    rng = np.random.default_rng(a.seed)
+    t0 = time.perf_counter()
+    samples, failed = [], 0
+    while len(samples) < a.train:
+        xyz = random_cluster(rng)
+        sup, corr, r = converged_sample(ctx, xyz, data_config(3 * len(xyz)), a.n)
+        if not r.converged:
+            failed += 1
+            continue
+        samples.append((xyz, sup, corr))
+        print(f"sample {len(samples)}: {len(xyz)} atoms, {r.iterations} SCF iterations, "
+              f"{r.seconds:.2f} s", flush=True)
+    t_data = time.perf_counter() - t0
+    net = train(ctx, samples, a.n, a.epochs, a.lr, seed=a.seed,
+                log=lambda m: print(m, flush=True))
+    t_train = time.perf_counter() - t0 - t_data
+    net.save(a.out, seed=a.seed, train_clusters=a.train, epochs=a.epochs)
+    test_rng = np.random.default_rng(a.seed + 1)
+    evals = []
+    for i in range(a.test):
+        res = compare_starts(ctx, net, random_cluster(test_rng))
+        evals.append(res)
+        print(json.dumps(res), flush=True)
+    sup_it = [e["superposition"]["to_1e-6"] or 80 for e in evals]
+    lrn_it = [e["learned"]["to_1e-6"] or 80 for e in evals]
+    report = {"train_clusters": a.train, "failed_scf": failed, "epochs": a.epochs,
+              "data_seconds": t_data, "train_seconds": t_train, "loss_history": net.history,
+              "params": net.nparams, "test": evals,
+              "mean_iterations_to_1e-6": {"superposition": float(np.mean(sup_it)),
+                                          "learned": float(np.mean(lrn_it))},
+              "mean_seconds": {"superposition": float(np.mean([e["superposition"]["seconds"]
+                                                               for e in evals])),
+                               "learned": float(np.mean([e["learned"]["seconds"]
+                                                         for e in evals]))},
+              "max_energy_diff_per_atom": max(e["energy_diff_per_atom"] for e in evals)}
+    with open(a.report, "w") as fh:
+        json.dump(report, fh, indent=1)
+    print(json.dumps({k: v for k, v in report.items() if k not in ("test", "loss_history")}),
+          flush=True)
+
+
+if __name__ == "__main__":
+    main()
