@@ -224,7 +224,7 @@ def _apply(ctx, ham, X, dtype):
 DEVICE_CASES = [
     # dims, Veff ranks, 1-D half-bandwidth, projector ranks, expected storage
     ((20, 16, 12), (4, 3, 5), 3, [2, 3, 1, 4], 1),   # Kronecker band fragments
-    ((10, 9, 8), (3, 3, 3), 9, [2, 2], 0),           # full coupling: CSR
+    ((10, 9, 8), (3, 3, 3), 9, [2, 2], 2),           # full coupling: dense
     ((12, 11, 10), (2, 4, 3), 2, [], 1),             # no projectors
 ]
 

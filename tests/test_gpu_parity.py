@@ -425,3 +425,25 @@ def test_chfsi_step_matches_oracle(ctx):
     X = dX.to_host()
     # subspaces agree (eigenvector signs are free)
     assert np.abs(np.abs(np.sum(X[:, :nocc] * Xo[:, :nocc], axis=0)) - 1).max() < 1e-6
+
+
+@pytest.mark.parametrize("tucker", [False, True])
+def test_dense_storage_matches_oracle(ctx, tucker):
+    """A small, nearly full H_loc (every 1-D factor couples all basis
+    functions) is stored dense and applied by one matrix product plus the
+    fused epilogue: H X, the filter and both precisions against the oracle."""
+    s = E.SynthHam(9, 8, 7, 8, 7, 6, 5, seed=17)
+    assert s.nnz > 0.2 * s.n ** 2
+    H = oracle.Ham.from_synth(s)
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=tucker)
+    assert ham.format()[0] == 2
+    for dtype, tol in ((E.F64, 1e-13), (E.F32, 1e-5)):
+        npdt = np.float64 if dtype == E.F64 else np.float32
+        X = np.random.default_rng(4).standard_normal((s.n, 40)).astype(npdt)
+        dX, dY = dev(ctx, X, npdt), E.DeviceArray(ctx, (s.n, 40), npdt)
+        ctx.ham_apply(ham, dX.ptr, 40, dY.ptr, 40, 40, dtype)
+        assert rel(dY.to_host().astype(np.float64), oracle.ham_apply(H, X.astype(np.float64))) < tol
+    X = np.random.default_rng(5).standard_normal((s.n, 24))
+    dX = dev(ctx, X)
+    ctx.cheb_filter(ham, dX.ptr, 24, 24, 8, -0.5, 30.0, -3.0, 1, E.F64)
+    assert rel(dX.to_host(), oracle.cheb_filter(H, X, 8, -0.5, 30.0, -3.0, 1)) < 1e-12

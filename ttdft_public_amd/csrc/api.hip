@@ -341,6 +341,7 @@ int ttdft_ctx_destroy(ttdft_ctx ctx) {
     timing_flush(ctx);
     for (auto &s : ctx->scratch) s.release();
     for (auto &s : ctx->nn_scratch) s.release();
+    ctx->dense_hx.release();
     dfree(ctx->d_info);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->own_stream) {
@@ -551,6 +552,7 @@ int ttdft_ham_upload(ttdft_ctx ctx, int64_t n, const int64_t *rowptr, const int3
         }
     }
 #undef UP
+    if (rc == TTDFT_OK) rc = maybe_build_dense(ctx, H, copies);
     if (rc == TTDFT_OK) {
         hipError_t e = hipStreamSynchronize(ctx->stream);  // host staging vectors go out of scope
         if (e != hipSuccess) {
@@ -578,7 +580,8 @@ int ttdft_ham_upload_tucker(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz,
     const int64_t n = (int64_t)rx * ry * rz;
     TT_TRY(ttdft_ham_upload(ctx, n, rowptr, colind, val, p, c_rowptr, c_colind, c_val,
                             inv_c_lm, copies, out));
-    const int rc = build_band(ctx, *out, rx, ry, rz, rowptr, colind, val, copies);
+    int rc = build_band(ctx, *out, rx, ry, rz, rowptr, colind, val, copies);
+    if (rc == TTDFT_OK && (*out)->format == 1) drop_dense(*out);
     if (rc != TTDFT_OK) {
         std::string msg = g_err;
         ttdft_ham_free(*out);
@@ -605,6 +608,8 @@ int ttdft_ham_free(ttdft_ham H) {
     (void)e;
     dfree(H->mf64);
     dfree(H->mf32);
+    dfree(H->dense64);
+    dfree(H->dense32);
     dfree(H->rowptr);
     dfree(H->col);
     dfree(H->val64);

@@ -954,6 +954,7 @@ static int ham_assemble(ttdft_ctx ctx, const HlocIn &a, int32_t p, const int32_t
     }
     if (ext[0] >= 0) TT_TRY(build_band_device(ctx, H, a.rx, a.ry, a.rz, ext[0], ext[1], ext[2],
                                               copies));
+    TT_TRY(maybe_build_dense(ctx, H, copies));
     if (!(copies & TTDFT_HAM_F64)) {
         TT_HIP(hipStreamSynchronize(ctx->stream));
         TT_HIP(hipFree(H->val64));

@@ -89,6 +89,8 @@ struct ttdft_ctx_s {
     ttdft::Scratch scratch[8];
     // network-layer work: [0] transformed weights, [1] weight-gradient partials
     ttdft::Scratch nn_scratch[2];
+    // H_loc X of the dense format
+    ttdft::Scratch dense_hx;
     int *d_info = nullptr;  // rocSOLVER info
 };
 
@@ -123,6 +125,10 @@ struct ttdft_ham_s {
     int32_t gxb = 0, nxc = 0;   // x-bricks per workgroup, x-chunks
     double *mf64 = nullptr;
     float *mf32 = nullptr;
+    // dense storage of H_loc (format 2, n x n, row i contiguous) for small,
+    // nearly full matrices
+    double *dense64 = nullptr;
+    float *dense32 = nullptr;
 };
 
 namespace ttdft {
@@ -148,6 +154,10 @@ int build_band_device(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int
                       int by, int bz, int copies);
 // Y = alpha*(H_loc X + C^T W) + beta*X + delta*Z   (Z may alias Y; flags select terms)
 // use_beta / use_delta select whether X / Z are read.
+// Dense storage (format 2) of H_loc when it is nearly full and small enough
+// (no-op otherwise); drop_dense frees it.
+int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies);
+void drop_dense(ttdft_ham_s *H);
 int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                       const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
                       int32_t k, int dtype, double alpha, double beta, double delta,

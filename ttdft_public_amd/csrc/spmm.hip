@@ -198,6 +198,56 @@ int nl_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *X, int64_t ldx, int3
     return TTDFT_OK;
 }
 
+// D[row][col_e] = val_e (D zeroed beforehand); one wavefront per row
+template <typename T>
+__global__ __launch_bounds__(256) void csr_to_dense_kernel(int64_t n,
+                                                           const int64_t *__restrict__ rowptr,
+                                                           const int32_t *__restrict__ col,
+                                                           const T *__restrict__ val,
+                                                           T *__restrict__ D) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    for (int64_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64)
+        D[row * n + col[e]] = val[e];
+}
+
+// Y[i][j] = alpha (HX[i][j] + sum_e ct[e] W[ct_col_e][j]) (+ beta X[i][j]) (+ delta Z[i][j]),
+// the same epilogue as spmm_fused_kernel on a precomputed H_loc X (ld k)
+template <typename T>
+__global__ __launch_bounds__(256) void dense_epilogue_kernel(
+    int64_t n, int32_t k, const T *__restrict__ HX, const int64_t *__restrict__ ct_rowptr,
+    const int32_t *__restrict__ ct_col, const T *__restrict__ ct_val, const T *__restrict__ W,
+    const T *X, int64_t ldx, T *Y, int64_t ldy, const T *Z, int64_t ldz, T alpha, T beta,
+    T delta, int use_beta, int use_delta) {
+    const int64_t total = n * k;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t / k;
+        const int j = (int)(t - i * k);
+        T acc = HX[t];
+        if (ct_rowptr)
+            for (int64_t e = ct_rowptr[i]; e < ct_rowptr[i + 1]; ++e)
+                acc += ct_val[e] * W[(int64_t)ct_col[e] * k + j];
+        T y = acc * alpha;
+        if (use_beta) y = y + beta * X[i * ldx + j];
+        if (use_delta) y = y + delta * Z[i * ldz + j];
+        Y[i * ldy + j] = y;
+    }
+}
+
+__global__ __launch_bounds__(256) void dense_to_f32_kernel(const double *__restrict__ s,
+                                                           float *__restrict__ d, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = (float)s[i];
+}
+
+unsigned grid_blocks(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (unsigned)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
 template <typename T, int NC>
 int spmm_dispatch_nc(ttdft_ctx ctx, const ttdft_ham_s *H, const T *X, int64_t ldx, const T *W,
                      T *Y, int64_t ldy, const T *Z, int64_t ldz, int32_t k, double alpha,
@@ -293,6 +343,101 @@ int ham_spmv(ttdft_ctx ctx, const ttdft_ham_s *H, const double *x, double *y) {
     return TTDFT_OK;
 }
 
+void drop_dense(ttdft_ham_s *H) {
+    if (H->dense64) {
+        hipError_t e = hipFree(H->dense64);
+        (void)e;
+    }
+    if (H->dense32) {
+        hipError_t e = hipFree(H->dense32);
+        (void)e;
+    }
+    H->dense64 = nullptr;
+    H->dense32 = nullptr;
+    if (H->format == 2) H->format = 0;
+}
+
+// Dense storage when H_loc has no Kronecker band storage, n <= 32768 and at
+// least a fifth of the entries are nonzero: the CSR kernel then walks rows of
+// thousands of entries with the block's k states on one wavefront, while a
+// matrix product streams the n^2 values once at full bandwidth.
+int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
+    const int64_t n = H->n;
+    if (H->format != 0 || n > 32768 || n < 1 || (double)H->nnz < 0.2 * (double)n * n) return TTDFT_OK;
+    const size_t n2 = (size_t)n * n;
+    const unsigned rows = (unsigned)((n + 3) / 4);
+    if (H->val64) {
+        TT_HIP(hipMalloc(&H->dense64, sizeof(double) * n2));
+        TT_HIP(hipMemsetAsync(H->dense64, 0, sizeof(double) * n2, ctx->stream));
+        hipLaunchKernelGGL(csr_to_dense_kernel<double>, dim3(rows), dim3(256), 0, ctx->stream, n,
+                           H->rowptr, H->col, H->val64, H->dense64);
+        TT_LAUNCH_CHECK();
+        if (copies & TTDFT_HAM_F32) {
+            TT_HIP(hipMalloc(&H->dense32, sizeof(float) * n2));
+            hipLaunchKernelGGL(dense_to_f32_kernel, dim3(grid_blocks((int64_t)n2)), dim3(256), 0,
+                               ctx->stream, H->dense64, H->dense32, (int64_t)n2);
+            TT_LAUNCH_CHECK();
+        }
+        if (!(copies & TTDFT_HAM_F64)) {
+            TT_HIP(hipStreamSynchronize(ctx->stream));
+            TT_HIP(hipFree(H->dense64));
+            H->dense64 = nullptr;
+        }
+    } else if (H->val32) {
+        TT_HIP(hipMalloc(&H->dense32, sizeof(float) * n2));
+        TT_HIP(hipMemsetAsync(H->dense32, 0, sizeof(float) * n2, ctx->stream));
+        hipLaunchKernelGGL(csr_to_dense_kernel<float>, dim3(rows), dim3(256), 0, ctx->stream, n,
+                           H->rowptr, H->col, H->val32, H->dense32);
+        TT_LAUNCH_CHECK();
+    } else {
+        return TTDFT_OK;
+    }
+    H->format = 2;
+    return TTDFT_OK;
+}
+
+// H X by one library product on the dense storage (column-major view: the
+// basis-major block X is the k x n matrix X^T, and H is symmetric), then the
+// recurrence epilogue
+static int launch_dense_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
+                             const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
+                             int32_t k, int dtype, double alpha, double beta, double delta,
+                             int use_beta, int use_delta) {
+    const int64_t n = H->n;
+    const size_t es = dtype_size(dtype);
+    TT_TRY(ctx->dense_hx.ensure(es * (size_t)n * k));
+    void *HX = ctx->dense_hx.ptr;
+    const int nn = (int)n;
+    rocblas_status st;
+    if (dtype == TTDFT_F64) {
+        const double one = 1.0, zero = 0.0;
+        st = rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, k, nn, nn,
+                           &one, (const double *)X, (int)ldx, H->dense64, nn, &zero,
+                           (double *)HX, k);
+    } else {
+        const float one = 1.0f, zero = 0.0f;
+        st = rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, k, nn, nn,
+                           &one, (const float *)X, (int)ldx, H->dense32, nn, &zero, (float *)HX,
+                           k);
+    }
+    TT_CHECK(st == rocblas_status_success, TTDFT_ELIB, "dense H X: rocBLAS status %d", (int)st);
+    const unsigned blocks = grid_blocks(n * k);
+    const bool nl = H->p > 0;
+    if (dtype == TTDFT_F64)
+        hipLaunchKernelGGL(dense_epilogue_kernel<double>, dim3(blocks), dim3(256), 0, ctx->stream,
+                           n, k, (const double *)HX, nl ? H->ct_rowptr : nullptr, H->ct_col,
+                           H->ct_val64, (const double *)W, (const double *)X, ldx, (double *)Y,
+                           ldy, (const double *)Z, ldz, alpha, beta, delta, use_beta, use_delta);
+    else
+        hipLaunchKernelGGL(dense_epilogue_kernel<float>, dim3(blocks), dim3(256), 0, ctx->stream, n,
+                           k, (const float *)HX, nl ? H->ct_rowptr : nullptr, H->ct_col,
+                           H->ct_val32, (const float *)W, (const float *)X, ldx, (float *)Y, ldy,
+                           (const float *)Z, ldz, (float)alpha, (float)beta, (float)delta,
+                           use_beta, use_delta);
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
+}
+
 int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, void *Y,
              int64_t ldy, const void *Z, int64_t ldz, int32_t k, int dtype, double alpha,
              double beta, double delta, int use_beta, int use_delta) {
@@ -311,9 +456,14 @@ int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, vo
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_SPMM, &ts));
     const bool band = H->format == 1 && k >= 16 &&
                       (dtype == TTDFT_F64 ? H->mf64 != nullptr : H->mf32 != nullptr);
+    const bool dense = H->format == 2 && (dtype == TTDFT_F64 ? H->dense64 != nullptr
+                                                            : H->dense32 != nullptr);
     if (band)
         TT_TRY(launch_band_spmm(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
                                 alpha, beta, delta, use_beta, use_delta));
+    else if (dense)
+        TT_TRY(launch_dense_spmm(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
+                                 alpha, beta, delta, use_beta, use_delta));
     else
         TT_TRY(launch_spmm_fused(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
                                  alpha, beta, delta, use_beta, use_delta));
