@@ -544,6 +544,10 @@ int ttdft_grid_clamp(ttdft_ctx ctx, double *d_f, int64_t n, double lo);
  * {D, H, W}.  Convolutions: kernel ks = 1 or 3, stride 1, zero padding ks/2,
  * weights [co][ci][ks^3] (taps z-major), bias [co] (may be NULL).
  * accumulate = 1 adds into the output instead of writing it. */
+/* Arithmetic of the convolutions on this context: FP32 vector math, or BF16
+ * MFMA (operands rounded to BF16, FP32 accumulation).  Tensors stay FP32. */
+enum { TTDFT_NN_FP32 = 0, TTDFT_NN_BF16 = 1 };
+int ttdft_nn_set_math(ttdft_ctx ctx, int math);
 int ttdft_nn_conv3d_forward(ttdft_ctx ctx, const float *d_x, int32_t ci, const int32_t dims[3],
                             const float *d_w, const float *d_b, int32_t co, int32_t ks,
                             float *d_y, int accumulate);

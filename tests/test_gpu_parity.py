@@ -261,8 +261,9 @@ def test_rotate_and_orth(ctx, mode):
     assert np.abs(Xo @ (Xo.T @ ref) - ref).max() / np.abs(ref).max() < (1e-11 if not f32 else 1e-4)
 
 
-def test_orth_matches_oracle_f64(ctx):
-    n, k = 2000, 40
+@pytest.mark.parametrize("k", [40, 64, 80])  # k <= 64: one-workgroup Cholesky inverse
+def test_orth_matches_oracle_f64(ctx, k):
+    n = 2000
     X = np.random.default_rng(1).standard_normal((n, k))
     dX = dev(ctx, X)
     ctx.orth(dX.ptr, k, n, k)
@@ -279,7 +280,7 @@ def test_orth_not_spd_reports_error(ctx):
     assert ei.value.status in (E.ENOTSPD,)
 
 
-@pytest.mark.parametrize("k", [24, 300])
+@pytest.mark.parametrize("k", [1, 7, 24, 64, 65, 300])  # k <= 64: one-workgroup Jacobi
 def test_rr_eig(ctx, k):
     A = np.random.default_rng(k).standard_normal((k, k))
     A = A + A.T

@@ -144,3 +144,40 @@ def test_unet_forward_backward_match_oracle(ctx, n, widths):
         wo, bo = net.offsets[name]
         sl = slice(wo, bo + co)
         assert close(g[sl], g0[sl], 2e-4), name
+
+
+@pytest.mark.parametrize("ci,co,ks,shape", [(3, 37, 3, (5, 6, 7)), (48, 16, 3, (8, 8, 8)),
+                                            (16, 1, 1, (6, 6, 6))])
+def test_conv_bf16_mfma(ctx, ci, co, ks, shape):
+    """BF16 MFMA forms (operands rounded to BF16, FP32 accumulation): forward,
+    backward data and backward weight within BF16 rounding of the oracle,
+    2e-2 of the largest entry."""
+    rng = np.random.default_rng(ci * co)
+    x = rng.standard_normal((ci,) + shape)
+    w = rng.standard_normal((co, ci, ks ** 3)) / np.sqrt(ci * ks ** 3)
+    b = rng.standard_normal(co)
+    dy = rng.standard_normal((co,) + shape)
+    L, h = E.lib(), ctx.h
+    P = int(np.prod(shape))
+    dx_, dw_, db_, ddy = f32(ctx, x), f32(ctx, w), f32(ctx, b), f32(ctx, dy)
+    y = E.DeviceArray(ctx, (co * P,), np.float32)
+    gx = E.DeviceArray(ctx, (ci * P,), np.float32)
+    gw = E.DeviceArray(ctx, (w.size,), np.float32)
+    E._check(L.ttdft_nn_set_math(h, E.NN_BF16))
+    try:
+        print(f"conv3d bf16 ci={ci} co={co} ks={ks} shape={shape}", flush=True)
+        E._check(L.ttdft_nn_conv3d_forward(h, dx_.ptr, ci, dims(shape), dw_.ptr, db_.ptr, co, ks,
+                                           y.ptr, 0))
+        E._check(L.ttdft_nn_conv3d_backward_data(h, ddy.ptr, co, dims(shape), dw_.ptr, ci, ks,
+                                                 gx.ptr, 0))
+        E._check(L.ttdft_nn_conv3d_backward_weight(h, dx_.ptr, ci, dims(shape), ddy.ptr, co, ks,
+                                                   gw.ptr, None, 0))
+    finally:
+        E._check(L.ttdft_nn_set_math(h, E.NN_FP32))
+    y0 = U.conv3d(x, w, b, ks)
+    gx0, gw0, _ = U.conv3d_backward(x, w, dy, ks)
+    assert close(y.to_host().reshape(y0.shape), y0, 2e-2)
+    assert close(gx.to_host().reshape(gx0.shape), gx0, 2e-2)
+    assert close(gw.to_host().reshape(gw0.shape), gw0, 2e-2)
+    # and it is not the FP32 path: BF16 rounding shows at the 1e-4 level
+    assert not close(y.to_host().reshape(y0.shape), y0, 1e-6)

@@ -262,6 +262,15 @@ static int orth_once(ttdft_ctx ctx, void *X, int64_t ldx, int64_t n, int32_t k, 
     double *S = (double *)ctx->scratch[1].ptr;
     double *QT = S + kk;
     TT_TRY(launch_gram(ctx, X, ldx, X, ldx, n, k, dtype, mode, S));
+    if (small_dense_enabled(k)) {
+        TT_TRY(small_chol_inv(ctx, S, k, QT));
+        const size_t es = dtype_size(dtype);
+        TT_TRY(ctx->scratch[0].ensure((size_t)n * k * es));
+        void *Z = ctx->scratch[0].ptr;
+        TT_TRY(launch_rotate(ctx, X, ldx, Z, k, n, k, QT, dtype, mode));
+        TT_TRY(launch_copy_block(ctx, Z, k, dtype, X, ldx, dtype, n, k));
+        return TTDFT_OK;
+    }
     rocblas_status st = rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, k, S, k, ctx->d_info);
     TT_CHECK(st == rocblas_status_success, TTDFT_ELIB, "rocsolver_dpotrf: status %d", (int)st);
     int info = 0;
@@ -715,6 +724,9 @@ int ttdft_rr_eig(ttdft_ctx ctx, const double *dA, int32_t k, double *d_evals, do
     TT_CHECK(ctx && dA && d_evals && dQ && k >= 1, TTDFT_EBADARG, "rr_eig: bad arguments");
     TT_TRY(ctx->scratch[1].ensure((size_t)k * k * 2 * sizeof(double) + k * sizeof(double)));
     double *E = (double *)ctx->scratch[1].ptr + (size_t)k * k * 2;
+    // the one-workgroup kernel reads A whole into LDS before it writes Q, so
+    // dQ may alias dA
+    if (small_dense_enabled(k)) return small_syev(ctx, dA, k, d_evals, dQ);
     if (dQ != dA)
         TT_HIP(hipMemcpyAsync(dQ, dA, sizeof(double) * k * k, hipMemcpyDeviceToDevice,
                               ctx->stream));

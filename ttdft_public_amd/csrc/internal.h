@@ -91,6 +91,7 @@ struct ttdft_ctx_s {
     ttdft::Scratch nn_scratch[2];
     // H_loc X of the dense format
     ttdft::Scratch dense_hx;
+    int nn_math = 0;  // TTDFT_NN_FP32 or TTDFT_NN_BF16 for the network's convolutions
     int *d_info = nullptr;  // rocSOLVER info
 };
 
@@ -154,6 +155,10 @@ int build_band_device(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int
                       int by, int bz, int copies);
 // Y = alpha*(H_loc X + C^T W) + beta*X + delta*Z   (Z may alias Y; flags select terms)
 // use_beta / use_delta select whether X / Z are read.
+// One-workgroup k x k factorisations for k <= 64 (small_dense.hip)
+bool small_dense_enabled(int32_t k);
+int small_chol_inv(ttdft_ctx ctx, const double *S, int32_t k, double *QT);
+int small_syev(ttdft_ctx ctx, const double *A, int32_t k, double *evals, double *Q);
 // Dense storage (format 2) of H_loc when it is nearly full and small enough
 // (no-op otherwise); drop_dense frees it.
 int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies);

@@ -111,6 +111,152 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(const float *__restrict__
     }
 }
 
+// ---- BF16 MFMA forms -----------------------------------------------------------
+// The same products on v_mfma_f32_16x16x16_bf16: operands rounded to BF16
+// (nearest even) as they are staged into LDS, FP32 accumulation.  MFMA
+// layouts: A[i][kk] in lane i + 16 (kk/4), element kk%4; B[kk][j] in lane
+// j + 16 (kk/4), element kk%4; element v of lane l holds D[4 (l/16) + v][l%16].
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short to_bf16(float x) {
+    unsigned u = __builtin_bit_cast(unsigned, x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+// Forward: a workgroup computes 32 (co) x 128 (positions) of Y; wave w owns
+// positions 32w .. 32w+31 (two 16-column MFMA tiles) and both 16-row tiles.
+template <int KS>
+__global__ __launch_bounds__(256) void conv_fwd_bf16_kernel(const float *__restrict__ X, int ci,
+                                                            Geo g, const float *__restrict__ Wt,
+                                                            const float *__restrict__ bias,
+                                                            int co, float *__restrict__ Y,
+                                                            int accumulate) {
+    __shared__ unsigned short As[kTM][kTK + 4];   // [m][k]
+    __shared__ unsigned short Bs[kTN][kTK + 4];   // [position][k]
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int m0 = blockIdx.y * kTM;
+    const int64_t n0 = (int64_t)blockIdx.x * kTN;
+    constexpr int KT = KS * KS * KS;
+    const int K = ci * KT;
+    const int64_t pb = n0 + (t & 127);
+    int bx = 0, by = 0, bz = 0;
+    decompose(g, pb < g.P ? pb : 0, bx, by, bz);
+    f4v acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    const int li = lane & 15, lk = lane >> 4;
+    for (int kc = 0; kc < K; kc += kTK) {
+        for (int r = 0; r < 2; ++r) {  // A: 32 x 16
+            const int e = t + 256 * r;
+            const int m = e >> 4, kk = e & 15;
+            const int k = kc + kk;
+            As[m][kk] = to_bf16((m0 + m < co && k < K) ? Wt[(int64_t)(m0 + m) * K + k] : 0.0f);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {  // B: 16 x 128, gathered
+            const int kk = (t >> 7) + 2 * r;
+            const int k = kc + kk;
+            Bs[t & 127][kk] =
+                to_bf16((k < K && pb < g.P) ? gather<KS>(X, g, bx, by, bz, k) : 0.0f);
+        }
+        __syncthreads();
+        s4v a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[i][v] = (short)As[16 * i + li][4 * lk + v];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b[j][v] = (short)Bs[32 * wave + 16 * j + li][4 * lk + v];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = m0 + 16 * i + 4 * lk + v;
+            if (m >= co) continue;
+            const float bv = bias ? bias[m] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int64_t p = n0 + 32 * wave + 16 * j + li;
+                if (p >= g.P) continue;
+                const float val = acc[i][j][v] + bv;
+                float *dst = Y + (int64_t)m * g.P + p;
+                *dst = accumulate ? *dst + val : val;
+            }
+        }
+}
+
+// Weight gradient: a workgroup computes partial[s][m][k] for 32 (co) x 64 (k)
+// over one position range; the MFMA reduction dimension is 16 positions per
+// step; wave w owns k columns 16w .. 16w+15 and both 16-row tiles.
+template <int KS>
+__global__ __launch_bounds__(256) void conv_wgrad_bf16_kernel(const float *__restrict__ X, int ci,
+                                                              Geo g, const float *__restrict__ dY,
+                                                              int co, int64_t span,
+                                                              float *__restrict__ partial) {
+    __shared__ unsigned short Ds[kTM][16 + 4];  // [m][position]
+    __shared__ unsigned short Cs[64][16 + 4];   // [k][position]
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int m0 = blockIdx.y * kTM;
+    const int k0 = blockIdx.x * 64;
+    constexpr int KT = KS * KS * KS;
+    const int K = ci * KT;
+    const int64_t pb = (int64_t)blockIdx.z * span;
+    const int64_t pe = min(pb + span, g.P);
+    const int li = lane & 15, lk = lane >> 4;
+    f4v acc[2] = {f4v{0.0f, 0.0f, 0.0f, 0.0f}, f4v{0.0f, 0.0f, 0.0f, 0.0f}};
+    for (int64_t pc = pb; pc < pe; pc += 16) {
+        for (int r = 0; r < 2; ++r) {  // dY: 32 x 16
+            const int e = t + 256 * r;
+            const int pp = e & 15, m = e >> 4;
+            const int64_t p = pc + pp;
+            Ds[m][pp] = to_bf16((m0 + m < co && p < pe) ? dY[(int64_t)(m0 + m) * g.P + p] : 0.0f);
+        }
+        const int64_t p = pc + (t & 15);
+        int x = 0, y = 0, z = 0;
+        decompose(g, p < pe ? p : pb, x, y, z);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {  // X_k: 64 x 16, gathered
+            const int kk = (t >> 4) + 16 * r;
+            const int k = k0 + kk;
+            Cs[kk][t & 15] = to_bf16((k < K && p < pe) ? gather<KS>(X, g, x, y, z, k) : 0.0f);
+        }
+        __syncthreads();
+        s4v a[2], b;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[i][v] = (short)Ds[16 * i + li][4 * lk + v];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) b[v] = (short)Cs[16 * wave + li][4 * lk + v];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b, acc[i], 0, 0, 0);
+        __syncthreads();
+    }
+    float *out = partial + (int64_t)blockIdx.z * co * K;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = m0 + 16 * i + 4 * lk + v;
+            const int k = k0 + 16 * wave + li;
+            if (m < co && k < K) out[(int64_t)m * K + k] = acc[i][v];
+        }
+}
+
 // W'[c][m ks^3 + (ks^3 - 1 - tap)] = W[m][c ks^3 + tap]
 __global__ __launch_bounds__(256) void flip_weights_kernel(const float *__restrict__ Wt, int co,
                                                            int ci, int KT,
@@ -368,6 +514,16 @@ int conv_forward(ttdft_ctx ctx, const float *x, int ci, const Geo &g, const floa
                  const float *b, int co, int ks, float *y, int accumulate) {
     const dim3 grid((unsigned)((g.P + kTN - 1) / kTN), (unsigned)((co + kTM - 1) / kTM));
     TT_CHECK(grid.x <= 0x7fffffffu && grid.y <= 65535u, TTDFT_EBADARG, "conv3d: grid too large");
+    if (ctx->nn_math == TTDFT_NN_BF16) {
+        if (ks == 3)
+            hipLaunchKernelGGL(conv_fwd_bf16_kernel<3>, grid, dim3(256), 0, ctx->stream, x, ci, g,
+                               w, b, co, y, accumulate);
+        else
+            hipLaunchKernelGGL(conv_fwd_bf16_kernel<1>, grid, dim3(256), 0, ctx->stream, x, ci, g,
+                               w, b, co, y, accumulate);
+        TT_LAUNCH_CHECK();
+        return TTDFT_OK;
+    }
     if (ks == 3)
         hipLaunchKernelGGL(conv_fwd_kernel<3>, grid, dim3(256), 0, ctx->stream, x, ci, g, w, b, co,
                            y, accumulate);
@@ -392,6 +548,14 @@ int check_conv(ttdft_ctx ctx, int ci, int co, int ks) {
 using namespace ttdft;
 
 extern "C" {
+
+int ttdft_nn_set_math(ttdft_ctx ctx, int math) {
+    clear_error();
+    TT_CHECK(ctx && (math == TTDFT_NN_FP32 || math == TTDFT_NN_BF16), TTDFT_EBADARG,
+             "nn_set_math: unknown mode %d", math);
+    ctx->nn_math = math;
+    return TTDFT_OK;
+}
 
 int ttdft_nn_conv3d_forward(ttdft_ctx ctx, const float *d_x, int32_t ci, const int32_t dims[3],
                             const float *d_w, const float *d_b, int32_t co, int32_t ks,
@@ -442,12 +606,20 @@ int ttdft_nn_conv3d_backward_weight(ttdft_ctx ctx, const float *d_x, int32_t ci,
     const int64_t nw = (int64_t)co * K;
     TT_TRY(ctx->nn_scratch[1].ensure(sizeof(float) * nw * nsplit));
     float *partial = (float *)ctx->nn_scratch[1].ptr;
-    if (ks == 3)
-        hipLaunchKernelGGL(conv_wgrad_kernel<3>, dim3(gx, gy, (unsigned)nsplit), dim3(256), 0,
-                           ctx->stream, d_x, ci, g, d_dy, co, span, partial);
+    const dim3 wg(gx, gy, (unsigned)nsplit);
+    const bool bf = ctx->nn_math == TTDFT_NN_BF16;
+    if (ks == 3 && bf)
+        hipLaunchKernelGGL(conv_wgrad_bf16_kernel<3>, wg, dim3(256), 0, ctx->stream, d_x, ci, g,
+                           d_dy, co, span, partial);
+    else if (bf)
+        hipLaunchKernelGGL(conv_wgrad_bf16_kernel<1>, wg, dim3(256), 0, ctx->stream, d_x, ci, g,
+                           d_dy, co, span, partial);
+    else if (ks == 3)
+        hipLaunchKernelGGL(conv_wgrad_kernel<3>, wg, dim3(256), 0, ctx->stream, d_x, ci, g, d_dy,
+                           co, span, partial);
     else
-        hipLaunchKernelGGL(conv_wgrad_kernel<1>, dim3(gx, gy, (unsigned)nsplit), dim3(256), 0,
-                           ctx->stream, d_x, ci, g, d_dy, co, span, partial);
+        hipLaunchKernelGGL(conv_wgrad_kernel<1>, wg, dim3(256), 0, ctx->stream, d_x, ci, g, d_dy,
+                           co, span, partial);
     TT_LAUNCH_CHECK();
     hipLaunchKernelGGL(split_reduce_kernel, dim3(nblk(nw)), dim3(256), 0, ctx->stream, partial,
                        (int)nsplit, nw, d_dw, accumulate);

@@ -169,6 +169,7 @@ _SIGS = {
     "ttdft_gaussian_splat": (ctypes.c_int, [_vp, _i32, _c_dp, _d, _c_i32p, _c_dp, _c_dp, _vp]),
     "ttdft_grid_clamp": (ctypes.c_int, [_vp, _vp, _i64, _d]),
     "ttdft_grid_axpby": (ctypes.c_int, [_vp, _vp, _d, _vp, _d, _i64]),
+    "ttdft_nn_set_math": (ctypes.c_int, [_vp, ctypes.c_int]),
     "ttdft_nn_conv3d_forward": (ctypes.c_int, [_vp, _vp, _i32, _c_i32p, _vp, _vp, _i32, _i32,
                                                _vp, ctypes.c_int]),
     "ttdft_nn_conv3d_backward_data": (ctypes.c_int, [_vp, _vp, _i32, _c_i32p, _vp, _i32, _i32,
@@ -248,6 +249,7 @@ def rng_normal(seed, count):
 
 MESH_UNIFORM, MESH_ADAPTIVE, MESH_MANUAL1, MESH_HARTREE_OUTER = 0, 1, 3, 4
 LDA_PZ81, LDA_PZ81_REFERENCE = 0, 1
+NN_FP32, NN_BF16 = 0, 1
 MESH_TYPES = {"uniform": MESH_UNIFORM, "adaptive": MESH_ADAPTIVE, "manual1": MESH_MANUAL1,
               "hartree_outer": MESH_HARTREE_OUTER}
 

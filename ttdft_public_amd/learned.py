@@ -51,9 +51,10 @@ def _c(rc):
 class UNet:
     """The network's parameters, Adam state and activations on one device."""
 
-    def __init__(self, ctx, n=64, cin=2, widths=(16, 32, 64, 128), seed=0):
+    def __init__(self, ctx, n=64, cin=2, widths=(16, 32, 64, 128), seed=0, arith=E.NN_FP32):
         assert n % 8 == 0
         self.ctx, self.n, self.cin, self.widths = ctx, n, cin, tuple(widths)
+        self.arith = arith  # E.NN_FP32, or E.NN_BF16: convolutions on BF16 MFMA
         self.specs = layer_specs(cin, widths)
         self.offsets = {}
         o = 0
@@ -136,8 +137,18 @@ class UNet:
             _c(E.lib().ttdft_nn_conv3d_backward_data(self.ctx.h, gy_ptr, co, d, w, ci, ks, gx_ptr,
                                                      0))
 
+    def _math(self, on):
+        _c(E.lib().ttdft_nn_set_math(self.ctx.h, self.arith if on else E.NN_FP32))
+
     def forward(self):
         """y = net(x) on the buffers a["x"] -> a["y"]."""
+        self._math(True)
+        try:
+            self._forward()
+        finally:
+            self._math(False)
+
+    def _forward(self):
         a, P, (c0, c1, c2, c3) = self.a, self.P, self.widths
         L, h = E.lib(), self.ctx.h
         s0, s1, s2 = (self._sub(a["cat0"], c1, P[0]), self._sub(a["cat1"], c2, P[1]),
@@ -168,6 +179,13 @@ class UNet:
         """Parameter gradients for the output gradient g["y"], written or (with
         accumulate, over the samples of a batch) added to self.grads."""
         self._acc = 1 if accumulate else 0
+        self._math(True)
+        try:
+            self._backward()
+        finally:
+            self._math(False)
+
+    def _backward(self):
         a, g, P, (c0, c1, c2, c3) = self.a, self.g, self.P, self.widths
         L, h = E.lib(), self.ctx.h
         s0, s1, s2 = (self._sub(a["cat0"], c1, P[0]), self._sub(a["cat1"], c2, P[1]),
@@ -220,12 +238,15 @@ class UNet:
     def save(self, path, **meta):
         np.savez_compressed(path, params=self.get_params().astype(np.float32),
                             widths=np.array(self.widths), cin=self.cin, n=self.n,
-                            in_scale=self.in_scale, out_scale=self.out_scale, **meta)
+                            in_scale=self.in_scale, out_scale=self.out_scale, arith=self.arith,
+                            **meta)
 
     @classmethod
     def load(cls, ctx, path=DEFAULT_WEIGHTS):
         d = np.load(path, allow_pickle=False)
-        net = cls(ctx, int(d["n"]), int(d["cin"]), tuple(int(w) for w in d["widths"]))
+        arith = int(d["arith"]) if "arith" in d.files else E.NN_FP32
+        net = cls(ctx, int(d["n"]), int(d["cin"]), tuple(int(w) for w in d["widths"]),
+                  arith=arith)
         net.set_params(d["params"])
         net.in_scale, net.out_scale = float(d["in_scale"]), float(d["out_scale"])
         return net
@@ -316,10 +337,10 @@ def converged_sample(ctx, xyz, cfg, n):
     return sup, conv - sup, r
 
 
-def train(ctx, samples, n=64, epochs=40, lr=1e-3, seed=0, log=print):
+def train(ctx, samples, n=64, epochs=40, lr=1e-3, seed=0, log=print, arith=E.NN_FP32):
     """samples: list of (xyz, superposition, correction) on the n^3 grid.
     Returns the trained network (scales fitted to the data)."""
-    net = UNet(ctx, n, seed=seed)
+    net = UNet(ctx, n, seed=seed, arith=arith)
     sup_max = max(float(np.abs(s).max()) for _, s, _ in samples)
     corr_rms = math.sqrt(np.mean([np.mean(c ** 2) for _, _, c in samples]))
     net.in_scale, net.out_scale = 1.0 / sup_max, 1.0 / corr_rms
@@ -411,6 +432,8 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--n", type=int, default=64, help="uniform grid points per axis")
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--math", choices=["fp32", "bf16"], default="bf16",
+                    help="convolution arithmetic: FP32 vector math or BF16 MFMA")
     ap.add_argument("--out", required=True, help="weights (.npz)")
     ap.add_argument("--report", required=True, help="evaluation (.json)")
     a = ap.parse_args(argv)
@@ -428,8 +451,9 @@ def main(argv=None):
         print(f"sample {len(samples)}: {len(xyz)} atoms, {r.iterations} SCF iterations, "
               f"{r.seconds:.2f} s", flush=True)
     t_data = time.perf_counter() - t0
+    arith = E.NN_BF16 if a.math == "bf16" else E.NN_FP32
     net = train(ctx, samples, a.n, a.epochs, a.lr, seed=a.seed,
-                log=lambda m: print(m, flush=True))
+                log=lambda m: print(m, flush=True), arith=arith)
     t_train = time.perf_counter() - t0 - t_data
     net.save(a.out, seed=a.seed, train_clusters=a.train, epochs=a.epochs)
     test_rng = np.random.default_rng(a.seed + 1)
@@ -441,6 +465,7 @@ def main(argv=None):
     sup_it = [e["superposition"]["to_1e-6"] or 80 for e in evals]
     lrn_it = [e["learned"]["to_1e-6"] or 80 for e in evals]
     report = {"train_clusters": a.train, "failed_scf": failed, "epochs": a.epochs,
+              "math": a.math,
               "data_seconds": t_data, "train_seconds": t_train, "loss_history": net.history,
               "params": net.nparams, "test": evals,
               "mean_iterations_to_1e-6": {"superposition": float(np.mean(sup_it)),

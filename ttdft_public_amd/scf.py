@@ -189,7 +189,10 @@ class Axis:
     def mat_pot(self, d_U, rv):
         """MatPot (rv, R, R) of the V_eff factor d_U (np x rv column-major)."""
         r = self.cfg.rank
-        d_mp = E.DeviceArray(self.ctx, (rv * r * r,))
+        if getattr(self, "_mp_rank", None) != rv:
+            self._d_mp = E.DeviceArray(self.ctx, (rv * r * r,))
+            self._mp_rank = rv
+        d_mp = self._d_mp
         self.ctx.tucker_basis_matrices(self.np, r, self.d_B.ptr, self.d_DB.ptr, self.d_jw.ptr,
                                        self.d_inv_jw.ptr, rv, d_U.ptr, None, None, d_mp.ptr)
         return d_mp.to_host().reshape(rv, r, r)
@@ -274,10 +277,15 @@ class Scf:
 
     # ---- potential -----------------------------------------------------------------
     def tucker(self, d_field, rank):
-        r = (rank,) * 3
-        d_core = E.DeviceArray(self.ctx, (rank ** 3,))
-        d_U = [E.DeviceArray(self.ctx, (n * rank,)) for n in self.p]
-        self.ctx.sthosvd(d_field.ptr, self.p, r, d_core.ptr, [u.ptr for u in d_U])
+        """STHOSVD of a grid field into buffers kept per rank (reused every
+        iteration)."""
+        if not hasattr(self, "_tucker_bufs"):
+            self._tucker_bufs = {}
+        if rank not in self._tucker_bufs:
+            self._tucker_bufs[rank] = (E.DeviceArray(self.ctx, (rank ** 3,)),
+                                       [E.DeviceArray(self.ctx, (n * rank,)) for n in self.p])
+        d_core, d_U = self._tucker_bufs[rank]
+        self.ctx.sthosvd(d_field.ptr, self.p, (rank,) * 3, d_core.ptr, [u.ptr for u in d_U])
         return d_core, d_U
 
     def effective_potential(self, d_rho, d_veff, d_vh, d_vxc, d_exc):
