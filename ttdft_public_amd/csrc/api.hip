@@ -350,7 +350,6 @@ int ttdft_ctx_destroy(ttdft_ctx ctx) {
     timing_flush(ctx);
     for (auto &s : ctx->scratch) s.release();
     for (auto &s : ctx->nn_scratch) s.release();
-    ctx->dense_hx.release();
     dfree(ctx->d_info);
     if (ctx->blas) rocblas_destroy_handle(ctx->blas);
     if (ctx->own_stream) {

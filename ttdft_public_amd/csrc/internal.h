@@ -89,8 +89,6 @@ struct ttdft_ctx_s {
     ttdft::Scratch scratch[8];
     // network-layer work: [0] transformed weights, [1] weight-gradient partials
     ttdft::Scratch nn_scratch[2];
-    // H_loc X of the dense format
-    ttdft::Scratch dense_hx;
     int nn_math = 0;  // TTDFT_NN_FP32 or TTDFT_NN_BF16 for the network's convolutions
     int *d_info = nullptr;  // rocSOLVER info
 };

@@ -212,27 +212,67 @@ __global__ __launch_bounds__(256) void csr_to_dense_kernel(int64_t n,
         D[row * n + col[e]] = val[e];
 }
 
-// Y[i][j] = alpha (HX[i][j] + sum_e ct[e] W[ct_col_e][j]) (+ beta X[i][j]) (+ delta Z[i][j]),
-// the same epilogue as spmm_fused_kernel on a precomputed H_loc X (ld k)
+// Dense H_loc X with the recurrence epilogue, on FP64 MFMA (16x16x4): a
+// workgroup owns 16 rows of Y and 64 states; its 4 waves split the inner
+// dimension l in four and their partial tiles are summed through LDS.  The
+// MFMA computes D[j][i] = sum_l X[l][j] H[l][i] (H symmetric): the A operand
+// is X^T (16 states x 4 l, lane j + 16 kk), the B operand H (4 l x 16 rows,
+// lane i + 16 kk: 16 consecutive entries of one row of H per 16 lanes); D
+// element v of lane L is D[L/16 + 4v][L%16].  FP32 storage is widened to FP64
+// on load.  Then
+// Y[i][j] = alpha (HX[i][j] + sum_e ct[e] W[ct_col_e][j]) (+ beta X[i][j]) (+ delta Z[i][j]).
+typedef double dv4 __attribute__((ext_vector_type(4)));
+
 template <typename T>
-__global__ __launch_bounds__(256) void dense_epilogue_kernel(
-    int64_t n, int32_t k, const T *__restrict__ HX, const int64_t *__restrict__ ct_rowptr,
-    const int32_t *__restrict__ ct_col, const T *__restrict__ ct_val, const T *__restrict__ W,
-    const T *X, int64_t ldx, T *Y, int64_t ldy, const T *Z, int64_t ldz, T alpha, T beta,
-    T delta, int use_beta, int use_delta) {
-    const int64_t total = n * k;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = t / k;
-        const int j = (int)(t - i * k);
-        T acc = HX[t];
+__global__ __launch_bounds__(256) void dense_hx_kernel(
+    int64_t n, int32_t k, const T *__restrict__ H, const T *X, int64_t ldx,
+    const int64_t *__restrict__ ct_rowptr, const int32_t *__restrict__ ct_col,
+    const T *__restrict__ ct_val, const T *__restrict__ W, T *Y, int64_t ldy, const T *Z,
+    int64_t ldz, double alpha, double beta, double delta, int use_beta, int use_delta) {
+    __shared__ double red[4][64][17];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t i0 = (int64_t)blockIdx.x * 16;
+    const int c0 = blockIdx.y * 64;
+    const int64_t quarter = ((n + 3) / 4 + 3) / 4 * 4;
+    const int64_t lb = wave * quarter, le = lb + quarter < n ? lb + quarter : n;
+    const int64_t row = i0 + li;
+    const bool row_ok = row < n;
+    bool col_ok[4];
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) col_ok[jt] = c0 + 16 * jt + li < k;
+    dv4 acc[4];
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) acc[jt] = dv4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t l0 = lb; l0 < le; l0 += 4) {
+        const int64_t l = l0 + lk;
+        const bool l_ok = l < le;
+        const double h = (l_ok && row_ok) ? (double)H[l * n + row] : 0.0;
+        const T *xr = X + l * ldx + c0 + li;
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) {
+            const double x = (l_ok && col_ok[jt]) ? (double)xr[16 * jt] : 0.0;
+            acc[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, h, acc[jt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[wave][16 * jt + lk + 4 * v][li] = acc[jt][v];
+    __syncthreads();
+    for (int e = t; e < 16 * 64; e += 256) {
+        const int j = e & 63, ii = e >> 6;
+        const int64_t r = i0 + ii;
+        const int c = c0 + j;
+        if (r >= n || c >= k) continue;
+        double s = (red[0][j][ii] + red[1][j][ii]) + (red[2][j][ii] + red[3][j][ii]);
         if (ct_rowptr)
-            for (int64_t e = ct_rowptr[i]; e < ct_rowptr[i + 1]; ++e)
-                acc += ct_val[e] * W[(int64_t)ct_col[e] * k + j];
-        T y = acc * alpha;
-        if (use_beta) y = y + beta * X[i * ldx + j];
-        if (use_delta) y = y + delta * Z[i * ldz + j];
-        Y[i * ldy + j] = y;
+            for (int64_t f = ct_rowptr[r]; f < ct_rowptr[r + 1]; ++f)
+                s += (double)ct_val[f] * (double)W[(int64_t)ct_col[f] * k + c];
+        double y = s * alpha;
+        if (use_beta) y += beta * (double)X[r * ldx + c];
+        if (use_delta) y += delta * (double)Z[r * ldz + c];
+        Y[r * ldy + c] = (T)y;
     }
 }
 
@@ -396,44 +436,24 @@ int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
     return TTDFT_OK;
 }
 
-// H X by one library product on the dense storage (column-major view: the
-// basis-major block X is the k x n matrix X^T, and H is symmetric), then the
-// recurrence epilogue
+// H X on the dense storage with the recurrence epilogue in one kernel
 static int launch_dense_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                              const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
                              int32_t k, int dtype, double alpha, double beta, double delta,
                              int use_beta, int use_delta) {
     const int64_t n = H->n;
-    const size_t es = dtype_size(dtype);
-    TT_TRY(ctx->dense_hx.ensure(es * (size_t)n * k));
-    void *HX = ctx->dense_hx.ptr;
-    const int nn = (int)n;
-    rocblas_status st;
-    if (dtype == TTDFT_F64) {
-        const double one = 1.0, zero = 0.0;
-        st = rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, k, nn, nn,
-                           &one, (const double *)X, (int)ldx, H->dense64, nn, &zero,
-                           (double *)HX, k);
-    } else {
-        const float one = 1.0f, zero = 0.0f;
-        st = rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, k, nn, nn,
-                           &one, (const float *)X, (int)ldx, H->dense32, nn, &zero, (float *)HX,
-                           k);
-    }
-    TT_CHECK(st == rocblas_status_success, TTDFT_ELIB, "dense H X: rocBLAS status %d", (int)st);
-    const unsigned blocks = grid_blocks(n * k);
+    const dim3 grid((unsigned)((n + 15) / 16), (unsigned)((k + 63) / 64));
     const bool nl = H->p > 0;
     if (dtype == TTDFT_F64)
-        hipLaunchKernelGGL(dense_epilogue_kernel<double>, dim3(blocks), dim3(256), 0, ctx->stream,
-                           n, k, (const double *)HX, nl ? H->ct_rowptr : nullptr, H->ct_col,
-                           H->ct_val64, (const double *)W, (const double *)X, ldx, (double *)Y,
-                           ldy, (const double *)Z, ldz, alpha, beta, delta, use_beta, use_delta);
+        hipLaunchKernelGGL(dense_hx_kernel<double>, grid, dim3(256), 0, ctx->stream, n, k,
+                           H->dense64, (const double *)X, ldx, nl ? H->ct_rowptr : nullptr,
+                           H->ct_col, H->ct_val64, (const double *)W, (double *)Y, ldy,
+                           (const double *)Z, ldz, alpha, beta, delta, use_beta, use_delta);
     else
-        hipLaunchKernelGGL(dense_epilogue_kernel<float>, dim3(blocks), dim3(256), 0, ctx->stream, n,
-                           k, (const float *)HX, nl ? H->ct_rowptr : nullptr, H->ct_col,
-                           H->ct_val32, (const float *)W, (const float *)X, ldx, (float *)Y, ldy,
-                           (const float *)Z, ldz, (float)alpha, (float)beta, (float)delta,
-                           use_beta, use_delta);
+        hipLaunchKernelGGL(dense_hx_kernel<float>, grid, dim3(256), 0, ctx->stream, n, k,
+                           H->dense32, (const float *)X, ldx, nl ? H->ct_rowptr : nullptr,
+                           H->ct_col, H->ct_val32, (const float *)W, (float *)Y, ldy,
+                           (const float *)Z, ldz, alpha, beta, delta, use_beta, use_delta);
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }

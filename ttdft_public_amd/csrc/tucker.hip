@@ -4,8 +4,9 @@
 //
 // STHOSVD (fixed ranks, modes 0, 1, 2; the algorithm of the third-party
 // TuckerMPI library the reference calls): for each mode the Gram matrix of
-// the mode unfolding (one rocBLAS product; modes 1 and 2 see the tensor
-// already shrunk in the earlier modes), rocSOLVER dsyevd, the leading
+// the mode unfolding (the row-chunked MFMA Gram kernel for modes 0 and 1, a
+// rocBLAS product for mode 2; modes 1 and 2 see the tensor already shrunk in
+// the earlier modes), rocSOLVER dsyevd, the leading
 // eigenvectors with the largest |entry| of each made positive, and the
 // product that shrinks the tensor.  Tensors are first index fastest.
 //
@@ -35,16 +36,50 @@ namespace {
                  (int)st_);                                                               \
     } while (0)
 
-struct Dev {
-    void *p = nullptr;
-    ~Dev() {
-        if (p) {
+// Work buffers of STHOSVD and the Hartree expansion come from a cache of
+// blocks by (device, size): the SCF loop calls both with the same shapes every
+// iteration, and hipMalloc / hipFree of hundreds of MB (hipFree synchronises
+// the device) would otherwise dominate.  A returned block is reused in stream
+// order on the context's stream; the cache keeps at most 8 GiB per process.
+struct BlockCache {
+    std::vector<std::pair<std::pair<int, size_t>, void *>> blocks;
+    size_t bytes = 0;
+    void *take(int dev, size_t sz) {
+        for (size_t i = 0; i < blocks.size(); ++i)
+            if (blocks[i].first.first == dev && blocks[i].first.second == sz) {
+                void *p = blocks[i].second;
+                blocks.erase(blocks.begin() + i);
+                bytes -= sz;
+                return p;
+            }
+        return nullptr;
+    }
+    void give(int dev, size_t sz, void *p) {
+        if (bytes + sz > (size_t(8) << 30)) {
             hipError_t e = hipFree(p);
             (void)e;
+            return;
         }
+        blocks.push_back({{dev, sz}, p});
+        bytes += sz;
+    }
+};
+BlockCache g_blocks;
+
+struct Dev {
+    void *p = nullptr;
+    size_t sz = 0;
+    int dev = 0;
+    ~Dev() {
+        if (p) g_blocks.give(dev, sz, p);
     }
     int alloc(size_t bytes) {
-        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 8));
+        sz = (std::max<size_t>(bytes, 8) + 255) / 256 * 256;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) dev = 0;
+        p = g_blocks.take(dev, sz);
+        if (p) return TTDFT_OK;
+        e = hipMalloc(&p, sz);
         if (e != hipSuccess) {
             p = nullptr;
             set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
@@ -190,8 +225,10 @@ int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int3
     Dev S0, Y1, Z, S1, W, Y2, S2;
     // mode 0: S0 = T_(0) T_(0)^T, U0, Y1 = U0^T T_(0)  (r0 x n1 n2)
     TT_TRY(S0.alloc(sizeof(double) * n0 * n0));
-    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n0, n0,
-                          (rocblas_int)n12, &one, d_T, n0, d_T, n0, &zero, S0.d(), n0));
+    // T_(0) is the n12 x n0 row-major block with ld n0: its Gram on the
+    // engine's row-chunked MFMA kernel (a library product here would run on
+    // a handful of 300 x 300 output tiles with depth n12)
+    TT_TRY(launch_gram(ctx, d_T, n0, d_T, n0, n12, n0, TTDFT_F64, TTDFT_GEMM_F64, S0.d()));
     TT_TRY(gram_eigvecs(ctx, S0.d(), n0, r0, d_U0, ev));
     TT_TRY(Y1.alloc(sizeof(double) * r0 * n12));
     TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, r0,
@@ -202,8 +239,8 @@ int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int3
                        Y1.d(), r0, n1, (int64_t)n2, Z.d());
     TT_LAUNCH_CHECK();
     TT_TRY(S1.alloc(sizeof(double) * n1 * n1));
-    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n1, n1,
-                          r0 * n2, &one, Z.d(), n1, Z.d(), n1, &zero, S1.d(), n1));
+    TT_TRY(launch_gram(ctx, Z.d(), n1, Z.d(), n1, (int64_t)r0 * n2, n1, TTDFT_F64,
+                       TTDFT_GEMM_F64, S1.d()));
     TT_TRY(gram_eigvecs(ctx, S1.d(), n1, r1, d_U1, ev));
     TT_TRY(W.alloc(sizeof(double) * (int64_t)r1 * r0 * n2));
     TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, r1,
