@@ -430,7 +430,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU "
-                                   "sharding is not built (see DESIGN.md), run with --gpus 1"}))
+                                   "sharding is not built, run with --gpus 1"}))
         sys.exit(2)
 
     from ttdft_public_amd import engine as E

@@ -213,59 +213,70 @@ __global__ __launch_bounds__(256) void csr_to_dense_kernel(int64_t n,
 }
 
 // Dense H_loc X with the recurrence epilogue, on FP64 MFMA (16x16x4): a
-// workgroup owns 16 rows of Y and 64 states; its 4 waves split the inner
-// dimension l in four and their partial tiles are summed through LDS.  The
-// MFMA computes D[j][i] = sum_l X[l][j] H[l][i] (H symmetric): the A operand
-// is X^T (16 states x 4 l, lane j + 16 kk), the B operand H (4 l x 16 rows,
-// lane i + 16 kk: 16 consecutive entries of one row of H per 16 lanes); D
-// element v of lane L is D[L/16 + 4v][L%16].  FP32 storage is widened to FP64
-// on load.  Then
+// workgroup owns 16 rows of Y and 16 JT states; its 8 waves split the inner
+// dimension l in eight, each keeping four 4-deep steps of loads in flight,
+// and their partial tiles are summed through LDS.  The MFMA computes
+// D[j][i] = sum_l X[l][j] H[l][i] (H symmetric): the A operand is X^T (16
+// states x 4 l, lane j + 16 kk), the B operand H (4 l x 16 rows, lane
+// i + 16 kk: 16 consecutive entries of one row of H per 16 lanes); D element
+// v of lane L is D[L/16 + 4v][L%16].  FP32 storage is widened to FP64 on
+// load.  Then
 // Y[i][j] = alpha (HX[i][j] + sum_e ct[e] W[ct_col_e][j]) (+ beta X[i][j]) (+ delta Z[i][j]).
 typedef double dv4 __attribute__((ext_vector_type(4)));
+constexpr int kDenseWaves = 8;
 
-template <typename T>
-__global__ __launch_bounds__(256) void dense_hx_kernel(
+template <typename T, int JT>
+__global__ __launch_bounds__(512) void dense_hx_kernel(
     int64_t n, int32_t k, const T *__restrict__ H, const T *X, int64_t ldx,
     const int64_t *__restrict__ ct_rowptr, const int32_t *__restrict__ ct_col,
     const T *__restrict__ ct_val, const T *__restrict__ W, T *Y, int64_t ldy, const T *Z,
     int64_t ldz, double alpha, double beta, double delta, int use_beta, int use_delta) {
-    __shared__ double red[4][64][17];
+    __shared__ double red[kDenseWaves][16 * JT][17];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int64_t i0 = (int64_t)blockIdx.x * 16;
-    const int c0 = blockIdx.y * 64;
-    const int64_t quarter = ((n + 3) / 4 + 3) / 4 * 4;
-    const int64_t lb = wave * quarter, le = lb + quarter < n ? lb + quarter : n;
+    const int c0 = blockIdx.y * 16 * JT;
+    const int64_t part = ((n + kDenseWaves - 1) / kDenseWaves + 15) / 16 * 16;
+    const int64_t lb = wave * part, le = lb + part < n ? lb + part : n;
     const int64_t row = i0 + li;
     const bool row_ok = row < n;
-    bool col_ok[4];
+    bool col_ok[JT];
 #pragma unroll
-    for (int jt = 0; jt < 4; ++jt) col_ok[jt] = c0 + 16 * jt + li < k;
-    dv4 acc[4];
+    for (int jt = 0; jt < JT; ++jt) col_ok[jt] = c0 + 16 * jt + li < k;
+    dv4 acc[JT];
 #pragma unroll
-    for (int jt = 0; jt < 4; ++jt) acc[jt] = dv4{0.0, 0.0, 0.0, 0.0};
-    for (int64_t l0 = lb; l0 < le; l0 += 4) {
-        const int64_t l = l0 + lk;
-        const bool l_ok = l < le;
-        const double h = (l_ok && row_ok) ? (double)H[l * n + row] : 0.0;
-        const T *xr = X + l * ldx + c0 + li;
+    for (int jt = 0; jt < JT; ++jt) acc[jt] = dv4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t l0 = lb; l0 < le; l0 += 16) {
+        double h[4], x[4][JT];
 #pragma unroll
-        for (int jt = 0; jt < 4; ++jt) {
-            const double x = (l_ok && col_ok[jt]) ? (double)xr[16 * jt] : 0.0;
-            acc[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, h, acc[jt], 0, 0, 0);
+        for (int u = 0; u < 4; ++u) {
+            const int64_t l = l0 + 4 * u + lk;
+            const bool l_ok = l < le;
+            h[u] = (l_ok && row_ok) ? (double)H[l * n + row] : 0.0;
+            const T *xr = X + l * ldx + c0 + li;
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+                x[u][jt] = (l_ok && col_ok[jt]) ? (double)xr[16 * jt] : 0.0;
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+                acc[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[u][jt], h[u], acc[jt], 0, 0, 0);
     }
 #pragma unroll
-    for (int jt = 0; jt < 4; ++jt)
+    for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
         for (int v = 0; v < 4; ++v) red[wave][16 * jt + lk + 4 * v][li] = acc[jt][v];
     __syncthreads();
-    for (int e = t; e < 16 * 64; e += 256) {
-        const int j = e & 63, ii = e >> 6;
+    for (int e = t; e < 16 * 16 * JT; e += 512) {
+        const int j = e % (16 * JT), ii = e / (16 * JT);
         const int64_t r = i0 + ii;
         const int c = c0 + j;
         if (r >= n || c >= k) continue;
-        double s = (red[0][j][ii] + red[1][j][ii]) + (red[2][j][ii] + red[3][j][ii]);
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kDenseWaves; ++w) s += red[w][j][ii];
         if (ct_rowptr)
             for (int64_t f = ct_rowptr[r]; f < ct_rowptr[r + 1]; ++f)
                 s += (double)ct_val[f] * (double)W[(int64_t)ct_col[f] * k + c];
@@ -436,24 +447,44 @@ int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
     return TTDFT_OK;
 }
 
-// H X on the dense storage with the recurrence epilogue in one kernel
+template <typename T, int JT>
+static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const T *ctv,
+                         const void *X, int64_t ldx, const void *W, void *Y, int64_t ldy,
+                         const void *Z, int64_t ldz, int32_t k, double alpha, double beta,
+                         double delta, int use_beta, int use_delta) {
+    const int64_t n = H->n;
+    const dim3 grid((unsigned)((n + 15) / 16), (unsigned)((k + 16 * JT - 1) / (16 * JT)));
+    hipLaunchKernelGGL((dense_hx_kernel<T, JT>), grid, dim3(64 * kDenseWaves), 0, ctx->stream, n,
+                       k, Hd, (const T *)X, ldx, H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col,
+                       ctv, (const T *)W, (T *)Y, ldy, (const T *)Z, ldz, alpha, beta, delta,
+                       use_beta, use_delta);
+}
+
+// H X on the dense storage with the recurrence epilogue in one kernel; the
+// state tiles per workgroup follow k (16, 32, 48 or 64 states)
 static int launch_dense_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                              const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
                              int32_t k, int dtype, double alpha, double beta, double delta,
                              int use_beta, int use_delta) {
-    const int64_t n = H->n;
-    const dim3 grid((unsigned)((n + 15) / 16), (unsigned)((k + 63) / 64));
-    const bool nl = H->p > 0;
-    if (dtype == TTDFT_F64)
-        hipLaunchKernelGGL(dense_hx_kernel<double>, grid, dim3(256), 0, ctx->stream, n, k,
-                           H->dense64, (const double *)X, ldx, nl ? H->ct_rowptr : nullptr,
-                           H->ct_col, H->ct_val64, (const double *)W, (double *)Y, ldy,
-                           (const double *)Z, ldz, alpha, beta, delta, use_beta, use_delta);
-    else
-        hipLaunchKernelGGL(dense_hx_kernel<float>, grid, dim3(256), 0, ctx->stream, n, k,
-                           H->dense32, (const float *)X, ldx, nl ? H->ct_rowptr : nullptr,
-                           H->ct_col, H->ct_val32, (const float *)W, (float *)Y, ldy,
-                           (const float *)Z, ldz, alpha, beta, delta, use_beta, use_delta);
+    const int jt = k >= 64 ? 4 : (k + 15) / 16;
+#define DENSE_CASE(JTV)                                                                         \
+    case JTV:                                                                                   \
+        if (dtype == TTDFT_F64)                                                                 \
+            dense_launch<double, JTV>(ctx, H, H->dense64, H->ct_val64, X, ldx, W, Y, ldy, Z,    \
+                                      ldz, k, alpha, beta, delta, use_beta, use_delta);         \
+        else                                                                                    \
+            dense_launch<float, JTV>(ctx, H, H->dense32, H->ct_val32, X, ldx, W, Y, ldy, Z,     \
+                                     ldz, k, alpha, beta, delta, use_beta, use_delta);          \
+        break;
+    switch (jt) {
+        DENSE_CASE(1)
+        DENSE_CASE(2)
+        DENSE_CASE(3)
+        DENSE_CASE(4)
+    default:
+        break;
+    }
+#undef DENSE_CASE
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
