@@ -423,6 +423,18 @@ int ttdft_mixer_history_size(ttdft_mixer m, int32_t *size);
 int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int32_t n2,
                   int32_t r0, int32_t r1, int32_t r2, double *d_core, double *d_U0, double *d_U1,
                   double *d_U2, double *evals);
+/* The same decomposition from the factors of a nearby tensor (d_U0..2 in/out,
+ * orthonormal columns on input, e.g. the previous SCF iteration's): for each
+ * mode, a block of min(r + 16, 64) vectors starting from the old factors is
+ * multiplied by the mode Gram matrix iters times (orthonormalised after each
+ * product) and Rayleigh-Ritz picks the leading r.  Repeated calls on a fixed
+ * tensor converge to ttdft_sthosvd's factors; a mode with r >= 64 or
+ * n min(r + 16, 64) > 18432 uses the full eigensolve instead.  No
+ * reference counterpart: it replaces the per-iteration TuckerMPI call of
+ * main.cc:882-899 inside an SCF loop. */
+int ttdft_sthosvd_refine(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int32_t n2,
+                         int32_t r0, int32_t r1, int32_t r2, double *d_core, double *d_U0,
+                         double *d_U1, double *d_U2, int32_t iters, double *evals);
 int ttdft_hartree_expansion(ttdft_ctx ctx, const int32_t r[3], const double *d_core,
                             const double *const d_U[3], const int32_t ne[3],
                             const double *const d_xe[3], const double *const d_jw[3],

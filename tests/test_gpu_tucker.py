@@ -136,3 +136,37 @@ def test_hartree_coulomb_potential_of_gaussian_grid(ctx):
     r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
     ref = np.pi ** 1.5 * erf(r) / r
     assert np.abs(V / ref - 1).max() < 2e-5
+
+
+def low_rank_plus_noise(seed, dims, ranks, noise):
+    rng = np.random.default_rng(seed)
+    G0 = rng.standard_normal(ranks) * np.logspace(0, -2, ranks[0])[:, None, None]
+    Us0 = [np.linalg.qr(rng.standard_normal((n, r)))[0] for n, r in zip(dims, ranks)]
+    return reconstruct(G0, Us0) + noise * rng.standard_normal(dims)
+
+
+def test_sthosvd_refine_converges_to_exact(ctx):
+    """From the exact factors of a nearby tensor, repeated refinement reaches
+    the exact STHOSVD of the new one.  The tensors have multilinear rank 8 x 7
+    x 6 plus 1e-6 noise, so the Gram spectra drop by orders of magnitude after
+    the kept ranks and each block step contracts the error strongly."""
+    dims, ranks = (60, 52, 44), (8, 7, 6)
+    T = low_rank_plus_noise(11, dims, ranks, 1e-6)
+    T2 = T + 1e-4 * low_rank_plus_noise(12, dims, ranks, 0.0)
+    G0, Us0, ev0 = oracle.sthosvd(T2, ranks)
+    dT = dev(ctx, flat(T))
+    dG = E.DeviceArray(ctx, (int(np.prod(ranks)),))
+    dU = [E.DeviceArray(ctx, (n * r,)) for n, r in zip(dims, ranks)]
+    ctx.sthosvd(dT.ptr, dims, ranks, dG.ptr, [u.ptr for u in dU])
+    dT2 = dev(ctx, flat(T2))
+    print("sthosvd_refine", flush=True)
+    for _ in range(6):
+        ev = ctx.sthosvd_refine(dT2.ptr, dims, ranks, dG.ptr, [u.ptr for u in dU], iters=2)
+    Us = [u.to_host().reshape(r, n).T for u, n, r in zip(dU, dims, ranks)]
+    G = unflat(dG.to_host(), ranks)
+    for u, u0 in zip(Us, Us0):
+        assert np.abs(u - u0).max() < 1e-8
+    assert np.abs(G - G0).max() < 1e-8 * np.abs(G0).max()
+    assert np.abs(ev - np.concatenate(ev0)).max() < 1e-9 * ev0[0][0]
+    with pytest.raises(E.TtdftError):
+        ctx.sthosvd_refine(dT2.ptr, dims, ranks, dG.ptr, [u.ptr for u in dU], iters=0)

@@ -157,6 +157,11 @@ int build_band_device(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int
 bool small_dense_enabled(int32_t k);
 int small_chol_inv(ttdft_ctx ctx, const double *S, int32_t k, double *QT);
 int small_syev(ttdft_ctx ctx, const double *A, int32_t k, double *evals, double *Q);
+// Orthonormal basis of an n x b column-major block in place (n b <= 18432, b <= 64)
+bool small_orth_fits(int32_t n, int32_t b);
+// f[i] = lo + (hi - lo) u_i, seeded hash (misc.hip)
+int dev_fill_uniform(ttdft_ctx ctx, double *f, int64_t n, uint64_t seed, double lo, double hi);
+int small_orth(ttdft_ctx ctx, double *Y, int32_t n, int32_t b);
 // Dense storage (format 2) of H_loc when it is nearly full and small enough
 // (no-op otherwise); drop_dense frees it.
 int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies);

@@ -206,6 +206,49 @@ __global__ __launch_bounds__(256) void small_jacobi_kernel(const double *__restr
     if (t == 0) *info = sweep >= 50 ? 1 : 0;
 }
 
+// Orthonormal basis of the columns of Y (n x b column-major, in place) by
+// classical Gram-Schmidt with one reorthogonalisation pass (CGS2), the block
+// held in LDS; a column that vanishes against the earlier ones is left 0.
+constexpr int kOrthMax = 18432;  // doubles of LDS for the block (n b)
+
+__global__ __launch_bounds__(256) void small_orth_kernel(double *__restrict__ Y, int n, int b) {
+    __shared__ double Ys[kOrthMax];
+    __shared__ double dots[kMax];
+    __shared__ double red[4];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int e = t; e < n * b; e += 256) Ys[e] = Y[e];
+    __syncthreads();
+    for (int j = 0; j < b; ++j) {
+        double *y = Ys + (int64_t)n * j;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int i = wave; i < j; i += 4) {  // dots[i] = q_i . y_j, one wave per i
+                const double *q = Ys + (int64_t)n * i;
+                double s = 0.0;
+                for (int r = lane; r < n; r += 64) s += q[r] * y[r];
+                for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+                if (lane == 0) dots[i] = s;
+            }
+            __syncthreads();
+            for (int r = t; r < n; r += 256) {
+                double v = y[r];
+                for (int i = 0; i < j; ++i) v -= dots[i] * Ys[(int64_t)n * i + r];
+                y[r] = v;
+            }
+            __syncthreads();
+        }
+        double s = 0.0;
+        for (int r = t; r < n; r += 256) s += y[r] * y[r];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) red[wave] = s;
+        __syncthreads();
+        const double nrm = sqrt((red[0] + red[1]) + (red[2] + red[3]));
+        const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        for (int r = t; r < n; r += 256) y[r] *= inv;
+        __syncthreads();
+    }
+    for (int e = t; e < n * b; e += 256) Y[e] = Ys[e];
+}
+
 }  // namespace
 
 bool small_dense_enabled(int32_t k) {
@@ -235,6 +278,21 @@ int small_syev(ttdft_ctx ctx, const double *A, int32_t k, double *evals, double 
     TT_HIP(hipMemcpyAsync(&info, ctx->d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     TT_HIP(hipStreamSynchronize(ctx->stream));
     TT_CHECK(info == 0, TTDFT_ENOCONV, "Jacobi eigensolver did not converge in 50 sweeps");
+    return TTDFT_OK;
+}
+
+}  // namespace ttdft
+
+namespace ttdft {
+
+bool small_orth_fits(int32_t n, int32_t b) {
+    return n >= 1 && b >= 1 && b <= kMax && (int64_t)n * b <= kOrthMax;
+}
+
+int small_orth(ttdft_ctx ctx, double *Y, int32_t n, int32_t b) {
+    TT_CHECK(small_orth_fits(n, b), TTDFT_EBADARG, "small_orth: %d x %d block too large", n, b);
+    hipLaunchKernelGGL(small_orth_kernel, dim3(1), dim3(256), 0, ctx->stream, Y, n, b);
+    TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
 

@@ -92,15 +92,15 @@ struct Dev {
 
 unsigned nblk(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65535)); }
 
-// U[:, j] = s_j Q[:, n - 1 - j] (leading eigenvectors of an ascending
-// dsyevd result), s_j making the largest |entry| positive (first on ties).
+// U[:, j] = s_j Q[:, ncols - 1 - j] (leading eigenvectors of an ascending
+// eigensolver result with ncols columns), s_j making the largest |entry| positive (first on ties).
 __global__ __launch_bounds__(256) void leading_vectors_kernel(const double *__restrict__ Q,
-                                                              int n, int r,
+                                                              int n, int ncols, int r,
                                                               double *__restrict__ U) {
     __shared__ double bv[256];
     __shared__ int bi[256];
     const int j = blockIdx.x;
-    const double *q = Q + (int64_t)n * (n - 1 - j);
+    const double *q = Q + (int64_t)n * (ncols - 1 - j);
     double best = -1.0;
     int bidx = n;
     for (int i = threadIdx.x; i < n; i += 256) {
@@ -186,7 +186,7 @@ int gram_eigvecs(ttdft_ctx ctx, double *S, int n, int r, double *U, std::vector<
                                                rocblas_fill_lower, n, S, n, w.d(), e.d(),
                                                ctx->d_info);
     TT_CHECK(st == rocblas_status_success, TTDFT_ELIB, "rocsolver_dsyevd: status %d", (int)st);
-    hipLaunchKernelGGL(leading_vectors_kernel, dim3(r), dim3(256), 0, ctx->stream, S, n, r, U);
+    hipLaunchKernelGGL(leading_vectors_kernel, dim3(r), dim3(256), 0, ctx->stream, S, n, n, r, U);
     TT_LAUNCH_CHECK();
     int info = 0;
     std::vector<double> wh(n);
@@ -199,6 +199,60 @@ int gram_eigvecs(ttdft_ctx ctx, double *S, int n, int r, double *U, std::vector<
     return TTDFT_OK;
 }
 
+// Leading r eigenvectors of the symmetric n x n G from the previous ones (U,
+// in place): a block of b = min(r + 16, 64) vectors (U and seeded random
+// columns) is orthonormalised, multiplied by G iters times with an
+// orthonormalisation after each product, and Rayleigh-Ritz on the block gives
+// the Ritz vectors (largest |entry| of each made positive) and values.  At a
+// fixed G repeated calls converge to the exact leading eigenvectors; the
+// oversampling makes each step contract by lambda_{b+1} / lambda_r.
+int refine_mode(ttdft_ctx ctx, const double *G, int n, int r, int iters, double *U,
+                std::vector<double> &ev) {
+    const int b = std::min(n, std::min(64, r + 16));
+    const double one = 1.0, zero = 0.0;
+    Dev Y, Z, B, V, W;
+    TT_TRY(Y.alloc(sizeof(double) * (size_t)n * b));
+    TT_TRY(Z.alloc(sizeof(double) * (size_t)n * b));
+    TT_TRY(B.alloc(sizeof(double) * (size_t)b * b));
+    TT_TRY(V.alloc(sizeof(double) * (size_t)b * b));
+    TT_TRY(W.alloc(sizeof(double) * (size_t)b));
+    TT_HIP(hipMemcpyAsync(Y.d(), U, sizeof(double) * (size_t)n * r, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    if (b > r) TT_TRY(dev_fill_uniform(ctx, Y.d() + (size_t)n * r, (int64_t)n * (b - r), 977,
+                                       -1.0, 1.0));
+    TT_TRY(small_orth(ctx, Y.d(), n, b));
+    for (int it = 0; it < iters; ++it) {
+        TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, b, n,
+                              &one, G, n, Y.d(), n, &zero, Z.d(), n));
+        TT_TRY(small_orth(ctx, Z.d(), n, b));
+        std::swap(Y.p, Z.p);
+    }
+    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, b, n,
+                          &one, G, n, Y.d(), n, &zero, Z.d(), n));
+    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, b, b,
+                          n, &one, Y.d(), n, Z.d(), n, &zero, B.d(), b));
+    TT_TRY(small_syev(ctx, B.d(), b, W.d(), V.d()));
+    TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, b, b,
+                          &one, Y.d(), n, V.d(), b, &zero, Z.d(), n));
+    hipLaunchKernelGGL(leading_vectors_kernel, dim3(r), dim3(256), 0, ctx->stream, Z.d(), n, b,
+                       r, U);
+    TT_LAUNCH_CHECK();
+    std::vector<double> wh(b);
+    TT_HIP(hipMemcpyAsync(wh.data(), W.p, sizeof(double) * b, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    for (int j = 0; j < r; ++j) ev.push_back(wh[b - 1 - j]);
+    return TTDFT_OK;
+}
+
+// the exact eigensolve, or refinement from the incoming factors when it fits
+int mode_factors(ttdft_ctx ctx, double *G, int n, int r, int iters, double *U,
+                 std::vector<double> &ev) {
+    const int b = std::min(n, std::min(64, r + 16));
+    if (iters > 0 && b > r && small_orth_fits(n, b)) return refine_mode(ctx, G, n, r, iters, U, ev);
+    return gram_eigvecs(ctx, G, n, r, U, ev);
+}
+
 }  // namespace
 }  // namespace ttdft
 
@@ -206,10 +260,9 @@ using namespace ttdft;
 
 extern "C" {
 
-int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int32_t n2,
-                  int32_t r0, int32_t r1, int32_t r2, double *d_core, double *d_U0, double *d_U1,
-                  double *d_U2, double *evals) {
-    clear_error();
+static int sthosvd_impl(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int32_t n2,
+                        int32_t r0, int32_t r1, int32_t r2, double *d_core, double *d_U0,
+                        double *d_U1, double *d_U2, double *evals, int iters) {
     TT_CHECK(ctx && d_T && d_core && d_U0 && d_U1 && d_U2, TTDFT_EBADARG,
              "sthosvd: NULL argument");
     TT_CHECK(n0 > 0 && n1 > 0 && n2 > 0 && r0 > 0 && r1 > 0 && r2 > 0 && r0 <= n0 && r1 <= n1 &&
@@ -229,7 +282,7 @@ int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int3
     // engine's row-chunked MFMA kernel (a library product here would run on
     // a handful of 300 x 300 output tiles with depth n12)
     TT_TRY(launch_gram(ctx, d_T, n0, d_T, n0, n12, n0, TTDFT_F64, TTDFT_GEMM_F64, S0.d()));
-    TT_TRY(gram_eigvecs(ctx, S0.d(), n0, r0, d_U0, ev));
+    TT_TRY(mode_factors(ctx, S0.d(), n0, r0, iters, d_U0, ev));
     TT_TRY(Y1.alloc(sizeof(double) * r0 * n12));
     TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, r0,
                           (rocblas_int)n12, n0, &one, d_U0, n0, d_T, n0, &zero, Y1.d(), r0));
@@ -241,7 +294,7 @@ int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int3
     TT_TRY(S1.alloc(sizeof(double) * n1 * n1));
     TT_TRY(launch_gram(ctx, Z.d(), n1, Z.d(), n1, (int64_t)r0 * n2, n1, TTDFT_F64,
                        TTDFT_GEMM_F64, S1.d()));
-    TT_TRY(gram_eigvecs(ctx, S1.d(), n1, r1, d_U1, ev));
+    TT_TRY(mode_factors(ctx, S1.d(), n1, r1, iters, d_U1, ev));
     TT_TRY(W.alloc(sizeof(double) * (int64_t)r1 * r0 * n2));
     TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, r1,
                           r0 * n2, n1, &one, d_U1, n1, Z.d(), n1, &zero, W.d(), r1));
@@ -254,7 +307,7 @@ int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int3
     TT_TRY(S2.alloc(sizeof(double) * n2 * n2));
     TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n2, n2,
                           r0 * r1, &one, Y2.d(), r0 * r1, Y2.d(), r0 * r1, &zero, S2.d(), n2));
-    TT_TRY(gram_eigvecs(ctx, S2.d(), n2, r2, d_U2, ev));
+    TT_TRY(mode_factors(ctx, S2.d(), n2, r2, iters, d_U2, ev));
     TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, r0 * r1, r2,
                           n2, &one, Y2.d(), r0 * r1, d_U2, n2, &zero, d_core, r0 * r1));
     const double flops = 2.0 * n0 * n0 * n12 + 2.0 * r0 * n0 * n12 + 2.0 * n1 * n1 * r0 * n2 +
@@ -264,6 +317,22 @@ int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int3
     TT_HIP(hipStreamSynchronize(ctx->stream));
     if (evals) std::copy(ev.begin(), ev.end(), evals);
     return TTDFT_OK;
+}
+
+int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int32_t n2,
+                  int32_t r0, int32_t r1, int32_t r2, double *d_core, double *d_U0, double *d_U1,
+                  double *d_U2, double *evals) {
+    clear_error();
+    return sthosvd_impl(ctx, d_T, n0, n1, n2, r0, r1, r2, d_core, d_U0, d_U1, d_U2, evals, 0);
+}
+
+int ttdft_sthosvd_refine(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int32_t n2,
+                         int32_t r0, int32_t r1, int32_t r2, double *d_core, double *d_U0,
+                         double *d_U1, double *d_U2, int32_t iters, double *evals) {
+    clear_error();
+    TT_CHECK(iters >= 1, TTDFT_EBADARG, "sthosvd_refine: iters must be >= 1");
+    return sthosvd_impl(ctx, d_T, n0, n1, n2, r0, r1, r2, d_core, d_U0, d_U1, d_U2, evals,
+                        iters);
 }
 
 int ttdft_hartree_expansion(ttdft_ctx ctx, const int32_t r[3], const double *d_core,

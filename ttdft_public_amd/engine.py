@@ -144,6 +144,8 @@ _SIGS = {
     "ttdft_mixer_history_size": (ctypes.c_int, [_vp, _c_i32p]),
     "ttdft_sthosvd": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp,
                                      _vp, _vp, _c_dp]),
+    "ttdft_sthosvd_refine": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp,
+                                            _vp, _vp, _vp, _i32, _c_dp]),
     "ttdft_hartree_expansion": (ctypes.c_int, [_vp, _c_i32p, _vp, ctypes.POINTER(_vp), _c_i32p,
                                                ctypes.POINTER(_vp), ctypes.POINTER(_vp), _c_i32p,
                                                ctypes.POINTER(_vp), _i32, _c_dp, _c_dp, _vp,
@@ -491,6 +493,13 @@ class Context:
         ev = np.zeros(sum(ranks))
         _check(lib().ttdft_sthosvd(self.h, d_T, *dims, *ranks, d_core, d_U[0], d_U[1], d_U[2],
                                    _dptr(ev)))
+        return ev
+
+    def sthosvd_refine(self, d_T, dims, ranks, d_core, d_U, iters=2):
+        """STHOSVD from the factors already in d_U (refined in place)."""
+        ev = np.zeros(sum(ranks))
+        _check(lib().ttdft_sthosvd_refine(self.h, d_T, *dims, *ranks, d_core, d_U[0], d_U[1],
+                                          d_U[2], iters, _dptr(ev)))
         return ev
 
     def hartree_expansion(self, ranks, d_core, d_U, d_xe, d_jw, ne, d_xq, p, alpha, omega, d_V,

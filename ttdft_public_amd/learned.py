@@ -299,9 +299,12 @@ def run_learned(net, scf, callback=None):
     residual of the superposition start (one SCF iteration, counted in the
     time) is the bar the learned start's first residual must meet."""
     t0 = time.perf_counter()
-    cfg1 = S.ScfConfig(**{**scf.cfg.__dict__, "max_iter": 1})
-    probe = S.Scf(scf.ctx, scf.xyz, float(scf.zval[0]), cfg1)
-    r0 = probe.run().residuals[0]
+    saved = scf.cfg.max_iter
+    scf.cfg.max_iter = 1  # the probe: one iteration from the superposition
+    try:
+        r0 = scf.run().residuals[0]
+    finally:
+        scf.cfg.max_iter = saved
     d_start = E.DeviceArray(scf.ctx, (scf.ngrid,))
     learned_start(net, scf, d_start)
     r = scf.run(d_start, callback, fallback_residual=r0)

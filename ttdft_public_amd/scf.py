@@ -59,6 +59,7 @@ class ScfConfig:
     kb_energy: float = 1.0
     well_depth: float = 1.0    # 1-D model wells for the basis (Ha)
     truncation: float = 1e-10
+    tucker_refine: int = 2     # STHOSVD from the previous iteration's factors (0: exact each time)
     restart_first: int = 10
     restart_other: int = 12
     lda_variant: int = E.LDA_PZ81
@@ -276,21 +277,27 @@ class Scf:
         return self.ctx.grid_dot(d_f.ptr, d_g.ptr, self.p, *w)
 
     # ---- potential -----------------------------------------------------------------
-    def tucker(self, d_field, rank):
-        """STHOSVD of a grid field into buffers kept per rank (reused every
-        iteration)."""
+    def tucker(self, d_field, rank, key, refine=False):
+        """STHOSVD of a grid field into buffers kept per field (reused every
+        iteration); with refine, from the field's previous factors
+        (ttdft_sthosvd_refine, cfg.tucker_refine iterations)."""
         if not hasattr(self, "_tucker_bufs"):
             self._tucker_bufs = {}
-        if rank not in self._tucker_bufs:
-            self._tucker_bufs[rank] = (E.DeviceArray(self.ctx, (rank ** 3,)),
-                                       [E.DeviceArray(self.ctx, (n * rank,)) for n in self.p])
-        d_core, d_U = self._tucker_bufs[rank]
-        self.ctx.sthosvd(d_field.ptr, self.p, (rank,) * 3, d_core.ptr, [u.ptr for u in d_U])
+        if key not in self._tucker_bufs:
+            self._tucker_bufs[key] = (E.DeviceArray(self.ctx, (rank ** 3,)),
+                                      [E.DeviceArray(self.ctx, (n * rank,)) for n in self.p])
+            refine = False
+        d_core, d_U = self._tucker_bufs[key]
+        if refine and self.cfg.tucker_refine > 0:
+            self.ctx.sthosvd_refine(d_field.ptr, self.p, (rank,) * 3, d_core.ptr,
+                                    [u.ptr for u in d_U], self.cfg.tucker_refine)
+        else:
+            self.ctx.sthosvd(d_field.ptr, self.p, (rank,) * 3, d_core.ptr, [u.ptr for u in d_U])
         return d_core, d_U
 
-    def effective_potential(self, d_rho, d_veff, d_vh, d_vxc, d_exc):
+    def effective_potential(self, d_rho, d_veff, d_vh, d_vxc, d_exc, refine=False):
         ctx, cfg = self.ctx, self.cfg
-        core, U = self.tucker(d_rho, cfg.rho_rank)
+        core, U = self.tucker(d_rho, cfg.rho_rank, "rho", refine)
         ax = self.axes
         ctx.hartree_expansion((cfg.rho_rank,) * 3, core.ptr, [u.ptr for u in U],
                               [a.d_xq.ptr for a in ax], [a.d_jw.ptr for a in ax], self.p,
@@ -300,10 +307,10 @@ class Scf:
         for d_src in (d_vh, d_vxc):
             ctx.grid_axpby(d_veff.ptr, 1.0, d_src.ptr, 1.0, self.ngrid)
 
-    def hamiltonian(self, d_veff, copies=E.HAM_F64):
+    def hamiltonian(self, d_veff, copies=E.HAM_F64, refine=False):
         cfg = self.cfg
         rv = cfg.veff_rank
-        core, U = self.tucker(d_veff, rv)
+        core, U = self.tucker(d_veff, rv, "veff", refine)
         PX, PY, PZ = (a.mat_pot(u, rv) for a, u in zip(self.axes, U))
         G = core.to_host().reshape(rv, rv, rv).transpose(2, 1, 0)  # [a, b, c]
         a = self.axes
@@ -346,7 +353,7 @@ class Scf:
         for it in range(cfg.max_iter):
             t = time.perf_counter()
             mixer.compute_rho_in(it, None, d_in.ptr)
-            self.effective_potential(d_in, d_veff, d_vh, d_vxc, d_exc)
+            self.effective_potential(d_in, d_veff, d_vh, d_vxc, d_exc, refine=it > 0)
             ctx.synchronize()
             phases["potential"] += time.perf_counter() - t
             t = time.perf_counter()
@@ -356,7 +363,8 @@ class Scf:
                                                      N, k))
                 dt = E.F64
             # the Lanczos bounds always run on the FP64 copy
-            ham = self.hamiltonian(d_veff, E.HAM_F64 if dt == E.F64 else E.HAM_F64 | E.HAM_F32)
+            ham = self.hamiltonian(d_veff, E.HAM_F64 if dt == E.F64 else E.HAM_F64 | E.HAM_F32,
+                                   refine=it > 0)
             phases["hamiltonian"] += time.perf_counter() - t
             t = time.perf_counter()
             dXc = dX if dt == E.F64 else dX32
