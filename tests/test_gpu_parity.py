@@ -448,3 +448,32 @@ def test_dense_storage_matches_oracle(ctx, tucker):
     dX = dev(ctx, X)
     ctx.cheb_filter(ham, dX.ptr, 24, 24, 8, -0.5, 30.0, -3.0, 1, E.F64)
     assert rel(dX.to_host(), oracle.cheb_filter(H, X, 8, -0.5, 30.0, -3.0, 1)) < 1e-12
+    X32 = X.astype(np.float32)
+    dX = dev(ctx, X32, np.float32)
+    ctx.cheb_filter(ham, dX.ptr, 24, 24, 8, -0.5, 30.0, -3.0, 1, E.F32)
+    ref = oracle.cheb_filter(H, X32.astype(np.float64), 8, -0.5, 30.0, -3.0, 1)
+    assert rel(dX.to_host().astype(np.float64), ref) < 5e-5
+
+
+@pytest.mark.parametrize("dims", [(4, 3, 5), (9, 8, 7), (11, 10, 13)])
+@pytest.mark.parametrize("k", [1, 8, 16, 24, 33, 64, 100])
+def test_dense_product_tilings(ctx, dims, k):
+    """The dense product over its state tilings (16 to 64 states per
+    workgroup, partial tiles), leading dimension of the block above k, and n
+    not a multiple of 32 (n = 60: six of the eight waves get no l range;
+    n = 1430: the last wave's range is shorter)."""
+    rx, ry, rz = dims
+    s = E.SynthHam(rx, ry, rz, rx - 1, ry - 1, rz - 1, 3, seed=k)
+    H = oracle.Ham.from_synth(s)
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=False)
+    assert ham.format()[0] == 2
+    ld = k + 3
+    print(f"dense product n={s.n} k={k}", flush=True)
+    for dtype, npdt, tol in ((E.F64, np.float64, 1e-13), (E.F32, np.float32, 1e-5)):
+        X = np.random.default_rng(k).standard_normal((s.n, ld)).astype(npdt)
+        dX, dY = dev(ctx, X, npdt), dev(ctx, np.full((s.n, ld), 7.0), npdt)
+        ctx.ham_apply(ham, dX.ptr, ld, dY.ptr, ld, k, dtype)
+        Y = dY.to_host().reshape(s.n, ld).astype(np.float64)
+        ref = oracle.ham_apply(H, X[:, :k].astype(np.float64))
+        assert rel(Y[:, :k], ref) < tol
+        assert np.all(Y[:, k:] == 7.0)  # columns past k untouched

@@ -124,10 +124,12 @@ struct ttdft_ham_s {
     int32_t gxb = 0, nxc = 0;   // x-bricks per workgroup, x-chunks
     double *mf64 = nullptr;
     float *mf32 = nullptr;
-    // dense storage of H_loc (format 2, n x n, row i contiguous) for small,
-    // nearly full matrices
+    // dense storage of H_loc (format 2) for small, nearly full matrices:
+    // row-major, leading dimension dense_ld (n rounded up to 32), rows padded
+    // to a multiple of 16, padding zero
     double *dense64 = nullptr;
     float *dense32 = nullptr;
+    int64_t dense_ld = 0;
 };
 
 namespace ttdft {

@@ -198,9 +198,10 @@ int nl_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *X, int64_t ldx, int3
     return TTDFT_OK;
 }
 
-// D[row][col_e] = val_e (D zeroed beforehand); one wavefront per row
+// D[row][col_e] = val_e (D zeroed beforehand, leading dimension ld); one
+// wavefront per row
 template <typename T>
-__global__ __launch_bounds__(256) void csr_to_dense_kernel(int64_t n,
+__global__ __launch_bounds__(256) void csr_to_dense_kernel(int64_t n, int64_t ld,
                                                            const int64_t *__restrict__ rowptr,
                                                            const int32_t *__restrict__ col,
                                                            const T *__restrict__ val,
@@ -209,65 +210,137 @@ __global__ __launch_bounds__(256) void csr_to_dense_kernel(int64_t n,
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
     for (int64_t e = rowptr[row] + lane; e < rowptr[row + 1]; e += 64)
-        D[row * n + col[e]] = val[e];
+        D[row * ld + col[e]] = val[e];
 }
 
-// Dense H_loc X with the recurrence epilogue, on FP64 MFMA (16x16x4): a
-// workgroup owns 16 rows of Y and 16 JT states; its 8 waves split the inner
-// dimension l in eight, each keeping four 4-deep steps of loads in flight,
-// and their partial tiles are summed through LDS.  The MFMA computes
-// D[j][i] = sum_l X[l][j] H[l][i] (H symmetric): the A operand is X^T (16
-// states x 4 l, lane j + 16 kk), the B operand H (4 l x 16 rows, lane
-// i + 16 kk: 16 consecutive entries of one row of H per 16 lanes); D element
-// v of lane L is D[L/16 + 4v][L%16].  FP32 storage is widened to FP64 on
-// load.  Then
+// Dense H_loc X with the recurrence epilogue, on matrix cores: a workgroup
+// owns 16 rows of Y and 16 JT states; its 8 waves split the inner dimension l
+// in eight and their partial tiles are summed through LDS.
+//
+// Storage: row-major, leading dimension ldh = n rounded up to 32, rows padded
+// to a multiple of 16, padding zero.  Per 32-wide step of l, lane (i, q)
+// (i = lane % 16, q = lane / 16) loads the 8 consecutive entries
+// H[i0 + i][l0 + 8q .. 8q + 7] with wide loads, so each row of the tile is one
+// contiguous 32-entry run, and the 8 matching rows of X; MFMA step u
+// (u = 0..7) then pairs l = l0 + 8q + u in both operands.  The next step's
+// loads are issued before this step's products (two register buffers).  X
+// rows past n are read clamped to row n - 1 (finite values times the zero
+// padding) and states past k clamped to k - 1 (never stored), so no load is
+// predicated.
+//
+// The product is D[j][i] = sum_l X[l][j] H[i][l]: A operand X^T (16 states x
+// 4 l, lane j + 16 kk), B operand H^T (4 l x 16 rows, lane i + 16 kk).  FP64
+// storage runs on 16x16x4 FP64 MFMA (element v of lane L is D[L/16 + 4v][L%16])
+// and FP32 storage on 16x16x4 FP32 MFMA with FP32 accumulation (element v of
+// lane L is D[4 (L/16) + v][L%16]).  Then
 // Y[i][j] = alpha (HX[i][j] + sum_e ct[e] W[ct_col_e][j]) (+ beta X[i][j]) (+ delta Z[i][j]).
 typedef double dv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
 constexpr int kDenseWaves = 8;
+
+template <typename T> struct DenseMma;
+template <> struct DenseMma<double> {
+    typedef dv4 acc_t;
+    static __device__ __forceinline__ dv4 step(double a, double b, dv4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int state(int q, int v) { return q + 4 * v; }
+};
+template <> struct DenseMma<float> {
+    typedef fv4 acc_t;
+    static __device__ __forceinline__ fv4 step(float a, float b, fv4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int state(int q, int v) { return 4 * q + v; }
+};
+
+template <typename T, int JT> struct DenseStage {
+    T h[8];
+    T x[8][JT];
+};
+
+template <typename T, int JT>
+__device__ __forceinline__ void dense_load(DenseStage<T, JT> &st, const T *hrow, const T *X,
+                                           int64_t ldx, int64_t n, const int *cidx, int64_t l0,
+                                           int q) {
+    // 8 consecutive entries: 32 B (FP32) or 64 B (FP64), aligned by the padding
+    typedef T v8 __attribute__((ext_vector_type(8)));
+    const v8 hv = *reinterpret_cast<const v8 *>(hrow + l0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) st.h[u] = hv[u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        int64_t l = l0 + 8 * q + u;
+        l = l < n ? l : n - 1;
+        const T *xr = X + l * ldx;
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) st.x[u][jt] = xr[cidx[jt]];
+    }
+}
+
+template <typename T, int JT>
+__device__ __forceinline__ void dense_mma(const DenseStage<T, JT> &st,
+                                          typename DenseMma<T>::acc_t (&acc)[2][JT]) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt)
+            acc[u & 1][jt] = DenseMma<T>::step(st.x[u][jt], st.h[u], acc[u & 1][jt]);
+}
 
 template <typename T, int JT>
 __global__ __launch_bounds__(512) void dense_hx_kernel(
-    int64_t n, int32_t k, const T *__restrict__ H, const T *X, int64_t ldx,
+    int64_t n, int64_t ldh, int32_t k, const T *__restrict__ H, const T *X, int64_t ldx,
     const int64_t *__restrict__ ct_rowptr, const int32_t *__restrict__ ct_col,
     const T *__restrict__ ct_val, const T *__restrict__ W, T *Y, int64_t ldy, const T *Z,
     int64_t ldz, double alpha, double beta, double delta, int use_beta, int use_delta) {
-    __shared__ double red[kDenseWaves][16 * JT][17];
+    typedef typename DenseMma<T>::acc_t acc_t;
+    __shared__ T red[kDenseWaves][16 * JT][17];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int li = lane & 15, lk = lane >> 4;
+    const int li = lane & 15, q = lane >> 4;
     const int64_t i0 = (int64_t)blockIdx.x * 16;
     const int c0 = blockIdx.y * 16 * JT;
-    const int64_t part = ((n + kDenseWaves - 1) / kDenseWaves + 15) / 16 * 16;
-    const int64_t lb = wave * part, le = lb + part < n ? lb + part : n;
-    const int64_t row = i0 + li;
-    const bool row_ok = row < n;
-    bool col_ok[JT];
+    const int64_t part = ((ldh + kDenseWaves - 1) / kDenseWaves + 31) / 32 * 32;
+    const int64_t lb = wave * part, le = lb + part < ldh ? lb + part : ldh;
+    const T *hrow = H + (i0 + li) * ldh + 8 * q;
+    int cidx[JT];
 #pragma unroll
-    for (int jt = 0; jt < JT; ++jt) col_ok[jt] = c0 + 16 * jt + li < k;
-    dv4 acc[JT];
+    for (int jt = 0; jt < JT; ++jt) {
+        const int c = c0 + 16 * jt + li;
+        cidx[jt] = c < k ? c : k - 1;
+    }
+    acc_t acc[2][JT];
 #pragma unroll
-    for (int jt = 0; jt < JT; ++jt) acc[jt] = dv4{0.0, 0.0, 0.0, 0.0};
-    for (int64_t l0 = lb; l0 < le; l0 += 16) {
-        double h[4], x[4][JT];
+    for (int e = 0; e < 2; ++e)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int64_t l = l0 + 4 * u + lk;
-            const bool l_ok = l < le;
-            h[u] = (l_ok && row_ok) ? (double)H[l * n + row] : 0.0;
-            const T *xr = X + l * ldx + c0 + li;
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-                x[u][jt] = (l_ok && col_ok[jt]) ? (double)xr[16 * jt] : 0.0;
+        for (int jt = 0; jt < JT; ++jt) acc[e][jt] = acc_t{0, 0, 0, 0};
+    if (lb < le) {
+        // two buffers, the loads of step s + 1 in flight during the products
+        // of step s (the last pair reloads the final step into the idle
+        // buffer; scheduling barriers keep the compiler from sinking the loads
+        // into the products)
+        DenseStage<T, JT> sa, sb;
+        const int64_t nsteps = (le - lb) / 32;
+        dense_load<T, JT>(sa, hrow, X, ldx, n, cidx, lb, q);
+        int64_t st = 0;
+        for (; st + 2 <= nsteps; st += 2) {
+            dense_load<T, JT>(sb, hrow, X, ldx, n, cidx, lb + 32 * (st + 1), q);
+            __builtin_amdgcn_sched_barrier(0);
+            dense_mma<T, JT>(sa, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            const int64_t s2 = st + 2 < nsteps ? st + 2 : nsteps - 1;
+            dense_load<T, JT>(sa, hrow, X, ldx, n, cidx, lb + 32 * s2, q);
+            __builtin_amdgcn_sched_barrier(0);
+            dense_mma<T, JT>(sb, acc);
+            __builtin_amdgcn_sched_barrier(0);
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int jt = 0; jt < JT; ++jt)
-                acc[jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[u][jt], h[u], acc[jt], 0, 0, 0);
+        if (st < nsteps) dense_mma<T, JT>(sa, acc);
     }
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) red[wave][16 * jt + lk + 4 * v][li] = acc[jt][v];
+        for (int v = 0; v < 4; ++v)
+            red[wave][16 * jt + DenseMma<T>::state(q, v)][li] = acc[0][jt][v] + acc[1][jt][v];
     __syncthreads();
     for (int e = t; e < 16 * 16 * JT; e += 512) {
         const int j = e % (16 * JT), ii = e / (16 * JT);
@@ -276,7 +349,7 @@ __global__ __launch_bounds__(512) void dense_hx_kernel(
         if (r >= n || c >= k) continue;
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < kDenseWaves; ++w) s += red[w][j][ii];
+        for (int w = 0; w < kDenseWaves; ++w) s += (double)red[w][j][ii];
         if (ct_rowptr)
             for (int64_t f = ct_rowptr[r]; f < ct_rowptr[r + 1]; ++f)
                 s += (double)ct_val[f] * (double)W[(int64_t)ct_col[f] * k + c];
@@ -415,13 +488,15 @@ void drop_dense(ttdft_ham_s *H) {
 int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
     const int64_t n = H->n;
     if (H->format != 0 || n > 32768 || n < 1 || (double)H->nnz < 0.2 * (double)n * n) return TTDFT_OK;
-    const size_t n2 = (size_t)n * n;
+    // rows padded to 16 and the leading dimension to 32 (the product's tiles)
+    const int64_t ld = (n + 31) / 32 * 32;
+    const size_t n2 = (size_t)((n + 15) / 16 * 16) * (size_t)ld;
     const unsigned rows = (unsigned)((n + 3) / 4);
     if (H->val64) {
         TT_HIP(hipMalloc(&H->dense64, sizeof(double) * n2));
         TT_HIP(hipMemsetAsync(H->dense64, 0, sizeof(double) * n2, ctx->stream));
         hipLaunchKernelGGL(csr_to_dense_kernel<double>, dim3(rows), dim3(256), 0, ctx->stream, n,
-                           H->rowptr, H->col, H->val64, H->dense64);
+                           ld, H->rowptr, H->col, H->val64, H->dense64);
         TT_LAUNCH_CHECK();
         if (copies & TTDFT_HAM_F32) {
             TT_HIP(hipMalloc(&H->dense32, sizeof(float) * n2));
@@ -438,11 +513,12 @@ int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
         TT_HIP(hipMalloc(&H->dense32, sizeof(float) * n2));
         TT_HIP(hipMemsetAsync(H->dense32, 0, sizeof(float) * n2, ctx->stream));
         hipLaunchKernelGGL(csr_to_dense_kernel<float>, dim3(rows), dim3(256), 0, ctx->stream, n,
-                           H->rowptr, H->col, H->val32, H->dense32);
+                           ld, H->rowptr, H->col, H->val32, H->dense32);
         TT_LAUNCH_CHECK();
     } else {
         return TTDFT_OK;
     }
+    H->dense_ld = ld;
     H->format = 2;
     return TTDFT_OK;
 }
@@ -455,9 +531,9 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
     const int64_t n = H->n;
     const dim3 grid((unsigned)((n + 15) / 16), (unsigned)((k + 16 * JT - 1) / (16 * JT)));
     hipLaunchKernelGGL((dense_hx_kernel<T, JT>), grid, dim3(64 * kDenseWaves), 0, ctx->stream, n,
-                       k, Hd, (const T *)X, ldx, H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col,
-                       ctv, (const T *)W, (T *)Y, ldy, (const T *)Z, ldz, alpha, beta, delta,
-                       use_beta, use_delta);
+                       H->dense_ld, k, Hd, (const T *)X, ldx,
+                       H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col, ctv, (const T *)W, (T *)Y,
+                       ldy, (const T *)Z, ldz, alpha, beta, delta, use_beta, use_delta);
 }
 
 // H X on the dense storage with the recurrence epilogue in one kernel; the
