@@ -360,6 +360,10 @@ def bench_scf(ctx, E):
                        "30 quartic elements per axis graded toward the cluster, 300^3 quadrature "
                        "grid, Tucker basis 20^3, 24 states, Chebyshev degree 20, 500 K, Anderson "
                        "0.5 / 10 (seeded geometry, synthetic pseudopotential)"}
+    # untimed warm-up of both precisions (code-object loading, library
+    # handles, the block cache), so the first timed form pays no start-up cost
+    for mixed in (False, True):
+        S.Scf(ctx, xyz, 3.0, S.ScfConfig(max_iter=3, mixed=mixed)).run()
     r64, st64 = run(False)
     e_ref = r64.energy
     forms = [("fp64", False, False), ("mixed", True, False)]
