@@ -383,6 +383,7 @@ def bench_scf(ctx, E):
                      "phase_seconds": {k: round(v, 3) for k, v in r.phase_seconds.items()}}
         if learned:
             out[name]["fell_back"] = r.fell_back
+            out[name]["probed"] = r.probed
         log(f"scf {name}: {out[name]}")
     if out["fp64"]["seconds_to_1e-5_Ha_per_atom"] and isinstance(out.get("mixed_learned"), dict) \
             and out["mixed_learned"]["seconds_to_1e-5_Ha_per_atom"]:

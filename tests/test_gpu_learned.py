@@ -51,3 +51,20 @@ def test_fallback_to_superposition(ctx, net):
     assert r.residuals[0] > 1e-3
     assert abs(r.iterations - (ref.iterations + 1)) <= 1
     assert abs(r.energy - ref.energy) < 1e-5 * len(xyz)
+
+
+def test_probe_only_when_in_doubt(ctx, net):
+    """The superposition probe runs only when the learned start's first
+    residual is not below probe_below; forced (probe_below None) it costs one
+    counted iteration and leads to the same converged energy."""
+    xyz = S.cuboctahedron(seed=6)
+    cfg = Ln.data_config(3 * len(xyz))
+    bar = Ln.default_probe_below()
+    assert bar is not None and bar > 0
+    r = Ln.run_learned(net, S.Scf(ctx, xyz, 3.0, cfg))
+    assert r.residuals[0] < bar and not r.probed and not r.fell_back
+    rp = Ln.run_learned(net, S.Scf(ctx, xyz, 3.0, cfg), probe_below=None)
+    assert rp.probed and not rp.fell_back
+    assert rp.iterations == r.iterations + 1
+    assert rp.residuals[0] == pytest.approx(r.residuals[0], rel=1e-8)
+    assert abs(rp.energy - r.energy) < 1e-7 * len(xyz)

@@ -10,11 +10,12 @@ the SCF loop (scf.py) converges on seeded clusters.
 
 The start handed to the SCF is the superposition on the quadrature grid plus
 the predicted correction (trilinear from the uniform grid), clamped at zero
-and renormalised to the electron count.  The plain superposition stays as the
-fallback: if the first SCF residual from the learned start exceeds the
-superposition's, the loop restarts from the superposition
-(InitialGuess::initialize_rho, dft/InitialGuess.cc:683-762, is the start
-this corrects).
+and renormalised to the electron count.  The plain superposition
+(InitialGuess::initialize_rho, dft/InitialGuess.cc:683-762, is the start this
+corrects) stays as the fallback: if the first SCF residual from the learned
+start exceeds the superposition's, the loop restarts from the superposition.
+The superposition's first residual is probed only when the learned start's
+is not clearly below the smallest seen on the evaluation clusters.
 """
 import ctypes
 import math
@@ -294,22 +295,53 @@ def learned_start(net, scf, d_rho):
     scf.renormalise(d_rho)
 
 
-def run_learned(net, scf, callback=None):
-    """SCF from the learned start with the superposition fallback: the first
-    residual of the superposition start (one SCF iteration, counted in the
-    time) is the bar the learned start's first residual must meet."""
-    t0 = time.perf_counter()
-    saved = scf.cfg.max_iter
-    scf.cfg.max_iter = 1  # the probe: one iteration from the superposition
+def default_probe_below(path=None):
+    """A quarter of the smallest superposition first residual over the
+    evaluation clusters in the shipped network's report, or None without the
+    report."""
+    import json
+    path = path or os.path.join(os.path.dirname(DEFAULT_WEIGHTS),
+                                "unet_initial_density_report.json")
     try:
-        r0 = scf.run().residuals[0]
-    finally:
-        scf.cfg.max_iter = saved
+        with open(path) as fh:
+            test = json.load(fh)["test"]
+        return 0.25 * min(c["superposition"]["first_residual"] for c in test)
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
+def run_learned(net, scf, callback=None, probe_below="report"):
+    """SCF from the learned start with the superposition fallback: the bar for
+    the learned start's first residual is the superposition start's first
+    residual, measured by a probe (one SCF iteration from the superposition,
+    counted in the time and the iterations).  The probe runs only when the
+    learned start's first residual is at least probe_below; below it the
+    learned start is kept without one.  probe_below defaults to a quarter of
+    the smallest superposition first residual on the evaluation clusters
+    (default_probe_below); None always probes."""
+    t0 = time.perf_counter()
+    if probe_below == "report":
+        probe_below = default_probe_below()
     d_start = E.DeviceArray(scf.ctx, (scf.ngrid,))
     learned_start(net, scf, d_start)
-    r = scf.run(d_start, callback, fallback_residual=r0)
+    probed = []
+
+    def bar(res):
+        if probe_below is not None and res < probe_below:
+            return None
+        saved, saved_bufs = scf.cfg.max_iter, getattr(scf, "_tucker_bufs", {})
+        scf.cfg.max_iter = 1  # the probe: one iteration from the superposition,
+        scf._tucker_bufs = {}  # with its own Tucker factors
+        try:
+            probed.append(scf.run().residuals[0])
+        finally:
+            scf.cfg.max_iter, scf._tucker_bufs = saved, saved_bufs
+        return probed[0]
+
+    r = scf.run(d_start, callback, fallback_residual=bar)
     r.seconds = time.perf_counter() - t0
-    r.iterations += 1  # the probe iteration
+    r.probed = bool(probed)
+    r.iterations += r.probed  # the probe iteration
     return r
 
 
@@ -411,7 +443,8 @@ def compare_starts(ctx, net, xyz, tol=1e-10):
     s1 = S.Scf(ctx, xyz, 3.0, cfg)
     r1 = run_learned(net, s1)
     to1 = iterations_to(r1.residuals, 1e-6)
-    to1 = None if to1 is None else to1 + 1  # the fallback probe's iteration
+    if to1 is not None and r1.probed:
+        to1 += 1  # the fallback probe's iteration
     return {"natoms": len(xyz),
             "superposition": {"iterations": r0.iterations, "to_1e-6": iterations_to(r0.residuals, 1e-6),
                               "first_residual": r0.residuals[0], "seconds": r0.seconds,

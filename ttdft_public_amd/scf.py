@@ -80,6 +80,7 @@ class ScfResult:
     seconds: float
     phase_seconds: dict = field(default_factory=dict)
     fell_back: bool = False
+    probed: bool = False
 
 
 def coulomb_expansion():
@@ -322,7 +323,9 @@ class Scf:
     def run(self, d_rho_start=None, callback=None, fallback_residual=None):
         """SCF from d_rho_start (default: the atomic superposition).  With
         fallback_residual, a first residual above it restarts the loop from
-        the superposition (the wasted iteration is counted)."""
+        the superposition (the wasted iteration is counted).  fallback_residual
+        may be a callable: given the first residual, it returns the bar, or
+        None for no fallback."""
         ctx, cfg = self.ctx, self.cfg
         n, k, R = self.ngrid, cfg.nstates, cfg.rank
         N = R ** 3
@@ -394,7 +397,10 @@ class Scf:
             phases["mixing"] += time.perf_counter() - t
             if callback:
                 callback(it, res, energy)
-            if it == 0 and fallback_residual is not None and res > fallback_residual:
+            bar = None
+            if it == 0 and fallback_residual is not None:
+                bar = fallback_residual(res) if callable(fallback_residual) else fallback_residual
+            if bar is not None and res > bar:
                 r = self.run(None, callback)
                 r.iterations += 1
                 r.residuals = residuals + r.residuals
