@@ -280,6 +280,9 @@ typedef struct {
     double occ_energy;     /* occupied_orbital_energy             */
     int32_t passes;        /* out: filter passes in the last call */
     int32_t lanczos_runs;  /* out: Lanczos calls in the last call */
+    double occ_tol;        /* in: relative change of eig[occ_index] that ends
+                              the restart loop (1e-3 from the init, the value
+                              of main.cc:1130-1145); no counterpart field */
 } ttdft_chfsi_state;
 
 /* Fills the initial state of main.cc:733-734 and :685.  */
@@ -288,7 +291,8 @@ int ttdft_chfsi_state_init(ttdft_chfsi_state *st);
 /* One SCF iteration's eigensolve on a device-resident block X:
  * bound update (Lanczos when err_upper > 0.01), restart loop of
  * filter -> orth -> Rayleigh-Ritz with the stop test on eig[occ_index]
- * (relative change < 1e-3), then the rotation X <- X Q.
+ * (relative change < st->occ_tol, 1e-3 as main.cc uses), then the rotation
+ * X <- X Q.
  * evals: host array of k (previous iteration in, new out).
  * precision: TTDFT_F64 (X FP64, FP64 GEMMs) or TTDFT_F32 (X FP32, FP32
  * filter, split-BF16 GEMMs; Cholesky and Rayleigh-Ritz stay FP64).

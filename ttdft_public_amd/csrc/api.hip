@@ -920,6 +920,7 @@ int ttdft_chfsi_state_init(ttdft_chfsi_state *st) {
     st->occ_energy = 1.0e9;
     st->passes = 0;
     st->lanczos_runs = 0;
+    st->occ_tol = 1.0e-3;
     return TTDFT_OK;
 }
 
@@ -1001,7 +1002,7 @@ int ttdft_chfsi_step(ttdft_ctx ctx, ttdft_ham H, void *X, int64_t ldx, int32_t k
         const double cur = evals[occ_index];
         const double err = std::fabs((cur - st->occ_energy) / st->occ_energy);
         st->occ_energy = cur;
-        if (err < 1.0e-3) break;  // main.cc:1130-1145
+        if (err < (st->occ_tol > 0.0 ? st->occ_tol : 1.0e-3)) break;  // main.cc:1130-1145
     }
     if (rc == TTDFT_OK && have_q) rc = ttdft_rotate(ctx, X, ldx, n, k, dQ, dtype, mode);
     if (rc == TTDFT_OK) {

@@ -50,6 +50,7 @@ class ChfsiState(ctypes.Structure):
         ("occ_energy", ctypes.c_double),
         ("passes", ctypes.c_int32),
         ("lanczos_runs", ctypes.c_int32),
+        ("occ_tol", ctypes.c_double),
     ]
 
 

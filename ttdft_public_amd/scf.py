@@ -62,6 +62,7 @@ class ScfConfig:
     tucker_refine: int = 2     # STHOSVD from the previous iteration's factors (0: exact each time)
     restart_first: int = 10
     restart_other: int = 12
+    occ_tol: float = 1e-3      # restart-loop stop on eig[occ] (main.cc:1130-1145 uses 1e-3)
     lda_variant: int = E.LDA_PZ81
     mixed: bool = False        # FP32 filter + split-BF16 GEMMs, then FP64 refinement
     refine_below: float = 1e-4  # residual at which a mixed run switches to FP64
@@ -348,6 +349,7 @@ class Scf:
             dt = E.F32
         ev = np.zeros(k)
         st = E.new_state()
+        st.occ_tol = cfg.occ_tol
         occ_index = min(self.nelec // 2 + 10, k - 1)
         self.fp32_iterations = 0
         residuals, energies = [], []
