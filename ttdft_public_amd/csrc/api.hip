@@ -561,6 +561,7 @@ int ttdft_ham_upload(ttdft_ctx ctx, int64_t n, const int64_t *rowptr, const int3
     }
 #undef UP
     if (rc == TTDFT_OK) rc = maybe_build_dense(ctx, H, copies);
+    if (rc == TTDFT_OK) rc = finish_dense(ctx, H, copies);
     if (rc == TTDFT_OK) {
         hipError_t e = hipStreamSynchronize(ctx->stream);  // host staging vectors go out of scope
         if (e != hipSuccess) {

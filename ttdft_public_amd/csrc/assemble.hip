@@ -961,7 +961,7 @@ static int ham_assemble(ttdft_ctx ctx, const HlocIn &a, int32_t p, const int32_t
         H->val64 = nullptr;
     }
     H->p = p;
-    if (p == 0) return TTDFT_OK;
+    if (p == 0) return finish_dense(ctx, H, copies);
     DevCsr c;
     TT_TRY(proj_run(ctx, p, ranks, a.rx, a.ry, a.rz, cores, mx, my, mz, c_truncation, true, c));
     H->nnz_c = c.nnz;
@@ -1004,7 +1004,7 @@ static int ham_assemble(ttdft_ctx ctx, const HlocIn &a, int32_t p, const int32_t
     TT_HIP(hipStreamSynchronize(ctx->stream));  // ctp is read by the copy above
     H->c_rowptr = c.rowptr.release<int64_t>();
     H->c_col = c.col.release<int32_t>();
-    return TTDFT_OK;
+    return finish_dense(ctx, H, copies);
 }
 
 int ttdft_ham_assemble_tucker(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz, int32_t rvx,

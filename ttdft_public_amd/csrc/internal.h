@@ -130,6 +130,7 @@ struct ttdft_ham_s {
     double *dense64 = nullptr;
     float *dense32 = nullptr;
     int64_t dense_ld = 0;
+    bool dense_nl = false;  // C^T D C is folded into the dense storage
 };
 
 namespace ttdft {
@@ -167,6 +168,7 @@ int small_orth(ttdft_ctx ctx, double *Y, int32_t n, int32_t b);
 // Dense storage (format 2) of H_loc when it is nearly full and small enough
 // (no-op otherwise); drop_dense frees it.
 int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies);
+int finish_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies);
 void drop_dense(ttdft_ham_s *H);
 int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                       const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,

@@ -478,6 +478,7 @@ void drop_dense(ttdft_ham_s *H) {
     }
     H->dense64 = nullptr;
     H->dense32 = nullptr;
+    H->dense_nl = false;
     if (H->format == 2) H->format = 0;
 }
 
@@ -486,6 +487,7 @@ void drop_dense(ttdft_ham_s *H) {
 // thousands of entries with the block's k states on one wavefront, while a
 // matrix product streams the n^2 values once at full bandwidth.
 int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
+    (void)copies;
     const int64_t n = H->n;
     if (H->format != 0 || n > 32768 || n < 1 || (double)H->nnz < 0.2 * (double)n * n) return TTDFT_OK;
     // rows padded to 16 and the leading dimension to 32 (the product's tiles)
@@ -498,17 +500,6 @@ int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
         hipLaunchKernelGGL(csr_to_dense_kernel<double>, dim3(rows), dim3(256), 0, ctx->stream, n,
                            ld, H->rowptr, H->col, H->val64, H->dense64);
         TT_LAUNCH_CHECK();
-        if (copies & TTDFT_HAM_F32) {
-            TT_HIP(hipMalloc(&H->dense32, sizeof(float) * n2));
-            hipLaunchKernelGGL(dense_to_f32_kernel, dim3(grid_blocks((int64_t)n2)), dim3(256), 0,
-                               ctx->stream, H->dense64, H->dense32, (int64_t)n2);
-            TT_LAUNCH_CHECK();
-        }
-        if (!(copies & TTDFT_HAM_F64)) {
-            TT_HIP(hipStreamSynchronize(ctx->stream));
-            TT_HIP(hipFree(H->dense64));
-            H->dense64 = nullptr;
-        }
     } else if (H->val32) {
         TT_HIP(hipMalloc(&H->dense32, sizeof(float) * n2));
         TT_HIP(hipMemsetAsync(H->dense32, 0, sizeof(float) * n2, ctx->stream));
@@ -519,7 +510,70 @@ int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
         return TTDFT_OK;
     }
     H->dense_ld = ld;
+    H->dense_nl = false;
     H->format = 2;
+    return TTDFT_OK;
+}
+
+// Completes the dense storage once the projectors are on the device: the
+// nonlocal term C^T D C is added into the FP64 matrix (one rank-p product,
+// D C from c_val64 and C from the transposed rows ct_val64), so a product
+// with the dense H needs no projection pass; then the FP32 copy is rounded
+// from it and the FP64 copy dropped if only FP32 was asked for.  An FP32-only
+// build (no FP64 values) keeps the projection pass.
+int finish_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
+    if (H->format != 2 || !H->dense64) return TTDFT_OK;
+    const int64_t n = H->n, ld = H->dense_ld;
+    const size_t n2 = (size_t)((n + 15) / 16 * 16) * (size_t)ld;
+    if (H->p > 0 && H->c_val64 && H->ct_val64) {
+        const int64_t p = H->p;
+        double *A1 = nullptr, *A2 = nullptr;  // D C as p x n rows, C^T as n x p rows
+        TT_HIP(hipMalloc(&A1, sizeof(double) * 2 * (size_t)p * n));
+        A2 = A1 + p * n;
+        int rc = TTDFT_OK;
+        hipError_t e = hipMemsetAsync(A1, 0, sizeof(double) * 2 * (size_t)p * n, ctx->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(csr_to_dense_kernel<double>, dim3((unsigned)((p + 3) / 4)),
+                               dim3(256), 0, ctx->stream, p, n, H->c_rowptr, H->c_col,
+                               H->c_val64, A1);
+            hipLaunchKernelGGL(csr_to_dense_kernel<double>, dim3((unsigned)((n + 3) / 4)),
+                               dim3(256), 0, ctx->stream, n, p, H->ct_rowptr, H->ct_col,
+                               H->ct_val64, A2);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess) {
+            set_error("finish_dense: %s", hipGetErrorString(e));
+            rc = TTDFT_EHIP;
+        } else {
+            // column-major view: H(l, i) = dense[i ld + l] += sum_r (D C)(r, l) C(r, i)
+            const double one = 1.0;
+            const rocblas_status st = rocblas_dgemm(
+                ctx->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n,
+                (rocblas_int)n, (rocblas_int)p, &one, A1, (rocblas_int)n, A2, (rocblas_int)p,
+                &one, H->dense64, (rocblas_int)ld);
+            if (st != rocblas_status_success) {
+                set_error("finish_dense: rocblas_dgemm status %d", (int)st);
+                rc = TTDFT_ELIB;
+            }
+        }
+        hipError_t se = hipStreamSynchronize(ctx->stream);
+        hipError_t fe = hipFree(A1);
+        if (rc != TTDFT_OK) return rc;
+        TT_HIP(se);
+        TT_HIP(fe);
+        H->dense_nl = true;
+    }
+    if (copies & TTDFT_HAM_F32) {
+        TT_HIP(hipMalloc(&H->dense32, sizeof(float) * n2));
+        hipLaunchKernelGGL(dense_to_f32_kernel, dim3(grid_blocks((int64_t)n2)), dim3(256), 0,
+                           ctx->stream, H->dense64, H->dense32, (int64_t)n2);
+        TT_LAUNCH_CHECK();
+    }
+    if (!(copies & TTDFT_HAM_F64)) {
+        TT_HIP(hipStreamSynchronize(ctx->stream));
+        TT_HIP(hipFree(H->dense64));
+        H->dense64 = nullptr;
+    }
     return TTDFT_OK;
 }
 
@@ -532,7 +586,8 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
     const dim3 grid((unsigned)((n + 15) / 16), (unsigned)((k + 16 * JT - 1) / (16 * JT)));
     hipLaunchKernelGGL((dense_hx_kernel<T, JT>), grid, dim3(64 * kDenseWaves), 0, ctx->stream, n,
                        H->dense_ld, k, Hd, (const T *)X, ldx,
-                       H->p > 0 ? H->ct_rowptr : nullptr, H->ct_col, ctv, (const T *)W, (T *)Y,
+                       H->p > 0 && !H->dense_nl ? H->ct_rowptr : nullptr, H->ct_col, ctv,
+                       (const T *)W, (T *)Y,
                        ldy, (const T *)Z, ldz, alpha, beta, delta, use_beta, use_delta);
 }
 
@@ -569,8 +624,12 @@ int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, vo
              int64_t ldy, const void *Z, int64_t ldz, int32_t k, int dtype, double alpha,
              double beta, double delta, int use_beta, int use_delta) {
     const size_t es = dtype_size(dtype);
+    const bool band = H->format == 1 && k >= 16 &&
+                      (dtype == TTDFT_F64 ? H->mf64 != nullptr : H->mf32 != nullptr);
+    const bool dense = H->format == 2 && (dtype == TTDFT_F64 ? H->dense64 != nullptr
+                                                            : H->dense32 != nullptr);
     void *W = nullptr;
-    if (H->p > 0) {
+    if (H->p > 0 && !(dense && H->dense_nl)) {  // folded into the dense storage otherwise
         TT_TRY(ctx->scratch[3].ensure((size_t)nl_workspace(H, k) * es));
         W = ctx->scratch[3].ptr;
         PendingTiming tn;
@@ -581,10 +640,6 @@ int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, vo
     }
     PendingTiming ts;
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_SPMM, &ts));
-    const bool band = H->format == 1 && k >= 16 &&
-                      (dtype == TTDFT_F64 ? H->mf64 != nullptr : H->mf32 != nullptr);
-    const bool dense = H->format == 2 && (dtype == TTDFT_F64 ? H->dense64 != nullptr
-                                                            : H->dense32 != nullptr);
     if (band)
         TT_TRY(launch_band_spmm(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
                                 alpha, beta, delta, use_beta, use_delta));
