@@ -101,7 +101,7 @@ __device__ __forceinline__ void jacobi_pair(int m, int r, int t, int &p, int &q)
 // (rows of pair j, columns of pair j') of A <- J^T A J (rows first, then
 // columns, the same operations as two separate passes) and pairs of columns
 // of V <- V J.
-constexpr int kJacobiThreads = 512;
+constexpr int kJacobiThreads = 1024;
 
 __global__ __launch_bounds__(kJacobiThreads) void small_jacobi_kernel(
     const double *__restrict__ Ain, int k, double *__restrict__ evals, double *__restrict__ Q,
