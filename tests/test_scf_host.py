@@ -1,7 +1,7 @@
 """Host-side pieces of the SCF driver and the learned start (no device work):
 cluster generators, piecewise-Lagrange point evaluation against the oracle's
-finite-element interpolation, the Coulomb expansion data and the network's
-parameter layout."""
+finite-element interpolation, the Coulomb expansion data, the network's
+parameter layout and the fallback probe's bar."""
 import math
 
 import numpy as np
@@ -61,3 +61,26 @@ def test_network_layout_matches_oracle():
     assert Ln.layer_specs() == U.layer_specs()
     n = sum(co * ci * ks ** 3 + co for _, ci, co, ks in Ln.layer_specs())
     assert n == 1460017
+
+
+def test_probe_bar_from_report(tmp_path):
+    """The lazy-probe bar is a quarter of the smallest superposition first
+    residual in the evaluation report, below every learned first residual
+    that report shows; a missing or malformed report gives None (always
+    probe)."""
+    bar = Ln.default_probe_below()
+    import json
+    import os
+    rep = json.load(open(os.path.join(os.path.dirname(Ln.DEFAULT_WEIGHTS),
+                                      "unet_initial_density_report.json")))
+    sup = [c["superposition"]["first_residual"] for c in rep["test"]]
+    lrn = [c["learned"]["first_residual"] for c in rep["test"]]
+    assert bar == 0.25 * min(sup)
+    assert max(lrn) < bar < min(sup)
+    p = tmp_path / "r.json"
+    p.write_text(json.dumps({"test": [{"superposition": {"first_residual": 8.0}},
+                                      {"superposition": {"first_residual": 4.0}}]}))
+    assert Ln.default_probe_below(str(p)) == 1.0
+    p.write_text("{}")
+    assert Ln.default_probe_below(str(p)) is None
+    assert Ln.default_probe_below(str(tmp_path / "absent.json")) is None
