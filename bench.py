@@ -409,6 +409,50 @@ def profiled_traffic(kernel_tag):
     return None
 
 
+def bench_large(ctx, E, reps=5):
+    """One H application at the large mixed-precision shape: N = 70^3 =
+    343,000 Tucker basis functions, 900 states, Kronecker half-bandwidths
+    (7, 7, 7) (about 2,900 nonzeros per row, 9.8e8 in all) and 500 projectors
+    (seeded synthetic H; the reference has no input at this size).  FP64 and
+    FP32 band products, timed with HIP events on the engine's stream."""
+    t0 = time.perf_counter()
+    s = E.SynthHam(70, 70, 70, 7, 7, 7, 500, seed=4)
+    n, nnz, k = s.n, s.nnz, 900
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=True)
+    setup = time.perf_counter() - t0
+    del s
+    out = {"workload": f"H application, N={n}, k={k}, {nnz} nonzeros "
+                       f"({nnz / n:.0f}/row), 500 projectors, Kronecker band storage",
+           "setup_seconds": round(setup, 1), "format": list(ham.format())}
+    X = np.random.default_rng(1).standard_normal((n, k))
+    Y64 = None
+    for dt, npdt, name in ((E.F64, np.float64, "fp64"), (E.F32, np.float32, "fp32")):
+        dX = E.DeviceArray.from_host(ctx, X.astype(npdt))
+        dY = E.DeviceArray(ctx, (n, k), npdt)
+        ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, dt)
+        ctx.synchronize()
+        ctx.reset_stats()
+        ctx.set_profiling(True)
+        for _ in range(reps):
+            ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, dt)
+        ctx.synchronize()
+        sp = ctx.kernel_stats(E.KSTAT_SPMM)
+        ctx.set_profiling(False)
+        ms = sp["ms"] / sp["launches"]
+        out[name] = {"ms_per_application": round(ms, 2),
+                     "algorithmic_GBps": round(sp["bytes"] / sp["launches"] / (ms * 1e-3) / 1e9, 1),
+                     "useful_TFLOPs": round(sp["flops"] / sp["launches"] / (ms * 1e-3) / 1e12, 1)}
+        Y = dY.to_host()
+        if Y64 is None:
+            Y64 = Y
+        else:
+            out[name]["max_rel_diff_vs_fp64"] = float(np.abs(Y - Y64).max() / np.abs(Y64).max())
+        dX.free()
+        dY.free()
+        log(f"large {name}: {out[name]}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,13 +462,14 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-density", action="store_true")
     ap.add_argument("--mode", default="step",
-                    choices=["step", "spmm", "assembly", "mixing", "hartree", "scf"],
+                    choices=["step", "spmm", "assembly", "mixing", "hartree", "scf", "large"],
                     help="spmm: only 3 FP64 H applications and 3 copies of one 125000 x 300 "
                          "FP64 block (copy = calibration of rocprofv3 byte counters); "
                          "assembly: only the device-resident H assembly (3 runs); "
                          "mixing: only the Anderson mixing step (4 runs); "
                          "hartree: only STHOSVD + Hartree potential (4 runs); "
-                         "scf: only the 13-atom SCF (FP64, mixed, mixed + learned start)")
+                         "scf: only the 13-atom SCF (FP64, mixed, mixed + learned start); "
+                         "large: one H application at N = 343,000, k = 900 (FP64 and FP32)")
     ap.add_argument("--no-hartree", action="store_true",
                     help="skip the STHOSVD + Hartree measurement")
     ap.add_argument("--no-mixing", action="store_true",
@@ -456,6 +501,10 @@ def main():
     if args.mode == "scf":
         ctx = E.Context(0)
         log(json.dumps(bench_scf(ctx, E)))
+        return
+    if args.mode == "large":
+        ctx = E.Context(0)
+        log(json.dumps(bench_large(ctx, E)))
         return
 
     t0 = time.perf_counter()

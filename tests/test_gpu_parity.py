@@ -111,7 +111,9 @@ def test_ham_upload_rejects_bad_csr(ctx):
                                           ((70, 5, 4), (3, 1, 1), 40),    # two x-chunks
                                           ((6, 7, 5), (1, 2, 2), 33),     # padded narrow chunk
                                           ((17, 9, 8), (7, 3, 2), 81),    # widest x band
-                                          ((22, 6, 5), (2, 2, 1), 50)])   # two split x-bricks
+                                          ((22, 6, 5), (2, 2, 1), 50),    # two split x-bricks
+                                          ((12, 12, 12), (6, 6, 7), 900),  # 900 states: 3 / 12 groups
+                                          ((17, 9, 8), (7, 3, 2), 900)])
 @pytest.mark.parametrize("dtype", [E.F64, E.F32])
 def test_band_spmm_matches_oracle(ctx, dims, bands, k, dtype):
     s = E.SynthHam(*dims, *bands, 4, seed=sum(dims))
