@@ -458,10 +458,11 @@ def test_dense_storage_matches_oracle(ctx, tucker):
 
 
 @pytest.mark.parametrize("dims", [(4, 3, 5), (9, 8, 7), (11, 10, 13)])
-@pytest.mark.parametrize("k", [1, 8, 16, 24, 33, 64, 100])
+@pytest.mark.parametrize("k", [1, 8, 16, 24, 33, 64, 80, 96, 100, 150])
 def test_dense_product_tilings(ctx, dims, k):
-    """The dense product over its state tilings (16 to 64 states per
-    workgroup, partial tiles), leading dimension of the block above k, and n
+    """The dense product over its state tilings (16 to 96 states per
+    workgroup, one or two workgroup columns, partial tiles), leading dimension
+    of the block above k, and n
     not a multiple of 32 (n = 60: six of the eight waves get no l range;
     n = 1430: the last wave's range is shorter)."""
     rx, ry, rz = dims

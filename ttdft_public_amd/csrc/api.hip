@@ -255,6 +255,70 @@ static void dfree(void *p) {
     }
 }
 
+namespace {
+constexpr size_t kBlockCacheCap = size_t(64) << 30;  // released bytes kept per context
+
+// size class: 256-byte granules below 1 MiB, else rounded up to 1/8 of the
+// power of two below the size (at most 12.5% slack)
+size_t block_class(size_t bytes) {
+    if (bytes < 8) bytes = 8;
+    if (bytes <= (size_t(1) << 20)) return (bytes + 255) / 256 * 256;
+    size_t top = size_t(1) << (63 - __builtin_clzll(bytes));
+    const size_t g = top / 8;
+    return (bytes + g - 1) / g * g;
+}
+}  // namespace
+
+int dev_alloc(ttdft_ctx ctx, void **p, size_t bytes) {
+    const size_t c = block_class(bytes);
+    auto it = ctx->blk_free.find(c);
+    if (it != ctx->blk_free.end()) {
+        *p = it->second;
+        ctx->blk_free.erase(it);
+        ctx->blk_free_bytes -= c;
+    } else {
+        hipError_t e = hipMalloc(p, c);
+        if (e != hipSuccess && !ctx->blk_free.empty()) {  // give the cached blocks back, retry
+            dev_cache_clear(ctx);
+            e = hipMalloc(p, c);
+        }
+        if (e != hipSuccess) {
+            *p = nullptr;
+            set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+            return TTDFT_ENOMEM;
+        }
+    }
+    ctx->blk_size[*p] = c;
+    return TTDFT_OK;
+}
+
+void dev_release(ttdft_ctx ctx, void *p) {
+    if (!p) return;
+    auto it = ctx->blk_size.find(p);
+    if (it == ctx->blk_size.end()) {
+        dfree(p);
+        return;
+    }
+    const size_t c = it->second;
+    ctx->blk_size.erase(it);
+    if (ctx->blk_free_bytes + c > kBlockCacheCap) {
+        dfree(p);
+        return;
+    }
+    ctx->blk_free.emplace(c, p);
+    ctx->blk_free_bytes += c;
+}
+
+void dev_cache_clear(ttdft_ctx ctx) {
+    if (!ctx->blk_free.empty()) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);  // pending work may still use them
+        (void)e;
+    }
+    for (auto &b : ctx->blk_free) dfree(b.second);
+    ctx->blk_free.clear();
+    ctx->blk_free_bytes = 0;
+}
+
 static int orth_once(ttdft_ctx ctx, void *X, int64_t ldx, int64_t n, int32_t k, int dtype,
                      int mode) {
     const int64_t kk = (int64_t)k * k;
@@ -348,6 +412,7 @@ int ttdft_ctx_destroy(ttdft_ctx ctx) {
         (void)e;
     }
     timing_flush(ctx);
+    dev_cache_clear(ctx);
     for (auto &s : ctx->scratch) s.release();
     for (auto &s : ctx->nn_scratch) s.release();
     dfree(ctx->d_info);
@@ -615,22 +680,22 @@ int ttdft_ham_free(ttdft_ham H) {
     if (!H) return TTDFT_OK;
     hipError_t e = hipStreamSynchronize(H->ctx->stream);
     (void)e;
-    dfree(H->mf64);
-    dfree(H->mf32);
-    dfree(H->dense64);
-    dfree(H->dense32);
-    dfree(H->rowptr);
-    dfree(H->col);
-    dfree(H->val64);
-    dfree(H->val32);
-    dfree(H->c_rowptr);
-    dfree(H->c_col);
-    dfree(H->c_val64);
-    dfree(H->c_val32);
-    dfree(H->ct_rowptr);
-    dfree(H->ct_col);
-    dfree(H->ct_val64);
-    dfree(H->ct_val32);
+    dev_release(H->ctx, H->mf64);
+    dev_release(H->ctx, H->mf32);
+    dev_release(H->ctx, H->dense64);
+    dev_release(H->ctx, H->dense32);
+    dev_release(H->ctx, H->rowptr);
+    dev_release(H->ctx, H->col);
+    dev_release(H->ctx, H->val64);
+    dev_release(H->ctx, H->val32);
+    dev_release(H->ctx, H->c_rowptr);
+    dev_release(H->ctx, H->c_col);
+    dev_release(H->ctx, H->c_val64);
+    dev_release(H->ctx, H->c_val32);
+    dev_release(H->ctx, H->ct_rowptr);
+    dev_release(H->ctx, H->ct_col);
+    dev_release(H->ctx, H->ct_val64);
+    dev_release(H->ctx, H->ct_val32);
     delete H;
     return TTDFT_OK;
 }

@@ -41,23 +41,18 @@ namespace {
     } while (0)
 
 // device buffers freed on scope exit
+// A device block from the context's block cache (dev_alloc), released back
+// to it when the owner goes out of scope unless handed to a Hamiltonian
+// (release), whose ttdft_ham_free returns it.
 struct DevBuf {
     void *p = nullptr;
+    ttdft_ctx ctx = nullptr;
     ~DevBuf() {
-        if (p) {
-            hipError_t e = hipFree(p);
-            (void)e;
-        }
+        if (p) dev_release(ctx, p);
     }
-    int alloc(size_t bytes) {
-        if (bytes == 0) bytes = 8;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            p = nullptr;
-            set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-            return TTDFT_ENOMEM;
-        }
-        return TTDFT_OK;
+    int alloc(ttdft_ctx c, size_t bytes) {
+        ctx = c;
+        return dev_alloc(c, &p, bytes == 0 ? 8 : bytes);
     }
     template <typename T>
     T *as() const {
@@ -72,7 +67,7 @@ struct DevBuf {
 };
 
 int upload_to(ttdft_ctx ctx, DevBuf &b, const void *src, size_t bytes) {
-    TT_TRY(b.alloc(bytes));
+    TT_TRY(b.alloc(ctx, bytes));
     if (bytes) TT_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     return TTDFT_OK;
 }
@@ -609,8 +604,8 @@ int hloc_run(ttdft_ctx ctx, const HlocIn &a, bool fill, DevCsr &out, int ext[3])
     const int64_t mp = (rz2 + 63) / 64 * 64;          // Ae rows, padded to the tile
     const int64_t ldb = fused ? ntup + 256 : ntup;    // Be plane stride, padded
     DevBuf dA, dB, dAe;
-    TT_TRY(dA.alloc(sizeof(double) * rx2 * rvy * rvz));
-    TT_TRY(dB.alloc(sizeof(double) * (fused ? K4 + 8 : rvz) * ldb));
+    TT_TRY(dA.alloc(ctx, sizeof(double) * rx2 * rvy * rvz));
+    TT_TRY(dB.alloc(ctx, sizeof(double) * (fused ? K4 + 8 : rvz) * ldb));
     if (fused) {
         std::vector<double> ae((size_t)mp * (K4 + 8), 0.0);
         for (int r = 0; r < rvz; ++r)
@@ -627,13 +622,13 @@ int hloc_run(ttdft_ctx ctx, const HlocIn &a, bool fill, DevCsr &out, int ext[3])
     gc = std::max<int64_t>(1, std::min<int64_t>(gc, ngroups));
     gc = env_cap("TTDFT_ASSEMBLE_GROUPS", gc);  // groups (xI, y, yI) per chunk
     DevBuf dC, dcnt, drel, drowlen, dflag;
-    TT_TRY(dflag.alloc(sizeof(int32_t) * 4));
+    TT_TRY(dflag.alloc(ctx, sizeof(int32_t) * 4));
     const int32_t flag0[4] = {0, -1, -1, -1};  // mismatch, half-bandwidths x, y, z
     TT_HIP(hipMemcpyAsync(dflag.p, flag0, sizeof(flag0), hipMemcpyHostToDevice, ctx->stream));
-    if (!fused) TT_TRY(dC.alloc(sizeof(double) * rz2 * rx * gc));
-    TT_TRY(dcnt.alloc(sizeof(uint16_t) * n * ry * rz));
-    TT_TRY(drel.alloc(sizeof(int32_t) * n * ry * rz));
-    TT_TRY(drowlen.alloc(sizeof(int64_t) * n));
+    if (!fused) TT_TRY(dC.alloc(ctx, sizeof(double) * rz2 * rx * gc));
+    TT_TRY(dcnt.alloc(ctx, sizeof(uint16_t) * n * ry * rz));
+    TT_TRY(drel.alloc(ctx, sizeof(int32_t) * n * ry * rz));
+    TT_TRY(drowlen.alloc(ctx, sizeof(int64_t) * n));
     FusedArgs fa{};
     if (fused) {
         fa.A = dAe.as<double>();
@@ -725,8 +720,8 @@ int hloc_run(ttdft_ctx ctx, const HlocIn &a, bool fill, DevCsr &out, int ext[3])
     out.nnz = out.rp[n];
     if (!fill) return TTDFT_OK;
     TT_TRY(upload_to(ctx, out.rowptr, out.rp.data(), sizeof(int64_t) * (n + 1)));
-    TT_TRY(out.col.alloc(sizeof(int32_t) * out.nnz));
-    TT_TRY(out.val.alloc(sizeof(double) * out.nnz));
+    TT_TRY(out.col.alloc(ctx, sizeof(int32_t) * out.nnz));
+    TT_TRY(out.val.alloc(ctx, sizeof(double) * out.nnz));
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ASSEMBLY, &tm));
     TT_TRY(run_pass(1, out.rowptr.as<int64_t>(), out.col.as<int32_t>(), out.val.as<double>()));
     TT_TRY(timing_end(ctx, &tm, 12.0 * out.nnz + 8.0 * (n + 1), flops));
@@ -806,9 +801,9 @@ int proj_run(ttdft_ctx ctx, int32_t p, const int32_t *ranks, int32_t rx, int32_t
     int64_t rb = std::max<int64_t>(1, std::min<int64_t>(p, ((int64_t)1 << 27) / n));
     rb = env_cap("TTDFT_ASSEMBLE_ROWS", rb);
     DevBuf dD, dcnt, dstart;
-    TT_TRY(dD.alloc(sizeof(double) * rb * n));
-    TT_TRY(dcnt.alloc(sizeof(int32_t) * (size_t)p * nchunk));
-    TT_TRY(dstart.alloc(sizeof(int64_t) * (size_t)p * nchunk));
+    TT_TRY(dD.alloc(ctx, sizeof(double) * rb * n));
+    TT_TRY(dcnt.alloc(ctx, sizeof(int32_t) * (size_t)p * nchunk));
+    TT_TRY(dstart.alloc(ctx, sizeof(int64_t) * (size_t)p * nchunk));
     auto run_pass = [&](int write, int32_t *dcol, double *dval) -> int {
         for (int64_t i0 = 0; i0 < p; i0 += rb) {
             const int64_t nr = std::min<int64_t>(rb, p - i0);
@@ -850,8 +845,8 @@ int proj_run(ttdft_ctx ctx, int32_t p, const int32_t *ranks, int32_t rx, int32_t
     TT_HIP(hipMemcpyAsync(dstart.p, start.data(), sizeof(int64_t) * start.size(),
                           hipMemcpyHostToDevice, ctx->stream));
     TT_TRY(upload_to(ctx, out.rowptr, out.rp.data(), sizeof(int64_t) * (p + 1)));
-    TT_TRY(out.col.alloc(sizeof(int32_t) * acc));
-    TT_TRY(out.val.alloc(sizeof(double) * acc));
+    TT_TRY(out.col.alloc(ctx, sizeof(int32_t) * acc));
+    TT_TRY(out.val.alloc(ctx, sizeof(double) * acc));
     TT_TRY(run_pass(1, out.col.as<int32_t>(), out.val.as<double>()));
     TT_HIP(hipStreamSynchronize(ctx->stream));  // start[] is read by the copy above
     return TTDFT_OK;
@@ -945,7 +940,7 @@ static int ham_assemble(ttdft_ctx ctx, const HlocIn &a, int32_t p, const int32_t
     H->col = h.col.release<int32_t>();
     H->val64 = h.val.release<double>();
     if (copies & TTDFT_HAM_F32) {
-        TT_HIP(hipMalloc(&H->val32, sizeof(float) * std::max<int64_t>(1, h.nnz)));
+        TT_TRY(dev_alloc(ctx, (void **)&H->val32, sizeof(float) * std::max<int64_t>(1, h.nnz)));
         if (h.nnz) {
             hipLaunchKernelGGL(to_f32_kernel, dim3(blocks_for(h.nnz, 256)), dim3(256), 0,
                                ctx->stream, H->val64, H->val32, h.nnz);
@@ -957,7 +952,7 @@ static int ham_assemble(ttdft_ctx ctx, const HlocIn &a, int32_t p, const int32_t
     TT_TRY(maybe_build_dense(ctx, H, copies));
     if (!(copies & TTDFT_HAM_F64)) {
         TT_HIP(hipStreamSynchronize(ctx->stream));
-        TT_HIP(hipFree(H->val64));
+        dev_release(ctx, H->val64);
         H->val64 = nullptr;
     }
     H->p = p;
@@ -969,20 +964,20 @@ static int ham_assemble(ttdft_ctx ctx, const HlocIn &a, int32_t p, const int32_t
     TT_TRY(upload_to(ctx, dinv, inv_c_lm, sizeof(double) * p));
     const size_t nc = (size_t)std::max<int64_t>(1, c.nnz);
     if (copies & TTDFT_HAM_F64) {
-        TT_HIP(hipMalloc(&H->c_val64, sizeof(double) * nc));
-        TT_HIP(hipMalloc(&H->ct_val64, sizeof(double) * nc));
+        TT_TRY(dev_alloc(ctx, (void **)&H->c_val64, sizeof(double) * nc));
+        TT_TRY(dev_alloc(ctx, (void **)&H->ct_val64, sizeof(double) * nc));
     }
     if (copies & TTDFT_HAM_F32) {
-        TT_HIP(hipMalloc(&H->c_val32, sizeof(float) * nc));
-        TT_HIP(hipMalloc(&H->ct_val32, sizeof(float) * nc));
+        TT_TRY(dev_alloc(ctx, (void **)&H->c_val32, sizeof(float) * nc));
+        TT_TRY(dev_alloc(ctx, (void **)&H->ct_val32, sizeof(float) * nc));
     }
-    TT_HIP(hipMalloc(&H->ct_col, sizeof(int32_t) * nc));
+    TT_TRY(dev_alloc(ctx, (void **)&H->ct_col, sizeof(int32_t) * nc));
     // D C (row-scaled) and C^T (unscaled), ProjectHamiltonianSparse.cc:316-325
     hipLaunchKernelGGL(scale_rows_kernel, dim3((unsigned)p), dim3(256), 0, ctx->stream,
                        c.rowptr.as<int64_t>(), c.val.as<double>(), dinv.as<double>(),
                        H->c_val64, H->c_val32);
     TT_LAUNCH_CHECK();
-    TT_TRY(ctlen.alloc(sizeof(int64_t) * n));
+    TT_TRY(ctlen.alloc(ctx, sizeof(int64_t) * n));
     const unsigned nbj = blocks_for(n, 256);
     hipLaunchKernelGGL(transpose_rows_kernel<0>, dim3(nbj), dim3(256), 0, ctx->stream,
                        c.rowptr.as<int64_t>(), c.col.as<int32_t>(), c.val.as<double>(), p, n,
@@ -994,7 +989,7 @@ static int ham_assemble(ttdft_ctx ctx, const HlocIn &a, int32_t p, const int32_t
     TT_HIP(hipStreamSynchronize(ctx->stream));
     for (int64_t j = 0; j < n; ++j) ctp[j + 1] += ctp[j];
     TT_CHECK(ctp[n] == c.nnz, TTDFT_ELIB, "ham_assemble: projector transpose lost entries");
-    TT_HIP(hipMalloc(&H->ct_rowptr, sizeof(int64_t) * (n + 1)));
+    TT_TRY(dev_alloc(ctx, (void **)&H->ct_rowptr, sizeof(int64_t) * (n + 1)));
     TT_HIP(hipMemcpyAsync(H->ct_rowptr, ctp.data(), sizeof(int64_t) * (n + 1),
                           hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(transpose_rows_kernel<1>, dim3(nbj), dim3(256), 0, ctx->stream,

@@ -8,6 +8,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <map>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/ttdft_chfsi.h"
@@ -91,6 +93,12 @@ struct ttdft_ctx_s {
     ttdft::Scratch nn_scratch[2];
     int nn_math = 0;  // TTDFT_NN_FP32 or TTDFT_NN_BF16 for the network's convolutions
     int *d_info = nullptr;  // rocSOLVER info
+    // device blocks handed out by dev_alloc (pointer -> size class) and the
+    // released ones kept for reuse on this context's stream (size class ->
+    // blocks); see dev_alloc
+    std::unordered_map<void *, size_t> blk_size;
+    std::multimap<size_t, void *> blk_free;
+    size_t blk_free_bytes = 0;
 };
 
 struct ttdft_ham_s {
@@ -169,6 +177,15 @@ int small_orth(ttdft_ctx ctx, double *Y, int32_t n, int32_t b);
 // (no-op otherwise); drop_dense frees it.
 int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies);
 int finish_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies);
+// Device blocks that live and die with Hamiltonians, rebuilt every SCF
+// iteration at multi-GB sizes: dev_alloc takes a released block of the same
+// size class (sizes rounded up by at most 1/8) or calls hipMalloc;
+// dev_release returns a dev_alloc block to the context (reused in the order
+// of its stream, so no synchronisation is needed) and hipFrees any other
+// pointer.  dev_cache_clear frees the released blocks.
+int dev_alloc(ttdft_ctx ctx, void **p, size_t bytes);
+void dev_release(ttdft_ctx ctx, void *p);
+void dev_cache_clear(ttdft_ctx ctx);
 void drop_dense(ttdft_ham_s *H);
 int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                       const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,

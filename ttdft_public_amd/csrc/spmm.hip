@@ -254,23 +254,34 @@ template <> struct DenseMma<float> {
     static __device__ __forceinline__ int state(int q, int v) { return 4 * q + v; }
 };
 
+// Stage depth U (MFMA steps per stage, U consecutive entries per lane) and
+// accumulator sets NA per state-tile count: up to 64 states, 8-deep stages and
+// two accumulator sets (alternate steps); 80 or 96 states, 4-deep stages and
+// one set, which keeps the two stages in registers.
+template <int JT> struct DenseCfg {
+    static constexpr int U = JT <= 4 ? 8 : 4;
+    static constexpr int NA = JT <= 4 ? 2 : 1;
+};
+
 template <typename T, int JT> struct DenseStage {
-    T h[8];
-    T x[8][JT];
+    static constexpr int U = DenseCfg<JT>::U;
+    T h[U];
+    T x[U][JT];
 };
 
 template <typename T, int JT>
 __device__ __forceinline__ void dense_load(DenseStage<T, JT> &st, const T *hrow, const T *X,
                                            int64_t ldx, int64_t n, const int *cidx, int64_t l0,
                                            int q) {
-    // 8 consecutive entries: 32 B (FP32) or 64 B (FP64), aligned by the padding
-    typedef T v8 __attribute__((ext_vector_type(8)));
-    const v8 hv = *reinterpret_cast<const v8 *>(hrow + l0);
+    constexpr int U = DenseCfg<JT>::U;
+    // U consecutive entries (16-64 B), aligned by the padding
+    typedef T vU __attribute__((ext_vector_type(U)));
+    const vU hv = *reinterpret_cast<const vU *>(hrow + l0);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) st.h[u] = hv[u];
+    for (int u = 0; u < U; ++u) st.h[u] = hv[u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        int64_t l = l0 + 8 * q + u;
+    for (int u = 0; u < U; ++u) {
+        int64_t l = l0 + U * q + u;
         l = l < n ? l : n - 1;
         const T *xr = X + l * ldx;
 #pragma unroll
@@ -279,13 +290,14 @@ __device__ __forceinline__ void dense_load(DenseStage<T, JT> &st, const T *hrow,
 }
 
 template <typename T, int JT>
-__device__ __forceinline__ void dense_mma(const DenseStage<T, JT> &st,
-                                          typename DenseMma<T>::acc_t (&acc)[2][JT]) {
+__device__ __forceinline__ void dense_mma(
+    const DenseStage<T, JT> &st, typename DenseMma<T>::acc_t (&acc)[DenseCfg<JT>::NA][JT]) {
+    constexpr int U = DenseCfg<JT>::U, NA = DenseCfg<JT>::NA;
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt)
-            acc[u & 1][jt] = DenseMma<T>::step(st.x[u][jt], st.h[u], acc[u & 1][jt]);
+            acc[u % NA][jt] = DenseMma<T>::step(st.x[u][jt], st.h[u], acc[u % NA][jt]);
 }
 
 template <typename T, int JT>
@@ -295,23 +307,24 @@ __global__ __launch_bounds__(512) void dense_hx_kernel(
     const T *__restrict__ ct_val, const T *__restrict__ W, T *Y, int64_t ldy, const T *Z,
     int64_t ldz, double alpha, double beta, double delta, int use_beta, int use_delta) {
     typedef typename DenseMma<T>::acc_t acc_t;
+    constexpr int U = DenseCfg<JT>::U, NA = DenseCfg<JT>::NA, SW = 4 * U;  // step width in l
     __shared__ T red[kDenseWaves][16 * JT][17];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int li = lane & 15, q = lane >> 4;
     const int64_t i0 = (int64_t)blockIdx.x * 16;
     const int c0 = blockIdx.y * 16 * JT;
-    const int64_t part = ((ldh + kDenseWaves - 1) / kDenseWaves + 31) / 32 * 32;
+    const int64_t part = ((ldh + kDenseWaves - 1) / kDenseWaves + SW - 1) / SW * SW;
     const int64_t lb = wave * part, le = lb + part < ldh ? lb + part : ldh;
-    const T *hrow = H + (i0 + li) * ldh + 8 * q;
+    const T *hrow = H + (i0 + li) * ldh + U * q;
     int cidx[JT];
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) {
         const int c = c0 + 16 * jt + li;
         cidx[jt] = c < k ? c : k - 1;
     }
-    acc_t acc[2][JT];
+    acc_t acc[NA][JT];
 #pragma unroll
-    for (int e = 0; e < 2; ++e)
+    for (int e = 0; e < NA; ++e)
 #pragma unroll
         for (int jt = 0; jt < JT; ++jt) acc[e][jt] = acc_t{0, 0, 0, 0};
     if (lb < le) {
@@ -320,16 +333,16 @@ __global__ __launch_bounds__(512) void dense_hx_kernel(
         // buffer; scheduling barriers keep the compiler from sinking the loads
         // into the products)
         DenseStage<T, JT> sa, sb;
-        const int64_t nsteps = (le - lb) / 32;
+        const int64_t nsteps = (le - lb) / SW;
         dense_load<T, JT>(sa, hrow, X, ldx, n, cidx, lb, q);
         int64_t st = 0;
         for (; st + 2 <= nsteps; st += 2) {
-            dense_load<T, JT>(sb, hrow, X, ldx, n, cidx, lb + 32 * (st + 1), q);
+            dense_load<T, JT>(sb, hrow, X, ldx, n, cidx, lb + SW * (st + 1), q);
             __builtin_amdgcn_sched_barrier(0);
             dense_mma<T, JT>(sa, acc);
             __builtin_amdgcn_sched_barrier(0);
             const int64_t s2 = st + 2 < nsteps ? st + 2 : nsteps - 1;
-            dense_load<T, JT>(sa, hrow, X, ldx, n, cidx, lb + 32 * s2, q);
+            dense_load<T, JT>(sa, hrow, X, ldx, n, cidx, lb + SW * s2, q);
             __builtin_amdgcn_sched_barrier(0);
             dense_mma<T, JT>(sb, acc);
             __builtin_amdgcn_sched_barrier(0);
@@ -340,7 +353,8 @@ __global__ __launch_bounds__(512) void dense_hx_kernel(
     for (int jt = 0; jt < JT; ++jt)
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-            red[wave][16 * jt + DenseMma<T>::state(q, v)][li] = acc[0][jt][v] + acc[1][jt][v];
+            red[wave][16 * jt + DenseMma<T>::state(q, v)][li] =
+                NA == 2 ? acc[0][jt][v] + acc[NA - 1][jt][v] : acc[0][jt][v];
     __syncthreads();
     for (int e = t; e < 16 * 16 * JT; e += 512) {
         const int j = e % (16 * JT), ii = e / (16 * JT);
@@ -468,14 +482,8 @@ int ham_spmv(ttdft_ctx ctx, const ttdft_ham_s *H, const double *x, double *y) {
 }
 
 void drop_dense(ttdft_ham_s *H) {
-    if (H->dense64) {
-        hipError_t e = hipFree(H->dense64);
-        (void)e;
-    }
-    if (H->dense32) {
-        hipError_t e = hipFree(H->dense32);
-        (void)e;
-    }
+    dev_release(H->ctx, H->dense64);
+    dev_release(H->ctx, H->dense32);
     H->dense64 = nullptr;
     H->dense32 = nullptr;
     H->dense_nl = false;
@@ -495,13 +503,13 @@ int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
     const size_t n2 = (size_t)((n + 15) / 16 * 16) * (size_t)ld;
     const unsigned rows = (unsigned)((n + 3) / 4);
     if (H->val64) {
-        TT_HIP(hipMalloc(&H->dense64, sizeof(double) * n2));
+        TT_TRY(dev_alloc(ctx, (void **)&H->dense64, sizeof(double) * n2));
         TT_HIP(hipMemsetAsync(H->dense64, 0, sizeof(double) * n2, ctx->stream));
         hipLaunchKernelGGL(csr_to_dense_kernel<double>, dim3(rows), dim3(256), 0, ctx->stream, n,
                            ld, H->rowptr, H->col, H->val64, H->dense64);
         TT_LAUNCH_CHECK();
     } else if (H->val32) {
-        TT_HIP(hipMalloc(&H->dense32, sizeof(float) * n2));
+        TT_TRY(dev_alloc(ctx, (void **)&H->dense32, sizeof(float) * n2));
         TT_HIP(hipMemsetAsync(H->dense32, 0, sizeof(float) * n2, ctx->stream));
         hipLaunchKernelGGL(csr_to_dense_kernel<float>, dim3(rows), dim3(256), 0, ctx->stream, n,
                            ld, H->rowptr, H->col, H->val32, H->dense32);
@@ -528,7 +536,7 @@ int finish_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
     if (H->p > 0 && H->c_val64 && H->ct_val64) {
         const int64_t p = H->p;
         double *A1 = nullptr, *A2 = nullptr;  // D C as p x n rows, C^T as n x p rows
-        TT_HIP(hipMalloc(&A1, sizeof(double) * 2 * (size_t)p * n));
+        TT_TRY(dev_alloc(ctx, (void **)&A1, sizeof(double) * 2 * (size_t)p * n));
         A2 = A1 + p * n;
         int rc = TTDFT_OK;
         hipError_t e = hipMemsetAsync(A1, 0, sizeof(double) * 2 * (size_t)p * n, ctx->stream);
@@ -556,22 +564,18 @@ int finish_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
                 rc = TTDFT_ELIB;
             }
         }
-        hipError_t se = hipStreamSynchronize(ctx->stream);
-        hipError_t fe = hipFree(A1);
+        dev_release(ctx, A1);  // reused in stream order
         if (rc != TTDFT_OK) return rc;
-        TT_HIP(se);
-        TT_HIP(fe);
         H->dense_nl = true;
     }
     if (copies & TTDFT_HAM_F32) {
-        TT_HIP(hipMalloc(&H->dense32, sizeof(float) * n2));
+        TT_TRY(dev_alloc(ctx, (void **)&H->dense32, sizeof(float) * n2));
         hipLaunchKernelGGL(dense_to_f32_kernel, dim3(grid_blocks((int64_t)n2)), dim3(256), 0,
                            ctx->stream, H->dense64, H->dense32, (int64_t)n2);
         TT_LAUNCH_CHECK();
     }
     if (!(copies & TTDFT_HAM_F64)) {
-        TT_HIP(hipStreamSynchronize(ctx->stream));
-        TT_HIP(hipFree(H->dense64));
+        dev_release(ctx, H->dense64);
         H->dense64 = nullptr;
     }
     return TTDFT_OK;
@@ -592,12 +596,15 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
 }
 
 // H X on the dense storage with the recurrence epilogue in one kernel; the
-// state tiles per workgroup follow k (16, 32, 48 or 64 states)
+// state tiles per workgroup follow k (16 to 96 states)
 static int launch_dense_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                              const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
                              int32_t k, int dtype, double alpha, double beta, double delta,
                              int use_beta, int use_delta) {
-    const int jt = k >= 64 ? 4 : (k + 15) / 16;
+    // fewest workgroup columns of at most 96 states, then the fewest 16-state
+    // tiles that cover k in that many columns (each column streams H once)
+    const int ncol = (k + 95) / 96;
+    const int jt = (k + 16 * ncol - 1) / (16 * ncol);
 #define DENSE_CASE(JTV)                                                                         \
     case JTV:                                                                                   \
         if (dtype == TTDFT_F64)                                                                 \
@@ -612,6 +619,8 @@ static int launch_dense_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X,
         DENSE_CASE(2)
         DENSE_CASE(3)
         DENSE_CASE(4)
+        DENSE_CASE(5)
+        DENSE_CASE(6)
     default:
         break;
     }
