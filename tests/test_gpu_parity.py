@@ -458,7 +458,7 @@ def test_dense_storage_matches_oracle(ctx, tucker):
 
 
 @pytest.mark.parametrize("dims", [(4, 3, 5), (9, 8, 7), (11, 10, 13)])
-@pytest.mark.parametrize("k", [1, 8, 16, 24, 33, 64, 80, 96, 100, 150])
+@pytest.mark.parametrize("k", [1, 8, 16, 24, 33, 64, 80, 96, 100, 150, 192])
 def test_dense_product_tilings(ctx, dims, k):
     """The dense product over its state tilings (16 to 96 states per
     workgroup, one or two workgroup columns, partial tiles), leading dimension
@@ -470,9 +470,11 @@ def test_dense_product_tilings(ctx, dims, k):
     H = oracle.Ham.from_synth(s)
     ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=False)
     assert ham.format()[0] == 2
-    ld = k + 3
     print(f"dense product n={s.n} k={k}", flush=True)
-    for dtype, npdt, tol in ((E.F64, np.float64, 1e-13), (E.F32, np.float32, 1e-5)):
+    # an odd and an even leading dimension (96-state multiples with an even
+    # one take the six-consecutive-states layout)
+    for ld, dtype, npdt, tol in ((k + 3, E.F64, np.float64, 1e-13), (k + 3, E.F32, np.float32, 1e-5),
+                                 (k + 4, E.F64, np.float64, 1e-13), (k + 4, E.F32, np.float32, 1e-5)):
         X = np.random.default_rng(k).standard_normal((s.n, ld)).astype(npdt)
         dX, dY = dev(ctx, X, npdt), dev(ctx, np.full((s.n, ld), 7.0), npdt)
         ctx.ham_apply(ham, dX.ptr, ld, dY.ptr, ld, k, dtype)

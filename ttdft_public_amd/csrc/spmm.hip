@@ -269,13 +269,14 @@ template <typename T, int JT> struct DenseStage {
     T x[U][JT];
 };
 
-template <typename T, int JT>
+template <typename T, int JT, bool CONTIG>
 __device__ __forceinline__ void dense_load(DenseStage<T, JT> &st, const T *hrow, const T *X,
                                            int64_t ldx, int64_t n, const int *cidx, int64_t l0,
                                            int q) {
     constexpr int U = DenseCfg<JT>::U;
     // U consecutive entries (16-64 B), aligned by the padding
     typedef T vU __attribute__((ext_vector_type(U)));
+    typedef T v2 __attribute__((ext_vector_type(2)));
     const vU hv = *reinterpret_cast<const vU *>(hrow + l0);
 #pragma unroll
     for (int u = 0; u < U; ++u) st.h[u] = hv[u];
@@ -284,8 +285,19 @@ __device__ __forceinline__ void dense_load(DenseStage<T, JT> &st, const T *hrow,
         int64_t l = l0 + U * q + u;
         l = l < n ? l : n - 1;
         const T *xr = X + l * ldx;
+        if constexpr (CONTIG) {
+            // the lane's JT states are consecutive from cidx[0] (even, with
+            // ldx even): pairs in one load each
 #pragma unroll
-        for (int jt = 0; jt < JT; ++jt) st.x[u][jt] = xr[cidx[jt]];
+            for (int jt = 0; jt < JT; jt += 2) {
+                const v2 w = *reinterpret_cast<const v2 *>(xr + cidx[0] + jt);
+                st.x[u][jt] = w[0];
+                st.x[u][jt + 1] = w[1];
+            }
+        } else {
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt) st.x[u][jt] = xr[cidx[jt]];
+        }
     }
 }
 
@@ -300,7 +312,7 @@ __device__ __forceinline__ void dense_mma(
             acc[u % NA][jt] = DenseMma<T>::step(st.x[u][jt], st.h[u], acc[u % NA][jt]);
 }
 
-template <typename T, int JT>
+template <typename T, int JT, bool CONTIG>
 __global__ __launch_bounds__(512) void dense_hx_kernel(
     int64_t n, int64_t ldh, int32_t k, const T *__restrict__ H, const T *X, int64_t ldx,
     const int64_t *__restrict__ ct_rowptr, const int32_t *__restrict__ ct_col,
@@ -316,10 +328,12 @@ __global__ __launch_bounds__(512) void dense_hx_kernel(
     const int64_t part = ((ldh + kDenseWaves - 1) / kDenseWaves + SW - 1) / SW * SW;
     const int64_t lb = wave * part, le = lb + part < ldh ? lb + part : ldh;
     const T *hrow = H + (i0 + li) * ldh + U * q;
+    // state of tile jt, tile row i: c0 + 16 jt + i, or with CONTIG (every tile
+    // full, ldx even) c0 + JT i + jt, so a lane's JT states are consecutive
     int cidx[JT];
 #pragma unroll
     for (int jt = 0; jt < JT; ++jt) {
-        const int c = c0 + 16 * jt + li;
+        const int c = CONTIG ? c0 + JT * li + jt : c0 + 16 * jt + li;
         cidx[jt] = c < k ? c : k - 1;
     }
     acc_t acc[NA][JT];
@@ -334,15 +348,15 @@ __global__ __launch_bounds__(512) void dense_hx_kernel(
         // into the products)
         DenseStage<T, JT> sa, sb;
         const int64_t nsteps = (le - lb) / SW;
-        dense_load<T, JT>(sa, hrow, X, ldx, n, cidx, lb, q);
+        dense_load<T, JT, CONTIG>(sa, hrow, X, ldx, n, cidx, lb, q);
         int64_t st = 0;
         for (; st + 2 <= nsteps; st += 2) {
-            dense_load<T, JT>(sb, hrow, X, ldx, n, cidx, lb + SW * (st + 1), q);
+            dense_load<T, JT, CONTIG>(sb, hrow, X, ldx, n, cidx, lb + SW * (st + 1), q);
             __builtin_amdgcn_sched_barrier(0);
             dense_mma<T, JT>(sa, acc);
             __builtin_amdgcn_sched_barrier(0);
             const int64_t s2 = st + 2 < nsteps ? st + 2 : nsteps - 1;
-            dense_load<T, JT>(sa, hrow, X, ldx, n, cidx, lb + SW * s2, q);
+            dense_load<T, JT, CONTIG>(sa, hrow, X, ldx, n, cidx, lb + SW * s2, q);
             __builtin_amdgcn_sched_barrier(0);
             dense_mma<T, JT>(sb, acc);
             __builtin_amdgcn_sched_barrier(0);
@@ -359,7 +373,7 @@ __global__ __launch_bounds__(512) void dense_hx_kernel(
     for (int e = t; e < 16 * 16 * JT; e += 512) {
         const int j = e % (16 * JT), ii = e / (16 * JT);
         const int64_t r = i0 + ii;
-        const int c = c0 + j;
+        const int c = CONTIG ? c0 + JT * (j % 16) + j / 16 : c0 + j;
         if (r >= n || c >= k) continue;
         double s = 0.0;
 #pragma unroll
@@ -588,11 +602,19 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
                          double delta, int use_beta, int use_delta) {
     const int64_t n = H->n;
     const dim3 grid((unsigned)((n + 15) / 16), (unsigned)((k + 16 * JT - 1) / (16 * JT)));
-    hipLaunchKernelGGL((dense_hx_kernel<T, JT>), grid, dim3(64 * kDenseWaves), 0, ctx->stream, n,
-                       H->dense_ld, k, Hd, (const T *)X, ldx,
-                       H->p > 0 && !H->dense_nl ? H->ct_rowptr : nullptr, H->ct_col, ctv,
-                       (const T *)W, (T *)Y,
-                       ldy, (const T *)Z, ldz, alpha, beta, delta, use_beta, use_delta);
+    const int64_t *ctr = H->p > 0 && !H->dense_nl ? H->ct_rowptr : nullptr;
+    // six-state runs per lane when every tile is full and rows of X are 8- or
+    // 16-byte aligned
+    if (JT == 6 && k % (16 * JT) == 0 && ldx % 2 == 0)
+        hipLaunchKernelGGL((dense_hx_kernel<T, JT, (JT == 6)>), grid, dim3(64 * kDenseWaves), 0,
+                           ctx->stream, n, H->dense_ld, k, Hd, (const T *)X, ldx, ctr, H->ct_col,
+                           ctv, (const T *)W, (T *)Y, ldy, (const T *)Z, ldz, alpha, beta, delta,
+                           use_beta, use_delta);
+    else
+        hipLaunchKernelGGL((dense_hx_kernel<T, JT, false>), grid, dim3(64 * kDenseWaves), 0,
+                           ctx->stream, n, H->dense_ld, k, Hd, (const T *)X, ldx, ctr, H->ct_col,
+                           ctv, (const T *)W, (T *)Y, ldy, (const T *)Z, ldz, alpha, beta, delta,
+                           use_beta, use_delta);
 }
 
 // H X on the dense storage with the recurrence epilogue in one kernel; the
