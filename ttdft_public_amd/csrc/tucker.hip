@@ -37,47 +37,60 @@ namespace {
     } while (0)
 
 // Work buffers of STHOSVD and the Hartree expansion come from a cache of
-// blocks by (device, size): the SCF loop calls both with the same shapes every
-// iteration, and hipMalloc / hipFree of hundreds of MB (hipFree synchronises
-// the device) would otherwise dominate.  A returned block is reused in stream
-// order on the context's stream; the cache keeps at most 8 GiB per process.
+// blocks by (device, stream, size): the SCF loop calls both with the same
+// shapes every iteration, and hipMalloc / hipFree of hundreds of MB (hipFree
+// synchronises the device) would otherwise dominate.  A returned block is
+// reused only on the stream it was released from, in that stream's order (an
+// early error return may release blocks that queued work still uses); the
+// cache keeps at most 8 GiB per process.
+struct BlockKey {
+    int dev;
+    hipStream_t stream;
+    size_t sz;
+    bool operator==(const BlockKey &o) const {
+        return dev == o.dev && stream == o.stream && sz == o.sz;
+    }
+};
 struct BlockCache {
-    std::vector<std::pair<std::pair<int, size_t>, void *>> blocks;
+    std::vector<std::pair<BlockKey, void *>> blocks;
     size_t bytes = 0;
-    void *take(int dev, size_t sz) {
+    void *take(const BlockKey &k) {
         for (size_t i = 0; i < blocks.size(); ++i)
-            if (blocks[i].first.first == dev && blocks[i].first.second == sz) {
+            if (blocks[i].first == k) {
                 void *p = blocks[i].second;
                 blocks.erase(blocks.begin() + i);
-                bytes -= sz;
+                bytes -= k.sz;
                 return p;
             }
         return nullptr;
     }
-    void give(int dev, size_t sz, void *p) {
-        if (bytes + sz > (size_t(8) << 30)) {
+    void give(const BlockKey &k, void *p) {
+        if (bytes + k.sz > (size_t(8) << 30)) {
             hipError_t e = hipFree(p);
             (void)e;
             return;
         }
-        blocks.push_back({{dev, sz}, p});
-        bytes += sz;
+        blocks.push_back({k, p});
+        bytes += k.sz;
     }
 };
 BlockCache g_blocks;
+// the stream of the entry point running on this thread (set on entry)
+thread_local hipStream_t g_stream = nullptr;
 
 struct Dev {
     void *p = nullptr;
-    size_t sz = 0;
-    int dev = 0;
+    BlockKey key{0, nullptr, 0};
     ~Dev() {
-        if (p) g_blocks.give(dev, sz, p);
+        if (p) g_blocks.give(key, p);
     }
     int alloc(size_t bytes) {
-        sz = (std::max<size_t>(bytes, 8) + 255) / 256 * 256;
-        hipError_t e = hipGetDevice(&dev);
-        if (e != hipSuccess) dev = 0;
-        p = g_blocks.take(dev, sz);
+        key.sz = (std::max<size_t>(bytes, 8) + 255) / 256 * 256;
+        key.stream = g_stream;
+        hipError_t e = hipGetDevice(&key.dev);
+        if (e != hipSuccess) key.dev = 0;
+        const size_t sz = key.sz;
+        p = g_blocks.take(key);
         if (p) return TTDFT_OK;
         e = hipMalloc(&p, sz);
         if (e != hipSuccess) {
@@ -323,6 +336,7 @@ int ttdft_sthosvd(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n1, int3
                   int32_t r0, int32_t r1, int32_t r2, double *d_core, double *d_U0, double *d_U1,
                   double *d_U2, double *evals) {
     clear_error();
+    if (ctx) g_stream = ctx->stream;
     return sthosvd_impl(ctx, d_T, n0, n1, n2, r0, r1, r2, d_core, d_U0, d_U1, d_U2, evals, 0);
 }
 
@@ -330,6 +344,7 @@ int ttdft_sthosvd_refine(ttdft_ctx ctx, const double *d_T, int32_t n0, int32_t n
                          int32_t r0, int32_t r1, int32_t r2, double *d_core, double *d_U0,
                          double *d_U1, double *d_U2, int32_t iters, double *evals) {
     clear_error();
+    if (ctx) g_stream = ctx->stream;
     TT_CHECK(iters >= 1, TTDFT_EBADARG, "sthosvd_refine: iters must be >= 1");
     return sthosvd_impl(ctx, d_T, n0, n1, n2, r0, r1, r2, d_core, d_U0, d_U1, d_U2, evals,
                         iters);
@@ -341,6 +356,7 @@ int ttdft_hartree_expansion(ttdft_ctx ctx, const int32_t r[3], const double *d_c
                             const int32_t p[3], const double *const d_xq[3], int32_t nterms,
                             const double *alpha, const double *omega, double *d_V, int add) {
     clear_error();
+    if (ctx) g_stream = ctx->stream;
     TT_CHECK(ctx && r && d_core && d_U && ne && d_xe && d_jw && p && d_xq && alpha && omega &&
                  d_V,
              TTDFT_EBADARG, "hartree_expansion: NULL argument");
