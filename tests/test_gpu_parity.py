@@ -482,3 +482,21 @@ def test_dense_product_tilings(ctx, dims, k):
         ref = oracle.ham_apply(H, X[:, :k].astype(np.float64))
         assert rel(Y[:, :k], ref) < tol
         assert np.all(Y[:, k:] == 7.0)  # columns past k untouched
+
+
+def test_dense_storage_reuse_across_hamiltonians(ctx):
+    """Hamiltonians of different sizes built and freed in turn on one context:
+    the dense storage and the projector fold come from the context's block
+    cache (a freed block of the same size class is reused), and every product
+    still matches the oracle."""
+    for dims in [(9, 8, 7), (4, 3, 5), (11, 10, 13), (9, 8, 7), (8, 8, 8), (4, 3, 5)]:
+        s = E.SynthHam(*dims, dims[0] - 1, dims[1] - 1, dims[2] - 1, 5, seed=sum(dims))
+        H = oracle.Ham.from_synth(s)
+        ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=False)
+        assert ham.format()[0] == 2
+        X = np.random.default_rng(sum(dims)).standard_normal((s.n, 40))
+        dX, dY = dev(ctx, X), E.DeviceArray(ctx, (s.n, 40))
+        print(f"reuse n={s.n}", flush=True)
+        ctx.ham_apply(ham, dX.ptr, 40, dY.ptr, 40, 40, E.F64)
+        assert rel(dY.to_host(), oracle.ham_apply(H, X)) < 1e-13
+        ham.free()
