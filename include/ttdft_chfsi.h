@@ -93,10 +93,16 @@ int ttdft_get_device_count(int *count);
  * replaces device_set_device (DeviceUtils.cu:30-31), the device binding of
  * DeviceCommUtils (DeviceCommUtils.cc:69-71) and MultTM::MultTM (MultTm.cu:31-84) */
 int ttdft_ctx_create(int device, ttdft_ctx *out);
-/* replaces MultTM::~MultTM (MultTm.cu:469-473) */
+/* Ends the context.  Hamiltonians and mixers created on it keep it alive:
+ * with any of them not yet freed, the call synchronises the stream and
+ * returns, and the last ttdft_ham_free / ttdft_mixer_destroy completes the
+ * destruction.  No other call may use the context after this one.
+ * replaces MultTM::~MultTM (MultTm.cu:469-473) */
 int ttdft_ctx_destroy(ttdft_ctx ctx);
 /* Make the context issue its work on an existing HIP stream (e.g. the
- * caller's); NULL restores the context's own stream. */
+ * caller's); NULL restores the context's own stream.  The outgoing stream is
+ * synchronised first, since the context's scratch and released storage
+ * blocks are reused in the order of one stream. */
 int ttdft_ctx_set_stream(ttdft_ctx ctx, void *hip_stream);
 int ttdft_ctx_synchronize(ttdft_ctx ctx);
 /* Enable per-launch HIP-event timing of the kernel classes TTDFT_KSTAT_*. */

@@ -61,6 +61,10 @@ int Scratch::ensure(size_t need) {
         bytes = 0;
     }
     hipError_t e = hipMalloc(&ptr, need);
+    if (e != hipSuccess && owner && !owner->blk_free.empty()) {
+        dev_cache_clear(owner);
+        e = hipMalloc(&ptr, need);
+    }
     if (e != hipSuccess) {
         ptr = nullptr;
         set_error("scratch allocation of %zu bytes failed: %s", need, hipGetErrorString(e));
@@ -238,12 +242,7 @@ static void spline_build(int n, const double *x, const double *y, double *coef) 
 static int upload(ttdft_ctx ctx, void **dptr, const void *src, size_t bytes) {
     *dptr = nullptr;
     if (bytes == 0) return TTDFT_OK;
-    hipError_t e = hipMalloc(dptr, bytes);
-    if (e != hipSuccess) {
-        *dptr = nullptr;
-        set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-        return TTDFT_ENOMEM;
-    }
+    TT_TRY(dev_alloc(ctx, dptr, bytes));
     TT_HIP(hipMemcpyAsync(*dptr, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     return TTDFT_OK;
 }
@@ -319,6 +318,29 @@ void dev_cache_clear(ttdft_ctx ctx) {
     ctx->blk_free_bytes = 0;
 }
 
+int dev_malloc_retry(ttdft_ctx ctx, void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+    if (e != hipSuccess && ctx && !ctx->blk_free.empty()) {
+        dev_cache_clear(ctx);
+        e = hipMalloc(p, bytes ? bytes : 8);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return TTDFT_ENOMEM;
+    }
+    return TTDFT_OK;
+}
+
+void ctx_attach(ttdft_ctx ctx) { ctx->live_children++; }
+
+void ctx_detach(ttdft_ctx ctx) {
+    if (--ctx->live_children == 0 && ctx->destroy_pending) {
+        ctx->destroy_pending = false;
+        ttdft_ctx_destroy(ctx);
+    }
+}
+
 static int orth_once(ttdft_ctx ctx, void *X, int64_t ldx, int64_t n, int32_t k, int dtype,
                      int mode) {
     const int64_t kk = (int64_t)k * k;
@@ -380,6 +402,8 @@ int ttdft_ctx_create(int device, ttdft_ctx *out) {
     ttdft_ctx ctx = new ttdft_ctx_s();
     ctx->device = device;
     ctx->pending = new std::vector<PendingTiming>();
+    for (auto &s : ctx->scratch) s.owner = ctx;
+    for (auto &s : ctx->nn_scratch) s.owner = ctx;
     hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete ctx->pending;
@@ -405,6 +429,7 @@ int ttdft_ctx_create(int device, ttdft_ctx *out) {
 
 int ttdft_ctx_destroy(ttdft_ctx ctx) {
     if (!ctx) return TTDFT_OK;
+    if (ctx->destroy_pending) return TTDFT_OK;  // already handed to the last child
     hipError_t e = hipSetDevice(ctx->device);
     (void)e;
     if (ctx->stream) {
@@ -412,6 +437,13 @@ int ttdft_ctx_destroy(ttdft_ctx ctx) {
         (void)e;
     }
     timing_flush(ctx);
+    if (ctx->live_children > 0) {
+        // Hamiltonians / mixers still hold this context (their storage goes
+        // back to its block cache): the last of them to be freed destroys it
+        ctx->destroy_pending = true;
+        return TTDFT_OK;
+    }
+    free_tucker_blocks(ctx->device, ctx->stream);
     dev_cache_clear(ctx);
     for (auto &s : ctx->scratch) s.release();
     for (auto &s : ctx->nn_scratch) s.release();
@@ -428,7 +460,12 @@ int ttdft_ctx_destroy(ttdft_ctx ctx) {
 
 int ttdft_ctx_set_stream(ttdft_ctx ctx, void *hip_stream) {
     TT_CHECK(ctx, TTDFT_EBADARG, "ctx is NULL");
-    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    hipStream_t next = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
+    if (next == ctx->stream) return TTDFT_OK;
+    // Released cache blocks and scratch buffers are reused in the order of
+    // one stream: finish what the outgoing stream still has queued on them.
+    TT_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stream = next;
     TT_CHECK(rocblas_set_stream(ctx->blas, ctx->stream) == rocblas_status_success, TTDFT_ELIB,
              "rocblas_set_stream failed");
     return TTDFT_OK;
@@ -564,6 +601,7 @@ int ttdft_ham_upload(ttdft_ctx ctx, int64_t n, const int64_t *rowptr, const int3
     TT_HIP(hipSetDevice(ctx->device));
     ttdft_ham H = new ttdft_ham_s();
     H->ctx = ctx;
+    ctx_attach(ctx);
     H->n = n;
     H->nnz = nnz;
     H->p = p;
@@ -696,7 +734,9 @@ int ttdft_ham_free(ttdft_ham H) {
     dev_release(H->ctx, H->ct_col);
     dev_release(H->ctx, H->ct_val64);
     dev_release(H->ctx, H->ct_val32);
+    ttdft_ctx ctx = H->ctx;
     delete H;
+    ctx_detach(ctx);
     return TTDFT_OK;
 }
 
@@ -1026,14 +1066,13 @@ int ttdft_chfsi_step(ttdft_ctx ctx, ttdft_ham H, void *X, int64_t ldx, int32_t k
     }
     const int64_t kk = (int64_t)k * k;
     const size_t es = dtype_size(dtype);
-    // persistent per-call work: HX block, S, Q, eigenvalues
+    // per-call work (HX block, S, Q, eigenvalues) from the context's block
+    // cache: after the first step of a shape no runtime allocation happens
     void *HX = nullptr;
     double *dS = nullptr;
-    TT_HIP(hipMalloc(&HX, (size_t)n * k * es));
-    hipError_t e = hipMalloc(&dS, sizeof(double) * (2 * kk + k));
-    if (e != hipSuccess) {
-        dfree(HX);
-        set_error("chfsi_step: hipMalloc: %s", hipGetErrorString(e));
+    TT_TRY(dev_alloc(ctx, &HX, (size_t)n * k * es));
+    if (dev_alloc(ctx, (void **)&dS, sizeof(double) * (2 * kk + k)) != TTDFT_OK) {
+        dev_release(ctx, HX);
         return TTDFT_ENOMEM;
     }
     double *dQ = dS + kk, *dEv = dQ + kk;
@@ -1078,14 +1117,9 @@ int ttdft_chfsi_step(ttdft_ctx ctx, ttdft_ham H, void *X, int64_t ldx, int32_t k
             rc = TTDFT_EHIP;
         }
     }
-    std::string msg = g_err;
-    {
-        hipError_t se = hipStreamSynchronize(ctx->stream);
-        (void)se;
-    }
-    dfree(HX);
-    dfree(dS);
-    g_err = msg;
+    // back to the cache in stream order (an early error may leave work queued)
+    dev_release(ctx, HX);
+    dev_release(ctx, dS);
     return rc;
 }
 

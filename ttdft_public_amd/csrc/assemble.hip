@@ -1022,6 +1022,7 @@ int ttdft_ham_assemble_tucker(ttdft_ctx ctx, int32_t rx, int32_t ry, int32_t rz,
     TT_HIP(hipSetDevice(ctx->device));
     ttdft_ham H = new ttdft_ham_s();
     H->ctx = ctx;
+    ctx_attach(ctx);
     const int rc = ham_assemble(ctx, a, p, ranks, cores, mx, my, mz, c_truncation, inv_c_lm,
                                 copies, H);
     if (rc != TTDFT_OK) {

@@ -705,7 +705,7 @@ int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz
         for (auto &x : th) x.join();
     }
     if (copies & TTDFT_HAM_F64) {
-        TT_HIP(hipMalloc(&H->mf64, sizeof(double) * mf.size()));
+        TT_TRY(dev_alloc(ctx, (void **)&H->mf64, sizeof(double) * mf.size()));
         TT_HIP(hipMemcpyAsync(H->mf64, mf.data(), sizeof(double) * mf.size(),
                               hipMemcpyHostToDevice, ctx->stream));
         TT_HIP(hipStreamSynchronize(ctx->stream));
@@ -713,7 +713,7 @@ int build_band(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int32_t rz
     if (copies & TTDFT_HAM_F32) {
         std::vector<float> m32(mf.size());
         for (size_t q = 0; q < mf.size(); ++q) m32[q] = (float)mf[q];
-        TT_HIP(hipMalloc(&H->mf32, sizeof(float) * m32.size()));
+        TT_TRY(dev_alloc(ctx, (void **)&H->mf32, sizeof(float) * m32.size()));
         TT_HIP(hipMemcpyAsync(H->mf32, m32.data(), sizeof(float) * m32.size(),
                               hipMemcpyHostToDevice, ctx->stream));
         TT_HIP(hipStreamSynchronize(ctx->stream));
@@ -729,7 +729,7 @@ int build_band_device(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int
     if (!band_shape(rx, ry, rz, bx, by, bz, H->nnz, &b)) return TTDFT_OK;  // keep CSR
     TT_CHECK(H->val64, TTDFT_EBADARG, "build_band_device: FP64 CSR values missing");
     double *mf = nullptr;
-    TT_HIP(hipMalloc(&mf, sizeof(double) * b.slots));
+    TT_TRY(dev_alloc(ctx, (void **)&mf, sizeof(double) * b.slots));
     H->mf64 = mf;  // owned by H from here on
     TT_HIP(hipMemsetAsync(mf, 0, sizeof(double) * b.slots, ctx->stream));
     const unsigned nb = (unsigned)((n + 3) / 4);
@@ -738,14 +738,14 @@ int build_band_device(ttdft_ctx ctx, ttdft_ham_s *H, int32_t rx, int32_t ry, int
                        mf);
     TT_LAUNCH_CHECK();
     if (copies & TTDFT_HAM_F32) {
-        TT_HIP(hipMalloc(&H->mf32, sizeof(float) * b.slots));
+        TT_TRY(dev_alloc(ctx, (void **)&H->mf32, sizeof(float) * b.slots));
         hipLaunchKernelGGL(f64_to_f32_kernel, dim3((unsigned)((b.slots + 255) / 256)),
                            dim3(256), 0, ctx->stream, mf, H->mf32, b.slots);
         TT_LAUNCH_CHECK();
     }
     if (!(copies & TTDFT_HAM_F64)) {
-        TT_HIP(hipStreamSynchronize(ctx->stream));
-        TT_HIP(hipFree(mf));
+        // reused by later work on this stream only, after the conversion
+        dev_release(ctx, mf);
         H->mf64 = nullptr;
     }
     set_band_fields(H, rx, ry, rz, bx, by, bz, b);

@@ -55,9 +55,12 @@ void clear_error();
     } while (0)
 
 // ---- grow-only device scratch buffer ---------------------------------------
+// owner: the context whose released blocks are given back to the device
+// (dev_cache_clear) and the allocation retried when hipMalloc fails.
 struct Scratch {
     void *ptr = nullptr;
     size_t bytes = 0;
+    ttdft_ctx owner = nullptr;
     int ensure(size_t need);
     void release();
 };
@@ -99,6 +102,11 @@ struct ttdft_ctx_s {
     std::unordered_map<void *, size_t> blk_size;
     std::multimap<size_t, void *> blk_free;
     size_t blk_free_bytes = 0;
+    // Hamiltonians and mixers created on this context and not yet freed.
+    // ttdft_ctx_destroy with any alive only marks the context; the last
+    // ttdft_ham_free / ttdft_mixer_destroy then destroys it.
+    int live_children = 0;
+    bool destroy_pending = false;
 };
 
 struct ttdft_ham_s {
@@ -186,6 +194,15 @@ int finish_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies);
 int dev_alloc(ttdft_ctx ctx, void **p, size_t bytes);
 void dev_release(ttdft_ctx ctx, void *p);
 void dev_cache_clear(ttdft_ctx ctx);
+// hipMalloc that, when it fails, gives the context's released blocks back to
+// the device and tries once more (plain, uncached memory: free with hipFree)
+int dev_malloc_retry(ttdft_ctx ctx, void **p, size_t bytes);
+// A Hamiltonian or mixer was created on / released from ctx (the release
+// may complete a deferred ttdft_ctx_destroy, after which ctx is gone).
+void ctx_attach(ttdft_ctx ctx);
+void ctx_detach(ttdft_ctx ctx);
+// Frees the STHOSVD / Hartree work blocks released on (device, stream).
+void free_tucker_blocks(int device, hipStream_t stream);
 void drop_dense(ttdft_ham_s *H);
 int launch_spmm_fused(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                       const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,

@@ -280,7 +280,7 @@ int mixing_constants(ttdft_mixer_s *M) {
         if (M->partial) TT_HIP(hipFree(M->partial));
         M->partial = nullptr;
         M->partial_bytes = 0;
-        TT_HIP(hipMalloc(&M->partial, need));
+        TT_TRY(dev_malloc_retry(ctx, (void **)&M->partial, need));
         M->partial_bytes = need;
     }
     PendingTiming tm;
@@ -342,6 +342,7 @@ int ttdft_mixer_create(ttdft_ctx ctx, int64_t n_grid, int64_t n_nodal, const dou
     TT_HIP(hipSetDevice(ctx->device));
     ttdft_mixer M = new ttdft_mixer_s();
     M->ctx = ctx;
+    ctx_attach(ctx);
     M->n_grid = n_grid;
     M->n_nodal = n_nodal;
     M->alpha = alpha;
@@ -377,7 +378,9 @@ int ttdft_mixer_destroy(ttdft_mixer M) {
             e = hipFree(p);
             (void)e;
         }
+    ttdft_ctx ctx = M->ctx;
     delete M;
+    ctx_detach(ctx);
     return TTDFT_OK;
 }
 
@@ -397,14 +400,14 @@ int ttdft_mixer_update(ttdft_mixer M, const double *d_rho_nodal_in, const double
         M->spare.pop_back();
     } else {
         const size_t g = sizeof(double) * M->n_grid, nd = sizeof(double) * M->n_nodal;
-        hipError_t e = hipMalloc(&en.grid_in, g);
-        if (e == hipSuccess) e = hipMalloc(&en.grid_out, g);
-        if (e == hipSuccess && nd) e = hipMalloc(&en.nodal_in, nd);
-        if (e == hipSuccess && nd) e = hipMalloc(&en.nodal_out, nd);
-        if (e != hipSuccess) {
+        int rc = dev_malloc_retry(ctx, (void **)&en.grid_in, g);
+        if (rc == TTDFT_OK) rc = dev_malloc_retry(ctx, (void **)&en.grid_out, g);
+        if (rc == TTDFT_OK && nd) rc = dev_malloc_retry(ctx, (void **)&en.nodal_in, nd);
+        if (rc == TTDFT_OK && nd) rc = dev_malloc_retry(ctx, (void **)&en.nodal_out, nd);
+        if (rc != TTDFT_OK) {
+            const std::string msg = ttdft_last_error();
             free_entry(en);
-            set_error("mixer_update: hipMalloc of a history entry failed: %s",
-                      hipGetErrorString(e));
+            set_error("mixer_update: history entry: %s", msg.c_str());
             return TTDFT_ENOMEM;
         }
     }

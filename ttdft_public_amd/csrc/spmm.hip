@@ -603,9 +603,11 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
     const int64_t n = H->n;
     const dim3 grid((unsigned)((n + 15) / 16), (unsigned)((k + 16 * JT - 1) / (16 * JT)));
     const int64_t *ctr = H->p > 0 && !H->dense_nl ? H->ct_rowptr : nullptr;
-    // six-state runs per lane when every tile is full and rows of X are 8- or
-    // 16-byte aligned
-    if (JT == 6 && k % (16 * JT) == 0 && ldx % 2 == 0)
+    // six-state runs per lane, fetched as pairs, when every tile is full and
+    // every row of X starts on a pair boundary (even leading dimension and a
+    // pair-aligned X: a filter block may start at an odd column)
+    if (JT == 6 && k % (16 * JT) == 0 && ldx % 2 == 0 &&
+        reinterpret_cast<uintptr_t>(X) % (2 * sizeof(T)) == 0)
         hipLaunchKernelGGL((dense_hx_kernel<T, JT, (JT == 6)>), grid, dim3(64 * kDenseWaves), 0,
                            ctx->stream, n, H->dense_ld, k, Hd, (const T *)X, ldx, ctr, H->ct_col,
                            ctv, (const T *)W, (T *)Y, ldy, (const T *)Z, ldz, alpha, beta, delta,

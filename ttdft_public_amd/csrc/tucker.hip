@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -52,9 +53,11 @@ struct BlockKey {
     }
 };
 struct BlockCache {
+    std::mutex mu;  // contexts may run on several host threads
     std::vector<std::pair<BlockKey, void *>> blocks;
     size_t bytes = 0;
     void *take(const BlockKey &k) {
+        std::lock_guard<std::mutex> lk(mu);
         for (size_t i = 0; i < blocks.size(); ++i)
             if (blocks[i].first == k) {
                 void *p = blocks[i].second;
@@ -65,13 +68,38 @@ struct BlockCache {
         return nullptr;
     }
     void give(const BlockKey &k, void *p) {
-        if (bytes + k.sz > (size_t(8) << 30)) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (bytes + k.sz <= (size_t(8) << 30)) {
+                blocks.push_back({k, p});
+                bytes += k.sz;
+                return;
+            }
+        }
+        hipError_t e = hipFree(p);
+        (void)e;
+    }
+    // frees every block released on (dev, stream); the caller has
+    // synchronised that stream
+    void drop(int dev, hipStream_t stream) {
+        std::vector<void *> gone;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            size_t w = 0;
+            for (size_t i = 0; i < blocks.size(); ++i) {
+                if (blocks[i].first.dev == dev && blocks[i].first.stream == stream) {
+                    gone.push_back(blocks[i].second);
+                    bytes -= blocks[i].first.sz;
+                } else {
+                    blocks[w++] = blocks[i];
+                }
+            }
+            blocks.resize(w);
+        }
+        for (void *p : gone) {
             hipError_t e = hipFree(p);
             (void)e;
-            return;
         }
-        blocks.push_back({k, p});
-        bytes += k.sz;
     }
 };
 BlockCache g_blocks;
@@ -267,6 +295,9 @@ int mode_factors(ttdft_ctx ctx, double *G, int n, int r, int iters, double *U,
 }
 
 }  // namespace
+
+void free_tucker_blocks(int device, hipStream_t stream) { g_blocks.drop(device, stream); }
+
 }  // namespace ttdft
 
 using namespace ttdft;
