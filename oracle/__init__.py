@@ -69,88 +69,110 @@ def build() -> str:
     return _LIB_PATH
 
 
+_NATIVE_PATH = os.path.join(_HERE, "liborc_chfsi_native.so")
+
+
+def use_native_build(timeout=300) -> bool:
+    """Rebuild the oracle for this host (-O3 -march=native, OpenMP) and route
+    every later call through that build.  For timing the CPU baseline only:
+    parity checks use the default build.  False if the build failed (the
+    default build stays in use)."""
+    global _lib
+    try:
+        subprocess.run(["make", "-s", "-B", "-C", _HERE, "native"], check=True, timeout=timeout,
+                       capture_output=True)
+    except (subprocess.SubprocessError, OSError):
+        return False
+    _lib = _bind(ctypes.CDLL(_NATIVE_PATH))
+    return True
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.orc_csr_spmm.argtypes = [ctypes.c_int64, _i64p, _i32p, _dp, _dp, ctypes.c_int64,
-                                   ctypes.c_int, _dp, ctypes.c_int64]
-        L.orc_ham_apply.argtypes = [ctypes.POINTER(OrcHam), _dp, ctypes.c_int64, ctypes.c_int,
-                                    _dp, ctypes.c_int64]
-        L.orc_cheb_filter.argtypes = [ctypes.POINTER(OrcHam), _dp, ctypes.c_int64, ctypes.c_int,
-                                      ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                      ctypes.c_double, ctypes.c_int, ctypes.c_int]
-        L.orc_gram.argtypes = [ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int64, _dp,
-                               ctypes.c_int64, _dp]
-        L.orc_cholesky_lower.argtypes = [ctypes.c_int, _dp]
-        L.orc_orth.argtypes = [ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int64]
-        L.orc_syev.argtypes = [ctypes.c_int, _dp, _dp, _dp]
-        L.orc_rotate.argtypes = [ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int64, _dp]
-        L.orc_lanczos_bounds.argtypes = [ctypes.POINTER(OrcHam), ctypes.c_int, ctypes.c_uint64,
-                                         _dp, _dp]
-        L.orc_fermi_energy.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                       ctypes.c_double, ctypes.c_int]
-        L.orc_fermi_energy.restype = ctypes.c_double
-        L.orc_occupancy.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                    ctypes.c_double, _dp]
-        L.orc_band_energy.argtypes = [_dp, _dp, ctypes.c_int]
-        L.orc_band_energy.restype = ctypes.c_double
-        L.orc_density.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int,
-                                  ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
-                                  ctypes.c_int, ctypes.c_int, _dp]
-        L.orc_spline_build.argtypes = [ctypes.c_int, _dp, _dp, _dp]
-        L.orc_spline_eval.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double]
-        L.orc_spline_eval.restype = ctypes.c_double
-        L.orc_initial_density.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
-                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
-                                          _dp, _dp]
-        L.orc_grid_integral.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
-                                        _dp, _dp]
-        L.orc_grid_integral.restype = ctypes.c_double
-        L.orc_chfsi_step.argtypes = [ctypes.POINTER(OrcHam), _dp, ctypes.c_int64, ctypes.c_int,
-                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
-                                     ctypes.c_int, _dp, ctypes.POINTER(OrcChfsiState)]
-        L.orc_assemble_hloc.argtypes = [ctypes.c_int] * 6 + [_dp] * 10 + [ctypes.c_double,
-                                                                           _i64p, _i32p, _dp]
-        L.orc_assemble_hloc.restype = ctypes.c_int64
-        L.orc_assemble_hloc_rows.argtypes = ([ctypes.c_int] * 6 + [_dp] * 10
-                                             + [ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
-                                                _i64p, _i32p, _dp])
-        L.orc_assemble_hloc_rows.restype = ctypes.c_int64
-        L.orc_assemble_projectors.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int,
-                                              ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double,
-                                              _i64p, _i32p, _dp]
-        L.orc_assemble_projectors.restype = ctypes.c_int64
-        L.orc_anderson_constants.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp,
-                                             _dp]
-        L.orc_anderson_constants.restype = ctypes.c_int
-        L.orc_anderson_mix.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp,
-                                       ctypes.c_double, _dp]
-        L.orc_anderson_mix.restype = None
-        L.orc_sthosvd.argtypes = [ctypes.c_int] * 6 + [_dp] * 6
-        L.orc_sthosvd.restype = ctypes.c_int
-        _dpa = ctypes.POINTER(_dp)
-        _ip = ctypes.POINTER(ctypes.c_int)
-        L.orc_hartree_expansion.argtypes = [ctypes.c_int] * 3 + [_dp, _dpa, _ip, _dpa, _dpa, _ip,
-                                                                 _dpa, ctypes.c_int, _dp, _dp,
-                                                                 _dp, ctypes.c_int]
-        L.orc_hartree_expansion.restype = None
-        L.orc_fem_mesh.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 5 + [_dp]
-        L.orc_fem_mesh.restype = ctypes.c_int64
-        L.orc_fem_shape.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
-        L.orc_fem_shape.restype = None
-        L.orc_fem_interpolate.argtypes = [ctypes.c_int] * 3 + [_dp] * 3 + [ctypes.c_int] + [_dp] * 6
-        L.orc_fem_interpolate.restype = None
-        L.orc_tucker_basis_matrices.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 4 + [
-            ctypes.c_int] + [_dp] * 4
-        L.orc_tucker_basis_matrices.restype = None
-        L.orc_rng_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
-        L.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def _bind(L):
+    """Declare the argument types of the oracle's C functions on library L."""
+    L.orc_csr_spmm.argtypes = [ctypes.c_int64, _i64p, _i32p, _dp, _dp, ctypes.c_int64,
+                               ctypes.c_int, _dp, ctypes.c_int64]
+    L.orc_ham_apply.argtypes = [ctypes.POINTER(OrcHam), _dp, ctypes.c_int64, ctypes.c_int,
+                                _dp, ctypes.c_int64]
+    L.orc_cheb_filter.argtypes = [ctypes.POINTER(OrcHam), _dp, ctypes.c_int64, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.orc_gram.argtypes = [ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int64, _dp,
+                           ctypes.c_int64, _dp]
+    L.orc_cholesky_lower.argtypes = [ctypes.c_int, _dp]
+    L.orc_orth.argtypes = [ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int64]
+    L.orc_syev.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+    L.orc_rotate.argtypes = [ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int64, _dp]
+    L.orc_lanczos_bounds.argtypes = [ctypes.POINTER(OrcHam), ctypes.c_int, ctypes.c_uint64,
+                                     _dp, _dp]
+    L.orc_fermi_energy.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int]
+    L.orc_fermi_energy.restype = ctypes.c_double
+    L.orc_occupancy.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, _dp]
+    L.orc_band_energy.argtypes = [_dp, _dp, ctypes.c_int]
+    L.orc_band_energy.restype = ctypes.c_double
+    L.orc_density.argtypes = [_dp, ctypes.c_int64, ctypes.c_int, _dp, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, _dp]
+    L.orc_spline_build.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+    L.orc_spline_eval.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double]
+    L.orc_spline_eval.restype = ctypes.c_double
+    L.orc_initial_density.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                      _dp, _dp]
+    L.orc_grid_integral.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                    _dp, _dp]
+    L.orc_grid_integral.restype = ctypes.c_double
+    L.orc_chfsi_step.argtypes = [ctypes.POINTER(OrcHam), _dp, ctypes.c_int64, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                 ctypes.c_int, _dp, ctypes.POINTER(OrcChfsiState)]
+    L.orc_assemble_hloc.argtypes = [ctypes.c_int] * 6 + [_dp] * 10 + [ctypes.c_double,
+                                                                       _i64p, _i32p, _dp]
+    L.orc_assemble_hloc.restype = ctypes.c_int64
+    L.orc_assemble_hloc_rows.argtypes = ([ctypes.c_int] * 6 + [_dp] * 10
+                                         + [ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
+                                            _i64p, _i32p, _dp])
+    L.orc_assemble_hloc_rows.restype = ctypes.c_int64
+    L.orc_assemble_projectors.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                          _i64p, _i32p, _dp]
+    L.orc_assemble_projectors.restype = ctypes.c_int64
+    L.orc_anderson_constants.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp,
+                                         _dp]
+    L.orc_anderson_constants.restype = ctypes.c_int
+    L.orc_anderson_mix.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp,
+                                   ctypes.c_double, _dp]
+    L.orc_anderson_mix.restype = None
+    L.orc_sthosvd.argtypes = [ctypes.c_int] * 6 + [_dp] * 6
+    L.orc_sthosvd.restype = ctypes.c_int
+    _dpa = ctypes.POINTER(_dp)
+    _ip = ctypes.POINTER(ctypes.c_int)
+    L.orc_hartree_expansion.argtypes = [ctypes.c_int] * 3 + [_dp, _dpa, _ip, _dpa, _dpa, _ip,
+                                                             _dpa, ctypes.c_int, _dp, _dp,
+                                                             _dp, ctypes.c_int]
+    L.orc_hartree_expansion.restype = None
+    L.orc_fem_mesh.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 5 + [_dp]
+    L.orc_fem_mesh.restype = ctypes.c_int64
+    L.orc_fem_shape.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+    L.orc_fem_shape.restype = None
+    L.orc_fem_interpolate.argtypes = [ctypes.c_int] * 3 + [_dp] * 3 + [ctypes.c_int] + [_dp] * 6
+    L.orc_fem_interpolate.restype = None
+    L.orc_tucker_basis_matrices.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 4 + [
+        ctypes.c_int] + [_dp] * 4
+    L.orc_tucker_basis_matrices.restype = None
+    L.orc_rng_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
+    L.orc_rng_normal.argtypes = [ctypes.c_uint64, ctypes.c_int64, _dp]
+    return L
 
 
 def _d(a):

@@ -205,15 +205,41 @@ int orc_cheb_filter(const orc_ham *H, double *X, int64_t ldx, int k, int m,
 /* ------------------------------------------------------------------------ */
 void orc_gram(int64_t n, int k, const double *X, int64_t ldx, const double *Y,
               int64_t ldy, double *S) {
-    /* MultTM::mult: S = X^T Y (dgemm T,N), eigensolver/MultTm.cu:177-190 */
+    /* MultTM::mult: S = X^T Y (dgemm T,N), eigensolver/MultTm.cu:177-190.
+     * The basis rows go in a fixed number of chunks, each summed into its own
+     * k x k partial (64 rows at a time, 8 states of X against a whole row of
+     * Y, so the partial rows stay in cache); the partials are added in chunk
+     * order, so the result does not depend on the thread count. */
+    enum { NCH = 64, RB = 64, AB = 8 };
+    const size_t kk = (size_t)k * k;
+    double *P = (double *)calloc((size_t)NCH * kk, sizeof(double));
 #pragma omp parallel for schedule(dynamic)
-    for (int bcol = 0; bcol < k; ++bcol) {
-        for (int arow = 0; arow < k; ++arow) {
-            double s = 0.0;
-            for (int64_t i = 0; i < n; ++i) s += X[i * ldx + arow] * Y[i * ldy + bcol];
-            S[arow + (int64_t)k * bcol] = s;
+    for (int c = 0; c < NCH; ++c) {
+        double *Pc = P + (size_t)c * kk; /* Pc[a k + b] */
+        const int64_t c0 = n * c / NCH, c1 = n * (c + 1) / NCH;
+        for (int64_t i0 = c0; i0 < c1; i0 += RB) {
+            const int64_t i1 = i0 + RB < c1 ? i0 + RB : c1;
+            for (int a0 = 0; a0 < k; a0 += AB) {
+                const int na = a0 + AB <= k ? AB : k - a0;
+                for (int64_t i = i0; i < i1; ++i) {
+                    const double *x = X + i * ldx + a0, *y = Y + i * ldy;
+                    for (int u = 0; u < na; ++u) {
+                        const double xa = x[u];
+                        double *row = Pc + (size_t)(a0 + u) * k;
+                        for (int b = 0; b < k; ++b) row[b] += xa * y[b];
+                    }
+                }
+            }
         }
     }
+#pragma omp parallel for schedule(static)
+    for (int a = 0; a < k; ++a)
+        for (int b = 0; b < k; ++b) {
+            double s = 0.0;
+            for (int c = 0; c < NCH; ++c) s += P[(size_t)c * kk + (size_t)a * k + b];
+            S[a + (int64_t)k * b] = s;
+        }
+    free(P);
 }
 
 int orc_cholesky_lower(int k, double *S) {
@@ -245,16 +271,22 @@ int orc_orth(int64_t n, int k, double *X, int64_t ldx) {
         free(S);
         return st;
     }
-    /* row-wise forward substitution: solve z L^T = x for each basis row */
+    /* row-wise forward substitution: solve z L^T = x for each basis row
+     * (L copied row-major so the inner sum runs along contiguous memory) */
+    double *Lr = (double *)malloc(sizeof(double) * k * k);
+    for (int j = 0; j < k; ++j)
+        for (int t = 0; t <= j; ++t) Lr[(int64_t)j * k + t] = S[j + (int64_t)k * t];
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         double *x = X + i * ldx;
         for (int j = 0; j < k; ++j) {
+            const double *l = Lr + (int64_t)j * k;
             double s = x[j];
-            for (int t = 0; t < j; ++t) s -= x[t] * S[j + (int64_t)k * t];
-            x[j] = s / S[j + (int64_t)k * j];
+            for (int t = 0; t < j; ++t) s -= x[t] * l[t];
+            x[j] = s / l[j];
         }
     }
+    free(Lr);
     free(S);
     return 0;
 }

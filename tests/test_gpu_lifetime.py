@@ -43,8 +43,22 @@ def test_ctx_destroy_before_children_are_freed():
     assert c.h is None and h
 
 
+def _hip():
+    """The HIP runtime the engine links (already loaded by it): a second
+    stream is made there, at the default priority."""
+    import ctypes
+
+    E.lib()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    return hip
+
+
 def test_stream_switch_then_rebuild():
-    torch = pytest.importorskip("torch")
+    import ctypes
+
+    hip = _hip()
     c = E.Context(0)
     s1 = E.SynthHam(10, 9, 8, 3, 3, 3, 4, seed=11)
     ham = c.ham_upload(s1, E.HAM_F64 | E.HAM_F32)
@@ -52,8 +66,9 @@ def test_stream_switch_then_rebuild():
     dX, dY = E.DeviceArray.from_host(c, X1), E.DeviceArray(c, (s1.n, 40))
     c.ham_apply(ham, dX.ptr, 40, dY.ptr, 40, 40, E.F64)
     ham.free()  # its storage goes back to the cache, work may still be queued
-    st = torch.cuda.Stream()
-    c.use_stream(st.cuda_stream)
+    st = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0
+    c.use_stream(st.value)
     s2 = E.SynthHam(10, 9, 8, 3, 3, 3, 4, seed=12)  # same sizes: reuses the blocks
     H2 = oracle.Ham.from_synth(s2)
     ham2 = c.ham_upload(s2, E.HAM_F64 | E.HAM_F32)
@@ -68,6 +83,7 @@ def test_stream_switch_then_rebuild():
     dX.free()
     dY.free()
     c.close()
+    assert hip.hipStreamDestroy(st) == 0
 
 
 @pytest.mark.parametrize("dtype", [E.F64, E.F32])

@@ -245,7 +245,7 @@ int main(int argc, char **argv) {
 
         const int dtype = f32 ? TTDFT_F32 : TTDFT_F64;
         MultTM multtm(ctx);
-        EigenSolver eigenSolver;
+        EigenSolver eigenSolver(ctx);
         ChebyShevFilter chebyshevFilter;
 
         // ---- initial subspace: seeded N(0,1), Cholesky-QR (main.cc:807-829) ----
@@ -279,7 +279,7 @@ int main(int argc, char **argv) {
             ax.setup_system(ctx, H, copies);
         }
         DenseBlock HP(ctx, n, k, dtype);
-        std::vector<double> Q((size_t)k * k);
+        SeqDenseMatrix Q{k, k, std::vector<double>((size_t)k * k)};
         double *dQ = nullptr;
         check(ttdft_malloc(ctx.handle(), sizeof(double) * k * k, (void **)&dQ));
 
@@ -289,8 +289,12 @@ int main(int argc, char **argv) {
             double temp_lower_bound = 0.0;
             if (error_in_upper_bound > 0.01) {
                 double current_spectrum;
-                eigenSolver.computeUpperBoundWithLanczos(ax, maxLanczosIter, current_spectrum,
-                                                         temp_lower_bound);
+                if (assembly && device_assembly)  // H exists only on the device
+                    eigenSolver.computeUpperBoundWithLanczos(ax, maxLanczosIter,
+                                                             current_spectrum, temp_lower_bound);
+                else  // main.cc:989-992, the reference's own call
+                    eigenSolver.computeUpperBoundWithLanczos(H, maxLanczosIter,
+                                                             current_spectrum, temp_lower_bound);
                 error_in_upper_bound =
                     (upperBoundUnWantedSpectrum - current_spectrum) / upperBoundUnWantedSpectrum;
                 upperBoundUnWantedSpectrum =
@@ -313,9 +317,10 @@ int main(int argc, char **argv) {
                                                         lowerBoundWantedSpectrum, 1);
                 multtm.orth(eig_vecs);
                 ax.perform_ax(eig_vecs, HP);
-                std::vector<double> PtHP;
-                multtm.mult(eig_vecs, HP, PtHP);
-                eigenSolver.computeEigenPairsSeq(ctx, PtHP, k, k, eig_vals, Q, true);
+                SeqDenseMatrix PtHP{k, k, {}};
+                multtm.mult(eig_vecs, HP, PtHP.data);
+                // main.cc:1109-1114
+                eigenSolver.computeEigenPairsSeq(PtHP, k, k, eig_vals, Q, true);
                 for (int i = 0; i < k; ++i) printf("%dth e-vals:\t%.18e\n", i, eig_vals[i]);
                 printf("Projected Eigenvalues of Chebyshev Pass %d\n", restart_iter);
                 lowerBoundWantedSpectrum = eig_vals[0];
@@ -329,7 +334,7 @@ int main(int argc, char **argv) {
                 }
             }
             // rotation X <- X Q (main.cc:1185-1207)
-            check(ttdft_memcpy_h2d(ctx.handle(), dQ, Q.data(), sizeof(double) * k * k));
+            check(ttdft_memcpy_h2d(ctx.handle(), dQ, Q.data.data(), sizeof(double) * k * k));
             check(ttdft_rotate(ctx.handle(), eig_vecs.data(), eig_vecs.ld(), n, k, dQ, dtype,
                                dtype == TTDFT_F64 ? TTDFT_GEMM_F64 : TTDFT_GEMM_BF16X3));
             const double ef = computeFermiEnergy(eig_vals.data(), k, nelec, 3.166811429e-06, 500.0);

@@ -198,6 +198,35 @@ void MultTM::orth(DenseBlock &X) {
 }
 
 // ---- EigenSolver -----------------------------------------------------------------
+EigenSolver::~EigenSolver() { delete own_; }
+
+DeviceContext &EigenSolver::context() {
+    if (!ctx_) {
+        own_ = new DeviceContext(0);
+        ctx_ = own_;
+    }
+    return *ctx_;
+}
+
+void EigenSolver::computeEigenPairsSeq(SeqDenseMatrix &A, int numberEigenPairs,
+                                       int numGlobalRows, std::vector<double> &eigVals,
+                                       SeqDenseMatrix &eigMat, bool lapack) {
+    if (A.rows != numGlobalRows || A.cols != numGlobalRows)
+        throw Error(TTDFT_EBADARG, "computeEigenPairsSeq: A is not numGlobalRows square");
+    computeEigenPairsSeq(context(), A.data, numberEigenPairs, numGlobalRows, eigVals,
+                         eigMat.data, lapack);
+    eigMat.rows = numGlobalRows;
+    eigMat.cols = numberEigenPairs;
+}
+
+void EigenSolver::computeUpperBoundWithLanczos(ProjectHamiltonianSparse &H,
+                                               int maxLanczosIteration, double &upperBound,
+                                               double &lowerBound) {
+    AX ax(1, 0);
+    ax.setup_system(context(), H, TTDFT_HAM_F64);
+    computeUpperBoundWithLanczos(ax, maxLanczosIteration, upperBound, lowerBound);
+}
+
 void EigenSolver::computeEigenPairsSeq(DeviceContext &ctx, const std::vector<double> &A,
                                        int numberEigenPairs, int numGlobalRows,
                                        std::vector<double> &eigVals,

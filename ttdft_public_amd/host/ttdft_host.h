@@ -193,19 +193,49 @@ private:
     DeviceContext *ctx_;
 };
 
+// A sequential dense matrix (the MATSEQDENSE Mat of the reference, e.g. PtHP
+// and Q in main.cc:1103-1114): host, column-major rows x cols.
+struct SeqDenseMatrix {
+    int rows = 0, cols = 0;
+    std::vector<double> data;
+};
+
 // eigensolver/EigenSolver.h:29-45
 class EigenSolver {
 public:
-    // A: numGlobalRows x numGlobalRows column-major (lower triangle used);
-    // eigVals ascending, eigMat column i <-> eigVals[i] (column-major).
+    // main.cc:672 default-constructs the solver: it then binds device 0 on
+    // its first call.  The second form shares a caller's context.
+    EigenSolver() = default;
+    explicit EigenSolver(DeviceContext &ctx) : ctx_(&ctx) {}
+    ~EigenSolver();
+    EigenSolver(const EigenSolver &) = delete;
+    EigenSolver &operator=(const EigenSolver &) = delete;
+
+    // The reference's signature (EigenSolver.h:33-39): A numGlobalRows x
+    // numGlobalRows (lower triangle used); eigVals ascending, eigMat
+    // numGlobalRows x numberEigenPairs with column i <-> eigVals[i].
     // `lapack` selects the dense solver in the reference; the engine always
     // uses its dense FP64 solver.
+    void computeEigenPairsSeq(SeqDenseMatrix &A, int numberEigenPairs, int numGlobalRows,
+                              std::vector<double> &eigVals, SeqDenseMatrix &eigMat, bool lapack);
+    // The same with plain column-major vectors on an explicit context.
     void computeEigenPairsSeq(DeviceContext &ctx, const std::vector<double> &A,
                               int numberEigenPairs, int numGlobalRows,
                               std::vector<double> &eigVals, std::vector<double> &eigMat,
                               bool lapack);
+    // The reference's signature (EigenSolver.h:41-45): main.cc calls it
+    // before AX::setup_system (main.cc:989, 1028 against 1034), so H_loc and
+    // the projectors are uploaded (FP64) for this call and freed after it.
+    void computeUpperBoundWithLanczos(ProjectHamiltonianSparse &H, int maxLanczosIteration,
+                                      double &upperBound, double &lowerBound);
+    // On an operator already on the device (no upload).
     void computeUpperBoundWithLanczos(AX &ax, int maxLanczosIteration, double &upperBound,
                                       double &lowerBound, uint64_t seed = 20261202);
+
+private:
+    DeviceContext &context();
+    DeviceContext *ctx_ = nullptr;
+    DeviceContext *own_ = nullptr;  // created by the default form on first use
 };
 
 // KSDFTEnergyFunctional::computeFermiEnergy (dft/KSDFTEnergyFunctional.h:46-51)
