@@ -179,6 +179,19 @@ int ttdft_ham_info(ttdft_ham ham, int64_t *n, int64_t *nnz, int32_t *p);
 int ttdft_ham_apply(ttdft_ctx ctx, ttdft_ham ham, const void *X, int64_t ldx,
                     void *Y, int64_t ldy, int32_t k, int dtype);
 
+/* Y = H X as ttdft_ham_apply (same kernels, same result), but allocating
+ * nothing: no hipMalloc / hipFree, no library allocation.  No counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  The nonlocal projections go to a static
+ * device work space of 32 MiB per GPU (held until the process ends): the
+ * call fails with TTDFT_EBADARG when p * k * (1 + splits) * sizeof(dtype)
+ * exceeds it (k = 300 and p = 172 need at most 2.5 MB in FP64).  Every call on one
+ * GPU shares that work space, so calls that overlap on one GPU (different
+ * contexts or streams) race; calls on one context are stream-ordered and
+ * safe.  Not recorded by the context's kernel timing. */
+int ttdft_ham_apply_noalloc(ttdft_ctx ctx, ttdft_ham ham, const void *X, int64_t ldx,
+                            void *Y, int64_t ldy, int32_t k, int dtype);
+
 /* In-place Chebyshev filter of the k states of X (device, dtype).
  * Filter interval [a, b], a0 = lowest wanted bound, degree m, the columns
  * processed in num_blocks chunks, `degree_convention` TTDFT_DEGREE_*.

@@ -1,0 +1,68 @@
+"""ttdft_ham_apply_noalloc against ttdft_ham_apply, through the C ABI.
+
+The allocation-free entry point runs the same kernels with the nonlocal
+projections in the library's static per-GPU work space, so its result must
+equal ttdft_ham_apply's bit for bit.  Two calls in a row, the second on a
+smaller input (fewer basis functions, projectors and states), show that the
+work space left by one call does not leak into the next.
+"""
+import numpy as np
+import pytest
+
+from ttdft_public_amd import engine as E
+
+pytestmark = pytest.mark.gpu
+
+
+def _apply_both(c, ham, n, k, ld, dtype, seed):
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = np.random.default_rng(seed).standard_normal((n, ld)).astype(npdt)
+    dX = E.DeviceArray.from_host(c, X, npdt)
+    dY = E.DeviceArray.from_host(c, np.full((n, ld), 3.0, npdt), npdt)
+    dZ = E.DeviceArray.from_host(c, np.full((n, ld), 5.0, npdt), npdt)
+    c.ham_apply(ham, dX.ptr, ld, dY.ptr, ld, k, dtype)
+    c.ham_apply_noalloc(ham, dX.ptr, ld, dZ.ptr, ld, k, dtype)
+    Y = dY.to_host().reshape(n, ld)
+    Z = dZ.to_host().reshape(n, ld)
+    for d in (dX, dY, dZ):
+        d.free()
+    return Y, Z
+
+
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+def test_noalloc_matches_ham_apply_two_calls(dtype):
+    c = E.Context(0)
+    try:
+        # band format with projectors (the work space holds W and its split partials)
+        big = E.SynthHam(12, 10, 9, 3, 3, 3, 9, seed=31)
+        ham = c.ham_upload(big, E.HAM_F64 | E.HAM_F32)
+        assert ham.format()[0] == 1
+        print(f"noalloc call 1: n={big.n} p=9 k=300 dtype={dtype}", flush=True)
+        Y, Z = _apply_both(c, ham, big.n, 300, 301, dtype, 1)
+        assert np.array_equal(Y[:, :300], Z[:, :300])
+        assert np.all(Z[:, 300:] == 5.0)
+        ham.free()
+        # smaller: fewer functions, projectors and states; CSR format
+        small = E.SynthHam(6, 5, 7, 2, 2, 2, 3, seed=32)
+        ham = c.ham_upload(small, E.HAM_F64 | E.HAM_F32, tucker=False)
+        print(f"noalloc call 2: n={small.n} p=3 k=40 dtype={dtype}", flush=True)
+        Y, Z = _apply_both(c, ham, small.n, 40, 40, dtype, 2)
+        assert np.array_equal(Y, Z)
+        ham.free()
+    finally:
+        c.close()
+
+
+def test_noalloc_dense_storage():
+    """Dense storage folds C^T D C into H, so no work space is used."""
+    c = E.Context(0)
+    try:
+        s = E.SynthHam(9, 8, 7, 8, 7, 6, 5, seed=17)
+        ham = c.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=False)
+        assert ham.format()[0] == 2
+        for dtype in (E.F64, E.F32):
+            Y, Z = _apply_both(c, ham, s.n, 96, 96, dtype, 3)
+            assert np.array_equal(Y, Z)
+        ham.free()
+    finally:
+        c.close()

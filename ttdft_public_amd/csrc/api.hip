@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -247,6 +248,39 @@ static int upload(ttdft_ctx ctx, void **dptr, const void *src, size_t bytes) {
     return TTDFT_OK;
 }
 
+// ---- allocation-free H application -------------------------------------------
+// Work space of ttdft_ham_apply_noalloc: the nonlocal projections W = D C X
+// (p x k) and their split partials.  Each GPU holds its own copy until the
+// process ends.  In host code `scratch` names a host-side stand-in, not the
+// device memory: the device address comes from hipGetSymbolAddress, taken
+// once per device in ttdft_ctx_create with that device current.
+constexpr size_t kNoAllocBytes = size_t(32) << 20;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[kNoAllocDevices] = {};
+
+// called with `device` current
+static int noalloc_setup(int device) {
+    TT_CHECK(device >= 0 && device < kNoAllocDevices, TTDFT_EBADARG,
+             "device %d beyond the %d devices of the static work space table", device,
+             kNoAllocDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return TTDFT_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    TT_CHECK(e == hipSuccess && ptr, TTDFT_EHIP,
+             "hipGetSymbolAddress(static work space) on device %d: %s", device,
+             hipGetErrorString(e));
+    g_noalloc_ptr[device] = ptr;
+    return TTDFT_OK;
+}
+
+static void *noalloc_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return device >= 0 && device < kNoAllocDevices ? g_noalloc_ptr[device] : nullptr;
+}
+
 static void dfree(void *p) {
     if (p) {
         hipError_t e = hipFree(p);
@@ -399,6 +433,7 @@ int ttdft_ctx_create(int device, ttdft_ctx *out) {
     TT_HIP(hipGetDeviceCount(&n));
     TT_CHECK(device >= 0 && device < n, TTDFT_EBADARG, "device %d not in [0,%d)", device, n);
     TT_HIP(hipSetDevice(device));
+    TT_TRY(noalloc_setup(device));
     ttdft_ctx ctx = new ttdft_ctx_s();
     ctx->device = device;
     ctx->pending = new std::vector<PendingTiming>();
@@ -755,6 +790,24 @@ int ttdft_ham_apply(ttdft_ctx ctx, ttdft_ham H, const void *X, int64_t ldx, void
              "ham_apply: bad arguments (k=%d)", k);
     TT_CHECK(X != Y, TTDFT_EBADARG, "ham_apply: X and Y must not alias");
     return ham_step(ctx, H, X, ldx, Y, ldy, nullptr, 0, k, dtype, 1.0, 0.0, 0.0, 0, 0);
+}
+
+int ttdft_ham_apply_noalloc(ttdft_ctx ctx, ttdft_ham H, const void *X, int64_t ldx, void *Y,
+                            int64_t ldy, int32_t k, int dtype) {
+    clear_error();
+    TT_CHECK(ctx && H && X && Y && k >= 1 && k <= 1024 && ldx >= k && ldy >= k, TTDFT_EBADARG,
+             "ham_apply_noalloc: bad arguments (k=%d)", k);
+    TT_CHECK(X != Y, TTDFT_EBADARG, "ham_apply_noalloc: X and Y must not alias");
+    void *ws = noalloc_ptr(ctx->device);
+    TT_CHECK(ws, TTDFT_EHIP, "ham_apply_noalloc: no static work space on device %d",
+             ctx->device);
+    // the same kernels as ttdft_ham_apply; not timed (timing creates events)
+    const int prof = ctx->profiling;
+    ctx->profiling = 0;
+    const int rc = ham_step(ctx, H, X, ldx, Y, ldy, nullptr, 0, k, dtype, 1.0, 0.0, 0.0, 0, 0,
+                            ws, kNoAllocBytes);
+    ctx->profiling = prof;
+    return rc;
 }
 
 int ttdft_cheb_filter(ttdft_ctx ctx, ttdft_ham H, void *X, int64_t ldx, int32_t k, int32_t m,

@@ -216,9 +216,12 @@ int launch_band_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t
                      int32_t k, int dtype, double alpha, double beta, double delta,
                      int use_beta, int use_delta);
 // Full H application (nl projection + fused SpMM) with the given epilogue.
+// The projections go to ws (ws_bytes long; an error if they do not fit) when
+// ws is given, else to the context's scratch.
 int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, void *Y,
              int64_t ldy, const void *Z, int64_t ldz, int32_t k, int dtype, double alpha,
-             double beta, double delta, int use_beta, int use_delta);
+             double beta, double delta, int use_beta, int use_delta, void *ws = nullptr,
+             size_t ws_bytes = 0);
 
 // S = X^T Y, k x k column-major FP64 (device)
 int launch_gram(ttdft_ctx ctx, const void *X, int64_t ldx, const void *Y, int64_t ldy,

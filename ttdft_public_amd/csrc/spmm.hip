@@ -655,7 +655,8 @@ static int launch_dense_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X,
 
 int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, void *Y,
              int64_t ldy, const void *Z, int64_t ldz, int32_t k, int dtype, double alpha,
-             double beta, double delta, int use_beta, int use_delta) {
+             double beta, double delta, int use_beta, int use_delta, void *ws,
+             size_t ws_bytes) {
     const size_t es = dtype_size(dtype);
     const bool band = H->format == 1 && k >= 16 &&
                       (dtype == TTDFT_F64 ? H->mf64 != nullptr : H->mf32 != nullptr);
@@ -663,8 +664,17 @@ int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, vo
                                                             : H->dense32 != nullptr);
     void *W = nullptr;
     if (H->p > 0 && !(dense && H->dense_nl)) {  // folded into the dense storage otherwise
-        TT_TRY(ctx->scratch[3].ensure((size_t)nl_workspace(H, k) * es));
-        W = ctx->scratch[3].ptr;
+        const size_t need = (size_t)nl_workspace(H, k) * es;
+        if (ws) {
+            TT_CHECK(need <= ws_bytes, TTDFT_EBADARG,
+                     "the nonlocal projection of %d projectors x %d states needs %zu bytes of "
+                     "work space, more than the %zu available",
+                     H->p, k, need, ws_bytes);
+            W = ws;
+        } else {
+            TT_TRY(ctx->scratch[3].ensure(need));
+            W = ctx->scratch[3].ptr;
+        }
         PendingTiming tn;
         TT_TRY(timing_begin(ctx, TTDFT_KSTAT_NLPROJ, &tn));
         TT_TRY(launch_nl_project(ctx, H, X, ldx, k, dtype, W));

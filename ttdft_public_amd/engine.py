@@ -97,6 +97,8 @@ _SIGS = {
     "ttdft_ham_free": (ctypes.c_int, [_vp]),
     "ttdft_ham_info": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _c_i32p]),
     "ttdft_ham_apply": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _i64, _i32, ctypes.c_int]),
+    "ttdft_ham_apply_noalloc": (ctypes.c_int,
+                                [_vp, _vp, _vp, _i64, _vp, _i64, _i32, ctypes.c_int]),
     "ttdft_cheb_filter": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _d, _d, _d, _i32,
                                          ctypes.c_int, ctypes.c_int]),
     "ttdft_gram": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _i64, _i32, ctypes.c_int,
@@ -438,6 +440,11 @@ class Context:
 
     def ham_apply(self, ham, X, ldx, Y, ldy, k, dtype=F64):
         _check(lib().ttdft_ham_apply(self.h, ham.h, X, ldx, Y, ldy, k, dtype))
+
+    def ham_apply_noalloc(self, ham, X, ldx, Y, ldy, k, dtype=F64):
+        """ham_apply with the projections in the library's static work space
+        (allocates nothing)."""
+        _check(lib().ttdft_ham_apply_noalloc(self.h, ham.h, X, ldx, Y, ldy, k, dtype))
 
     def cheb_filter(self, ham, X, ldx, k, m, a, b, a0, num_blocks=1, dtype=F64,
                     convention=DEGREE_REFERENCE):
