@@ -151,6 +151,34 @@ def test_band_spmm_state_groups_in_one_workgroup(ctx, monkeypatch, dims, bands, 
     assert rel(out["1"], oracle.ham_apply(H, X)) < 1e-13
 
 
+@pytest.mark.parametrize("dims,bands,k", [((20, 18, 16), (6, 6, 7), 300),
+                                          ((13, 11, 9), (7, 3, 2), 320),
+                                          ((9, 10, 7), (1, 2, 2), 257),
+                                          ((17, 9, 5), (7, 3, 2), 80),
+                                          ((22, 6, 3), (2, 2, 1), 50)])
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+def test_band_spmm_two_planes_per_workgroup(ctx, monkeypatch, dims, bands, k, dtype):
+    """Workgroups over two z planes (a staged line serves both planes in its
+    z band) give the same bits as one plane per workgroup, odd plane counts
+    included (the last workgroup has one plane), and match the oracle."""
+    s = E.SynthHam(*dims, *bands, 4, seed=sum(dims) + k)
+    H = oracle.Ham.from_synth(s)
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=True)
+    assert ham.format()[0] == 1
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = np.random.default_rng(k).standard_normal((s.n, k)).astype(npdt)
+    dX, dY = dev(ctx, X, npdt), E.DeviceArray(ctx, (s.n, k), npdt)
+    out = {}
+    for zb in ("1", "2"):
+        monkeypatch.setenv("TTDFT_BRICK_ZB", zb)
+        print(f"band two planes dims={dims} k={k} dtype={dtype} zb={zb}", flush=True)
+        ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, dtype)
+        out[zb] = dY.to_host()
+    assert np.array_equal(out["1"], out["2"])
+    ref = oracle.ham_apply(H, X.astype(np.float64))
+    assert rel(out["2"].astype(np.float64), ref) < (1e-13 if dtype == E.F64 else 1e-5)
+
+
 def test_band_filter_with_holes_matches_oracle(ctx):
     """Truncation leaves holes inside the band: stored as zeros, same result."""
     s = E.SynthHam(10, 10, 10, 4, 4, 4, 6, truncation=2e-5, seed=3)
@@ -239,6 +267,26 @@ def test_gram_bf16x3(ctx, n, k):
     ref = X.astype(np.float64).T @ Y.astype(np.float64)
     # error relative to the natural scale sqrt(n) of a Gram entry
     assert np.abs(dS.to_host().T - ref).max() / np.sqrt(n) < 5e-5
+
+
+@pytest.mark.parametrize("n,k,ld", [(1000, 24, 24), (5001, 131, 133), (3000, 300, 301),
+                                     (2000, 300, 300), (129, 17, 20)])
+def test_rotate_bf16x3_shapes(ctx, n, k, ld):
+    """Split-BF16 rotation over its tilings: several groups of eight column
+    tiles (k = 300), k not a multiple of 16, rows not 16-byte aligned
+    (odd leading dimension: scalar loads), n not a multiple of the 128-row
+    workgroup.  Columns past k are left alone."""
+    rng = np.random.default_rng(n + k)
+    X = rng.standard_normal((n, ld)).astype(np.float32)
+    Q = rng.standard_normal((k, k)) / np.sqrt(k)
+    dX = dev(ctx, X, np.float32)
+    dQ = dev(ctx, Q.T.copy())  # column-major
+    print(f"rotate bf16x3 n={n} k={k} ld={ld}", flush=True)
+    ctx.rotate(dX.ptr, ld, n, k, dQ.ptr, E.F32, E.GEMM_BF16X3)
+    got = dX.to_host().reshape(n, ld).astype(np.float64)
+    ref = oracle.rotate(X[:, :k].astype(np.float64), Q)
+    assert rel(got[:, :k], ref) < 5e-5
+    assert np.array_equal(got[:, k:], X[:, k:].astype(np.float64))
 
 
 @pytest.mark.parametrize("mode", [E.GEMM_F64, E.GEMM_BF16X3])

@@ -183,7 +183,13 @@ __device__ __forceinline__ void lds_read_n(const T *p, T (&v)[N]) {
 // load the same A fragments at the same line step (the fragments come from
 // HBM once instead of once per state group); workgroups then hold exactly GXM
 // x-bricks.
-template <typename T, int KS, int NST, int DB, int DEPTH, int GXM, int G>
+// ZB: z planes per workgroup (1 or 2).  With 2, every wave accumulates its
+// brick in two consecutive planes: a staged line (y', z') serves both planes
+// that lie within the z band of z', so the X staged per matrix-core flop
+// halves.  A line only a plane outside the band of z' is skipped for that
+// plane; each plane sees its lines in the same order as with ZB = 1, so
+// the sums are bitwise the same.
+template <typename T, int KS, int NST, int DB, int DEPTH, int GXM, int G, int ZB = 1>
 __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     typedef typename Acc4<T>::type acc_t;
     constexpr int S = 16 * NST;                  // states per wave
@@ -198,6 +204,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     constexpr int VE = DB >= ES ? DB / ES : 1;   // elements per lane-load
     constexpr int NB = DEPTH + 1;  // LDS buffers: lines i .. i+DEPTH
     static_assert(DEPTH >= 1 && DEPTH <= 3, "1 to 3 lines in flight");
+    static_assert(ZB == 1 || ZB == 2, "one or two z planes per workgroup");
     static_assert((4 * (KS - 1) * ST + 16 * (NST - 1)) * ES < 65536, "ds offset field");
     __shared__ __attribute__((aligned(16))) unsigned char lb0[BUF];
     __shared__ __attribute__((aligned(16))) unsigned char lb1[BUF];
@@ -220,7 +227,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     const int xc = (int)(t % a.nxc);
     t /= a.nxc;
     const int h = (int)(t % a.hyn);
-    const int z = (int)(t / a.hyn);
+    const int z0 = ZB * (int)(t / a.hyn);  // first z plane of the workgroup
     const int s0 = sg * ST;
     // Waves are spread round-robin over the 4 SIMDs and the MFMA work of a
     // line ends at a barrier, so the SIMD with the most tile work sets the
@@ -244,13 +251,16 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     const int NPY = 2 * a.by + 4;
     const int NP = NPY * (2 * a.bz + 1);
 
-    acc_t acc[NST];
+    acc_t acc[ZB][NST];
 #pragma unroll
-    for (int st = 0; st < NST; ++st) acc[st] = acc_t{0, 0, 0, 0};
+    for (int zi = 0; zi < ZB; ++zi)
+#pragma unroll
+        for (int st = 0; st < NST; ++st) acc[zi][st] = acc_t{0, 0, 0, 0};
 
     const int y0 = 4 * h;
-    const int zlo = z - a.bz < 0 ? 0 : z - a.bz;
-    const int zhi = z + a.bz > a.rz - 1 ? a.rz - 1 : z + a.bz;
+    const int zlast = z0 + ZB - 1 < a.rz ? z0 + ZB - 1 : a.rz - 1;  // last plane in the domain
+    const int zlo = z0 - a.bz < 0 ? 0 : z0 - a.bz;
+    const int zhi = zlast + a.bz > a.rz - 1 ? a.rz - 1 : zlast + a.bz;
     const int ylo = y0 - a.by < 0 ? 0 : y0 - a.by;
     const int yhi = y0 + 3 + a.by > a.ry - 1 ? a.ry - 1 : y0 + 3 + a.by;
     const int ny = yhi - ylo + 1;
@@ -303,16 +313,29 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
 #endif
         }
     };
-    // A fragments of a line (wave-private, from HBM/L2)
-    auto load_a = [&](int li_, T *av) {
+    // Does plane zi of the workgroup take line zz (inside the domain and
+    // within the z band)?
+    auto plane_on = [&](int zi, int zz) {
+        const int dz = zz - (z0 + zi);
+        return z0 + zi < a.rz && dz >= -a.bz && dz <= a.bz;
+    };
+    // A fragments of a line for every plane (wave-private, from HBM/L2).  A
+    // plane that does not take the line loads a valid fragment it never uses,
+    // so every wave issues the same loads per line.
+    auto load_a = [&](int li_, T (&av)[ZB][KS]) {
         int yy, zz;
         line_of(li_, yy, zz);
         const int gc = active ? g : a.gxn - 1;
         const int tt = yy - y0;  // in [-by, 3 + by] by construction
-        const int p = (tt + a.by) + NPY * (zz - z + a.bz);
-        const T *Ab = a.mf + ((((int64_t)(z * a.hyn + h) * a.gxn + gc) * NP + p) * KS) * 64 + lane;
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) av[ks] = Ab[ks * 64];
+        for (int zi = 0; zi < ZB; ++zi) {
+            const int zp = z0 + zi < a.rz ? z0 + zi : a.rz - 1;
+            const int p = (tt + a.by) + NPY * (plane_on(zi, zz) ? zz - zp + a.bz : 0);
+            const T *Ab =
+                a.mf + ((((int64_t)(zp * a.hyn + h) * a.gxn + gc) * NP + p) * KS) * 64 + lane;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) av[zi][ks] = Ab[ks * 64];
+        }
     };
     // reads the B operands of k-step ks (compile-time after unrolling is not
     // enough for asm immediates, so dispatch on ks)
@@ -338,7 +361,9 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     };
     // The MFMAs of k-step ks are issued before the B operands of ks+1 are
     // read, so the LDS latency overlaps their execution.
-    auto mfma_tiles = [&](const T *Xb, const T *av, auto lo_c, auto hi_c) {
+    // With two planes the B operands of a k-step serve both planes' MFMAs.
+    auto mfma_tiles = [&](const T *Xb, const T (&av)[ZB][KS], const bool (&on)[ZB], auto lo_c,
+                          auto hi_c) {
         constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
         constexpr int NT = HI - LO;
         T b[KS][NT];
@@ -346,21 +371,33 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
 #pragma unroll
-            for (int t = 0; t < NT; ++t)
-                acc[LO + t] = mfma16x16x4(av[ks], b[ks][t], acc[LO + t]);
+            for (int zi = 0; zi < ZB; ++zi)
+                if (on[zi]) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        acc[zi][LO + t] = mfma16x16x4(av[zi][ks], b[ks][t], acc[zi][LO + t]);
+                }
             if constexpr (true) {
                 if (ks + 1 < KS) read_ks(Xb, b, ks + 1, lo_c, hi_c);
             }
         }
     };
-    auto compute = [&](const unsigned char *buf, const T *av) {
+    auto compute = [&](const unsigned char *buf, const T (&av)[ZB][KS], int li_) {
+        int yy, zz;
+        line_of(li_, yy, zz);
+        bool on[ZB];
+#pragma unroll
+        for (int zi = 0; zi < ZB; ++zi) on[zi] = plane_on(zi, zz);
         const T *Xb = reinterpret_cast<const T *>(buf) + (4 * bl + lk) * ST + sgw * S + li;
         if (mode == 0)
-            mfma_tiles(Xb, av, std::integral_constant<int, 0>(), std::integral_constant<int, NST>());
+            mfma_tiles(Xb, av, on, std::integral_constant<int, 0>(),
+                       std::integral_constant<int, NST>());
         else if (mode == 1)
-            mfma_tiles(Xb, av, std::integral_constant<int, 0>(), std::integral_constant<int, NH>());
+            mfma_tiles(Xb, av, on, std::integral_constant<int, 0>(),
+                       std::integral_constant<int, NH>());
         else
-            mfma_tiles(Xb, av, std::integral_constant<int, NH>(), std::integral_constant<int, NST>());
+            mfma_tiles(Xb, av, on, std::integral_constant<int, NH>(),
+                       std::integral_constant<int, NST>());
     };
 
     // Line i is in LDS buffer i % NB.  While it is multiplied, the loads of
@@ -382,9 +419,9 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
         else
             return lb3;
     };
-    constexpr int WAIT = (DEPTH - 1) * (NQ + KS);
+    constexpr int WAIT = (DEPTH - 1) * (NQ + ZB * KS);
     constexpr int PERIOD = NB;
-    T av[NB][KS];
+    T av[NB][ZB][KS];
     fetch(0, lb0);
     load_a(0, av[0]);
     if constexpr (DEPTH >= 2) {
@@ -401,7 +438,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
         constexpr int J = decltype(jc)::value;
         fetch(li_ + DEPTH, buf_of(std::integral_constant<int, (J + DEPTH) % NB>()));
         load_a(li_ + DEPTH, av[(J + DEPTH) % NB]);
-        if (active) compute(buf_of(std::integral_constant<int, J % NB>()), av[J % NB]);
+        if (active) compute(buf_of(std::integral_constant<int, J % NB>()), av[J % NB], li_);
         TT_WAIT_VM_LDS(WAIT);
         line_barrier();
     };
@@ -425,6 +462,10 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     // FP32 element v is D[4(l/16) + v][l%16].  D rows are brick rows i =
     // (x - 4g) + 4 (y - 4h), D columns are states.
 #pragma unroll
+    for (int zi = 0; zi < ZB; ++zi) {
+    const int z = z0 + zi;
+    if (z >= a.rz) break;
+#pragma unroll
     for (int st = 0; st < NST; ++st) {
         if ((mode == 1 && st >= NH) || (mode == 2 && st < NH)) continue;
         const int s = s0 + sgw * S + 16 * st + li;
@@ -434,7 +475,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
             const int x = 4 * g + (i & 3), y = y0 + (i >> 2);
             if (x < a.rx && y < a.ry && s < a.k) {
                 const int64_t row = x + (int64_t)a.rx * (y + (int64_t)a.ry * z);
-                T val = acc[st][v];
+                T val = acc[zi][st][v];
                 if (a.ct_rowptr) {
                     const int64_t f0 = a.ct_rowptr[row], f1 = a.ct_rowptr[row + 1];
                     for (int64_t e = f0; e < f1; ++e)
@@ -446,6 +487,7 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
                 a.Y[row * a.ldy + s] = out;
             }
         }
+    }
     }
 }
 
@@ -463,13 +505,13 @@ struct CoShape<float, V> {
     static constexpr int GXM = 4, G = 4, DEPTH = 2;
 };
 
-template <typename T, int KS, int DB>
+template <typename T, int KS, int DB, int ZB>
 int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
     constexpr int NST = kStateTiles<T>();
     const int64_t blocks = (a.nblocks + 7) / 8 * 8;
     if (cogroups == 1) {
         typedef CoShape<T, 0> C;
-        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, C::DEPTH, C::GXM, C::G>),
+        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, C::DEPTH, C::GXM, C::G, ZB>),
                            dim3((unsigned)blocks), dim3(64 * C::G * C::GXM), 0, ctx->stream, a);
         TT_LAUNCH_CHECK();
         return TTDFT_OK;
@@ -479,17 +521,17 @@ int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
     constexpr int DEPTH = 2;
     const int nthr = 64 * (a.gxb + a.nsplit);
     if (a.gxb <= 8)
-        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 8, 1>),
+        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 8, 1, ZB>),
                            dim3((unsigned)blocks), dim3(nthr), 0, ctx->stream, a);
     else
-        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 16, 1>),
+        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 16, 1, ZB>),
                            dim3((unsigned)blocks), dim3(nthr), 0, ctx->stream, a);
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
 
 template <typename T, int KS>
-int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
+int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups, int zb) {
     TT_CHECK(a.gxb >= 1 && (cogroups ? CoShape<T, 0>::G * a.gxb : a.gxb + a.nsplit) <= 16,
              TTDFT_EBADARG, "band SpMM: %d x-bricks per chunk", a.gxb);
     TT_CHECK((int64_t)a.rx * a.ldx * (int64_t)sizeof(T) < ((int64_t)1 << 31), TTDFT_EBADARG,
@@ -497,8 +539,11 @@ int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
     // 16-byte loads need every X row 16-byte aligned
     const bool wide = (a.ldx * (int64_t)sizeof(T)) % 16 == 0 &&
                       (reinterpret_cast<uintptr_t>(a.X) & 15) == 0;
-    if (wide) return launch_db<T, KS, 16>(ctx, a, cogroups);
-    return launch_db<T, KS, 4>(ctx, a, cogroups);
+    if (zb == 2)
+        return wide ? launch_db<T, KS, 16, 2>(ctx, a, cogroups)
+                    : launch_db<T, KS, 4, 2>(ctx, a, cogroups);
+    return wide ? launch_db<T, KS, 16, 1>(ctx, a, cogroups)
+                : launch_db<T, KS, 4, 1>(ctx, a, cogroups);
 }
 
 template <typename T>
@@ -532,16 +577,24 @@ int launch_brick(ttdft_ctx ctx, BrickArgs<T> a, int ks) {
         a.nsg = (a.k + S - 1) / S;
         a.nsplit = a.gxb % 4 == 1 ? 1 : (a.gxb % 4 == 2 ? 2 : 0);
     }
-    a.nblocks = (int64_t)a.nsg * a.nxc * a.hyn * a.rz;
+    // z planes per workgroup: two for the FP64 state-group workgroups (the X
+    // stage then serves twice the matrix-core work), else one.
+    // TTDFT_BRICK_ZB=1 or 2 overrides.
+    int zb = cogroups ? 2 : 1;
+    if (const char *env = std::getenv("TTDFT_BRICK_ZB")) {
+        const int v = std::atoi(env);
+        if (v == 1 || v == 2) zb = v;
+    }
+    a.nblocks = (int64_t)a.nsg * a.nxc * a.hyn * ((a.rz + zb - 1) / zb);
     switch (ks) {
     case 2:
-        return launch_ks<T, 2>(ctx, a, cogroups);
+        return launch_ks<T, 2>(ctx, a, cogroups, zb);
     case 3:
-        return launch_ks<T, 3>(ctx, a, cogroups);
+        return launch_ks<T, 3>(ctx, a, cogroups, zb);
     case 4:
-        return launch_ks<T, 4>(ctx, a, cogroups);
+        return launch_ks<T, 4>(ctx, a, cogroups, zb);
     case 5:
-        return launch_ks<T, 5>(ctx, a, cogroups);
+        return launch_ks<T, 5>(ctx, a, cogroups, zb);
     default:
         break;
     }

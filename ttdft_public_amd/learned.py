@@ -254,15 +254,21 @@ class UNet:
 
 
 # ---- inputs and targets on the uniform grid --------------------------------------
-def uniform_points(scf, n):
-    """n points per axis spanning the box (point i at -box/2 + i box/(n-1))."""
-    return np.linspace(-scf.cfg.box / 2, scf.cfg.box / 2, n)
+# The network sees a fixed cube centred on the origin (the cluster's centre),
+# whatever the SCF box: 40 bohr, the box of the training clusters.  Larger
+# SCF boxes take the correction inside the cube and zero outside it.
+NET_BOX = 40.0
+
+
+def uniform_points(n, box=NET_BOX):
+    """n points per axis spanning the cube (point i at -box/2 + i box/(n-1))."""
+    return np.linspace(-box / 2, box / 2, n)
 
 
 def network_input(net, scf, d_sup_uniform=None):
     """Fills net.a["x"]: [splats, superposition * in_scale] on the n^3 grid."""
     n = net.n
-    pts = uniform_points(scf, n)
+    pts = uniform_points(n)
     h = pts[1] - pts[0]
     d_pts = E.DeviceArray.from_host(scf.ctx, pts)
     d_splat = E.DeviceArray(scf.ctx, (n ** 3,))
@@ -287,7 +293,7 @@ def learned_start(net, scf, d_rho):
     _c(E.lib().ttdft_nn_to_f64(scf.ctx.h, net.a["y"].ptr, d_corr.ptr, n ** 3,
                                1.0 / net.out_scale))
     scf.superposition(d_rho)
-    pts = uniform_points(scf, n)
+    pts = uniform_points(n)
     h = pts[1] - pts[0]
     scf.ctx.grid_resample(d_corr.ptr, (n, n, n), (pts[0],) * 3, (h,) * 3, scf.p,
                           *(a.d_xq.ptr for a in scf.axes), d_rho.ptr, add=True)
@@ -364,10 +370,10 @@ def converged_sample(ctx, xyz, cfg, n):
     float64 host arrays, and the SCF result."""
     s = S.Scf(ctx, xyz, 3.0, cfg)
     r = s.run()
-    pts = E.DeviceArray.from_host(ctx, uniform_points(s, n))
+    pts = E.DeviceArray.from_host(ctx, uniform_points(n))
     d_sup = E.DeviceArray(ctx, (n ** 3,))
     s.superposition(d_sup, points=pts, n=n)
-    d_conv = s.density_on_uniform(n)
+    d_conv = s.density_on_uniform(n, box=NET_BOX)
     sup, conv = d_sup.to_host(), d_conv.to_host()
     return sup, conv - sup, r
 

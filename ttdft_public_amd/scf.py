@@ -58,6 +58,10 @@ class ScfConfig:
     rc: float = 0.8            # local pseudopotential core radius
     kb_energy: float = 1.0
     well_depth: float = 1.0    # 1-D model wells for the basis (Ha)
+    basis: str = "eigen"       # "eigen": lowest eigenvectors of the 1-D model; "windowed":
+                               # compact, orthonormal, localized (H_loc a Kronecker band)
+    basis_window: int = 20     # mesh nodes per windowed basis function
+    basis_span: float = 23.0   # windows cover [-span, span] (bohr)
     truncation: float = 1e-10
     tucker_refine: int = 2     # STHOSVD from the previous iteration's factors (0: exact each time)
     restart_first: int = 10
@@ -143,6 +147,49 @@ def eval_nodal(nodes, npe, values, points):
     return out
 
 
+def windowed_basis(nodes, M, H1, rank, window, span):
+    """Localized 1-D basis with compact support, orthonormal in the mass
+    matrix M (nodal coefficients, rank x nodes).
+
+    Function k lives on a window of `window` consecutive mesh nodes; the
+    windows start at evenly spaced nodes over [-span, span] and are taken left
+    to right.  In its window, function k is the lowest-energy function (1-D
+    model Hamiltonian H1, generalised eigenproblem with M) that is
+    M-orthogonal to the earlier functions whose windows reach into it.
+    Functions whose windows do not meet share no element, so they are
+    orthogonal without constraint: the basis is exactly orthonormal, and two
+    functions couple in any 1-D overlap, kinetic or potential matrix only
+    when their windows meet.  The Tucker-product H_loc is then a Kronecker
+    band whose half-width is the number of neighbouring windows reached.
+
+    This is the localization of basis/BasisLocalization.cc:110-168, 247-308
+    (per-centre lowest eigenvectors of a localizing operator, functions
+    ordered by centre) restated for exact compact support: the reference
+    localizes by a |x - R|^p penalty over the whole axis, and its
+    localized functions are neither compact nor mutually orthogonal."""
+    nn = len(nodes)
+    interior = np.arange(1, nn - 1)  # zero Dirichlet ends
+    cand = interior[(nodes[interior] > -span) & (nodes[interior] < span)]
+    if len(cand) < window + rank // 2:
+        raise ValueError("windowed basis: too few mesh nodes inside the span")
+    starts = np.round(np.linspace(cand[0], cand[-1] - window + 1, rank)).astype(int)
+    # how far (in nodes) the mass matrix couples: nodes of one element
+    reach = max(int(np.nonzero(M[i])[0].max()) - i for i in range(nn))
+    phi = np.zeros((nn, rank))
+    for k, s in enumerate(starts):
+        idx = np.arange(s, s + window)
+        prev = [i for i in range(k) if starts[i] + window - 1 + reach >= s]
+        A, Hw = M[np.ix_(idx, idx)], H1[np.ix_(idx, idx)]
+        if prev:
+            Z = scipy.linalg.null_space((M[idx, :] @ phi[:, prev]).T)
+        else:
+            Z = np.eye(window)
+        _, v = scipy.linalg.eigh(Z.T @ Hw @ Z, Z.T @ A @ Z, subset_by_index=[0, 0])
+        c = Z @ v[:, 0]
+        phi[idx, k] = c / math.sqrt(c @ A @ c)
+    return phi.T
+
+
 class Axis:
     """Mesh, quadrature, 1-D basis and its matrices for one axis."""
 
@@ -170,11 +217,15 @@ class Axis:
         ctx.tucker_basis_matrices(self.np, nn, d_bn.ptr, d_dbn.ptr, self.d_jw.ptr,
                                   self.d_inv_jw.ptr, 1, d_v1.ptr, d_m.ptr, d_k.ptr, d_p.ptr)
         M, K, P = (d.to_host().reshape(nn, nn) for d in (d_m, d_k, d_p))
-        inner = slice(1, nn - 1)  # zero Dirichlet ends
-        lam, c = scipy.linalg.eigh(0.5 * K[inner, inner] + P[inner, inner], M[inner, inner],
-                                   subset_by_index=[0, cfg.rank - 1])
-        self.basis = np.zeros((cfg.rank, nn))
-        self.basis[:, 1:-1] = c.T
+        if cfg.basis == "windowed":
+            self.basis = windowed_basis(self.nodes, M, 0.5 * K + P, cfg.rank, cfg.basis_window,
+                                        cfg.basis_span)
+        else:
+            inner = slice(1, nn - 1)  # zero Dirichlet ends
+            lam, c = scipy.linalg.eigh(0.5 * K[inner, inner] + P[inner, inner], M[inner, inner],
+                                       subset_by_index=[0, cfg.rank - 1])
+            self.basis = np.zeros((cfg.rank, nn))
+            self.basis[:, 1:-1] = c.T
         # basis values and derivatives at the quadrature points (device)
         d_nodal = _dev(ctx, self.basis)
         self.d_B = E.DeviceArray(ctx, (cfg.rank * self.np,))
@@ -352,6 +403,7 @@ class Scf:
         st.occ_tol = cfg.occ_tol
         occ_index = min(self.nelec // 2 + 10, k - 1)
         self.fp32_iterations = 0
+        self.chfsi_passes = []  # filter passes of every iteration's ChFSI step
         residuals, energies = [], []
         converged, ef = False, 0.0
         it = 0
@@ -377,6 +429,7 @@ class Scf:
                            cfg.restart_first if it == 0 else cfg.restart_other,
                            precision=dt)
             self.fp32_iterations += dt == E.F32
+            self.chfsi_passes.append(st.passes)
             ham.free()
             phases["chfsi"] += time.perf_counter() - t
             t = time.perf_counter()
@@ -419,10 +472,12 @@ class Scf:
         return ScfResult(converged, it + 1, residuals, energies, energies[-1], ef, ev.copy(),
                          time.perf_counter() - t_start, phases)
 
-    def density_on_uniform(self, n, d_out=None):
-        """rho of the last iteration's states on an n^3 uniform grid spanning the
-        box (point i at -box/2 + i box/(n-1))."""
-        pts = np.linspace(-self.cfg.box / 2, self.cfg.box / 2, n)
+    def density_on_uniform(self, n, d_out=None, box=None):
+        """rho of the last iteration's states on an n^3 uniform grid spanning a
+        cube centred on the origin, by default the SCF box (point i at
+        -box/2 + i box/(n-1))."""
+        box = self.cfg.box if box is None else box
+        pts = np.linspace(-box / 2, box / 2, n)
         R, k = self.cfg.rank, self.cfg.nstates
         d_b = [_dev(self.ctx, a.at_points(pts)) for a in self.axes]
         d_out = d_out or E.DeviceArray(self.ctx, (n ** 3,))
