@@ -374,10 +374,13 @@ def test_density(ctx, dtype):
                 dRho.ptr, dtype)
     ref = oracle.density(X.astype(np.float64), occ, (rx, ry, rz), Bx, By, Bz)
     got = dRho.to_host()
-    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+    # an FP32 block is contracted in FP32 up to the y-factor density matrices
+    tol = 1e-12 if dtype == E.F64 else 1e-5
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
 
 
-def test_density_chunked_with_sparse_weights(ctx, monkeypatch):
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+def test_density_chunked_with_sparse_weights(ctx, monkeypatch, dtype):
     """Several qz chunks, zero weights scattered among the states, ry > rx, rz."""
     monkeypatch.setenv("TTDFT_DENSITY_PLANES", "4")
     rng = np.random.default_rng(11)
@@ -385,36 +388,41 @@ def test_density_chunked_with_sparse_weights(ctx, monkeypatch):
     px, py, pz = 40, 31, 23
     Bx, By, Bz = (rng.standard_normal((p, r)) for p, r in ((px, rx), (py, ry), (pz, rz)))
     k = 37
-    X = rng.standard_normal((rx * ry * rz, k + 3))  # ld > k
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = rng.standard_normal((rx * ry * rz, k + 3)).astype(npdt)  # ld > k
     occ = rng.uniform(0.0, 1.0, k)
     occ[[0, 5, 6, 20, 36]] = 0.0
     dRho = E.DeviceArray(ctx, (pz, py, px))
-    dX = dev(ctx, X)
+    dX = dev(ctx, X, npdt)
     dBx, dBy, dBz = dev(ctx, Bx.T.copy()), dev(ctx, By.T.copy()), dev(ctx, Bz.T.copy())
     ctx.density(dX.ptr, k + 3, k, occ, (rx, ry, rz), dBx.ptr, dBy.ptr, dBz.ptr, (px, py, pz),
-                dRho.ptr, E.F64)
-    ref = oracle.density(X[:, :k].copy(), occ, (rx, ry, rz), Bx, By, Bz)
+                dRho.ptr, dtype)
+    ref = oracle.density(X[:, :k].astype(np.float64), occ, (rx, ry, rz), Bx, By, Bz)
     got = dRho.to_host()
-    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+    tol = 1e-12 if dtype == E.F64 else 1e-5
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
 
 
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
 @pytest.mark.parametrize("ry", [33, 50, 70, 97, 128])
-def test_density_wide_y(ctx, ry):
+def test_density_wide_y(ctx, ry, dtype):
     """y ranks spanning 3..8 MFMA row tiles (the y contraction's tile groups)."""
     rng = np.random.default_rng(ry)
     rx, rz = 3, 2
     px, py, pz = 7, 19, 5
     Bx, By, Bz = (rng.standard_normal((p, r)) for p, r in ((px, rx), (py, ry), (pz, rz)))
     k = 22
-    X = rng.standard_normal((rx * ry * rz, k))
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = rng.standard_normal((rx * ry * rz, k)).astype(npdt)
     occ = rng.uniform(0.0, 1.0, k)
     dRho = E.DeviceArray(ctx, (pz, py, px))
-    dX = dev(ctx, X)
+    dX = dev(ctx, X, npdt)
     dBx, dBy, dBz = dev(ctx, Bx.T.copy()), dev(ctx, By.T.copy()), dev(ctx, Bz.T.copy())
     ctx.density(dX.ptr, k, k, occ, (rx, ry, rz), dBx.ptr, dBy.ptr, dBz.ptr, (px, py, pz),
-                dRho.ptr, E.F64)
-    ref = oracle.density(X, occ, (rx, ry, rz), Bx, By, Bz)
-    assert np.linalg.norm(dRho.to_host() - ref) / np.linalg.norm(ref) < 1e-12
+                dRho.ptr, dtype)
+    ref = oracle.density(X.astype(np.float64), occ, (rx, ry, rz), Bx, By, Bz)
+    tol = 1e-12 if dtype == E.F64 else 1e-5
+    assert np.linalg.norm(dRho.to_host() - ref) / np.linalg.norm(ref) < tol
 
 
 def test_initial_density_and_integrals(ctx):

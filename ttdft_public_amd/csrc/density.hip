@@ -9,17 +9,21 @@
 // which needs 2 R^2 Q^2 k + Q^3 R(R+1) flops instead of the 2 R Q^3 k of
 // squaring every psi_i (R ranks, Q points per axis, k states): at R = 50,
 // Q = 600, k = 300 about 4x fewer, and all states are summed in one pass.
-// Every step is a dense FP64 matrix product on the matrix cores:
+// Every step is a dense matrix product on the matrix cores:
 //   1. Cg[c][a][s][b]  = sqrt(w_s) X[a + rx b + rx ry c][idx_s]   (gather)
 //   2. T1[qz][a][s][b] = sum_c Bz[qz,c] Cg[c][a][s][b]   (rocBLAS GEMM)
 //   3. T2[qz][qx][s][b]= sum_a Bx[qx,a] T1[qz][a][s][b]  (rocBLAS GEMM per plane)
 //   4. m[qz][qx][pair] = upper triangle of T2[qz][qx] T2[qz][qx]^T, packed
-//                        (syrk_pack_kernel, FP64 MFMA)
+//                        (syrk_pack_kernel, MFMA)
 //   5. rho[qz]^T       = m[qz] E^T,  E[qy][pair] = (2 - [b=b']) By[qy,b] By[qy,b']
-//                        (rocBLAS GEMM per plane)
-// Steps 2-5 run over chunks of qz planes to bound the workspace.
+//                        (rocBLAS FP64 GEMM per plane)
+// Steps 2-5 run over chunks of qz planes to bound the workspace.  Steps 1-4
+// run in FP64 for an FP64 block and in FP32 (FP32 accumulation) for an FP32
+// block, which halves the T2 intermediate that dominates the traffic; m and
+// step 5 are FP64 in both.
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -30,11 +34,11 @@ namespace {
 // Cg[((c rx + a) K + s) ry + b] = sqrt(w_s) X[(a + rx b + rx ry c) ldx + idx_s]
 // (states before b: one state's y-factor coefficients are contiguous all the
 // way to the T2 columns read by syrk_pack_kernel)
-template <typename T>
+template <typename T, typename Tm>
 __global__ void gather_scaled_kernel(const T *__restrict__ X, int64_t ldx,
                                      const int32_t *__restrict__ idx,
                                      const double *__restrict__ sw, int32_t K, int32_t rx,
-                                     int32_t ry, int64_t N, double *__restrict__ Cg) {
+                                     int32_t ry, int64_t N, Tm *__restrict__ Cg) {
     const int64_t total = N * K;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
@@ -45,11 +49,12 @@ __global__ void gather_scaled_kernel(const T *__restrict__ X, int64_t ldx,
         const int a = (int)(ca % rx);
         const int64_t c = ca / rx;
         const int64_t I = a + (int64_t)rx * (b + (int64_t)ry * c);
-        Cg[e] = sw[s] * (double)X[I * ldx + idx[s]];
+        Cg[e] = (Tm)(sw[s] * (double)X[I * ldx + idx[s]]);
     }
 }
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
 
 constexpr int tri_col(int p) {  // column b' of upper-triangle pair p (column-major order)
     int bp = 0;
@@ -73,16 +78,20 @@ constexpr int tri_col(int p) {  // column b' of upper-triangle pair p (column-ma
 // once; rows past R load a valid row whose products land only in tiles that
 // are never stored, and states past K (last chunk only) are zeroed, reading
 // into the 16 R doubles of slack that follow the T2 buffer.
-template <int RT, int P0, int NP>
-__global__ __launch_bounds__(256) void syrk_pack_kernel(const double *__restrict__ T2, int32_t R,
+// Tm = float (the density of an FP32 block): T2 in FP32, FP32 16x16x4 MFMAs
+// with FP32 accumulation (operands in the same lane layout; element v of
+// lane l then holds D[4(l/16) + v][l%16]), M written in FP64.
+template <typename Tm, int RT, int P0, int NP>
+__global__ __launch_bounds__(256) void syrk_pack_kernel(const Tm *__restrict__ T2, int32_t R,
                                                          int32_t K, int64_t ncols,
                                                          double *__restrict__ mp) {
+    typedef typename std::conditional<sizeof(Tm) == 8, d4_t, f4_t>::type acc_t;
     constexpr int NBLK = tri_col(P0 + NP - 1) + 1;  // row blocks the tiles use
     static_assert(NBLK <= RT, "tile range");
     const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= ncols) return;  // no barriers: waves are independent
     const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
-    const double *V = T2 + t * R * K;
+    const Tm *V = T2 + t * R * K;
     int32_t off[NBLK][4];
 #pragma unroll
     for (int bi = 0; bi < NBLK; ++bi) {
@@ -91,26 +100,26 @@ __global__ __launch_bounds__(256) void syrk_pack_kernel(const double *__restrict
 #pragma unroll
         for (int e = 0; e < 4; ++e) off[bi][e] = (4 * kq + e) * R + rc;
     }
-    d4_t acc[NP];
+    acc_t acc[NP];
 #pragma unroll
-    for (int q = 0; q < NP; ++q) acc[q] = d4_t{0, 0, 0, 0};
-    double b0[NBLK][4], b1[NBLK][4], b2[NBLK][4];
+    for (int q = 0; q < NP; ++q) acc[q] = acc_t{0, 0, 0, 0};
+    Tm b0[NBLK][4], b1[NBLK][4], b2[NBLK][4];
     const int nchunk = (K + 15) / 16;
-    auto load = [&](int ch, double (*v)[4]) {
-        const double *Vc = V + (int64_t)(ch < nchunk ? ch : nchunk - 1) * 16 * R;
+    auto load = [&](int ch, Tm (*v)[4]) {
+        const Tm *Vc = V + (int64_t)(ch < nchunk ? ch : nchunk - 1) * 16 * R;
 #pragma unroll
         for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[bi][e] = Vc[off[bi][e]];
     };
-    auto mul = [&](int ch, double (*v)[4]) {
+    auto mul = [&](int ch, Tm (*v)[4]) {
         const int c = ch * 16;
         if (c + 16 > K) {  // last chunk: states past K contribute nothing
 #pragma unroll
             for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    if (c + 4 * kq + e >= K) v[bi][e] = 0.0;
+                    if (c + 4 * kq + e >= K) v[bi][e] = Tm(0);
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e)
@@ -118,7 +127,12 @@ __global__ __launch_bounds__(256) void syrk_pack_kernel(const double *__restrict
             for (int q = 0; q < NP; ++q) {
                 const int p = P0 + q;
                 const int bj = tri_col(p), bi = p - bj * (bj + 1) / 2;
-                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[bi][e], v[bj][e], acc[q], 0, 0, 0);
+                if constexpr (sizeof(Tm) == 8)
+                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[bi][e], v[bj][e], acc[q], 0, 0,
+                                                                  0);
+                else
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[bi][e], v[bj][e], acc[q], 0, 0,
+                                                                  0);
             }
     };
     load(0, b0);
@@ -133,7 +147,8 @@ __global__ __launch_bounds__(256) void syrk_pack_kernel(const double *__restrict
         load(ch + 4, b1);
         mul(ch + 2, b2);
     }
-    // FP64 16x16x4 D layout: element v of lane l is D[l/16 + 4v][l%16]
+    // 16x16x4 D layout: element v of lane l is D[l/16 + 4v][l%16] (FP64),
+    // D[4(l/16) + v][l%16] (FP32)
     const int64_t np = (int64_t)R * (R + 1) / 2;
     double *out = mp + t * np;
 #pragma unroll
@@ -142,49 +157,50 @@ __global__ __launch_bounds__(256) void syrk_pack_kernel(const double *__restrict
         const int bj = tri_col(p), bi = p - bj * (bj + 1) / 2;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int b = 16 * bi + kq + 4 * v, bp = 16 * bj + i;
-            if (b <= bp && bp < R) out[(int64_t)bp * (bp + 1) / 2 + b] = acc[q][v];
+            const int b = 16 * bi + (sizeof(Tm) == 8 ? kq + 4 * v : 4 * kq + v), bp = 16 * bj + i;
+            if (b <= bp && bp < R) out[(int64_t)bp * (bp + 1) / 2 + b] = (double)acc[q][v];
         }
     }
 }
 
-template <int RT, int P0, int NP>
-int launch_syrk_group(ttdft_ctx ctx, const double *T2, int32_t R, int32_t K, int64_t ncols,
+template <typename Tm, int RT, int P0, int NP>
+int launch_syrk_group(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t ncols,
                       double *mp) {
     const int64_t blocks = (ncols + 3) / 4;
-    hipLaunchKernelGGL((syrk_pack_kernel<RT, P0, NP>), dim3((unsigned)blocks), dim3(256), 0,
+    hipLaunchKernelGGL((syrk_pack_kernel<Tm, RT, P0, NP>), dim3((unsigned)blocks), dim3(256), 0,
                        ctx->stream, T2, R, K, ncols, mp);
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
 
 // all upper-triangle tiles of an R x R result, at most 11 tiles per wave
-int launch_syrk_pack(ttdft_ctx ctx, const double *T2, int32_t R, int32_t K, int64_t ncols,
+template <typename Tm>
+int launch_syrk_pack(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t ncols,
                      double *mp) {
     switch ((R + 15) / 16) {
     case 1:
-        return launch_syrk_group<1, 0, 1>(ctx, T2, R, K, ncols, mp);
+        return launch_syrk_group<Tm, 1, 0, 1>(ctx, T2, R, K, ncols, mp);
     case 2:
-        return launch_syrk_group<2, 0, 3>(ctx, T2, R, K, ncols, mp);
+        return launch_syrk_group<Tm, 2, 0, 3>(ctx, T2, R, K, ncols, mp);
     case 3:
-        return launch_syrk_group<3, 0, 6>(ctx, T2, R, K, ncols, mp);
+        return launch_syrk_group<Tm, 3, 0, 6>(ctx, T2, R, K, ncols, mp);
     case 4:
-        return launch_syrk_group<4, 0, 10>(ctx, T2, R, K, ncols, mp);
+        return launch_syrk_group<Tm, 4, 0, 10>(ctx, T2, R, K, ncols, mp);
     case 5:
-        TT_TRY((launch_syrk_group<5, 0, 8>(ctx, T2, R, K, ncols, mp)));
-        return launch_syrk_group<5, 8, 7>(ctx, T2, R, K, ncols, mp);
+        TT_TRY((launch_syrk_group<Tm, 5, 0, 8>(ctx, T2, R, K, ncols, mp)));
+        return launch_syrk_group<Tm, 5, 8, 7>(ctx, T2, R, K, ncols, mp);
     case 6:
-        TT_TRY((launch_syrk_group<6, 0, 11>(ctx, T2, R, K, ncols, mp)));
-        return launch_syrk_group<6, 11, 10>(ctx, T2, R, K, ncols, mp);
+        TT_TRY((launch_syrk_group<Tm, 6, 0, 11>(ctx, T2, R, K, ncols, mp)));
+        return launch_syrk_group<Tm, 6, 11, 10>(ctx, T2, R, K, ncols, mp);
     case 7:
-        TT_TRY((launch_syrk_group<7, 0, 10>(ctx, T2, R, K, ncols, mp)));
-        TT_TRY((launch_syrk_group<7, 10, 9>(ctx, T2, R, K, ncols, mp)));
-        return launch_syrk_group<7, 19, 9>(ctx, T2, R, K, ncols, mp);
+        TT_TRY((launch_syrk_group<Tm, 7, 0, 10>(ctx, T2, R, K, ncols, mp)));
+        TT_TRY((launch_syrk_group<Tm, 7, 10, 9>(ctx, T2, R, K, ncols, mp)));
+        return launch_syrk_group<Tm, 7, 19, 9>(ctx, T2, R, K, ncols, mp);
     case 8:
-        TT_TRY((launch_syrk_group<8, 0, 9>(ctx, T2, R, K, ncols, mp)));
-        TT_TRY((launch_syrk_group<8, 9, 9>(ctx, T2, R, K, ncols, mp)));
-        TT_TRY((launch_syrk_group<8, 18, 9>(ctx, T2, R, K, ncols, mp)));
-        return launch_syrk_group<8, 27, 9>(ctx, T2, R, K, ncols, mp);
+        TT_TRY((launch_syrk_group<Tm, 8, 0, 9>(ctx, T2, R, K, ncols, mp)));
+        TT_TRY((launch_syrk_group<Tm, 8, 9, 9>(ctx, T2, R, K, ncols, mp)));
+        TT_TRY((launch_syrk_group<Tm, 8, 18, 9>(ctx, T2, R, K, ncols, mp)));
+        return launch_syrk_group<Tm, 8, 27, 9>(ctx, T2, R, K, ncols, mp);
     default:
         break;
     }
@@ -224,6 +240,135 @@ unsigned grid_for(int64_t total) {
                  (int)st_);                                                               \
     } while (0)
 
+// Level-3 BLAS of the intermediate type (FP64 for FP64 blocks, FP32 for FP32)
+rocblas_status gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, rocblas_int m,
+                    rocblas_int n, rocblas_int k, const double *A, rocblas_int lda,
+                    const double *B, rocblas_int ldb, double *C, rocblas_int ldc) {
+    const double one = 1.0, zero = 0.0;
+    return rocblas_dgemm(h, ta, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
+}
+rocblas_status gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, rocblas_int m,
+                    rocblas_int n, rocblas_int k, const float *A, rocblas_int lda,
+                    const float *B, rocblas_int ldb, float *C, rocblas_int ldc) {
+    const float one = 1.0f, zero = 0.0f;
+    return rocblas_sgemm(h, ta, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
+}
+rocblas_status gemm_batched(rocblas_handle h, rocblas_operation ta, rocblas_operation tb,
+                            rocblas_int m, rocblas_int n, rocblas_int k, const double *A,
+                            rocblas_int lda, int64_t sa, const double *B, rocblas_int ldb,
+                            int64_t sb, double *C, rocblas_int ldc, int64_t sc, rocblas_int cnt) {
+    const double one = 1.0, zero = 0.0;
+    return rocblas_dgemm_strided_batched(h, ta, tb, m, n, k, &one, A, lda, sa, B, ldb, sb, &zero,
+                                         C, ldc, sc, cnt);
+}
+rocblas_status gemm_batched(rocblas_handle h, rocblas_operation ta, rocblas_operation tb,
+                            rocblas_int m, rocblas_int n, rocblas_int k, const float *A,
+                            rocblas_int lda, int64_t sa, const float *B, rocblas_int ldb,
+                            int64_t sb, float *C, rocblas_int ldc, int64_t sc, rocblas_int cnt) {
+    const float one = 1.0f, zero = 0.0f;
+    return rocblas_sgemm_strided_batched(h, ta, tb, m, n, k, &one, A, lda, sa, B, ldb, sb, &zero,
+                                         C, ldc, sc, cnt);
+}
+
+__global__ void to_float_kernel(const double *__restrict__ a, float *__restrict__ b,
+                                int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) b[i] = (float)a[i];
+}
+
+// Steps 1-5 with the T1 / T2 intermediates and their contractions in Tm.
+// Workspace (scratch 5, bytes): Cg (N K Tm), E (py np double), sqrt(w) (K
+// double), state indices (K int32, padded to K doubles), Bx / Bz in Tm (FP32
+// only), then per qz plane T1 (rx ry K Tm), T2 (px ry K Tm) and the packed
+// M (px np double), with 16 ry Tm of slack after T2 (read, never used, by
+// the last state chunk of the y contraction).
+template <typename Tm>
+int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std::vector<int32_t> &idx,
+                const std::vector<double> &sw, int32_t rx, int32_t ry, int32_t rz,
+                const double *Bx, const double *By, const double *Bz, int32_t px, int32_t py,
+                int32_t pz, double *rho) {
+    const int64_t N = (int64_t)rx * ry * rz;
+    const int32_t K = (int32_t)idx.size();
+    const int64_t rxy = (int64_t)rx * ry;
+    const int64_t np = (int64_t)ry * (ry + 1) / 2;
+    constexpr int64_t ES = sizeof(Tm);
+    const bool f32 = ES == 4;
+    const int64_t per_plane = (rxy * K + (int64_t)px * ry * K) * ES + (int64_t)px * np * 8;
+    const int64_t budget = (int64_t)6 << 30;  // bytes for the chunk buffers
+    int64_t QC = budget / per_plane;
+    if (QC < 1) QC = 1;
+    if (QC > pz) QC = pz;
+    // TTDFT_DENSITY_PLANES caps the planes per chunk (smaller workspace)
+    if (const char *env = std::getenv("TTDFT_DENSITY_PLANES")) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v > 0 && v < QC) QC = v;
+    }
+    auto align = [](int64_t b) { return (b + 255) / 256 * 256; };
+    const int64_t o_cg = 0, o_e = align(N * K * ES), o_sw = o_e + align(py * np * 8),
+                  o_idx = o_sw + align(K * 8), o_bx = o_idx + align(K * 8),
+                  o_bz = o_bx + (f32 ? align((int64_t)px * rx * ES) : 0),
+                  o_t1 = o_bz + (f32 ? align((int64_t)pz * rz * ES) : 0),
+                  o_t2 = o_t1 + align(QC * rxy * K * ES),
+                  o_mp = o_t2 + align(QC * px * ry * K * ES + 16 * ry * ES),
+                  total = o_mp + align(QC * px * np * 8);
+    TT_TRY(ctx->scratch[5].ensure((size_t)total));
+    char *base = (char *)ctx->scratch[5].ptr;
+    Tm *Cg = (Tm *)(base + o_cg);
+    double *E = (double *)(base + o_e);
+    double *swd = (double *)(base + o_sw);
+    int32_t *idxd = (int32_t *)(base + o_idx);
+    const Tm *Bxm = (const Tm *)Bx, *Bzm = (const Tm *)Bz;
+    if (f32) {
+        Bxm = (const Tm *)(base + o_bx);
+        Bzm = (const Tm *)(base + o_bz);
+        hipLaunchKernelGGL(to_float_kernel, dim3(grid_for((int64_t)px * rx)), dim3(256), 0,
+                           ctx->stream, Bx, (float *)(base + o_bx), (int64_t)px * rx);
+        TT_LAUNCH_CHECK();
+        hipLaunchKernelGGL(to_float_kernel, dim3(grid_for((int64_t)pz * rz)), dim3(256), 0,
+                           ctx->stream, Bz, (float *)(base + o_bz), (int64_t)pz * rz);
+        TT_LAUNCH_CHECK();
+    }
+    Tm *T1 = (Tm *)(base + o_t1);
+    Tm *T2 = (Tm *)(base + o_t2);
+    double *mp = (double *)(base + o_mp);
+
+    TT_HIP(hipMemcpyAsync(swd, sw.data(), sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
+    TT_HIP(hipMemcpyAsync(idxd, idx.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice,
+                          ctx->stream));
+    if (dtype == TTDFT_F64)
+        hipLaunchKernelGGL((gather_scaled_kernel<double, Tm>), dim3(grid_for(N * K)), dim3(256), 0,
+                           ctx->stream, (const double *)X, ldx, idxd, swd, K, rx, ry, N, Cg);
+    else
+        hipLaunchKernelGGL((gather_scaled_kernel<float, Tm>), dim3(grid_for(N * K)), dim3(256), 0,
+                           ctx->stream, (const float *)X, ldx, idxd, swd, K, rx, ry, N, Cg);
+    TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(pair_table_kernel, dim3(grid_for((int64_t)py * np)), dim3(256), 0,
+                       ctx->stream, By, py, ry, E);
+    TT_LAUNCH_CHECK();
+
+    const double one = 1.0, zero = 0.0;
+    for (int64_t q0 = 0; q0 < pz; q0 += QC) {
+        const int qc = (int)std::min<int64_t>(QC, pz - q0);
+        // 2. T1[qzl][a][s][b] = sum_c Cg[c][a][s][b] Bz[q0 + qzl, c]
+        TT_BLAS(gemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
+                     (rocblas_int)(rxy * K), qc, rz, Cg, (rocblas_int)(rxy * K), Bzm + q0, pz, T1,
+                     (rocblas_int)(rxy * K)));
+        // 3. T2[qzl][qx][(s, b)] = sum_a T1[qzl][a][(s, b)] Bx[qx, a], one GEMM per plane
+        TT_BLAS(gemm_batched(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
+                             (rocblas_int)((int64_t)ry * K), px, rx, T1,
+                             (rocblas_int)((int64_t)ry * K), rxy * K, Bxm, px, 0, T2,
+                             (rocblas_int)((int64_t)ry * K), (int64_t)px * ry * K, qc));
+        // 4. packed upper triangles of T2[qzl][qx] T2[qzl][qx]^T
+        TT_TRY(launch_syrk_pack<Tm>(ctx, T2, ry, K, (int64_t)qc * px, mp));
+        // 5. rho[q0 + qzl](qx, qy) = sum_pair m[qzl][qx][pair] E[qy][pair]  (FP64)
+        TT_BLAS(rocblas_dgemm_strided_batched(
+            ctx->blas, rocblas_operation_transpose, rocblas_operation_none, px, py,
+            (rocblas_int)np, &one, mp, (rocblas_int)np, (int64_t)px * np, E, (rocblas_int)np, 0,
+            &zero, rho + q0 * (int64_t)px * py, px, (int64_t)px * py, qc));
+    }
+    return TTDFT_OK;
+}
+
 }  // namespace
 
 int density_compute(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, int32_t k,
@@ -253,64 +398,16 @@ int density_compute(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, int32_
     TT_CHECK(rxy * K < ((int64_t)1 << 31), TTDFT_EBADARG,
              "density: rx*ry*states = %lld exceeds the 32-bit GEMM dimension",
              (long long)(rxy * K));
-    // per-qz-plane workspace: T1, T2, packed M
-    const int64_t per_plane = rxy * K + (int64_t)px * ry * K + (int64_t)px * np;
-    const int64_t budget = (int64_t)6 << 30;  // bytes for the chunk buffers
-    int64_t QC = budget / (8 * per_plane);
-    if (QC < 1) QC = 1;
-    if (QC > pz) QC = pz;
-    // TTDFT_DENSITY_PLANES caps the planes per chunk (smaller workspace)
-    if (const char *env = std::getenv("TTDFT_DENSITY_PLANES")) {
-        const long v = std::strtol(env, nullptr, 10);
-        if (v > 0 && v < QC) QC = v;
-    }
-    const size_t fixed = (size_t)N * K + (size_t)py * np + 2 * (size_t)K + 16 * (size_t)ry;
-    TT_TRY(ctx->scratch[5].ensure(8 * (fixed + (size_t)QC * per_plane)));
-    double *Cg = (double *)ctx->scratch[5].ptr;
-    double *E = Cg + (size_t)N * K;
-    double *swd = E + (size_t)py * np;
-    int32_t *idxd = (int32_t *)(swd + K);
-    double *T1 = swd + 2 * (size_t)K;
-    double *T2 = T1 + (size_t)QC * rxy * K;
-    double *mp = T2 + (size_t)QC * px * ry * K + 16 * (size_t)ry;  // 16 ry doubles of slack after T2
-
     PendingTiming t;
     TT_TRY(timing_begin(ctx, TTDFT_KSTAT_DENSITY, &t));
-    TT_HIP(hipMemcpyAsync(swd, sw.data(), sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
-    TT_HIP(hipMemcpyAsync(idxd, idx.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice,
-                          ctx->stream));
-    if (dtype == TTDFT_F64)
-        hipLaunchKernelGGL(gather_scaled_kernel<double>, dim3(grid_for(N * K)), dim3(256), 0,
-                           ctx->stream, (const double *)X, ldx, idxd, swd, K, rx, ry, N, Cg);
+    // An FP32 block takes FP32 intermediates (T1, T2, the state sums of M);
+    // M and the expansion onto the y points stay FP64.
+    if (dtype == TTDFT_F32)
+        TT_TRY(density_run<float>(ctx, X, ldx, dtype, idx, sw, rx, ry, rz, Bx, By, Bz, px, py,
+                                  pz, rho));
     else
-        hipLaunchKernelGGL(gather_scaled_kernel<float>, dim3(grid_for(N * K)), dim3(256), 0,
-                           ctx->stream, (const float *)X, ldx, idxd, swd, K, rx, ry, N, Cg);
-    TT_LAUNCH_CHECK();
-    hipLaunchKernelGGL(pair_table_kernel, dim3(grid_for((int64_t)py * np)), dim3(256), 0,
-                       ctx->stream, By, py, ry, E);
-    TT_LAUNCH_CHECK();
-
-    const double one = 1.0, zero = 0.0;
-    for (int64_t q0 = 0; q0 < pz; q0 += QC) {
-        const int qc = (int)std::min<int64_t>(QC, pz - q0);
-        // 2. T1[qzl][a][s][b] = sum_c Cg[c][a][s][b] Bz[q0 + qzl, c]
-        TT_BLAS(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
-                              (rocblas_int)(rxy * K), qc, rz, &one, Cg, (rocblas_int)(rxy * K),
-                              Bz + q0, pz, &zero, T1, (rocblas_int)(rxy * K)));
-        // 3. T2[qzl][qx][(s, b)] = sum_a T1[qzl][a][(s, b)] Bx[qx, a], one GEMM per plane
-        TT_BLAS(rocblas_dgemm_strided_batched(
-            ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
-            (rocblas_int)((int64_t)ry * K), px, rx, &one, T1, (rocblas_int)((int64_t)ry * K),
-            rxy * K, Bx, px, 0, &zero, T2, (rocblas_int)((int64_t)ry * K),
-            (int64_t)px * ry * K, qc));
-        // 4. packed upper triangles of T2[qzl][qx] T2[qzl][qx]^T
-        TT_TRY(launch_syrk_pack(ctx, T2, ry, K, (int64_t)qc * px, mp));
-        // 5. rho[q0 + qzl](qx, qy) = sum_pair m[qzl][qx][pair] E[qy][pair]
-        TT_BLAS(rocblas_dgemm_strided_batched(
-            ctx->blas, rocblas_operation_transpose, rocblas_operation_none, px, py,
-            (rocblas_int)np, &one, mp, (rocblas_int)np, (int64_t)px * np, E, (rocblas_int)np, 0,
-            &zero, rho + q0 * (int64_t)px * py, px, (int64_t)px * py, qc));
-    }
+        TT_TRY(density_run<double>(ctx, X, ldx, dtype, idx, sw, rx, ry, rz, Bx, By, Bz, px, py,
+                                   pz, rho));
     const double nst = (double)K;
     const double flops = 2.0 * nst * ((double)N * pz + (double)pz * rx * ry * px) +
                          nst * (double)ry * (ry + 1) * px * pz +

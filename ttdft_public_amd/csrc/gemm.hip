@@ -206,71 +206,105 @@ __global__ __launch_bounds__(256) void rotate_f64_kernel(const double *__restric
     }
 }
 
-// Rotation, split-BF16: Z = X Q for FP32 X; Q given as bf16 hi/lo planes in
-// column-major order (Qh[a + k*b]).  A[i][kk] = X[n0+i][a+kk] (lane i +
-// 16 (kk/4), element kk%4), B[kk][j] = Q[a+kk][b0+j].
-__global__ __launch_bounds__(256) void rotate_bf16x3_kernel(const float *__restrict__ X,
-                                                            int64_t ldx,
-                                                            const short *__restrict__ Qh,
-                                                            const short *__restrict__ Ql,
-                                                            float *__restrict__ Z, int64_t ldz,
-                                                            int64_t n, int32_t k) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t n0 = (int64_t)blockIdx.x * 64 + wave * 16;
-    const int b0 = blockIdx.y * 64;
-    const int li = lane & 15, lk = lane >> 4;
-    const int64_t arow = n0 + li;
-    const bool rok = arow < n;
-    const float *xr = X + (rok ? arow : 0) * ldx;
-    f4 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f4{0, 0, 0, 0};
-    for (int a = 0; a < k; a += 16) {
-        s4 xh, xl;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int aa = a + 4 * lk + v;
-            short h, l;
-            split_bf16((rok && aa < k) ? xr[aa] : 0.0f, h, l);
-            xh[v] = h;
-            xl[v] = l;
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int b = b0 + 16 * t + li;
-            s4 qh, ql;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int aa = a + 4 * lk + v;
-                const bool ok = aa < k && b < k;
-                qh[v] = ok ? Qh[aa + (int64_t)k * b] : (short)0;
-                ql[v] = ok ? Ql[aa + (int64_t)k * b] : (short)0;
-            }
-            acc[t] = mfma_bf16(xl, qh, acc[t]);
-            acc[t] = mfma_bf16(xh, ql, acc[t]);
-            acc[t] = mfma_bf16(xh, qh, acc[t]);
-        }
-    }
+// Q (k x k column-major FP64) split once into bf16 hi/lo B fragments of the
+// 16x16x16 MFMA: fragment (step s, tile t) holds Q[16s + kk][16t + j] in lane
+// j + 16 (kk/4), element kk%4, as 8 shorts per lane (hi[4], lo[4]), zero past
+// k.  Qf[((s * ntile + t) * 64 + lane)].
+struct QFrag {
+    s4 hi, lo;
+};
+__global__ void split_q_frag_kernel(const double *__restrict__ Q, int32_t k, int32_t nstep,
+                                    int32_t ntile, QFrag *__restrict__ Qf) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nstep * ntile * 64) return;
+    const int lane = (int)(idx & 63);
+    const int64_t st = idx >> 6;
+    const int t = (int)(st % ntile), s = (int)(st / ntile);
+    const int b = 16 * t + (lane & 15);
+    QFrag f;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int64_t row = n0 + 4 * lk + v;
-        if (row >= n) continue;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int b = b0 + 16 * t + li;
-            if (b < k) Z[row * ldz + b] = acc[t][v];
-        }
+        const int a = 16 * s + 4 * (lane >> 4) + v;
+        short h = 0, l = 0;
+        if (a < k && b < k) split_bf16((float)Q[a + (int64_t)k * b], h, l);
+        f.hi[v] = h;
+        f.lo[v] = l;
     }
+    Qf[idx] = f;
 }
 
-__global__ void split_q_kernel(const double *__restrict__ Q, int64_t count,
-                               short *__restrict__ Qh, short *__restrict__ Ql) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    short h, l;
-    split_bf16((float)Q[i], h, l);
-    Qh[i] = h;
-    Ql[i] = l;
+// Rotation, split-BF16: Z = X Q for FP32 X.  A wave computes 32 rows (two
+// 16-row MFMA tiles) by RT16 column tiles: per 16-deep step it loads its two
+// rows' 4 values each as one 16-byte vector (split once into hi/lo), then one
+// 16-byte pre-split Q fragment per column tile, shared by both row tiles
+// (three MFMAs each).  A[i][kk] = X[row i][a + kk] in lane i + 16 (kk/4),
+// element kk%4.
+template <int RT16>
+__global__ __launch_bounds__(256) void rotate_bf16x3_frag_kernel(
+    const float *__restrict__ X, int64_t ldx, const QFrag *__restrict__ Qf, int32_t nstep,
+    int32_t ntile, float *__restrict__ Z, int64_t ldz, int64_t n, int32_t k, int vec) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t n0 = (int64_t)blockIdx.x * 128 + wave * 32;
+    const int t0 = blockIdx.y * RT16;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t row0 = n0 + li, row1 = n0 + 16 + li;
+    const float *x0 = X + (row0 < n ? row0 : n - 1) * ldx;
+    const float *x1 = X + (row1 < n ? row1 : n - 1) * ldx;
+    f4 acc[2][RT16];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int t = 0; t < RT16; ++t) acc[r][t] = f4{0, 0, 0, 0};
+    for (int s = 0; s < nstep; ++s) {
+        const int a = 16 * s + 4 * lk;
+        f4 v0, v1;
+        if (vec && a + 3 < k) {  // rows 16-byte aligned (checked on the host)
+            v0 = *reinterpret_cast<const f4 *>(x0 + a);
+            v1 = *reinterpret_cast<const f4 *>(x1 + a);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                v0[v] = a + v < k ? x0[a + v] : 0.0f;
+                v1[v] = a + v < k ? x1[a + v] : 0.0f;
+            }
+        }
+        s4 h0, l0, h1, l1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            short h, l;
+            split_bf16(v0[v], h, l);
+            h0[v] = h;
+            l0[v] = l;
+            split_bf16(v1[v], h, l);
+            h1[v] = h;
+            l1[v] = l;
+        }
+        const QFrag *qs = Qf + ((int64_t)s * ntile + t0) * 64 + lane;
+#pragma unroll
+        for (int t = 0; t < RT16; ++t) {
+            if (t0 + t >= ntile) break;
+            const QFrag q = qs[t * 64];
+            acc[0][t] = mfma_bf16(l0, q.hi, acc[0][t]);
+            acc[0][t] = mfma_bf16(h0, q.lo, acc[0][t]);
+            acc[0][t] = mfma_bf16(h0, q.hi, acc[0][t]);
+            acc[1][t] = mfma_bf16(l1, q.hi, acc[1][t]);
+            acc[1][t] = mfma_bf16(h1, q.lo, acc[1][t]);
+            acc[1][t] = mfma_bf16(h1, q.hi, acc[1][t]);
+        }
+    }
+    // D[4(l/16) + v][l%16]: rows n0 + 16 r + 4 lk + v, column 16 (t0 + t) + li
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = n0 + 16 * r + 4 * lk + v;
+            if (row >= n) continue;
+#pragma unroll
+            for (int t = 0; t < RT16; ++t) {
+                const int b = 16 * (t0 + t) + li;
+                if (t0 + t < ntile && b < k) Z[row * ldz + b] = acc[r][t][v];
+            }
+        }
 }
 
 __global__ void transpose_small_kernel(const double *__restrict__ A, double *__restrict__ AT,
@@ -355,15 +389,19 @@ int launch_rotate(ttdft_ctx ctx, const void *X, int64_t ldx, void *Z, int64_t ld
         TT_LAUNCH_CHECK();
         TT_TRY(timing_end(ctx, &t, 2.0 * n * k * 8.0 + kk * 8.0, 2.0 * n * kk));
     } else {
-        TT_TRY(ctx->scratch[4].ensure((size_t)kk * 2 * sizeof(short)));
-        short *Qh = (short *)ctx->scratch[4].ptr;
-        short *Ql = Qh + kk;
-        hipLaunchKernelGGL(split_q_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0,
-                           ctx->stream, dQ, kk, Qh, Ql);
+        constexpr int RT16 = 8;  // column tiles per wave
+        const int nstep = (k + 15) / 16, ntile = (k + 15) / 16;
+        const int64_t nfrag = (int64_t)nstep * ntile * 64;
+        TT_TRY(ctx->scratch[4].ensure((size_t)nfrag * sizeof(QFrag)));
+        QFrag *Qf = (QFrag *)ctx->scratch[4].ptr;
+        hipLaunchKernelGGL(split_q_frag_kernel, dim3((unsigned)((nfrag + 255) / 256)), dim3(256),
+                           0, ctx->stream, dQ, k, nstep, ntile, Qf);
         TT_LAUNCH_CHECK();
+        const int vec = (ldx % 4 == 0) && (reinterpret_cast<uintptr_t>(X) % 16 == 0);
+        const dim3 g((unsigned)((n + 127) / 128), (unsigned)((ntile + RT16 - 1) / RT16));
         TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ROTATE, &t));
-        hipLaunchKernelGGL(rotate_bf16x3_kernel, grid, dim3(256), 0, ctx->stream,
-                           (const float *)X, ldx, Qh, Ql, (float *)Z, ldz, n, k);
+        hipLaunchKernelGGL(rotate_bf16x3_frag_kernel<RT16>, g, dim3(256), 0, ctx->stream,
+                           (const float *)X, ldx, Qf, nstep, ntile, (float *)Z, ldz, n, k, vec);
         TT_LAUNCH_CHECK();
         TT_TRY(timing_end(ctx, &t, 2.0 * n * k * 4.0 + kk * 4.0, 3.0 * 2.0 * n * kk));
     }
