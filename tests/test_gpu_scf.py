@@ -59,3 +59,21 @@ def test_scf_mixed_precision_matches_fp64(ctx, reference_run):
     assert r3.converged
     assert s3.fp32_iterations > 0
     assert abs(r3.energy - r.energy) < 1e-5 * s.natoms
+
+
+def test_scf_reference_energy_form_agrees_at_self_consistency(ctx, reference_run):
+    """The reference's total energy at rho_out (main.cc:1368-1401, with a
+    second Hartree solve) and the Harris-Foulkes energy at rho_in are
+    different functionals that coincide at self-consistency: equal to 1e-6
+    Ha/atom at residual 1e-10, far apart at the first iteration.  The energy
+    form does not feed back into the iteration, so both runs follow the same
+    densities."""
+    s, r = reference_run
+    s2 = S.Scf(ctx, s.xyz, 3.0, small_config(energy_form="reference"))
+    print("scf fp64 with the reference's energy at rho_out", flush=True)
+    r2 = s2.run()
+    assert r2.converged and abs(r2.iterations - r.iterations) <= 1
+    m = min(len(r.residuals), len(r2.residuals)) - 1
+    assert np.allclose(r2.residuals[:m], r.residuals[:m], rtol=1e-6, atol=0.0)
+    assert abs(r2.energy - r.energy) < 1e-6 * s.natoms
+    assert abs(r2.energies[0] - r.energies[0]) > 1e-3

@@ -71,6 +71,10 @@ class ScfConfig:
     mixed: bool = False        # FP32 filter + split-BF16 GEMMs, then FP64 refinement
     refine_below: float = 1e-4  # residual at which a mixed run switches to FP64
     seed: int = 20261202
+    # total energy: "harris" (Harris-Foulkes functional of rho_in, one Hartree
+    # solve per iteration) or "reference" (the reference's evaluation at
+    # rho_out, main.cc:1368-1401, which needs V_H[rho_out]: a second solve)
+    energy_form: str = "harris"
 
 
 @dataclass
@@ -360,6 +364,28 @@ class Scf:
         for d_src in (d_vh, d_vxc):
             ctx.grid_axpby(d_veff.ptr, 1.0, d_src.ptr, 1.0, self.ngrid)
 
+    def energy_at_output(self, d_out, d_vh_in, d_vxc_in, e_band, refine=False):
+        """The reference's total energy (main.cc:1368-1401): with rho_out and
+        its Hartree potential,
+          E = int[eps_xc(rho_out) rho_out + 1/2 rho_out V_H[rho_out]
+                  - rho_out (V_eff,in - V_loc)] + E_nn + E_band,
+        V_eff,in - V_loc = V_H[rho_in] + V_xc[rho_in].  V_H[rho_out] comes
+        from the same Tucker compression and kernel expansion as V_H[rho_in]
+        (main.cc:1288-1302), with factors kept for rho_out."""
+        ctx, cfg = self.ctx, self.cfg
+        if not hasattr(self, "_out_bufs"):
+            self._out_bufs = [E.DeviceArray(ctx, (self.ngrid,)) for _ in range(3)]
+        d_vh, d_vxc, d_exc = self._out_bufs
+        core, U = self.tucker(d_out, cfg.rho_rank, "rho_out", refine)
+        ax = self.axes
+        ctx.hartree_expansion((cfg.rho_rank,) * 3, core.ptr, [u.ptr for u in U],
+                              [a.d_xq.ptr for a in ax], [a.d_jw.ptr for a in ax], self.p,
+                              [a.d_xq.ptr for a in ax], self.p, self.alpha, self.omega, d_vh.ptr)
+        ctx.lda_xc(d_out.ptr, self.ngrid, d_vxc.ptr, d_exc.ptr, variant=cfg.lda_variant)
+        return (self.integral(d_out, d_exc) + 0.5 * self.integral(d_out, d_vh)
+                - self.integral(d_out, d_vh_in) - self.integral(d_out, d_vxc_in)
+                + self.e_nn + e_band)
+
     def hamiltonian(self, d_veff, copies=E.HAM_F64, refine=False):
         cfg = self.cfg
         rv = cfg.veff_rank
@@ -436,12 +462,15 @@ class Scf:
             ef, occ = E.fermi_occupancy(ev, self.nelec, KB_HARTREE, cfg.temperature)
             ctx.density(dXc.ptr, k, k, occ, (R, R, R), *(a.d_B.ptr for a in self.axes), self.p,
                         d_out.ptr, dtype=dt)
-            # Harris-Foulkes energy of the input density
             e_band = float(E.band_energy(ev, occ))
-            e_h = 0.5 * self.integral(d_in, d_vh)
-            e_xc = self.integral(d_in, d_exc)
-            e_dc = self.integral(d_in, d_vh) + self.integral(d_in, d_vxc)
-            energy = e_band - e_dc + e_h + e_xc + self.e_nn
+            if cfg.energy_form == "reference":
+                energy = self.energy_at_output(d_out, d_vh, d_vxc, e_band, refine=it > 0)
+            else:
+                # Harris-Foulkes energy of the input density
+                e_h = 0.5 * self.integral(d_in, d_vh)
+                e_xc = self.integral(d_in, d_exc)
+                e_dc = self.integral(d_in, d_vh) + self.integral(d_in, d_vxc)
+                energy = e_band - e_dc + e_h + e_xc + self.e_nn
             energies.append(energy)
             res = ctx.grid_residual(d_out.ptr, d_in.ptr, self.p,
                                     *(a.d_jw.ptr for a in self.axes))
