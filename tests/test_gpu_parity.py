@@ -269,6 +269,40 @@ def test_gram_bf16x3(ctx, n, k):
     assert np.abs(dS.to_host().T - ref).max() / np.sqrt(n) < 5e-5
 
 
+@pytest.mark.parametrize("n,k,ld,same", [(5000, 300, 300, True), (5000, 300, 300, False),
+                                          (3001, 161, 163, True), (3001, 161, 163, False),
+                                          (777, 333, 336, True), (100, 17, 17, False),
+                                          (20000, 900, 900, True), (40, 160, 160, False)])
+def test_gram_bf16x3_shapes(ctx, n, k, ld, same):
+    """Split-BF16 Gram over its tilings: one to six 160-wide blocks per
+    dimension, k not a multiple of 16 or of 4, rows not 16-byte aligned (odd
+    leading dimension: scalar loads), n below one 16-row step and not a
+    multiple of it.  X^T X (same pointer) runs the upper-triangle form and
+    must come out exactly symmetric.
+
+    The bar is elementwise against |X|^T |Y|: the three-term split drops
+    lo*lo' (up to 2^-18 of each product) and rounds x = hi + lo to ~2^-17,
+    which on the diagonal of X^T X is a one-signed bias growing with n (a
+    numpy emulation of the split on the 5000 x 300 case gives 3.1e-6 of
+    |X|^T |X|, the same 0.0155 maximum error the kernel shows), so the
+    sqrt(n) scale of a random-sign sum does not bound it."""
+    rng = np.random.default_rng(n + k + same)
+    X = rng.standard_normal((n, ld)).astype(np.float32)
+    Y = X if same else rng.standard_normal((n, ld)).astype(np.float32)
+    dX = dev(ctx, X, np.float32)
+    dY = dX if same else dev(ctx, Y, np.float32)
+    dS = E.DeviceArray(ctx, (k, k))
+    print(f"gram bf16x3 n={n} k={k} ld={ld} same={same}", flush=True)
+    ctx.gram(dX.ptr, ld, dY.ptr, ld, n, k, dS.ptr, E.F32, E.GEMM_BF16X3)
+    S = dS.to_host().reshape(k, k).T
+    X64, Y64 = X[:, :k].astype(np.float64), Y[:, :k].astype(np.float64)
+    ref = X64.T @ Y64
+    scale = np.abs(X64).T @ np.abs(Y64)
+    assert np.all(np.abs(S - ref) <= 1e-5 * scale + 1e-12)
+    if same:
+        assert np.array_equal(S, S.T)
+
+
 @pytest.mark.parametrize("n,k,ld", [(1000, 24, 24), (5001, 131, 133), (3000, 300, 301),
                                      (2000, 300, 300), (129, 17, 20)])
 def test_rotate_bf16x3_shapes(ctx, n, k, ld):

@@ -40,6 +40,12 @@ __device__ __forceinline__ void split_bf16(float x, short &hi, short &lo) {
     lo = f2bf(x - bf2f(hi));
 }
 
+// One lane's split bf16 operand of a 16x16x16 MFMA: four values as hi and lo
+// halves (16 bytes).
+struct QFrag {
+    s4 hi, lo;
+};
+
 // ---------------------------------------------------------------------------
 // Gram, FP64.  Block = 4 waves = 64x64 tile of S; wave = 32x32 (2x2 MFMA
 // tiles).  MFMA operand layout (16x16x4 f64): A[i][kk] in lane i + 16 kk,
@@ -89,67 +95,189 @@ __global__ __launch_bounds__(256) void gram_f64_kernel(const double *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Gram, split-BF16.  MFMA 16x16x16 bf16: A[i][kk] in lane i + 16 (kk/4),
-// element kk%4; B[kk][j] in lane j + 16 (kk/4), element kk%4.  16 rows per
-// step.
-__global__ __launch_bounds__(256) void gram_bf16x3_kernel(const float *__restrict__ X,
-                                                          int64_t ldx,
-                                                          const float *__restrict__ Y,
-                                                          int64_t ldy, int64_t n, int32_t k,
-                                                          int64_t rows_per_chunk,
-                                                          double *__restrict__ P) {
-    const int ntile = (k + 63) / 64;
-    const int ta = blockIdx.x % ntile, tb = blockIdx.x / ntile;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int a0 = ta * 64 + (wave & 1) * 32, b0 = tb * 64 + (wave >> 1) * 32;
-    const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
-    const int64_t r1 = (r0 + rows_per_chunk) < n ? (r0 + rows_per_chunk) : n;
-    const int li = lane & 15, lk = lane >> 4;
-    const bool ma0 = a0 + li < k, ma1 = a0 + 16 + li < k;
-    const bool mb0 = b0 + li < k, mb1 = b0 + 16 + li < k;
-    f4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
-    for (int64_t r = r0; r < r1; r += 16) {
-        s4 xh0, xl0, xh1, xl1, yh0, yl0, yh1, yl1;
+// Gram, split-BF16, staged through LDS.  A workgroup (4 waves) owns a block of
+// 160 x 160 of S (10 x 10 tiles of 16) and a chunk of rows; a wave owns 5 x 5
+// tiles.  Per 16-row step the workgroup loads the block's 160 columns of X
+// and of Y once (float4 along a row when aligned), splits every value once
+// into bf16 hi + lo and stores it in LDS in fragment order: column c holds,
+// for each row quad q, {hi[4], lo[4]} of rows 4q..4q+3 as 16 bytes at
+// c * 80 + 16 q (the 80-byte column pitch keeps a quarter-wave's 16-byte
+// reads on distinct banks).  A fragment (lane i + 16 q) is then one 16-byte
+// LDS read; each A fragment serves 5 B fragments and each B fragment 5 A
+// fragments (3 MFMAs per pair).  The next step's global loads are issued
+// before the current step's MFMAs and two LDS stages alternate, so one
+// barrier per step suffices.
+//
+// With sym (X == Y) only blocks bi <= bj are launched and, within diagonal
+// blocks, only tiles ta <= tb are computed; the reduction reads S[a][b] from
+// the tile of (min(a,b), max(a,b)), so S comes out exactly symmetric.
+// Partials are written per chunk in tile order: tile (ta, tb) of chunk c at
+// P + (c * nt * nt + tb * nt + ta) * 256, lane l's accumulator as a float4 at
+// 4 l (element v = D[4 (l/16) + v][l % 16]).
+constexpr int kGramTiles = 5;                 // tiles per wave per dimension
+constexpr int kGramBlock = 32 * kGramTiles;   // block edge: 2 waves x 5 tiles x 16
+constexpr int kGramPitch = 80;                // LDS bytes per staged column
+constexpr int kGramItems = 2 * (kGramBlock / 4) * 4;  // (matrix, column quad, row quad)
+
+struct StageRegs {
+    f4 v[2][4];  // up to two items of 4 rows x 4 columns
+};
+
+__device__ __forceinline__ void gram_stage_load(const float *__restrict__ X, int64_t ldx,
+                                                const float *__restrict__ Y, int64_t ldy,
+                                                int64_t r, int64_t r1, int32_t k, int a0, int b0,
+                                                int vec, StageRegs &s) {
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int item = threadIdx.x + it * 256;
+        if (item >= kGramItems) break;
+        const int m = item / (kGramItems / 2);
+        const int rem = item % (kGramItems / 2);
+        const int cq = rem >> 2, q = rem & 3;
+        const float *M = m ? Y : X;
+        const int64_t ld = m ? ldy : ldx;
+        const int c = (m ? b0 : a0) + 4 * cq;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int64_t row = r + 4 * lk + v;
-            const bool ok = row < r1;
-            const float *xr = X + row * ldx + a0 + li;
-            const float *yr = Y + row * ldy + b0 + li;
-            short h, l;
-            split_bf16((ok && ma0) ? xr[0] : 0.0f, h, l);
-            xh0[v] = h; xl0[v] = l;
-            split_bf16((ok && ma1) ? xr[16] : 0.0f, h, l);
-            xh1[v] = h; xl1[v] = l;
-            split_bf16((ok && mb0) ? yr[0] : 0.0f, h, l);
-            yh0[v] = h; yl0[v] = l;
-            split_bf16((ok && mb1) ? yr[16] : 0.0f, h, l);
-            yh1[v] = h; yl1[v] = l;
-        }
-        acc00 = mfma_bf16(xl0, yh0, acc00);
-        acc00 = mfma_bf16(xh0, yl0, acc00);
-        acc00 = mfma_bf16(xh0, yh0, acc00);
-        acc01 = mfma_bf16(xl0, yh1, acc01);
-        acc01 = mfma_bf16(xh0, yl1, acc01);
-        acc01 = mfma_bf16(xh0, yh1, acc01);
-        acc10 = mfma_bf16(xl1, yh0, acc10);
-        acc10 = mfma_bf16(xh1, yl0, acc10);
-        acc10 = mfma_bf16(xh1, yh0, acc10);
-        acc11 = mfma_bf16(xl1, yh1, acc11);
-        acc11 = mfma_bf16(xh1, yl1, acc11);
-        acc11 = mfma_bf16(xh1, yh1, acc11);
-    }
-    double *Pc = P + (int64_t)blockIdx.y * k * k;
-    const int j = lane & 15;
+            const int64_t row = r + 4 * q + v;
+            f4 x = {0.f, 0.f, 0.f, 0.f};
+            if (row < r1) {
+                const float *p = M + row * ld + c;
+                if (vec && c + 3 < k) {
+                    x = *reinterpret_cast<const f4 *>(p);
+                } else {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int i = 4 * (lane >> 4) + v;
-        const int aA = a0 + i, aB = a0 + 16 + i, bA = b0 + j, bB = b0 + 16 + j;
-        if (aA < k && bA < k) Pc[aA + (int64_t)k * bA] = (double)acc00[v];
-        if (aA < k && bB < k) Pc[aA + (int64_t)k * bB] = (double)acc01[v];
-        if (aB < k && bA < k) Pc[aB + (int64_t)k * bA] = (double)acc10[v];
-        if (aB < k && bB < k) Pc[aB + (int64_t)k * bB] = (double)acc11[v];
+                    for (int j = 0; j < 4; ++j) x[j] = c + j < k ? p[j] : 0.f;
+                }
+            }
+            s.v[it][v] = x;
+        }
     }
+}
+
+__device__ __forceinline__ void gram_stage_store(unsigned char *lds, const StageRegs &s) {
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int item = threadIdx.x + it * 256;
+        if (item >= kGramItems) break;
+        const int m = item / (kGramItems / 2);
+        const int rem = item % (kGramItems / 2);
+        const int cq = rem >> 2, q = rem & 3;
+        unsigned char *base = lds + m * (kGramBlock * kGramPitch);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s4 hi, lo;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                short h, l;
+                split_bf16(s.v[it][v][j], h, l);
+                hi[v] = h;
+                lo[v] = l;
+            }
+            QFrag f;
+            f.hi = hi;
+            f.lo = lo;
+            *reinterpret_cast<QFrag *>(base + (4 * cq + j) * kGramPitch + 16 * q) = f;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void gram_bf16x3_lds_kernel(
+    const float *__restrict__ X, int64_t ldx, const float *__restrict__ Y, int64_t ldy,
+    int64_t n, int32_t k, int32_t nb, int32_t nt, int sym, int64_t rows_per_chunk, int vec,
+    float *__restrict__ P) {
+    __shared__ __align__(16) unsigned char lds[2][2 * kGramBlock * kGramPitch];
+    int bi, bj;
+    if (sym) {  // upper triangle of blocks, row by row
+        int p = blockIdx.x, len = nb;
+        bi = 0;
+        while (p >= len) {
+            p -= len;
+            ++bi;
+            --len;
+        }
+        bj = bi + p;
+    } else {
+        bi = blockIdx.x % nb;
+        bj = blockIdx.x / nb;
+    }
+    const int a0 = bi * kGramBlock, b0 = bj * kGramBlock;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wa = (wave & 1) * 16 * kGramTiles, wb = (wave >> 1) * 16 * kGramTiles;
+    const int ta0 = (a0 + wa) / 16, tb0 = (b0 + wb) / 16;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+    const int64_t r1 = (r0 + rows_per_chunk) < n ? (r0 + rows_per_chunk) : n;
+    // tile (ta0 + i, tb0 + j) is computed when inside S and, with sym, on or
+    // above the diagonal (wave-uniform)
+    bool on[kGramTiles][kGramTiles];
+#pragma unroll
+    for (int i = 0; i < kGramTiles; ++i)
+#pragma unroll
+        for (int j = 0; j < kGramTiles; ++j)
+            on[i][j] = ta0 + i < nt && tb0 + j < nt && (!sym || ta0 + i <= tb0 + j);
+    f4 acc[kGramTiles][kGramTiles];
+#pragma unroll
+    for (int i = 0; i < kGramTiles; ++i)
+#pragma unroll
+        for (int j = 0; j < kGramTiles; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+
+    const int li = lane & 15, lq = lane >> 4;
+    StageRegs sr;
+    gram_stage_load(X, ldx, Y, ldy, r0, r1, k, a0, b0, vec, sr);
+    int buf = 0;
+    for (int64_t r = r0; r < r1; r += 16) {
+        gram_stage_store(lds[buf], sr);
+        __syncthreads();
+        // the next step's loads fly while this step's MFMAs run
+        if (r + 16 < r1) gram_stage_load(X, ldx, Y, ldy, r + 16, r1, k, a0, b0, vec, sr);
+        const unsigned char *lx = lds[buf] + (wa + li) * kGramPitch + 16 * lq;
+        const unsigned char *ly = lds[buf] + kGramBlock * kGramPitch + (wb + li) * kGramPitch +
+                                  16 * lq;
+        QFrag fa[kGramTiles];
+#pragma unroll
+        for (int i = 0; i < kGramTiles; ++i)
+            fa[i] = *reinterpret_cast<const QFrag *>(lx + i * 16 * kGramPitch);
+#pragma unroll
+        for (int j = 0; j < kGramTiles; ++j) {
+            const QFrag fb = *reinterpret_cast<const QFrag *>(ly + j * 16 * kGramPitch);
+#pragma unroll
+            for (int i = 0; i < kGramTiles; ++i) {
+                if (!on[i][j]) continue;
+                acc[i][j] = mfma_bf16(fa[i].lo, fb.hi, acc[i][j]);
+                acc[i][j] = mfma_bf16(fa[i].hi, fb.lo, acc[i][j]);
+                acc[i][j] = mfma_bf16(fa[i].hi, fb.hi, acc[i][j]);
+            }
+        }
+        buf ^= 1;
+    }
+    float *Pc = P + (int64_t)blockIdx.y * nt * nt * 256;
+#pragma unroll
+    for (int i = 0; i < kGramTiles; ++i)
+#pragma unroll
+        for (int j = 0; j < kGramTiles; ++j)
+            if (on[i][j])
+                *reinterpret_cast<f4 *>(Pc + ((int64_t)(tb0 + j) * nt + ta0 + i) * 256 +
+                                        4 * lane) = acc[i][j];
+}
+
+// S[a + k b] = sum over chunks (FP64, fixed order) of the tile partials; with
+// sym the pair is read as (min(a,b), max(a,b)).
+__global__ void reduce_tile_partials_kernel(const float *__restrict__ P, int nchunks, int32_t k,
+                                            int32_t nt, int sym, double *__restrict__ S) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)k * k) return;
+    int a = (int)(idx % k), b = (int)(idx / k);
+    if (sym && a > b) {
+        const int t = a;
+        a = b;
+        b = t;
+    }
+    const int ta = a >> 4, tb = b >> 4, ai = a & 15, bj = b & 15;
+    const int64_t off = ((int64_t)tb * nt + ta) * 256 + 4 * (bj + 16 * (ai >> 2)) + (ai & 3);
+    const int64_t stride = (int64_t)nt * nt * 256;
+    double s = 0.0;
+    for (int c = 0; c < nchunks; ++c) s += (double)P[c * stride + off];
+    S[idx] = s;
 }
 
 __global__ void reduce_partials_kernel(const double *__restrict__ P, int nchunks, int64_t count,
@@ -210,9 +338,6 @@ __global__ __launch_bounds__(256) void rotate_f64_kernel(const double *__restric
 // 16x16x16 MFMA: fragment (step s, tile t) holds Q[16s + kk][16t + j] in lane
 // j + 16 (kk/4), element kk%4, as 8 shorts per lane (hi[4], lo[4]), zero past
 // k.  Qf[((s * ntile + t) * 64 + lane)].
-struct QFrag {
-    s4 hi, lo;
-};
 __global__ void split_q_frag_kernel(const double *__restrict__ Q, int32_t k, int32_t nstep,
                                     int32_t ntile, QFrag *__restrict__ Qf) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -335,33 +460,57 @@ int launch_gram(ttdft_ctx ctx, const void *X, int64_t ldx, const void *Y, int64_
     TT_CHECK((mode == TTDFT_GEMM_F64 && dtype == TTDFT_F64) ||
                  (mode == TTDFT_GEMM_BF16X3 && dtype == TTDFT_F32),
              TTDFT_EBADARG, "gram: mode %d does not match dtype %d", mode, dtype);
-    const int ntile = (k + 63) / 64;
-    const int tiles = ntile * ntile;
-    const int64_t step = (mode == TTDFT_GEMM_F64) ? 4 : 16;
-    int64_t nchunks = 2048 / tiles;
-    if (nchunks < 1) nchunks = 1;
-    const int64_t max_chunks = (n + 255) / 256;
-    if (nchunks > max_chunks) nchunks = max_chunks;
-    int64_t rows = (n + nchunks - 1) / nchunks;
-    rows = (rows + step - 1) / step * step;
-    nchunks = (n + rows - 1) / rows;
     const int64_t kk = (int64_t)k * k;
-    TT_TRY(ctx->scratch[2].ensure((size_t)nchunks * kk * sizeof(double)));
-    double *P = (double *)ctx->scratch[2].ptr;
-    const dim3 grid((unsigned)tiles, (unsigned)nchunks);
     PendingTiming t;
-    TT_TRY(timing_begin(ctx, TTDFT_KSTAT_GRAM, &t));
     if (mode == TTDFT_GEMM_F64) {
-        hipLaunchKernelGGL(gram_f64_kernel, grid, dim3(256), 0, ctx->stream, (const double *)X,
-                           ldx, (const double *)Y, ldy, n, k, rows, P);
+        const int ntile = (k + 63) / 64;
+        const int tiles = ntile * ntile;
+        int64_t nchunks = 2048 / tiles;
+        if (nchunks < 1) nchunks = 1;
+        const int64_t max_chunks = (n + 255) / 256;
+        if (nchunks > max_chunks) nchunks = max_chunks;
+        int64_t rows = (n + nchunks - 1) / nchunks;
+        rows = (rows + 3) / 4 * 4;
+        nchunks = (n + rows - 1) / rows;
+        TT_TRY(ctx->scratch[2].ensure((size_t)nchunks * kk * sizeof(double)));
+        double *P = (double *)ctx->scratch[2].ptr;
+        TT_TRY(timing_begin(ctx, TTDFT_KSTAT_GRAM, &t));
+        hipLaunchKernelGGL(gram_f64_kernel, dim3((unsigned)tiles, (unsigned)nchunks), dim3(256),
+                           0, ctx->stream, (const double *)X, ldx, (const double *)Y, ldy, n, k,
+                           rows, P);
+        TT_LAUNCH_CHECK();
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256),
+                           0, ctx->stream, P, (int)nchunks, kk, dS);
+        TT_LAUNCH_CHECK();
     } else {
-        hipLaunchKernelGGL(gram_bf16x3_kernel, grid, dim3(256), 0, ctx->stream,
-                           (const float *)X, ldx, (const float *)Y, ldy, n, k, rows, P);
+        // X^T X is computed as the upper triangle of blocks and mirrored
+        const int sym = (X == Y && ldx == ldy) ? 1 : 0;
+        const int nb = (k + kGramBlock - 1) / kGramBlock;
+        const int nt = (k + 15) / 16;
+        const int blocks = sym ? nb * (nb + 1) / 2 : nb * nb;
+        // about two workgroups per CU in all, each chunk at least 8 steps
+        int64_t nchunks = (512 + blocks - 1) / blocks;
+        const int64_t max_chunks = (n + 127) / 128;
+        if (nchunks > max_chunks) nchunks = max_chunks;
+        if (nchunks < 1) nchunks = 1;
+        int64_t rows = (n + nchunks - 1) / nchunks;
+        rows = (rows + 15) / 16 * 16;
+        nchunks = (n + rows - 1) / rows;
+        const size_t pbytes = (size_t)nchunks * nt * nt * 256 * sizeof(float);
+        TT_TRY(ctx->scratch[2].ensure(pbytes));
+        float *P = (float *)ctx->scratch[2].ptr;
+        const int vec = (ldx % 4 == 0) && (ldy % 4 == 0) &&
+                        (reinterpret_cast<uintptr_t>(X) % 16 == 0) &&
+                        (reinterpret_cast<uintptr_t>(Y) % 16 == 0);
+        TT_TRY(timing_begin(ctx, TTDFT_KSTAT_GRAM, &t));
+        hipLaunchKernelGGL(gram_bf16x3_lds_kernel, dim3((unsigned)blocks, (unsigned)nchunks),
+                           dim3(256), 0, ctx->stream, (const float *)X, ldx, (const float *)Y,
+                           ldy, n, k, nb, nt, sym, rows, vec, P);
+        TT_LAUNCH_CHECK();
+        hipLaunchKernelGGL(reduce_tile_partials_kernel, dim3((unsigned)((kk + 255) / 256)),
+                           dim3(256), 0, ctx->stream, P, (int)nchunks, k, nt, sym, dS);
+        TT_LAUNCH_CHECK();
     }
-    TT_LAUNCH_CHECK();
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0,
-                       ctx->stream, P, (int)nchunks, kk, dS);
-    TT_LAUNCH_CHECK();
     const double es = (double)dtype_size(dtype);
     TT_TRY(timing_end(ctx, &t, 2.0 * n * k * es + kk * 8.0,
                       2.0 * n * kk * (mode == TTDFT_GEMM_F64 ? 1.0 : 3.0)));
