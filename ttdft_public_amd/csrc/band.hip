@@ -362,6 +362,12 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     // The MFMAs of k-step ks are issued before the B operands of ks+1 are
     // read, so the LDS latency overlaps their execution.
     // With two planes the B operands of a k-step serve both planes' MFMAs.
+    // State tiles of this wave that hold a state below k (wave-uniform): in
+    // FP64 the tiles past it, padding of the last state group, get no MFMA
+    // (4% of the 172-atom product).  The FP32 kernel keeps its loop free of
+    // the branches, which cost it more than the skipped work (3%).
+    const int tv = (a.k - (s0 + sgw * S) + 15) / 16;
+    constexpr bool SKIP = sizeof(T) == 8;
     auto mfma_tiles = [&](const T *Xb, const T (&av)[ZB][KS], const bool (&on)[ZB], auto lo_c,
                           auto hi_c) {
         constexpr int LO = decltype(lo_c)::value, HI = decltype(hi_c)::value;
@@ -375,7 +381,8 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
                 if (on[zi]) {
 #pragma unroll
                     for (int t = 0; t < NT; ++t)
-                        acc[zi][LO + t] = mfma16x16x4(av[zi][ks], b[ks][t], acc[zi][LO + t]);
+                        if (!SKIP || LO + t < tv)
+                            acc[zi][LO + t] = mfma16x16x4(av[zi][ks], b[ks][t], acc[zi][LO + t]);
                 }
             if constexpr (true) {
                 if (ks + 1 < KS) read_ks(Xb, b, ks + 1, lo_c, hi_c);
@@ -518,16 +525,26 @@ int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
     }
     // Two lines in flight.  (Three fit in LDS for FP64 but measured slower
     // on the 172-atom workload: 8.2 ms against 7.7 ms per H application.)
-    constexpr int DEPTH = 2;
-    const int nthr = 64 * (a.gxb + a.nsplit);
-    if (a.gxb <= 8)
-        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 8, 1, ZB>),
-                           dim3((unsigned)blocks), dim3(nthr), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 16, 1, ZB>),
-                           dim3((unsigned)blocks), dim3(nthr), 0, ctx->stream, a);
-    TT_LAUNCH_CHECK();
-    return TTDFT_OK;
+    if constexpr (ZB == 1) {  // the two-plane form exists for state-group workgroups only
+        constexpr int DEPTH = 2;
+        const int nthr = 64 * (a.gxb + a.nsplit);
+        if (a.gxb <= 8)
+            hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 8, 1, ZB>),
+                               dim3((unsigned)blocks), dim3(nthr), 0, ctx->stream, a);
+        else
+            hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, DEPTH, 16, 1, ZB>),
+                               dim3((unsigned)blocks), dim3(nthr), 0, ctx->stream, a);
+        TT_LAUNCH_CHECK();
+        return TTDFT_OK;
+    }
+    set_error("band SpMM: two z planes need state-group workgroups");
+    return TTDFT_EBADARG;
+}
+
+// 16-byte loads need every X row 16-byte aligned
+template <typename T>
+bool band_wide_loads(const BrickArgs<T> &a) {
+    return (a.ldx * (int64_t)sizeof(T)) % 16 == 0 && (reinterpret_cast<uintptr_t>(a.X) & 15) == 0;
 }
 
 template <typename T, int KS>
@@ -536,12 +553,14 @@ int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups, int zb) {
              TTDFT_EBADARG, "band SpMM: %d x-bricks per chunk", a.gxb);
     TT_CHECK((int64_t)a.rx * a.ldx * (int64_t)sizeof(T) < ((int64_t)1 << 31), TTDFT_EBADARG,
              "band SpMM: one x-line of X exceeds 2 GiB");
-    // 16-byte loads need every X row 16-byte aligned
-    const bool wide = (a.ldx * (int64_t)sizeof(T)) % 16 == 0 &&
-                      (reinterpret_cast<uintptr_t>(a.X) & 15) == 0;
-    if (zb == 2)
-        return wide ? launch_db<T, KS, 16, 2>(ctx, a, cogroups)
-                    : launch_db<T, KS, 4, 2>(ctx, a, cogroups);
+    const bool wide = band_wide_loads(a);
+    if constexpr (KS <= (sizeof(T) == 8 ? 4 : 3)) {  // the two-plane forms that fit
+        if (zb == 2) {
+            TT_CHECK(wide && cogroups, TTDFT_EBADARG, "band SpMM: no two-plane form here");
+            return launch_db<T, KS, 16, 2>(ctx, a, cogroups);
+        }
+    }
+    TT_CHECK(zb == 1, TTDFT_EBADARG, "band SpMM: no two-plane form at KS = %d", KS);
     return wide ? launch_db<T, KS, 16, 1>(ctx, a, cogroups)
                 : launch_db<T, KS, 4, 1>(ctx, a, cogroups);
 }
@@ -580,10 +599,16 @@ int launch_brick(ttdft_ctx ctx, BrickArgs<T> a, int ks) {
     // z planes per workgroup: two for the FP64 state-group workgroups (the X
     // stage then serves twice the matrix-core work), else one.
     // TTDFT_BRICK_ZB=1 or 2 overrides.
-    int zb = cogroups ? 2 : 1;
+    // Two planes need the accumulators and A fragments of both in the 128
+    // registers a wave has at 15-16 waves per workgroup: that fits the FP64
+    // state-group form up to KS = 4 and the FP32 one up to KS = 3; beyond it
+    // the compiler moves the arrays to scratch memory (20x slower).  The 4-byte
+    // load form (unaligned rows) does not fit either.
+    const bool zb2_fits = cogroups && ks <= (sizeof(T) == 8 ? 4 : 3) && band_wide_loads(a);
+    int zb = zb2_fits ? 2 : 1;
     if (const char *env = std::getenv("TTDFT_BRICK_ZB")) {
         const int v = std::atoi(env);
-        if (v == 1 || v == 2) zb = v;
+        if (v == 1 || (v == 2 && zb2_fits)) zb = v;
     }
     a.nblocks = (int64_t)a.nsg * a.nxc * a.hyn * ((a.rz + zb - 1) / zb);
     switch (ks) {
