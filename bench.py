@@ -506,6 +506,85 @@ def bench_large(ctx, E, reps=5):
     return out
 
 
+def bench_large_step(ctx, E, with_density=True):
+    """One ChFSI step at the 500-atom shape: N = 70^3 = 343,000, 900 states,
+    Chebyshev degree 40, Kronecker half-bandwidths (7, 7, 7), 500 projectors
+    (seeded synthetic H), then Fermi occupancy and the density on 850^3
+    points.  FP64 and mixed (FP32 filter, split-BF16 Gram and rotation), each
+    timed on its second step (the first, untimed, runs the Lanczos bounds and
+    loads the code objects) from the same seeded start; the mixed form's
+    Ritz values and band energy are compared with the FP64 step's."""
+    w = WL.CLUSTER500
+    t0 = time.perf_counter()
+    s = WL.synth_hamiltonian(w)
+    n, k, nnz = s.n, w.states, s.nnz
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32)
+    del s
+    setup = time.perf_counter() - t0
+    P = w.quad
+    _, wts, B = WL.basis_tables(w.dims[0], P, w.box)
+    dB = E.DeviceArray.from_host(ctx, np.ascontiguousarray(B.T))
+    dW = E.DeviceArray.from_host(ctx, wts)
+    dRho = E.DeviceArray(ctx, (P, P, P)) if with_density else None
+    occ_index = w.occ_index()
+    out = {"workload": f"500-atom ChFSI step: N={n}, k={k}, m={w.degree}, {nnz} nonzeros "
+                       f"({nnz / n:.0f}/row), {w.nproj} projectors, Kronecker band storage"
+                       + (f", density on {P}^3 points" if with_density else ""),
+           "setup_seconds": round(setup, 1)}
+    evs = {}
+    for name, dt, npdt in (("fp64", E.F64, np.float64), ("mixed", E.F32, np.float32)):
+        dX = E.DeviceArray(ctx, (n, k), npdt)
+        ctx.random_subspace(dX.ptr, k, n, k, seed=9, dtype=dt)
+        ev = np.zeros(k)
+        st = E.new_state()
+
+        def step(i):
+            ctx.chfsi_step(ham, dX.ptr, k, k, w.degree, ev, st, i, occ_index, restart_times=1,
+                           precision=dt)
+            ef, occ = E.fermi_occupancy(ev, w.nelec)
+            if dRho is not None:
+                ctx.density(dX.ptr, k, k, occ, w.dims, dB.ptr, dB.ptr, dB.ptr, (P,) * 3,
+                            dRho.ptr, dt)
+            return occ
+
+        step(0)
+        ctx.synchronize()
+        ctx.reset_stats()
+        ctx.set_profiling(True)
+        t = time.perf_counter()
+        occ = step(1)
+        ctx.synchronize()
+        wall = time.perf_counter() - t
+        stats = {c: ctx.kernel_stats(c) for c in range(5)}
+        ctx.set_profiling(False)
+        sp = stats[E.KSTAT_SPMM]
+        r = {"ms_per_step": round(1e3 * wall, 1),
+             "phases_ms": {"spmm": round(sp["ms"], 1),
+                           "nl_project": round(stats[E.KSTAT_NLPROJ]["ms"], 1),
+                           "gram": round(stats[E.KSTAT_GRAM]["ms"], 2),
+                           "rotate": round(stats[E.KSTAT_ROTATE]["ms"], 2),
+                           "density": round(stats[E.KSTAT_DENSITY]["ms"], 1)},
+             "spmm_ms_per_application": round(sp["ms"] / sp["launches"], 2),
+             "spmm_useful_TFLOPs": round(sp["flops"] / (sp["ms"] * 1e-3) / 1e12, 1)}
+        if dRho is not None:
+            r["charge"] = ctx.grid_integral(dRho.ptr, (P,) * 3, dW.ptr, dW.ptr, dW.ptr)
+        evs[name] = (ev.copy(), E.band_energy(ev, occ))
+        out[name] = r
+        dX.free()
+        log(f"large step {name}: {r}")
+    out["mixed"]["speedup_vs_fp64"] = round(out["fp64"]["ms_per_step"]
+                                            / out["mixed"]["ms_per_step"], 3)
+    nocc = occ_index + 1
+    (e64, b64), (e32, b32) = evs["fp64"], evs["mixed"]
+    out["mixed"]["error_vs_fp64"] = {
+        "max_ritz_occupied_Ha": float(np.abs(e32[:nocc] - e64[:nocc]).max()),
+        "band_energy_per_atom_Ha": abs(b32 - b64) / w.natoms}
+    for d in (dB, dW) + ((dRho,) if dRho is not None else ()):
+        d.free()
+    ham.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -516,7 +595,7 @@ def main():
     ap.add_argument("--no-density", action="store_true")
     ap.add_argument("--mode", default="step",
                     choices=["step", "spmm", "assembly", "mixing", "hartree", "scf", "scf55",
-                             "large"],
+                             "large", "large_step"],
                     help="spmm: only 3 FP64 H applications and 3 copies of one 125000 x 300 "
                          "FP64 block (copy = calibration of rocprofv3 byte counters); "
                          "assembly: only the device-resident H assembly (3 runs); "
@@ -524,7 +603,8 @@ def main():
                          "hartree: only STHOSVD + Hartree potential (4 runs); "
                          "scf: only the 13-atom SCF (FP64, mixed, mixed + learned start); "
                          "scf55: only the 55-atom SCF; "
-                         "large: one H application at N = 343,000, k = 900 (FP64 and FP32)")
+                         "large: one H application at N = 343,000, k = 900 (FP64 and FP32); "
+                         "large_step: only the 500-atom ChFSI step (FP64 and mixed)")
     ap.add_argument("--no-hartree", action="store_true",
                     help="skip the STHOSVD + Hartree measurement")
     ap.add_argument("--no-mixing", action="store_true",
@@ -533,6 +613,8 @@ def main():
                     help="skip the H assembly measurement")
     ap.add_argument("--no-scf", action="store_true", help="skip the 13-atom SCF measurement")
     ap.add_argument("--no-scf55", action="store_true", help="skip the 55-atom SCF measurement")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 500-atom ChFSI step (N = 343,000, k = 900, m = 40)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU "
@@ -561,6 +643,10 @@ def main():
     if args.mode == "large":
         ctx = E.Context(0)
         log(json.dumps(bench_large(ctx, E)))
+        return
+    if args.mode == "large_step":
+        ctx = E.Context(0)
+        log(json.dumps(bench_large_step(ctx, E, not args.no_density)))
         return
 
     t0 = time.perf_counter()
@@ -802,6 +888,8 @@ def main():
         result["scf"] = bench_scf(ctx, E)
     if not args.no_scf55:
         result["scf55"] = bench_scf(ctx, E, 55)
+    if not args.no_large:
+        result["large_step"] = bench_large_step(ctx, E, not args.no_density)
     ctx.synchronize()
     print(json.dumps(result))
 
