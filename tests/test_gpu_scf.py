@@ -77,3 +77,18 @@ def test_scf_reference_energy_form_agrees_at_self_consistency(ctx, reference_run
     assert np.allclose(r2.residuals[:m], r.residuals[:m], rtol=1e-6, atol=0.0)
     assert abs(r2.energy - r.energy) < 1e-6 * s.natoms
     assert abs(r2.energies[0] - r.energies[0]) > 1e-3
+
+
+def test_scf_both_energy_forms_in_one_run(ctx, reference_run):
+    """energy_form="both" keeps the Harris-Foulkes energies the iteration
+    reports and fills energies_out with the rho_out form in the same run;
+    the two meet at self-consistency (1e-6 Ha/atom)."""
+    s, r = reference_run
+    s2 = S.Scf(ctx, s.xyz, 3.0, small_config(energy_form="both"))
+    print("scf fp64 with both energy forms", flush=True)
+    r2 = s2.run()
+    assert r2.converged and len(r2.energies_out) == len(r2.energies) == r2.iterations
+    m = min(len(r.energies), len(r2.energies))
+    assert np.allclose(r2.energies[:m], r.energies[:m], rtol=1e-9, atol=0.0)
+    assert abs(r2.energies_out[-1] - r2.energies[-1]) < 1e-6 * s.natoms
+    assert abs(r2.energies_out[0] - r2.energies[0]) > 1e-3

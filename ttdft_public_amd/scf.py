@@ -72,8 +72,9 @@ class ScfConfig:
     refine_below: float = 1e-4  # residual at which a mixed run switches to FP64
     seed: int = 20261202
     # total energy: "harris" (Harris-Foulkes functional of rho_in, one Hartree
-    # solve per iteration) or "reference" (the reference's evaluation at
-    # rho_out, main.cc:1368-1401, which needs V_H[rho_out]: a second solve)
+    # solve per iteration), "reference" (the reference's evaluation at
+    # rho_out, main.cc:1368-1401, which needs V_H[rho_out]: a second solve), or
+    # "both" (Harris drives `energies`, the rho_out form fills `energies_out`)
     energy_form: str = "harris"
 
 
@@ -90,6 +91,7 @@ class ScfResult:
     phase_seconds: dict = field(default_factory=dict)
     fell_back: bool = False
     probed: bool = False
+    energies_out: list = field(default_factory=list)  # rho_out form, per iteration
 
 
 def coulomb_expansion():
@@ -430,7 +432,7 @@ class Scf:
         occ_index = min(self.nelec // 2 + 10, k - 1)
         self.fp32_iterations = 0
         self.chfsi_passes = []  # filter passes of every iteration's ChFSI step
-        residuals, energies = [], []
+        residuals, energies, energies_out = [], [], []
         converged, ef = False, 0.0
         it = 0
         for it in range(cfg.max_iter):
@@ -463,8 +465,13 @@ class Scf:
             ctx.density(dXc.ptr, k, k, occ, (R, R, R), *(a.d_B.ptr for a in self.axes), self.p,
                         d_out.ptr, dtype=dt)
             e_band = float(E.band_energy(ev, occ))
+            self.energy_out = None
+            if cfg.energy_form in ("reference", "both"):
+                self.energy_out = self.energy_at_output(d_out, d_vh, d_vxc, e_band,
+                                                        refine=it > 0)
+                energies_out.append(self.energy_out)
             if cfg.energy_form == "reference":
-                energy = self.energy_at_output(d_out, d_vh, d_vxc, e_band, refine=it > 0)
+                energy = self.energy_out
             else:
                 # Harris-Foulkes energy of the input density
                 e_h = 0.5 * self.integral(d_in, d_vh)
@@ -489,6 +496,7 @@ class Scf:
                 r.iterations += 1
                 r.residuals = residuals + r.residuals
                 r.energies = energies + r.energies
+                r.energies_out = energies_out + r.energies_out
                 r.seconds = time.perf_counter() - t_start
                 r.fell_back = True
                 return r
@@ -499,7 +507,7 @@ class Scf:
         self.d_rho = d_out
         self.X, self.X_dtype, self.occ = dXc, dt, occ
         return ScfResult(converged, it + 1, residuals, energies, energies[-1], ef, ev.copy(),
-                         time.perf_counter() - t_start, phases)
+                         time.perf_counter() - t_start, phases, energies_out=energies_out)
 
     def density_on_uniform(self, n, d_out=None, box=None):
         """rho of the last iteration's states on an n^3 uniform grid spanning a
