@@ -590,3 +590,28 @@ def test_dense_storage_reuse_across_hamiltonians(ctx):
         ctx.ham_apply(ham, dX.ptr, 40, dY.ptr, 40, 40, E.F64)
         assert rel(dY.to_host(), oracle.ham_apply(H, X)) < 1e-13
         ham.free()
+
+
+@pytest.mark.parametrize("dims,bands,k", [((50, 6, 5), (6, 2, 2), 300), ((22, 9, 7), (3, 2, 3), 81),
+                                          ((50, 8, 9), (6, 3, 2), 96)])
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+def test_band_spmm_light_chunks_last_same_bits(ctx, monkeypatch, dims, bands, k, dtype):
+    """TTDFT_BRICK_ORDER=1 dispatches the ragged last x-chunk's workgroups
+    after all full ones on every XCD; every output is still computed once, by
+    the same code: bitwise the default order's result."""
+    s = E.SynthHam(*dims, *bands, 4, seed=sum(dims) + k)
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=True)
+    assert ham.format()[0] == 1
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = np.random.default_rng(k).standard_normal((s.n, k)).astype(npdt)
+    dX = dev(ctx, X, npdt)
+    out = {}
+    for order in ("0", "1"):
+        monkeypatch.setenv("TTDFT_BRICK_ORDER", order)
+        dY = dev(ctx, np.full((s.n, k), np.nan), npdt)  # every output must be written
+        ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, dtype)
+        out[order] = dY.to_host()
+    assert np.array_equal(out["0"], out["1"])
+    if dtype == E.F64:
+        H = oracle.Ham.from_synth(s)
+        assert rel(out["1"], oracle.ham_apply(H, X)) < 1e-13

@@ -79,6 +79,7 @@ struct BrickArgs {
     int64_t nblocks;
     T alpha, beta, delta;
     int use_beta, use_delta;
+    int32_t light_last;  // ragged last x-chunk's units after all full ones, per XCD
 };
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -219,13 +220,43 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
 
     const unsigned nb = gridDim.x;  // a multiple of 8
     const unsigned bid = blockIdx.x;
-    const unsigned L = (bid & 7u) * (nb >> 3) + (bid >> 3);
-    if ((int64_t)L >= a.nblocks) return;
-    int64_t t = L;
+    // Work unit (state group, x-chunk, y-brick row, plane group) of this
+    // workgroup.  Dispatch deals workgroups round-robin over the 8 XCDs, so
+    // XCD bid % 8 runs positions bid / 8 in order; each XCD gets a contiguous
+    // range of units.  With light_last the units of a ragged last x-chunk
+    // (fewer x-bricks, less work) come after all full ones on every XCD, so
+    // the kernel's last round is made of short workgroups.
+    int64_t t;
+    int xc;
+    if (a.light_last) {
+        const int64_t per_chunk = a.nblocks / a.nxc;  // units of one x-chunk
+        const int64_t H = per_chunk * (a.nxc - 1);
+        const int64_t hs = (H + 7) / 8, ls = (per_chunk + 7) / 8;
+        const int64_t p = bid >> 3, xcd = bid & 7u;
+        if (p < hs) {
+            const int64_t u = xcd * hs + p;
+            if (u >= H) return;
+            t = u / a.nsg;
+            xc = (int)(t % (a.nxc - 1));
+            t /= a.nxc - 1;
+            t = t * a.nsg + u % a.nsg;  // (sg, rest) packed again below
+        } else {
+            const int64_t u = xcd * ls + (p - hs);
+            if (u >= per_chunk) return;
+            xc = a.nxc - 1;
+            t = u;
+        }
+    } else {
+        const unsigned L = (bid & 7u) * (nb >> 3) + (bid >> 3);
+        if ((int64_t)L >= a.nblocks) return;
+        t = L;
+        const int64_t q = t / a.nsg;
+        xc = (int)(q % a.nxc);
+        t = (q / a.nxc) * a.nsg + t % a.nsg;
+    }
+    // t = sg + nsg (h + hyn zgroup)
     const int sg = (int)(t % a.nsg);
     t /= a.nsg;
-    const int xc = (int)(t % a.nxc);
-    t /= a.nxc;
     const int h = (int)(t % a.hyn);
     const int z0 = ZB * (int)(t / a.hyn);  // first z plane of the workgroup
     const int s0 = sg * ST;
@@ -515,7 +546,11 @@ struct CoShape<float, V> {
 template <typename T, int KS, int DB, int ZB>
 int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
     constexpr int NST = kStateTiles<T>();
-    const int64_t blocks = (a.nblocks + 7) / 8 * 8;
+    int64_t blocks = (a.nblocks + 7) / 8 * 8;
+    if (a.light_last) {  // 8 XCDs x (full-chunk positions + last-chunk positions)
+        const int64_t per_chunk = a.nblocks / a.nxc;
+        blocks = 8 * ((per_chunk * (a.nxc - 1) + 7) / 8 + (per_chunk + 7) / 8);
+    }
     if (cogroups == 1) {
         typedef CoShape<T, 0> C;
         hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, C::DEPTH, C::GXM, C::G, ZB>),
@@ -523,8 +558,9 @@ int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
         TT_LAUNCH_CHECK();
         return TTDFT_OK;
     }
-    // Two lines in flight.  (Three fit in LDS for FP64 but measured slower
-    // on the 172-atom workload: 8.2 ms against 7.7 ms per H application.)
+    // Two lines in flight.  (Three fit in LDS but measured slower on the
+    // 172-atom workload: FP64 8.2 ms against 7.7 ms per H application, FP32
+    // 4.47 against 4.24 ms.)
     if constexpr (ZB == 1) {  // the two-plane form exists for state-group workgroups only
         constexpr int DEPTH = 2;
         const int nthr = 64 * (a.gxb + a.nsplit);
@@ -611,6 +647,12 @@ int launch_brick(ttdft_ctx ctx, BrickArgs<T> a, int ks) {
         if (v == 1 || (v == 2 && zb2_fits)) zb = v;
     }
     a.nblocks = (int64_t)a.nsg * a.nxc * a.hyn * ((a.rz + zb - 1) / zb);
+    // Short workgroups last when the last x-chunk is ragged: the kernel's
+    // final round is then made of them (172-atom FP64 product 6.05 -> 5.83 ms;
+    // FP32 unchanged).  TTDFT_BRICK_ORDER=0 keeps the plain order.
+    int order = 1;
+    if (const char *env = std::getenv("TTDFT_BRICK_ORDER")) order = std::atoi(env) == 0 ? 0 : 1;
+    a.light_last = (order == 1 && a.nxc >= 2 && a.gxn % a.gxb != 0) ? 1 : 0;
     switch (ks) {
     case 2:
         return launch_ks<T, 2>(ctx, a, cogroups, zb);
