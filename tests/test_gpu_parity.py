@@ -256,6 +256,49 @@ def test_gram_f64(ctx, n, k):
     assert rel(S, oracle.gram(X, Y)) < 1e-13
 
 
+@pytest.mark.parametrize("n,k,ld,same", [(5000, 300, 300, True), (5000, 300, 300, False),
+                                          (3001, 161, 163, True), (3001, 161, 163, False),
+                                          (777, 333, 333, True), (100, 17, 17, False),
+                                          (20000, 900, 900, True), (40, 160, 161, False)])
+def test_gram_f64_shapes(ctx, n, k, ld, same):
+    """FP64 Gram over its tilings (160-wide blocks, 16-row steps staged in LDS):
+    ragged k and n, odd leading dimension (scalar loads), fewer rows than
+    one step.  X^T X takes the upper-triangle form and is exactly
+    symmetric."""
+    rng = np.random.default_rng(n + k + same)
+    X = rng.standard_normal((n, ld))
+    Y = X if same else rng.standard_normal((n, ld))
+    dX = dev(ctx, X)
+    dY = dX if same else dev(ctx, Y)
+    dS = E.DeviceArray(ctx, (k, k))
+    print(f"gram f64 n={n} k={k} ld={ld} same={same}", flush=True)
+    ctx.gram(dX.ptr, ld, dY.ptr, ld, n, k, dS.ptr)
+    S = dS.to_host().reshape(k, k).T
+    assert rel(S, oracle.gram(np.ascontiguousarray(X[:, :k]), np.ascontiguousarray(Y[:, :k]))) \
+        < 1e-13
+    if same:
+        assert np.array_equal(S, S.T)
+
+
+@pytest.mark.parametrize("n,k,ld", [(1000, 24, 24), (5001, 131, 133), (3000, 300, 301),
+                                     (2000, 300, 300), (129, 17, 20), (700, 900, 900)])
+def test_rotate_f64_shapes(ctx, n, k, ld):
+    """FP64 rotation from pre-arranged Q fragments: several groups of eight
+    column tiles, k not a multiple of 16 or 4, odd leading dimension (scalar
+    loads), n not a multiple of the 128-row workgroup.  Columns past k are
+    left alone."""
+    rng = np.random.default_rng(n + k + ld)
+    X = rng.standard_normal((n, ld))
+    Q = rng.standard_normal((k, k)) / np.sqrt(k)
+    dX = dev(ctx, X)
+    dQ = dev(ctx, Q.T.copy())  # column-major
+    print(f"rotate f64 n={n} k={k} ld={ld}", flush=True)
+    ctx.rotate(dX.ptr, ld, n, k, dQ.ptr, E.F64, E.GEMM_F64)
+    got = dX.to_host().reshape(n, ld)
+    assert rel(got[:, :k], oracle.rotate(np.ascontiguousarray(X[:, :k]), Q)) < 1e-13
+    assert np.array_equal(got[:, k:], X[:, k:])
+
+
 @pytest.mark.parametrize("n,k", [(512, 24), (5000, 300)])
 def test_gram_bf16x3(ctx, n, k):
     rng = np.random.default_rng(n)

@@ -47,54 +47,6 @@ struct QFrag {
 };
 
 // ---------------------------------------------------------------------------
-// Gram, FP64.  Block = 4 waves = 64x64 tile of S; wave = 32x32 (2x2 MFMA
-// tiles).  MFMA operand layout (16x16x4 f64): A[i][kk] in lane i + 16 kk,
-// B[kk][j] in lane j + 16 kk, D[i][j] in lane j + 16 (i%4), element i/4.
-// A[i][kk] = X[r+kk][a0+i], B[kk][j] = Y[r+kk][b0+j].
-__global__ __launch_bounds__(256) void gram_f64_kernel(const double *__restrict__ X,
-                                                       int64_t ldx,
-                                                       const double *__restrict__ Y,
-                                                       int64_t ldy, int64_t n, int32_t k,
-                                                       int64_t rows_per_chunk,
-                                                       double *__restrict__ P) {
-    const int ntile = (k + 63) / 64;
-    const int ta = blockIdx.x % ntile, tb = blockIdx.x / ntile;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int a0 = ta * 64 + (wave & 1) * 32, b0 = tb * 64 + (wave >> 1) * 32;
-    const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
-    const int64_t r1 = (r0 + rows_per_chunk) < n ? (r0 + rows_per_chunk) : n;
-    const int li = lane & 15, lk = lane >> 4;
-    const bool ma0 = a0 + li < k, ma1 = a0 + 16 + li < k;
-    const bool mb0 = b0 + li < k, mb1 = b0 + 16 + li < k;
-    d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
-    for (int64_t r = r0; r < r1; r += 4) {
-        const int64_t row = r + lk;
-        const bool ok = row < r1;
-        const double *xr = X + row * ldx + a0 + li;
-        const double *yr = Y + row * ldy + b0 + li;
-        const double xa0 = (ok && ma0) ? xr[0] : 0.0;
-        const double xa1 = (ok && ma1) ? xr[16] : 0.0;
-        const double yb0 = (ok && mb0) ? yr[0] : 0.0;
-        const double yb1 = (ok && mb1) ? yr[16] : 0.0;
-        acc00 = mfma_f64(xa0, yb0, acc00);
-        acc01 = mfma_f64(xa0, yb1, acc01);
-        acc10 = mfma_f64(xa1, yb0, acc10);
-        acc11 = mfma_f64(xa1, yb1, acc11);
-    }
-    double *Pc = P + (int64_t)blockIdx.y * k * k;
-    const int j = lane & 15;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int i = (lane >> 4) + 4 * v;
-        const int aA = a0 + i, aB = a0 + 16 + i, bA = b0 + j, bB = b0 + 16 + j;
-        if (aA < k && bA < k) Pc[aA + (int64_t)k * bA] = acc00[v];
-        if (aA < k && bB < k) Pc[aA + (int64_t)k * bB] = acc01[v];
-        if (aB < k && bA < k) Pc[aB + (int64_t)k * bA] = acc10[v];
-        if (aB < k && bB < k) Pc[aB + (int64_t)k * bB] = acc11[v];
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Gram, split-BF16, staged through LDS.  A workgroup (4 waves) owns a block of
 // 160 x 160 of S (10 x 10 tiles of 16) and a chunk of rows; a wave owns 5 x 5
 // tiles.  Per 16-row step the workgroup loads the block's 160 columns of X
@@ -260,9 +212,147 @@ __global__ __launch_bounds__(256, 2) void gram_bf16x3_lds_kernel(
                                         4 * lane) = acc[i][j];
 }
 
+// ---------------------------------------------------------------------------
+// Gram, FP64, staged through LDS.  The blocking of the split-BF16 kernel
+// above: a workgroup (4 waves) owns a 160 x 160 block of S and a chunk of
+// rows, a wave 5 x 5 tiles, 16-row steps, only blocks bi <= bj and tiles
+// ta <= tb for X^T X.  The step's rows are staged row-major (a row's 160
+// block columns contiguous, 32-byte stores from coalesced global loads).  In
+// a K-step lane i + 16 kk reads row 4 ks + kk, column i; the row pitch of
+// 1408 bytes (352 dwords, 32 mod 64) puts rows kk and kk + 1 on opposite bank
+// halves, so each half-wave's 32 reads hit 64 distinct banks.  Per 16-row
+// step a wave does 4 K-steps x (5 + 5) 8-byte LDS reads
+// and 100 MFMAs.  Partials: tile (ta, tb) of chunk c at
+// P + (c nt nt + tb nt + ta) 256, lane l's d4 at 4 l (D[l/16 + 4v][l%16]).
+constexpr int kG64Row = 176;  // doubles per staged row (160 + 16 padding)
+constexpr int kG64Items = 2 * 16 * (kGramBlock / 4);  // (matrix, row, column quad)
+
+struct Stage64 {
+    d4 v[5];  // up to five items of 4 consecutive columns of one row
+};
+
+__device__ __forceinline__ void gram64_stage_load(const double *__restrict__ X, int64_t ldx,
+                                                  const double *__restrict__ Y, int64_t ldy,
+                                                  int64_t r, int64_t r1, int32_t k, int a0,
+                                                  int b0, int vec, Stage64 &s) {
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const int item = threadIdx.x + it * 256;
+        if (item >= kG64Items) break;
+        const int m = item / (kG64Items / 2);
+        const int rem = item % (kG64Items / 2);
+        const int row = rem / (kGramBlock / 4), cq = rem % (kGramBlock / 4);
+        const double *M = m ? Y : X;
+        const int64_t ld = m ? ldy : ldx;
+        const int c = (m ? b0 : a0) + 4 * cq;
+        const int64_t gr = r + row;
+        d4 x = {0.0, 0.0, 0.0, 0.0};
+        if (gr < r1) {
+            const double *p = M + gr * ld + c;
+            if (vec && c + 3 < k) {
+                typedef double d2 __attribute__((ext_vector_type(2)));
+                const d2 lo = *reinterpret_cast<const d2 *>(p);
+                const d2 hi = *reinterpret_cast<const d2 *>(p + 2);
+                x = d4{lo[0], lo[1], hi[0], hi[1]};
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x[j] = c + j < k ? p[j] : 0.0;
+            }
+        }
+        s.v[it] = x;
+    }
+}
+
+__device__ __forceinline__ void gram64_stage_store(double *lds, const Stage64 &s) {
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const int item = threadIdx.x + it * 256;
+        if (item >= kG64Items) break;
+        const int m = item / (kG64Items / 2);
+        const int rem = item % (kG64Items / 2);
+        const int row = rem / (kGramBlock / 4), cq = rem % (kGramBlock / 4);
+        *reinterpret_cast<d4 *>(lds + (m * 16 + row) * kG64Row + 4 * cq) = s.v[it];
+    }
+}
+
+__global__ __launch_bounds__(256, 1) void gram_f64_lds_kernel(
+    const double *__restrict__ X, int64_t ldx, const double *__restrict__ Y, int64_t ldy,
+    int64_t n, int32_t k, int32_t nb, int32_t nt, int sym, int64_t rows_per_chunk, int vec,
+    double *__restrict__ P) {
+    __shared__ __align__(32) double lds[2][2 * 16 * kG64Row];
+    int bi, bj;
+    if (sym) {
+        int p = blockIdx.x, len = nb;
+        bi = 0;
+        while (p >= len) {
+            p -= len;
+            ++bi;
+            --len;
+        }
+        bj = bi + p;
+    } else {
+        bi = blockIdx.x % nb;
+        bj = blockIdx.x / nb;
+    }
+    const int a0 = bi * kGramBlock, b0 = bj * kGramBlock;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wa = (wave & 1) * 16 * kGramTiles, wb = (wave >> 1) * 16 * kGramTiles;
+    const int ta0 = (a0 + wa) / 16, tb0 = (b0 + wb) / 16;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+    const int64_t r1 = (r0 + rows_per_chunk) < n ? (r0 + rows_per_chunk) : n;
+    bool on[kGramTiles][kGramTiles];
+#pragma unroll
+    for (int i = 0; i < kGramTiles; ++i)
+#pragma unroll
+        for (int j = 0; j < kGramTiles; ++j)
+            on[i][j] = ta0 + i < nt && tb0 + j < nt && (!sym || ta0 + i <= tb0 + j);
+    d4 acc[kGramTiles][kGramTiles];
+#pragma unroll
+    for (int i = 0; i < kGramTiles; ++i)
+#pragma unroll
+        for (int j = 0; j < kGramTiles; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    const int li = lane & 15, lq = lane >> 4;
+    Stage64 sr;
+    gram64_stage_load(X, ldx, Y, ldy, r0, r1, k, a0, b0, vec, sr);
+    int buf = 0;
+    for (int64_t r = r0; r < r1; r += 16) {
+        gram64_stage_store(lds[buf], sr);
+        __syncthreads();
+        if (r + 16 < r1) gram64_stage_load(X, ldx, Y, ldy, r + 16, r1, k, a0, b0, vec, sr);
+        const double *lx = lds[buf] + lq * kG64Row + wa + li;
+        const double *ly = lds[buf] + (16 + lq) * kG64Row + wb + li;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            double fa[kGramTiles];
+#pragma unroll
+            for (int i = 0; i < kGramTiles; ++i) fa[i] = lx[4 * ks * kG64Row + 16 * i];
+#pragma unroll
+            for (int j = 0; j < kGramTiles; ++j) {
+                const double fb = ly[4 * ks * kG64Row + 16 * j];
+#pragma unroll
+                for (int i = 0; i < kGramTiles; ++i)
+                    if (on[i][j]) acc[i][j] = mfma_f64(fa[i], fb, acc[i][j]);
+            }
+        }
+        buf ^= 1;
+    }
+    double *Pc = P + (int64_t)blockIdx.y * nt * nt * 256;
+#pragma unroll
+    for (int i = 0; i < kGramTiles; ++i)
+#pragma unroll
+        for (int j = 0; j < kGramTiles; ++j)
+            if (on[i][j])
+                *reinterpret_cast<d4 *>(Pc + ((int64_t)(tb0 + j) * nt + ta0 + i) * 256 +
+                                        4 * lane) = acc[i][j];
+}
+
 // S[a + k b] = sum over chunks (FP64, fixed order) of the tile partials; with
-// sym the pair is read as (min(a,b), max(a,b)).
-__global__ void reduce_tile_partials_kernel(const float *__restrict__ P, int nchunks, int32_t k,
+// sym the pair is read as (min(a,b), max(a,b)).  Within a tile, D[ai][bj]
+// is element v of lane l: for the bf16 MFMA (P float) l = bj + 16 (ai/4),
+// v = ai%4; for the f64 MFMA (P double) l = bj + 16 (ai%4), v = ai/4.
+template <typename P_t>
+__global__ void reduce_tile_partials_kernel(const P_t *__restrict__ P, int nchunks, int32_t k,
                                             int32_t nt, int sym, double *__restrict__ S) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)k * k) return;
@@ -273,67 +363,16 @@ __global__ void reduce_tile_partials_kernel(const float *__restrict__ P, int nch
         b = t;
     }
     const int ta = a >> 4, tb = b >> 4, ai = a & 15, bj = b & 15;
-    const int64_t off = ((int64_t)tb * nt + ta) * 256 + 4 * (bj + 16 * (ai >> 2)) + (ai & 3);
+    const int lv = sizeof(P_t) == 4 ? 4 * (bj + 16 * (ai >> 2)) + (ai & 3)
+                                    : 4 * (bj + 16 * (ai & 3)) + (ai >> 2);
+    const int64_t off = ((int64_t)tb * nt + ta) * 256 + lv;
     const int64_t stride = (int64_t)nt * nt * 256;
     double s = 0.0;
     for (int c = 0; c < nchunks; ++c) s += (double)P[c * stride + off];
     S[idx] = s;
 }
 
-__global__ void reduce_partials_kernel(const double *__restrict__ P, int nchunks, int64_t count,
-                                       double *__restrict__ S) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= count) return;
-    double s = 0.0;
-    for (int c = 0; c < nchunks; ++c) s += P[(int64_t)c * count + idx];
-    S[idx] = s;
-}
-
 // ---------------------------------------------------------------------------
-// Rotation, FP64: Z = X Qt with Qt row-major (Qt[a*k + b] = Q[a][b]).
-// Block = 4 waves = 64 rows x 64 columns of Z; wave = 16 rows x 64 columns
-// (1x4 MFMA tiles).  A[i][kk] = X[n0+i][a+kk], B[kk][j] = Qt[a+kk][b0+j].
-__global__ __launch_bounds__(256) void rotate_f64_kernel(const double *__restrict__ X,
-                                                         int64_t ldx,
-                                                         const double *__restrict__ Qt,
-                                                         double *__restrict__ Z, int64_t ldz,
-                                                         int64_t n, int32_t k) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t n0 = (int64_t)blockIdx.x * 64 + wave * 16;
-    const int b0 = blockIdx.y * 64;
-    const int li = lane & 15, lk = lane >> 4;
-    const int64_t arow = n0 + li;
-    const bool rok = arow < n;
-    const double *xr = X + (rok ? arow : 0) * ldx;
-    bool mb[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) mb[t] = b0 + 16 * t + li < k;
-    d4 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = d4{0, 0, 0, 0};
-    for (int a = 0; a < k; a += 4) {
-        const int aa = a + lk;
-        const bool aok = aa < k;
-        const double xa = (rok && aok) ? xr[aa] : 0.0;
-        const double *qr = Qt + (int64_t)(aok ? aa : 0) * k + b0 + li;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const double qb = (aok && mb[t]) ? qr[16 * t] : 0.0;
-            acc[t] = mfma_f64(xa, qb, acc[t]);
-        }
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int64_t row = n0 + lk + 4 * v;
-        if (row >= n) continue;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int b = b0 + 16 * t + li;
-            if (b < k) Z[row * ldz + b] = acc[t][v];
-        }
-    }
-}
-
 // Q (k x k column-major FP64) split once into bf16 hi/lo B fragments of the
 // 16x16x16 MFMA: fragment (step s, tile t) holds Q[16s + kk][16t + j] in lane
 // j + 16 (kk/4), element kk%4, as 8 shorts per lane (hi[4], lo[4]), zero past
@@ -432,6 +471,92 @@ __global__ __launch_bounds__(256) void rotate_bf16x3_frag_kernel(
         }
 }
 
+// Q (k x k column-major FP64) in B-fragment order of the f64 16x16x4 MFMA,
+// the K index of each 16-row chunk permuted so that an A-operand lane reads
+// four consecutive columns of X: chunk s, K-step t, column tile c, lane
+// j + 16 kk holds Q[16 s + 4 kk + t][16 c + j] (zero past k) at
+// Qd[((s * 4 + t) * ntile + c) * 64 + lane].
+__global__ void q_frag64_kernel(const double *__restrict__ Q, int32_t k, int32_t nstep,
+                                int32_t ntile, double *__restrict__ Qd) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nstep * 4 * ntile * 64) return;
+    const int lane = (int)(idx & 63);
+    int64_t rest = idx >> 6;
+    const int c = (int)(rest % ntile);
+    rest /= ntile;
+    const int t = (int)(rest & 3), s = (int)(rest >> 2);
+    const int a = 16 * s + 4 * (lane >> 4) + t, b = 16 * c + (lane & 15);
+    Qd[idx] = (a < k && b < k) ? Q[a + (int64_t)k * b] : 0.0;
+}
+
+// Rotation, FP64: Z = X Q.  A wave computes 32 rows (two 16-row tiles) by
+// RT16 column tiles, a workgroup 128 rows.  Per 16-column chunk s of X a
+// lane loads its two rows' columns 16 s + 4 kk .. + 3 once (two 16-byte
+// loads each when aligned); K-step t of the chunk takes column 16 s + 4 kk + t,
+// so the four values are the A operands of the chunk's four K-steps.  Each
+// pre-arranged Q fragment (one 8-byte load per lane, from L2) serves both
+// row tiles.
+template <int RT16>
+__global__ __launch_bounds__(256) void rotate_f64_frag_kernel(
+    const double *__restrict__ X, int64_t ldx, const double *__restrict__ Qd, int32_t nstep,
+    int32_t ntile, double *__restrict__ Z, int64_t ldz, int64_t n, int32_t k, int vec) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t n0 = (int64_t)blockIdx.x * 128 + wave * 32;
+    const int t0 = blockIdx.y * RT16;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t row0 = n0 + li, row1 = n0 + 16 + li;
+    // rows past n read row n - 1; their products are never stored
+    const double *x0 = X + (row0 < n ? row0 : n - 1) * ldx;
+    const double *x1 = X + (row1 < n ? row1 : n - 1) * ldx;
+    d4 acc[2][RT16];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < RT16; ++c) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < nstep; ++s) {
+        const int a = 16 * s + 4 * lk;
+        d4 v0, v1;
+        if (vec && a + 3 < k) {  // rows 16-byte aligned (checked on the host)
+            const d2 p0 = *reinterpret_cast<const d2 *>(x0 + a);
+            const d2 p1 = *reinterpret_cast<const d2 *>(x0 + a + 2);
+            const d2 q0 = *reinterpret_cast<const d2 *>(x1 + a);
+            const d2 q1 = *reinterpret_cast<const d2 *>(x1 + a + 2);
+            v0 = d4{p0[0], p0[1], p1[0], p1[1]};
+            v1 = d4{q0[0], q0[1], q1[0], q1[1]};
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                v0[j] = a + j < k ? x0[a + j] : 0.0;
+                v1[j] = a + j < k ? x1[a + j] : 0.0;
+            }
+        }
+        const double *qs = Qd + ((int64_t)s * 4 * ntile + t0) * 64 + lane;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int c = 0; c < RT16; ++c) {
+                if (t0 + c >= ntile) break;
+                const double q = qs[((int64_t)t * ntile + c) * 64];
+                acc[0][c] = mfma_f64(v0[t], q, acc[0][c]);
+                acc[1][c] = mfma_f64(v1[t], q, acc[1][c]);
+            }
+    }
+    // D[l/16 + 4v][l%16]: rows n0 + 16 r + lk + 4 v, column 16 (t0 + c) + li
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = n0 + 16 * r + lk + 4 * v;
+            if (row >= n) continue;
+#pragma unroll
+            for (int c = 0; c < RT16; ++c) {
+                const int b = 16 * (t0 + c) + li;
+                if (t0 + c < ntile && b < k) Z[row * ldz + b] = acc[r][c][v];
+            }
+        }
+}
+
 __global__ void transpose_small_kernel(const double *__restrict__ A, double *__restrict__ AT,
                                        int32_t k) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -462,53 +587,46 @@ int launch_gram(ttdft_ctx ctx, const void *X, int64_t ldx, const void *Y, int64_
              TTDFT_EBADARG, "gram: mode %d does not match dtype %d", mode, dtype);
     const int64_t kk = (int64_t)k * k;
     PendingTiming t;
+    // X^T X is computed as the upper triangle of blocks and mirrored
+    const int sym = (X == Y && ldx == ldy) ? 1 : 0;
+    const int nb = (k + kGramBlock - 1) / kGramBlock;
+    const int nt = (k + 15) / 16;
+    const int blocks = sym ? nb * (nb + 1) / 2 : nb * nb;
+    // about two workgroups per CU in all, each chunk at least 8 steps
+    int64_t nchunks = (512 + blocks - 1) / blocks;
+    const int64_t max_chunks = (n + 127) / 128;
+    if (nchunks > max_chunks) nchunks = max_chunks;
+    if (nchunks < 1) nchunks = 1;
+    int64_t rows = (n + nchunks - 1) / nchunks;
+    rows = (rows + 15) / 16 * 16;
+    nchunks = (n + rows - 1) / rows;
+    const size_t es_p = mode == TTDFT_GEMM_F64 ? sizeof(double) : sizeof(float);
+    TT_TRY(ctx->scratch[2].ensure((size_t)nchunks * nt * nt * 256 * es_p));
+    const dim3 grid((unsigned)blocks, (unsigned)nchunks), red((unsigned)((kk + 255) / 256));
+    TT_TRY(timing_begin(ctx, TTDFT_KSTAT_GRAM, &t));
     if (mode == TTDFT_GEMM_F64) {
-        const int ntile = (k + 63) / 64;
-        const int tiles = ntile * ntile;
-        int64_t nchunks = 2048 / tiles;
-        if (nchunks < 1) nchunks = 1;
-        const int64_t max_chunks = (n + 255) / 256;
-        if (nchunks > max_chunks) nchunks = max_chunks;
-        int64_t rows = (n + nchunks - 1) / nchunks;
-        rows = (rows + 3) / 4 * 4;
-        nchunks = (n + rows - 1) / rows;
-        TT_TRY(ctx->scratch[2].ensure((size_t)nchunks * kk * sizeof(double)));
         double *P = (double *)ctx->scratch[2].ptr;
-        TT_TRY(timing_begin(ctx, TTDFT_KSTAT_GRAM, &t));
-        hipLaunchKernelGGL(gram_f64_kernel, dim3((unsigned)tiles, (unsigned)nchunks), dim3(256),
-                           0, ctx->stream, (const double *)X, ldx, (const double *)Y, ldy, n, k,
-                           rows, P);
+        const int vec = (ldx % 2 == 0) && (ldy % 2 == 0) &&
+                        (reinterpret_cast<uintptr_t>(X) % 16 == 0) &&
+                        (reinterpret_cast<uintptr_t>(Y) % 16 == 0);
+        hipLaunchKernelGGL(gram_f64_lds_kernel, grid, dim3(256), 0, ctx->stream,
+                           (const double *)X, ldx, (const double *)Y, ldy, n, k, nb, nt, sym,
+                           rows, vec, P);
         TT_LAUNCH_CHECK();
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256),
-                           0, ctx->stream, P, (int)nchunks, kk, dS);
+        hipLaunchKernelGGL(reduce_tile_partials_kernel<double>, red, dim3(256), 0, ctx->stream, P,
+                           (int)nchunks, k, nt, sym, dS);
         TT_LAUNCH_CHECK();
     } else {
-        // X^T X is computed as the upper triangle of blocks and mirrored
-        const int sym = (X == Y && ldx == ldy) ? 1 : 0;
-        const int nb = (k + kGramBlock - 1) / kGramBlock;
-        const int nt = (k + 15) / 16;
-        const int blocks = sym ? nb * (nb + 1) / 2 : nb * nb;
-        // about two workgroups per CU in all, each chunk at least 8 steps
-        int64_t nchunks = (512 + blocks - 1) / blocks;
-        const int64_t max_chunks = (n + 127) / 128;
-        if (nchunks > max_chunks) nchunks = max_chunks;
-        if (nchunks < 1) nchunks = 1;
-        int64_t rows = (n + nchunks - 1) / nchunks;
-        rows = (rows + 15) / 16 * 16;
-        nchunks = (n + rows - 1) / rows;
-        const size_t pbytes = (size_t)nchunks * nt * nt * 256 * sizeof(float);
-        TT_TRY(ctx->scratch[2].ensure(pbytes));
         float *P = (float *)ctx->scratch[2].ptr;
         const int vec = (ldx % 4 == 0) && (ldy % 4 == 0) &&
                         (reinterpret_cast<uintptr_t>(X) % 16 == 0) &&
                         (reinterpret_cast<uintptr_t>(Y) % 16 == 0);
-        TT_TRY(timing_begin(ctx, TTDFT_KSTAT_GRAM, &t));
-        hipLaunchKernelGGL(gram_bf16x3_lds_kernel, dim3((unsigned)blocks, (unsigned)nchunks),
-                           dim3(256), 0, ctx->stream, (const float *)X, ldx, (const float *)Y,
-                           ldy, n, k, nb, nt, sym, rows, vec, P);
+        hipLaunchKernelGGL(gram_bf16x3_lds_kernel, grid, dim3(256), 0, ctx->stream,
+                           (const float *)X, ldx, (const float *)Y, ldy, n, k, nb, nt, sym, rows,
+                           vec, P);
         TT_LAUNCH_CHECK();
-        hipLaunchKernelGGL(reduce_tile_partials_kernel, dim3((unsigned)((kk + 255) / 256)),
-                           dim3(256), 0, ctx->stream, P, (int)nchunks, k, nt, sym, dS);
+        hipLaunchKernelGGL(reduce_tile_partials_kernel<float>, red, dim3(256), 0, ctx->stream, P,
+                           (int)nchunks, k, nt, sym, dS);
         TT_LAUNCH_CHECK();
     }
     const double es = (double)dtype_size(dtype);
@@ -524,17 +642,21 @@ int launch_rotate(ttdft_ctx ctx, const void *X, int64_t ldx, void *Z, int64_t ld
                  (mode == TTDFT_GEMM_BF16X3 && dtype == TTDFT_F32),
              TTDFT_EBADARG, "rotate: mode %d does not match dtype %d", mode, dtype);
     const int64_t kk = (int64_t)k * k;
-    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((k + 63) / 64));
     PendingTiming t;
     if (mode == TTDFT_GEMM_F64) {
-        TT_TRY(ctx->scratch[4].ensure((size_t)kk * sizeof(double)));
-        double *Qt = (double *)ctx->scratch[4].ptr;
-        hipLaunchKernelGGL(transpose_small_kernel, dim3((unsigned)((kk + 255) / 256)), dim3(256),
-                           0, ctx->stream, dQ, Qt, k);
+        constexpr int RT16 = 8;  // column tiles per wave
+        const int nstep = (k + 15) / 16, ntile = (k + 15) / 16;
+        const int64_t nq = (int64_t)nstep * 4 * ntile * 64;
+        TT_TRY(ctx->scratch[4].ensure((size_t)nq * sizeof(double)));
+        double *Qd = (double *)ctx->scratch[4].ptr;
+        hipLaunchKernelGGL(q_frag64_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                           ctx->stream, dQ, k, nstep, ntile, Qd);
         TT_LAUNCH_CHECK();
+        const int vec = (ldx % 2 == 0) && (reinterpret_cast<uintptr_t>(X) % 16 == 0);
+        const dim3 g((unsigned)((n + 127) / 128), (unsigned)((ntile + RT16 - 1) / RT16));
         TT_TRY(timing_begin(ctx, TTDFT_KSTAT_ROTATE, &t));
-        hipLaunchKernelGGL(rotate_f64_kernel, grid, dim3(256), 0, ctx->stream,
-                           (const double *)X, ldx, Qt, (double *)Z, ldz, n, k);
+        hipLaunchKernelGGL(rotate_f64_frag_kernel<RT16>, g, dim3(256), 0, ctx->stream,
+                           (const double *)X, ldx, Qd, nstep, ntile, (double *)Z, ldz, n, k, vec);
         TT_LAUNCH_CHECK();
         TT_TRY(timing_end(ctx, &t, 2.0 * n * k * 8.0 + kk * 8.0, 2.0 * n * kk));
     } else {
