@@ -132,23 +132,27 @@ def test_band_spmm_matches_oracle(ctx, dims, bands, k, dtype):
 @pytest.mark.parametrize("dims,bands", [((20, 18, 16), (6, 6, 7)), ((13, 11, 9), (7, 3, 2)),
                                          ((9, 10, 7), (1, 2, 2))])
 @pytest.mark.parametrize("k", [257, 300, 400])
-def test_band_spmm_state_groups_in_one_workgroup(ctx, monkeypatch, dims, bands, k):
-    """FP64 with more than 256 states runs all state groups of a row block in
-    one workgroup (x-chunks of 3 bricks, the last one ragged); it must give
-    the same bits as one workgroup per state group, and match the oracle."""
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+def test_band_spmm_state_groups_in_one_workgroup(ctx, monkeypatch, dims, bands, k, dtype):
+    """More than 4 state groups' worth of states (FP64 > 256, FP32 > 240) runs
+    all state groups of a row block in one workgroup (x-chunks of 3 or 4
+    bricks, the last one ragged); it must give the same bits as one workgroup
+    per state group, and match the oracle."""
     s = E.SynthHam(*dims, *bands, 4, seed=sum(dims) + k)
     H = oracle.Ham.from_synth(s)
-    ham = ctx.ham_upload(s, E.HAM_F64, tucker=True)
+    ham = ctx.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=True)
     assert ham.format()[0] == 1
-    X = np.random.default_rng(k).standard_normal((s.n, k))
-    dX, dY = dev(ctx, X), E.DeviceArray(ctx, (s.n, k))
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = np.random.default_rng(k).standard_normal((s.n, k)).astype(npdt)
+    dX, dY = dev(ctx, X, npdt), E.DeviceArray(ctx, (s.n, k), npdt)
     out = {}
     for co in ("1", "0"):
         monkeypatch.setenv("TTDFT_BRICK_COGROUPS", co)
-        ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, E.F64)
+        ctx.ham_apply(ham, dX.ptr, k, dY.ptr, k, k, dtype)
         out[co] = dY.to_host()
     assert np.array_equal(out["1"], out["0"])
-    assert rel(out["1"], oracle.ham_apply(H, X)) < 1e-13
+    ref = oracle.ham_apply(H, X.astype(np.float64))
+    assert rel(out["1"].astype(np.float64), ref) < (1e-13 if dtype == E.F64 else 1e-5)
 
 
 @pytest.mark.parametrize("dims,bands,k", [((20, 18, 16), (6, 6, 7), 300),

@@ -191,7 +191,7 @@ __device__ __forceinline__ void lds_read_n(const T *p, T (&v)[N]) {
 // plane; each plane sees its lines in the same order as with ZB = 1, so
 // the sums are bitwise the same.
 template <typename T, int KS, int NST, int DB, int DEPTH, int GXM, int G, int ZB = 1>
-__global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
+__global__ __launch_bounds__(G > 1 ? 64 * G * GXM : 1024) void brick_spmm_kernel(BrickArgs<T> a) {
     typedef typename Acc4<T>::type acc_t;
     constexpr int S = 16 * NST;                  // states per wave
     constexpr int ST = G * S;                    // states per staged row
@@ -529,19 +529,29 @@ __global__ __launch_bounds__(1024) void brick_spmm_kernel(BrickArgs<T> a) {
     }
 }
 
-// Workgroup shapes with all state groups co-resident (G > 1): FP64 3 x-bricks
-// x 5 groups of 64 states, one line in flight (a 320-state stage is 61-72 KB);
-// FP32 4 x-bricks x 4 groups of 80 states, two lines in flight.
+// Workgroup shapes with all state groups co-resident (G > 1):
+//   FP64: 3 x-bricks x 5 groups of 64 states, one line in flight (two 61 KB
+//         stages of 320 states at KS = 4);
+//   FP32: 4 x-bricks x 4 groups of 80 states, one line in flight (two 36 KB
+//         stages).
+// Measured alternatives on the 172-atom product: FP64 with 2 x-bricks and two
+// lines in flight 6.48 ms against 5.80; FP32 with two lines in flight 3.87
+// against 3.49 ms.
 template <typename T, int V>
 struct CoShape;
 template <>
 struct CoShape<double, 0> {
     static constexpr int GXM = 3, G = 5, DEPTH = 1;
 };
-template <int V>
-struct CoShape<float, V> {
-    static constexpr int GXM = 4, G = 4, DEPTH = 2;
+template <>
+struct CoShape<float, 0> {
+    static constexpr int GXM = 4, G = 4, DEPTH = 1;
 };
+
+// Does the two-plane form of the state-group shape fit the registers (no
+// scratch) at KS?  Checked with the compiler's resource report: up to KS = 4
+// in both precisions.
+constexpr bool two_planes_fit(int ks) { return ks <= 4; }
 
 template <typename T, int KS, int DB, int ZB>
 int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
@@ -590,7 +600,7 @@ int launch_ks(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups, int zb) {
     TT_CHECK((int64_t)a.rx * a.ldx * (int64_t)sizeof(T) < ((int64_t)1 << 31), TTDFT_EBADARG,
              "band SpMM: one x-line of X exceeds 2 GiB");
     const bool wide = band_wide_loads(a);
-    if constexpr (KS <= (sizeof(T) == 8 ? 4 : 3)) {  // the two-plane forms that fit
+    if constexpr (two_planes_fit(KS)) {
         if (zb == 2) {
             TT_CHECK(wide && cogroups, TTDFT_EBADARG, "band SpMM: no two-plane form here");
             return launch_db<T, KS, 16, 2>(ctx, a, cogroups);
@@ -611,10 +621,10 @@ int launch_brick(ttdft_ctx ctx, BrickArgs<T> a, int ks) {
         if (v == 8 || v == 16) chunk = v;
     }
     // All state groups of a row block in one workgroup when there are more
-    // than G - 1 full groups: FP64 only (FP32 measured slower that way).
-    // TTDFT_BRICK_COGROUPS=0 turns it off, 1 on.
+    // than G - 1 full groups (172-atom product: FP64 5.80 against 7.43 ms,
+    // FP32 3.49 against 4.18 ms).  TTDFT_BRICK_COGROUPS=0 turns it off, 1 on.
     const int S = 16 * kStateTiles<T>();
-    int cogroups = (sizeof(T) == 8 && a.k > (CoShape<T, 0>::G - 1) * S) ? 1 : 0;
+    int cogroups = a.k > (CoShape<T, 0>::G - 1) * S ? 1 : 0;
     if (const char *env = std::getenv("TTDFT_BRICK_COGROUPS")) {
         const int v = std::atoi(env);
         if (a.k > (CoShape<T, 0>::G - 1) * S && (v == 0 || v == 1)) cogroups = v;
@@ -632,15 +642,15 @@ int launch_brick(ttdft_ctx ctx, BrickArgs<T> a, int ks) {
         a.nsg = (a.k + S - 1) / S;
         a.nsplit = a.gxb % 4 == 1 ? 1 : (a.gxb % 4 == 2 ? 2 : 0);
     }
-    // z planes per workgroup: two for the FP64 state-group workgroups (the X
+    // z planes per workgroup: two for the state-group workgroups (the X
     // stage then serves twice the matrix-core work), else one.
     // TTDFT_BRICK_ZB=1 or 2 overrides.
     // Two planes need the accumulators and A fragments of both in the 128
-    // registers a wave has at 15-16 waves per workgroup: that fits the FP64
-    // state-group form up to KS = 4 and the FP32 one up to KS = 3; beyond it
-    // the compiler moves the arrays to scratch memory (20x slower).  The 4-byte
+    // registers a wave has at 15-16 waves per workgroup: that fits the
+    // state-group forms (one line in flight) up to KS = 4; beyond it the
+    // compiler moves the arrays to scratch memory (20x slower).  The 4-byte
     // load form (unaligned rows) does not fit either.
-    const bool zb2_fits = cogroups && ks <= (sizeof(T) == 8 ? 4 : 3) && band_wide_loads(a);
+    const bool zb2_fits = cogroups && two_planes_fit(ks) && band_wide_loads(a);
     int zb = zb2_fits ? 2 : 1;
     if (const char *env = std::getenv("TTDFT_BRICK_ZB")) {
         const int v = std::atoi(env);
