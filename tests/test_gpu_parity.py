@@ -662,3 +662,25 @@ def test_band_spmm_light_chunks_last_same_bits(ctx, monkeypatch, dims, bands, k,
     if dtype == E.F64:
         H = oracle.Ham.from_synth(s)
         assert rel(out["1"], oracle.ham_apply(H, X)) < 1e-13
+
+
+@pytest.mark.parametrize("n,k,ld", [(3000, 40, 40), (5000, 300, 301)])
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+def test_random_subspace_matches_oracle(ctx, n, k, ld, dtype):
+    """The initial block (main.cc:807-829 with a seeded generator): the
+    library's N(0,1) sequence is the oracle's bit for bit, and the block is
+    two Cholesky-QR passes of it -- in FP64 the oracle's twice-orthonormalized
+    block to 1e-11, in FP32 (split-BF16 Gram, whose XᵀX diagonal carries a
+    2^-18 one-signed bias) orthonormal to 3e-5."""
+    seed = 1234 + k
+    assert np.array_equal(E.rng_normal(seed, 1000), oracle.rng_normal(seed, 1000))
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    dX = E.DeviceArray(ctx, (n, ld), npdt)
+    ctx.random_subspace(dX.ptr, ld, n, k, seed=seed, dtype=dtype)
+    X = dX.to_host().reshape(n, ld)[:, :k].astype(np.float64)
+    ref = oracle.orth(oracle.orth(oracle.rng_normal(seed, n * k).reshape(n, k)))
+    if dtype == E.F64:
+        assert rel(X, ref) < 1e-11
+    else:
+        assert rel(X, ref) < 1e-4
+    assert np.abs(X.T @ X - np.eye(k)).max() < (1e-12 if dtype == E.F64 else 3e-5)
