@@ -252,8 +252,8 @@ double ttdft_band_energy(const double *evals, const double *occ, int32_t k);
  * psi_i = sum_abc X[abc][i] Bx[qx,a] By[qy,b] Bz[qz,c]; occ is a host array,
  * Bx/By/Bz device (px x rx, ... column-major), rho device (written).
  * An FP64 block is contracted in FP64; an FP32 block (the mixed-precision
- * form) in FP32 with FP32 accumulation up to the y-factor density matrices,
- * which are expanded onto the grid in FP64 (about 1e-6 relative).
+ * form) in FP32 with FP32 accumulation throughout, rho widened to FP64 at
+ * the end (about 1e-6 relative).
  * replaces ProjectHamiltonianSparse::computeRhoOut
  * (dft/ProjectHamiltonianSparse.cc:911-1221, direct form of :956-983,1081-1104)
  * and the RhoComputer stub (dft/RhoComputer.cc:28-53). */

@@ -16,11 +16,12 @@
 //   4. m[qz][qx][pair] = upper triangle of T2[qz][qx] T2[qz][qx]^T, packed
 //                        (syrk_pack_kernel, MFMA)
 //   5. rho[qz]^T       = m[qz] E^T,  E[qy][pair] = (2 - [b=b']) By[qy,b] By[qy,b']
-//                        (rocBLAS FP64 GEMM per plane)
-// Steps 2-5 run over chunks of qz planes to bound the workspace.  Steps 1-4
+//                        (rocBLAS GEMM per plane)
+// Steps 2-5 run over chunks of qz planes to bound the workspace.  All steps
 // run in FP64 for an FP64 block and in FP32 (FP32 accumulation) for an FP32
-// block, which halves the T2 intermediate that dominates the traffic; m and
-// step 5 are FP64 in both.
+// block, which halves the T2 intermediate that dominates the traffic and
+// runs the products at the FP32 rate; an FP32 plane's rho is widened into
+// the FP64 output.
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
@@ -84,7 +85,7 @@ constexpr int tri_col(int p) {  // column b' of upper-triangle pair p (column-ma
 template <typename Tm, int RT, int P0, int NP>
 __global__ __launch_bounds__(256) void syrk_pack_kernel(const Tm *__restrict__ T2, int32_t R,
                                                          int32_t K, int64_t ncols,
-                                                         double *__restrict__ mp) {
+                                                         Tm *__restrict__ mp) {
     typedef typename std::conditional<sizeof(Tm) == 8, d4_t, f4_t>::type acc_t;
     constexpr int NBLK = tri_col(P0 + NP - 1) + 1;  // row blocks the tiles use
     static_assert(NBLK <= RT, "tile range");
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(256) void syrk_pack_kernel(const Tm *__restrict__ T
     // 16x16x4 D layout: element v of lane l is D[l/16 + 4v][l%16] (FP64),
     // D[4(l/16) + v][l%16] (FP32)
     const int64_t np = (int64_t)R * (R + 1) / 2;
-    double *out = mp + t * np;
+    Tm *out = mp + t * np;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
         const int p = P0 + q;
@@ -158,14 +159,14 @@ __global__ __launch_bounds__(256) void syrk_pack_kernel(const Tm *__restrict__ T
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int b = 16 * bi + (sizeof(Tm) == 8 ? kq + 4 * v : 4 * kq + v), bp = 16 * bj + i;
-            if (b <= bp && bp < R) out[(int64_t)bp * (bp + 1) / 2 + b] = (double)acc[q][v];
+            if (b <= bp && bp < R) out[(int64_t)bp * (bp + 1) / 2 + b] = acc[q][v];
         }
     }
 }
 
 template <typename Tm, int RT, int P0, int NP>
 int launch_syrk_group(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t ncols,
-                      double *mp) {
+                      Tm *mp) {
     const int64_t blocks = (ncols + 3) / 4;
     hipLaunchKernelGGL((syrk_pack_kernel<Tm, RT, P0, NP>), dim3((unsigned)blocks), dim3(256), 0,
                        ctx->stream, T2, R, K, ncols, mp);
@@ -176,7 +177,7 @@ int launch_syrk_group(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t
 // all upper-triangle tiles of an R x R result, at most 11 tiles per wave
 template <typename Tm>
 int launch_syrk_pack(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t ncols,
-                     double *mp) {
+                     Tm *mp) {
     switch ((R + 15) / 16) {
     case 1:
         return launch_syrk_group<Tm, 1, 0, 1>(ctx, T2, R, K, ncols, mp);
@@ -209,8 +210,9 @@ int launch_syrk_pack(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t 
 }
 
 // E[qy][pair(b,b')] = (b == b' ? 1 : 2) By[qy,b] By[qy,b'];  By column-major py x R
+template <typename Tm>
 __global__ void pair_table_kernel(const double *__restrict__ By, int32_t py, int32_t R,
-                                  double *__restrict__ E) {
+                                  Tm *__restrict__ E) {
     const int64_t np = (int64_t)R * (R + 1) / 2;
     const int64_t total = (int64_t)py * np;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -222,7 +224,7 @@ __global__ void pair_table_kernel(const double *__restrict__ By, int32_t py, int
         while (bp * (bp + 1) / 2 > pr) --bp;
         const int b = pr - bp * (bp + 1) / 2;
         const double v = By[qy + (int64_t)py * b] * By[qy + (int64_t)py * bp];
-        E[e] = b == bp ? v : 2.0 * v;
+        E[e] = (Tm)(b == bp ? v : 2.0 * v);
     }
 }
 
@@ -275,13 +277,20 @@ __global__ void to_float_kernel(const double *__restrict__ a, float *__restrict_
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) b[i] = (float)a[i];
 }
+__global__ void to_double_kernel(const float *__restrict__ a, double *__restrict__ b,
+                                 int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
 
 // Steps 1-5 with the T1 / T2 intermediates and their contractions in Tm.
 // Workspace (scratch 5, bytes): Cg (N K Tm), E (py np double), sqrt(w) (K
 // double), state indices (K int32, padded to K doubles), Bx / Bz in Tm (FP32
-// only), then per qz plane T1 (rx ry K Tm), T2 (px ry K Tm) and the packed
-// M (px np double), with 16 ry Tm of slack after T2 (read, never used, by
-// the last state chunk of the y contraction).
+// only), then per qz plane T1 (rx ry K Tm), T2 (px ry K Tm), the packed
+// M (px np Tm) and, for FP32, the plane's rho (px py float) before it is
+// widened into the FP64 output; 16 ry Tm of slack follow T2 (read, never
+// used, by the last state chunk of the y contraction).
 template <typename Tm>
 int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std::vector<int32_t> &idx,
                 const std::vector<double> &sw, int32_t rx, int32_t ry, int32_t rz,
@@ -293,7 +302,8 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
     const int64_t np = (int64_t)ry * (ry + 1) / 2;
     constexpr int64_t ES = sizeof(Tm);
     const bool f32 = ES == 4;
-    const int64_t per_plane = (rxy * K + (int64_t)px * ry * K) * ES + (int64_t)px * np * 8;
+    const int64_t per_plane = (rxy * K + (int64_t)px * ry * K + (int64_t)px * np) * ES +
+                              (f32 ? (int64_t)px * py * 4 : 0);
     const int64_t budget = (int64_t)6 << 30;  // bytes for the chunk buffers
     int64_t QC = budget / per_plane;
     if (QC < 1) QC = 1;
@@ -304,17 +314,18 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
         if (v > 0 && v < QC) QC = v;
     }
     auto align = [](int64_t b) { return (b + 255) / 256 * 256; };
-    const int64_t o_cg = 0, o_e = align(N * K * ES), o_sw = o_e + align(py * np * 8),
+    const int64_t o_cg = 0, o_e = align(N * K * ES), o_sw = o_e + align(py * np * ES),
                   o_idx = o_sw + align(K * 8), o_bx = o_idx + align(K * 8),
                   o_bz = o_bx + (f32 ? align((int64_t)px * rx * ES) : 0),
                   o_t1 = o_bz + (f32 ? align((int64_t)pz * rz * ES) : 0),
                   o_t2 = o_t1 + align(QC * rxy * K * ES),
                   o_mp = o_t2 + align(QC * px * ry * K * ES + 16 * ry * ES),
-                  total = o_mp + align(QC * px * np * 8);
+                  o_rf = o_mp + align(QC * px * np * ES),
+                  total = o_rf + (f32 ? align(QC * px * py * 4) : 0);
     TT_TRY(ctx->scratch[5].ensure((size_t)total));
     char *base = (char *)ctx->scratch[5].ptr;
     Tm *Cg = (Tm *)(base + o_cg);
-    double *E = (double *)(base + o_e);
+    Tm *E = (Tm *)(base + o_e);
     double *swd = (double *)(base + o_sw);
     int32_t *idxd = (int32_t *)(base + o_idx);
     const Tm *Bxm = (const Tm *)Bx, *Bzm = (const Tm *)Bz;
@@ -330,7 +341,8 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
     }
     Tm *T1 = (Tm *)(base + o_t1);
     Tm *T2 = (Tm *)(base + o_t2);
-    double *mp = (double *)(base + o_mp);
+    Tm *mp = (Tm *)(base + o_mp);
+    float *rho_f = (float *)(base + o_rf);  // FP32 only
 
     TT_HIP(hipMemcpyAsync(swd, sw.data(), sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
     TT_HIP(hipMemcpyAsync(idxd, idx.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice,
@@ -342,11 +354,10 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
         hipLaunchKernelGGL((gather_scaled_kernel<float, Tm>), dim3(grid_for(N * K)), dim3(256), 0,
                            ctx->stream, (const float *)X, ldx, idxd, swd, K, rx, ry, N, Cg);
     TT_LAUNCH_CHECK();
-    hipLaunchKernelGGL(pair_table_kernel, dim3(grid_for((int64_t)py * np)), dim3(256), 0,
+    hipLaunchKernelGGL(pair_table_kernel<Tm>, dim3(grid_for((int64_t)py * np)), dim3(256), 0,
                        ctx->stream, By, py, ry, E);
     TT_LAUNCH_CHECK();
 
-    const double one = 1.0, zero = 0.0;
     for (int64_t q0 = 0; q0 < pz; q0 += QC) {
         const int qc = (int)std::min<int64_t>(QC, pz - q0);
         // 2. T1[qzl][a][s][b] = sum_c Cg[c][a][s][b] Bz[q0 + qzl, c]
@@ -360,11 +371,21 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
                              (rocblas_int)((int64_t)ry * K), (int64_t)px * ry * K, qc));
         // 4. packed upper triangles of T2[qzl][qx] T2[qzl][qx]^T
         TT_TRY(launch_syrk_pack<Tm>(ctx, T2, ry, K, (int64_t)qc * px, mp));
-        // 5. rho[q0 + qzl](qx, qy) = sum_pair m[qzl][qx][pair] E[qy][pair]  (FP64)
-        TT_BLAS(rocblas_dgemm_strided_batched(
-            ctx->blas, rocblas_operation_transpose, rocblas_operation_none, px, py,
-            (rocblas_int)np, &one, mp, (rocblas_int)np, (int64_t)px * np, E, (rocblas_int)np, 0,
-            &zero, rho + q0 * (int64_t)px * py, px, (int64_t)px * py, qc));
+        // 5. rho[q0 + qzl](qx, qy) = sum_pair m[qzl][qx][pair] E[qy][pair]
+        if constexpr (sizeof(Tm) == 8) {
+            TT_BLAS(gemm_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+                                 px, py, (rocblas_int)np, mp, (rocblas_int)np, (int64_t)px * np,
+                                 E, (rocblas_int)np, 0, rho + q0 * (int64_t)px * py, px,
+                                 (int64_t)px * py, qc));
+        } else {
+            TT_BLAS(gemm_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+                                 px, py, (rocblas_int)np, mp, (rocblas_int)np, (int64_t)px * np,
+                                 E, (rocblas_int)np, 0, rho_f, px, (int64_t)px * py, qc));
+            const int64_t cnt = (int64_t)qc * px * py;
+            hipLaunchKernelGGL(to_double_kernel, dim3(grid_for(cnt)), dim3(256), 0, ctx->stream,
+                               rho_f, rho + q0 * (int64_t)px * py, cnt);
+            TT_LAUNCH_CHECK();
+        }
     }
     return TTDFT_OK;
 }
