@@ -90,8 +90,9 @@ struct ttdft_ctx_s {
     ttdft::KStat stats[TTDFT_KSTAT_COUNT];
     std::vector<ttdft::PendingTiming> *pending = nullptr;
     // scratch: [0] second X block for the filter / rotation, [1] small k x k
-    // work, [2] Gram partials, [3] nonlocal projections, [4..7] misc
-    ttdft::Scratch scratch[8];
+    // work, [2] Gram partials, [3] nonlocal projections, [4..7] misc, [8] partial
+    // sums of the split dense product
+    ttdft::Scratch scratch[9];
     // network-layer work: [0] transformed weights, [1] weight-gradient partials
     ttdft::Scratch nn_scratch[2];
     int nn_math = 0;  // TTDFT_NN_FP32 or TTDFT_NN_BF16 for the network's convolutions

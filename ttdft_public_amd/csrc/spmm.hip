@@ -388,6 +388,150 @@ __global__ __launch_bounds__(512) void dense_hx_kernel(
     }
 }
 
+// Dense H_loc X with X staged through LDS, in two kernels.
+// dense_hx_part_kernel: workgroup (p, row block, state block) owns 64 rows (a
+// 16-row tile per wave, 4 waves) by 16 JT states over part p of the inner
+// dimension l (nparts parts of whole 32-wide steps, so there are enough
+// workgroups to fill the chip).  Per step the workgroup copies X[l0 .. l0 +
+// 31][its states] into LDS once (coalesced loads), so every X value read
+// from L2 serves four row tiles; each wave loads its 16 rows of H as before
+// (lane (i, q): H[i][l0 + 8q .. 8q + 7]) and MFMA step u pairs l = l0 + 8q + u
+// in both operands, the X operand read from LDS (row 8q + u, 16 consecutive
+// states per quarter-wave; the row pitch of 16 JT + 18 elements puts the
+// quarter-waves' rows 8 apart on distinct banks: 8 pitches are 16 mod 64
+// dwords in FP32, 32 in FP64).  Even and odd steps accumulate separately.
+// The next step's X and H loads are issued before this step's products; the
+// X stage alternates between two buffers with one barrier per step.  The
+// part's sums go to P[p][r][c] in FP64.  Rows past n read row n - 1 (never
+// stored); X rows past n meet zero H padding.
+// dense_combine_kernel: Y = alpha (sum_p P[p] + C^T W) (+ beta X) (+ delta Z),
+// the parts summed in a fixed order.
+template <typename T, int JT>
+__global__ __launch_bounds__(256) void dense_hx_part_kernel(int64_t n, int64_t ldh, int32_t k,
+                                                            const T *__restrict__ H,
+                                                            const T *__restrict__ X,
+                                                            int64_t ldx, int32_t nparts,
+                                                            double *__restrict__ P) {
+    typedef typename DenseMma<T>::acc_t acc_t;
+    constexpr int NS = 16 * JT;                 // states per workgroup
+    constexpr int XR = NS + 18;                 // LDS row pitch (elements)
+    constexpr int NX = (32 * NS + 255) / 256;   // X elements per thread per step
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    __shared__ __align__(16) T xs[2][32 * XR];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, q = lane >> 4;
+    const int64_t i0 = (int64_t)blockIdx.x * 64 + wave * 16;
+    const int c0 = blockIdx.y * NS;
+    const int part = blockIdx.z;
+    const int64_t nst = ldh / 32;
+    const int64_t sb = nst * part / nparts, se = nst * (part + 1) / nparts;
+    const int64_t row = i0 + li < n ? i0 + li : n - 1;
+    const T *hrow = H + row * ldh + 8 * q;
+    int xoff[NX], xlds[NX], xrow[NX];
+    bool xon[NX];
+#pragma unroll
+    for (int e = 0; e < NX; ++e) {
+        const int idx = t + 256 * e;
+        xon[e] = idx < 32 * NS;
+        const int r = xon[e] ? idx / NS : 0, col = xon[e] ? idx % NS : 0;
+        const int c = c0 + col < k ? c0 + col : k - 1;
+        xoff[e] = c;
+        xrow[e] = r;
+        xlds[e] = r * XR + col;
+    }
+    T xv[NX];
+    v4 ha[2], hb[2];
+    auto load_x = [&](int64_t l0) {
+#pragma unroll
+        for (int e = 0; e < NX; ++e) {
+            if (!xon[e]) continue;
+            int64_t l = l0 + xrow[e];
+            l = l < n ? l : n - 1;
+            xv[e] = X[l * ldx + xoff[e]];
+        }
+    };
+    auto store_x = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < NX; ++e)
+            if (xon[e]) xs[buf][xlds[e]] = xv[e];
+    };
+    auto load_h = [&](int64_t l0, v4 (&h)[2]) {
+        h[0] = *reinterpret_cast<const v4 *>(hrow + l0);
+        h[1] = *reinterpret_cast<const v4 *>(hrow + l0 + 4);
+    };
+    acc_t acc[2][JT];
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int jt = 0; jt < JT; ++jt) acc[e][jt] = acc_t{0, 0, 0, 0};
+    auto mma = [&](int buf, const v4 (&h)[2]) {
+        const T *xb = xs[buf] + 8 * q * XR + li;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const T hu = h[u >> 2][u & 3];
+#pragma unroll
+            for (int jt = 0; jt < JT; ++jt)
+                acc[u & 1][jt] = DenseMma<T>::step(xb[u * XR + 16 * jt], hu, acc[u & 1][jt]);
+        }
+    };
+    if (sb < se) {
+        load_x(32 * sb);
+        load_h(32 * sb, ha);
+        store_x(0);
+        __syncthreads();
+        int buf = 0;
+        for (int64_t st = sb; st < se; ++st) {
+            const bool more = st + 1 < se;
+            if (more) {
+                load_x(32 * (st + 1));
+                load_h(32 * (st + 1), hb);
+            }
+            mma(buf, ha);
+            if (more) {
+                store_x(buf ^ 1);
+                ha[0] = hb[0];
+                ha[1] = hb[1];
+            }
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    // D[j][i]: element v of lane L holds state DenseMma::state(L/16, v), row L%16
+    const int64_t r = i0 + li;
+    if (r >= n) return;
+    double *Pr = P + ((int64_t)part * n + r) * k;
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = c0 + 16 * jt + DenseMma<T>::state(q, v);
+            if (c < k) Pr[c] = (double)acc[0][jt][v] + (double)acc[1][jt][v];
+        }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void dense_combine_kernel(
+    int64_t n, int32_t k, int32_t nparts, const double *__restrict__ P, const T *X, int64_t ldx,
+    const int64_t *__restrict__ ct_rowptr, const int32_t *__restrict__ ct_col,
+    const T *__restrict__ ct_val, const T *__restrict__ W, T *Y, int64_t ldy, const T *Z,
+    int64_t ldz, double alpha, double beta, double delta, int use_beta, int use_delta) {
+    const int64_t total = n * k;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / k;
+        const int c = (int)(e - r * k);
+        double s = P[e];
+        for (int p = 1; p < nparts; ++p) s += P[(int64_t)p * total + e];
+        if (ct_rowptr)
+            for (int64_t f = ct_rowptr[r]; f < ct_rowptr[r + 1]; ++f)
+                s += (double)ct_val[f] * (double)W[(int64_t)ct_col[f] * k + c];
+        double y = s * alpha;
+        if (use_beta) y += beta * (double)X[r * ldx + c];
+        if (use_delta) y += delta * (double)Z[r * ldz + c];
+        Y[r * ldy + c] = (T)y;
+    }
+}
+
 __global__ __launch_bounds__(256) void dense_to_f32_kernel(const double *__restrict__ s,
                                                            float *__restrict__ d, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -595,17 +739,50 @@ int finish_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
     return TTDFT_OK;
 }
 
+constexpr int kDenseParts = 4;  // most parts of l in the split dense product
+
+// 0: one-kernel dense product; 1: split product, parts picked; 2..4: split
+// product with that many parts
+int dense_split_form(int dtype) {
+    const char *env = std::getenv("TTDFT_DENSE_LDS");
+    return env ? std::atoi(env) : (dtype == TTDFT_F32 ? 1 : 0);
+}
+
 template <typename T, int JT>
 static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const T *ctv,
                          const void *X, int64_t ldx, const void *W, void *Y, int64_t ldy,
                          const void *Z, int64_t ldz, int32_t k, double alpha, double beta,
-                         double delta, int use_beta, int use_delta) {
+                         double delta, int use_beta, int use_delta, double *P) {
     const int64_t n = H->n;
     const dim3 grid((unsigned)((n + 15) / 16), (unsigned)((k + 16 * JT - 1) / (16 * JT)));
     const int64_t *ctr = H->p > 0 && !H->dense_nl ? H->ct_rowptr : nullptr;
     // six-state runs per lane, fetched as pairs, when every tile is full and
     // every row of X starts on a pair boundary (even leading dimension and a
     // pair-aligned X: a filter block may start at an odd column)
+    // FP32: the split product with X staged in LDS (27,000 x 96: 2.54 -> 1.80
+    // ms); FP64 keeps the one-kernel product, whose registers the split form
+    // exceeds (3.27 against 4.73 ms).  TTDFT_DENSE_LDS=0 turns the split off,
+    // 1 picks the parts, 2..4 fixes them.
+    const int lds_form = dense_split_form(sizeof(T) == 8 ? TTDFT_F64 : TTDFT_F32);
+    if (lds_form >= 1 && P) {
+        // parts of l for about 4.5 workgroups per CU (measured best at 27,000)
+        const int64_t wgs = (n + 63) / 64 * grid.y;
+        int nparts = (int)((1152 + wgs - 1) / wgs);
+        if (lds_form > 1) nparts = lds_form;
+        const int64_t nst = H->dense_ld / 32;
+        if (nparts > nst) nparts = (int)nst;
+        if (nparts > kDenseParts) nparts = kDenseParts;
+        const dim3 g((unsigned)((n + 63) / 64), grid.y, (unsigned)nparts);
+        hipLaunchKernelGGL((dense_hx_part_kernel<T, JT>), g, dim3(256), 0, ctx->stream, n,
+                           H->dense_ld, k, Hd, (const T *)X, ldx, nparts, P);
+        const int64_t total = n * k;
+        const unsigned cb = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL((dense_combine_kernel<T>), dim3(cb), dim3(256), 0, ctx->stream, n, k,
+                           nparts, P, (const T *)X, ldx, ctr, H->ct_col, ctv, (const T *)W,
+                           (T *)Y, ldy, (const T *)Z, ldz, alpha, beta, delta, use_beta,
+                           use_delta);
+        return;
+    }
     if (JT == 6 && k % (16 * JT) == 0 && ldx % 2 == 0 &&
         reinterpret_cast<uintptr_t>(X) % (2 * sizeof(T)) == 0)
         hipLaunchKernelGGL((dense_hx_kernel<T, JT, (JT == 6)>), grid, dim3(64 * kDenseWaves), 0,
@@ -624,7 +801,7 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
 static int launch_dense_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx,
                              const void *W, void *Y, int64_t ldy, const void *Z, int64_t ldz,
                              int32_t k, int dtype, double alpha, double beta, double delta,
-                             int use_beta, int use_delta) {
+                             int use_beta, int use_delta, double *P) {
     // fewest workgroup columns of at most 96 states, then the fewest 16-state
     // tiles that cover k in that many columns (each column streams H once)
     const int ncol = (k + 95) / 96;
@@ -633,10 +810,10 @@ static int launch_dense_spmm(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X,
     case JTV:                                                                                   \
         if (dtype == TTDFT_F64)                                                                 \
             dense_launch<double, JTV>(ctx, H, H->dense64, H->ct_val64, X, ldx, W, Y, ldy, Z,    \
-                                      ldz, k, alpha, beta, delta, use_beta, use_delta);         \
+                                      ldz, k, alpha, beta, delta, use_beta, use_delta, P);      \
         else                                                                                    \
             dense_launch<float, JTV>(ctx, H, H->dense32, H->ct_val32, X, ldx, W, Y, ldy, Z,     \
-                                     ldz, k, alpha, beta, delta, use_beta, use_delta);          \
+                                     ldz, k, alpha, beta, delta, use_beta, use_delta, P);       \
         break;
     switch (jt) {
         DENSE_CASE(1)
@@ -686,9 +863,27 @@ int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, vo
     if (band)
         TT_TRY(launch_band_spmm(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
                                 alpha, beta, delta, use_beta, use_delta));
-    else if (dense)
+    else if (dense) {
+        // partial sums of the split product: after the projections in the
+        // caller's work space, else from the context
+        double *P = nullptr;
+        if (dense_split_form(dtype) >= 1) {
+            const size_t need_p = (size_t)kDenseParts * H->n * k * sizeof(double);
+            if (ws) {
+                const size_t off = W ? ((size_t)nl_workspace(H, k) * es + 255) / 256 * 256 : 0;
+                TT_CHECK(off + need_p <= ws_bytes, TTDFT_EBADARG,
+                         "the dense product of %lld functions x %d states needs %zu bytes of "
+                         "work space, more than the %zu available",
+                         (long long)H->n, k, off + need_p, ws_bytes);
+                P = (double *)((char *)ws + off);
+            } else {
+                TT_TRY(ctx->scratch[8].ensure(need_p));
+                P = (double *)ctx->scratch[8].ptr;
+            }
+        }
         TT_TRY(launch_dense_spmm(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
-                                 alpha, beta, delta, use_beta, use_delta));
+                                 alpha, beta, delta, use_beta, use_delta, P));
+    }
     else
         TT_TRY(launch_spmm_fused(ctx, H, X, ldx, W, Y, ldy, Z ? Z : Y, Z ? ldz : ldy, k, dtype,
                                  alpha, beta, delta, use_beta, use_delta));
