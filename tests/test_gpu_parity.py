@@ -506,6 +506,36 @@ def test_density_wide_y(ctx, ry, dtype):
     assert np.linalg.norm(dRho.to_host() - ref) / np.linalg.norm(ref) < tol
 
 
+@pytest.mark.parametrize("dtype", [E.F64, E.F32])
+@pytest.mark.parametrize("rx,ry,k", [(13, 16, 9), (30, 20, 17), (50, 40, 8), (61, 64, 33),
+                                     (4, 50, 3)])
+def test_density_fused_shapes(ctx, monkeypatch, rx, ry, k, dtype):
+    """The fused x + y-factor step over its tile classes (a steps of 4, 8, 13, 16; 1-4 blocks
+    of 16 b), a ragged last qx tile and ragged state chunks, against the oracle and against
+    the two-step form (x GEMM to memory, then the packed y-factor products)."""
+    rng = np.random.default_rng(rx * 100 + ry)
+    rz = 2
+    px, py, pz = 37, 11, 3
+    Bx, By, Bz = (rng.standard_normal((p, r)) for p, r in ((px, rx), (py, ry), (pz, rz)))
+    npdt = np.float64 if dtype == E.F64 else np.float32
+    X = rng.standard_normal((rx * ry * rz, k)).astype(npdt)
+    occ = rng.uniform(0.1, 1.0, k)
+    dX = dev(ctx, X, npdt)
+    dBx, dBy, dBz = dev(ctx, Bx.T.copy()), dev(ctx, By.T.copy()), dev(ctx, Bz.T.copy())
+    got = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("TTDFT_DENSITY_FUSED", fused)
+        dRho = E.DeviceArray(ctx, (pz, py, px))
+        ctx.density(dX.ptr, k, k, occ, (rx, ry, rz), dBx.ptr, dBy.ptr, dBz.ptr, (px, py, pz),
+                    dRho.ptr, dtype)
+        got[fused] = dRho.to_host()
+    ref = oracle.density(X.astype(np.float64), occ, (rx, ry, rz), Bx, By, Bz)
+    tol = 1e-12 if dtype == E.F64 else 1e-5
+    for f in ("1", "0"):
+        assert np.linalg.norm(got[f] - ref) / np.linalg.norm(ref) < tol, f
+    assert np.linalg.norm(got["1"] - got["0"]) / np.linalg.norm(ref) < tol
+
+
 def test_initial_density_and_integrals(ctx):
     r = np.linspace(0.0, 9.0, 901)
     rho_r = np.exp(-r**2) * (1 + 0.3 * r)

@@ -12,16 +12,16 @@
 // Every step is a dense matrix product on the matrix cores:
 //   1. Cg[c][a][s][b]  = sqrt(w_s) X[a + rx b + rx ry c][idx_s]   (gather)
 //   2. T1[qz][a][s][b] = sum_c Bz[qz,c] Cg[c][a][s][b]   (rocBLAS GEMM)
-//   3. T2[qz][qx][s][b]= sum_a Bx[qx,a] T1[qz][a][s][b]  (rocBLAS GEMM per plane)
+//   3. T2[qz][qx][s][b]= sum_a Bx[qx,a] T1[qz][a][s][b]
 //   4. m[qz][qx][pair] = upper triangle of T2[qz][qx] T2[qz][qx]^T, packed
-//                        (syrk_pack_kernel, MFMA)
+//      3 + 4 fused in xsyrk_kernel (rx, ry <= 64): T2 lives in LDS only.
+//      Above 64: a rocBLAS GEMM per plane writes T2, syrk_pack_kernel reads it.
 //   5. rho[qz]^T       = m[qz] E^T,  E[qy][pair] = (2 - [b=b']) By[qy,b] By[qy,b']
 //                        (rocBLAS GEMM per plane)
 // Steps 2-5 run over chunks of qz planes to bound the workspace.  All steps
 // run in FP64 for an FP64 block and in FP32 (FP32 accumulation) for an FP32
-// block, which halves the T2 intermediate that dominates the traffic and
-// runs the products at the FP32 rate; an FP32 plane's rho is widened into
-// the FP64 output.
+// block, which runs the products at the FP32 rate; an FP32 plane's rho is
+// widened into the FP64 output.
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
@@ -209,6 +209,229 @@ int launch_syrk_pack(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t 
     return TTDFT_EBADARG;
 }
 
+// ---- fused x contraction + y-factor density matrices (steps 3 and 4) ----
+// For one qz plane of T1 and 16 consecutive qx points the kernel forms
+//   V[qx][s][b] = sum_a Bx[qx,a] T1[qz][a][s][b]
+// eight states at a time on the matrix cores, hands it to the y step through
+// LDS and accumulates the packed upper triangle of M[qx] = V[qx]^T V[qx] over
+// all states in registers, so the x-contracted block never reaches memory.
+//
+// Work item = (plane, 16-qx tile); items are renumbered so that each XCD
+// takes a contiguous range (dispatch deals workgroups round-robin over the 8
+// XCDs): the 16-qx tiles of one plane then read that plane's T1 from one L2.
+// x step (per chunk of 8 states): the chunk's (s, b) pairs are flattened,
+//   n = s_local R + b, and cut into 16-wide tiles; wave w takes tiles w,
+//   w + W, ...  MFMA D[qx][n] += Bx[qx][a] T1[a][n] over a in steps of 4,
+//   with the Bx fragments read from LDS (written once per workgroup) and the
+//   T1 operand 16 consecutive values of one row a per lane group (one
+//   coalesced load); the next tile's loads are issued before the current
+//   tile's MFMAs, across chunk boundaries too.  Each finished tile is stored to LDS as V[s_local][qx][b]; b runs
+//   to 64 with zeros past R.
+// y step: wave w owns qx = CW w .. CW w + CW - 1 (CW = 2 FP64, 1 FP32).
+//   Per 4 states lane (i, kq) reads
+//   V[4 ks + kq][q][16 bi + i] and accumulates the M tiles (bi <= bj) as in
+//   syrk_pack_kernel (both MFMA operands from the same registers).
+// Two LDS buffers: the x step of chunk c writes buffer c & 1, so one barrier
+// per chunk orders it (every wave finished the y step of chunk c - 1, which
+// read the other buffer, before it reached the barrier of chunk c).
+// KA = a steps of 4 (rx <= 4 KA), NB = 16-row blocks of b (R <= 16 NB).
+constexpr int kXsQ = 16;   // qx points per workgroup
+constexpr int kXsSC = 8;   // states per chunk
+constexpr int kXsPB = 64;  // LDS pitch of b
+// waves per workgroup: 8 (FP64: two per SIMD, M tiles of two qx points per
+// wave) or 16 (FP32: four per SIMD, one qx point per wave)
+template <typename Tm>
+constexpr int xs_waves() { return sizeof(Tm) == 8 ? 8 : 16; }
+
+__device__ __forceinline__ d4_t mfma16(double a, double b, d4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// row of the 16x16 D tile held in element v of lane l
+template <typename Tm>
+__device__ __forceinline__ int drow(int l, int v) {
+    return sizeof(Tm) == 8 ? (l >> 4) + 4 * v : 4 * (l >> 4) + v;
+}
+
+template <typename Tm, int KA, int NB>
+__global__ __launch_bounds__(64 * xs_waves<Tm>(), sizeof(Tm) == 8 ? 1 : 2) void xsyrk_kernel(
+    const Tm *__restrict__ T1, const Tm *__restrict__ Bx, int32_t px, int32_t rx, int32_t R,
+    int32_t K, int32_t nplanes, int32_t ntx, int32_t per_xcd, Tm *__restrict__ mp) {
+    typedef typename std::conditional<sizeof(Tm) == 8, d4_t, f4_t>::type acc_t;
+    constexpr int NP = NB * (NB + 1) / 2;
+    constexpr int kXsW = xs_waves<Tm>(), kXsCW = kXsQ / kXsW;  // qx points per wave, y step
+    __shared__ __align__(16) Tm V[2][kXsSC][kXsQ][kXsPB];
+    __shared__ __align__(16) Tm Af[KA][64];
+    const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (item >= (int64_t)nplanes * ntx) return;  // whole workgroup, before any barrier
+    const int plane = (int)(item / ntx), qx0 = (int)(item % ntx) * kXsQ;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, li = lane & 15, kq = lane >> 4;
+    {
+        Tm *v = &V[0][0][0][0];
+        for (int e = tid; e < 2 * kXsSC * kXsQ * kXsPB; e += 64 * kXsW) v[e] = Tm(0);
+    }
+    // A operand of the x step, lane-major: Af[k][lane] = Bx[qx0 + i][4 k + kq]
+    for (int e = tid; e < KA * 64; e += 64 * kXsW) {
+        const int l = e & 63, a = 4 * (e >> 6) + (l >> 4), q = qx0 + (l & 15);
+        Af[e >> 6][l] = (a < rx && q < px) ? Bx[q + (int64_t)px * a] : Tm(0);
+    }
+    // T1 operand offsets in elements within the plane (< 2^31: checked by
+    // the launcher), from one uniform base so the loads take a 32-bit offset
+    const int32_t KR = K * R;
+    const char *T1p = reinterpret_cast<const char *>(T1 + (int64_t)plane * rx * KR);
+    const int nch = (K + kXsSC - 1) / kXsSC;
+    auto nnz = [&](int ch) { return min(kXsSC, K - ch * kXsSC) * R; };  // (s, b) pairs
+    // operands of tile t of chunk ch; rows past rx and columns past the chunk
+    // load a valid element: the rows meet zero Bx fragments, the columns are
+    // never stored
+    auto column = [&](int ch, int t) {
+        const int n = 16 * t + li, nn = nnz(ch);
+        return (uint32_t)(ch * kXsSC * R + (n < nn ? n : nn - 1));
+    };
+    auto load1 = [&](uint32_t col, int k) {
+        const int a = min(4 * k + kq, rx - 1);
+        return *reinterpret_cast<const Tm *>(
+            T1p + (uint32_t)((col + (uint32_t)a * (uint32_t)KR) * sizeof(Tm)));
+    };
+    // this wave's tiles in order: t = w, w + 8, ... of every chunk; operand k
+    // of the next tile is loaded into the register that MFMA k of the current
+    // tile has just read
+    auto next = [&](int &c, int &t) {
+        t += kXsW;
+        while (c < nch && t >= (nnz(c) + 15) >> 4) {
+            ++c;
+            t = w;
+        }
+    };
+    int lc = 0, lt = w - kXsW;
+    next(lc, lt);
+    Tm bc[KA];
+    if (lc < nch) {
+        const uint32_t col = column(lc, lt);
+#pragma unroll
+        for (int k = 0; k < KA; ++k) bc[k] = load1(col, k);
+    }
+    acc_t acc[kXsCW][NP];
+#pragma unroll
+    for (int c = 0; c < kXsCW; ++c)
+#pragma unroll
+        for (int p = 0; p < NP; ++p) acc[c][p] = acc_t{0, 0, 0, 0};
+    __syncthreads();  // LDS zeroed, Af written
+
+    for (int ch = 0; ch < nch; ++ch) {
+        const int buf = ch & 1, nv = min(kXsSC, K - ch * kXsSC), nn = nv * R;
+        while (lc == ch) {
+            int nc = lc, nt = lt;
+            next(nc, nt);
+            // unconditional (past the last tile: any valid column), so the
+            // wait before MFMA k counts exactly the KA - 1 younger loads
+            const uint32_t ncol = nc < nch ? column(nc, nt) : column(lc, lt);
+            acc_t d = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < KA; ++k) {
+                d = mfma16(Af[k][lane], bc[k], d);
+                bc[k] = load1(ncol, k);
+            }
+            // keep the order MFMA k, load k (each load one tile ahead of its use)
+#pragma unroll
+            for (int k = 0; k < KA; ++k) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one global load
+            }
+            const int n = 16 * lt + li;
+            if (n < nn) {
+                const int s = n / R, b = n - s * R;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) V[buf][s][drow<Tm>(lane, v)][b] = d[v];
+            }
+            lc = nc;
+            lt = nt;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kXsSC / 4; ++ks) {
+            if (4 * ks >= nv) break;
+            const int s = 4 * ks + kq;
+#pragma unroll
+            for (int c = 0; c < kXsCW; ++c) {
+                Tm op[NB];
+#pragma unroll
+                for (int bi = 0; bi < NB; ++bi) {
+                    const Tm x = V[buf][s < nv ? s : 0][kXsCW * w + c][16 * bi + li];
+                    op[bi] = s < nv ? x : Tm(0);
+                }
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    const int bj = tri_col(p), bi = p - bj * (bj + 1) / 2;
+                    acc[c][p] = mfma16(op[bi], op[bj], acc[c][p]);
+                }
+            }
+        }
+    }
+    const int64_t np = (int64_t)R * (R + 1) / 2;
+#pragma unroll
+    for (int c = 0; c < kXsCW; ++c) {
+        const int q = qx0 + kXsCW * w + c;
+        if (q >= px) continue;
+        Tm *out = mp + ((int64_t)plane * px + q) * np;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int bj = tri_col(p), bi = p - bj * (bj + 1) / 2;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int b = 16 * bi + drow<Tm>(lane, v), bp = 16 * bj + li;
+                if (b <= bp && bp < R) out[(int64_t)bp * (bp + 1) / 2 + b] = acc[c][p][v];
+            }
+        }
+    }
+}
+
+// the fused path covers rx, ry <= 64 (register budget of the M tiles)
+bool xsyrk_fits(int32_t rx, int32_t ry) { return rx >= 1 && rx <= 64 && ry >= 1 && ry <= 64; }
+
+template <typename Tm, int KA>
+int launch_xsyrk_ka(ttdft_ctx ctx, const Tm *T1, const Tm *Bx, int32_t px, int32_t rx,
+                    int32_t R, int32_t K, int32_t nplanes, Tm *mp) {
+    TT_CHECK((int64_t)rx * K * R * (int64_t)sizeof(Tm) < ((int64_t)1 << 32), TTDFT_EBADARG,
+             "density: one plane of %d x %d x %d exceeds 4 GiB", rx, K, R);
+    const int32_t ntx = (px + kXsQ - 1) / kXsQ;
+    const int64_t items = (int64_t)nplanes * ntx;
+    const int32_t per_xcd = (int32_t)((items + 7) / 8);
+    const dim3 grid((unsigned)(8 * (int64_t)per_xcd)), block(64 * xs_waves<Tm>());
+    switch ((R + 15) / 16) {
+    case 1:
+        hipLaunchKernelGGL((xsyrk_kernel<Tm, KA, 1>), grid, block, 0, ctx->stream, T1, Bx, px, rx,
+                           R, K, nplanes, ntx, per_xcd, mp);
+        break;
+    case 2:
+        hipLaunchKernelGGL((xsyrk_kernel<Tm, KA, 2>), grid, block, 0, ctx->stream, T1, Bx, px, rx,
+                           R, K, nplanes, ntx, per_xcd, mp);
+        break;
+    case 3:
+        hipLaunchKernelGGL((xsyrk_kernel<Tm, KA, 3>), grid, block, 0, ctx->stream, T1, Bx, px, rx,
+                           R, K, nplanes, ntx, per_xcd, mp);
+        break;
+    default:
+        hipLaunchKernelGGL((xsyrk_kernel<Tm, KA, 4>), grid, block, 0, ctx->stream, T1, Bx, px, rx,
+                           R, K, nplanes, ntx, per_xcd, mp);
+        break;
+    }
+    TT_LAUNCH_CHECK();
+    return TTDFT_OK;
+}
+
+// a steps rounded up to 4, 8, 13 or 16 (rx <= 16, 32, 52, 64)
+template <typename Tm>
+int launch_xsyrk(ttdft_ctx ctx, const Tm *T1, const Tm *Bx, int32_t px, int32_t rx, int32_t R,
+                 int32_t K, int32_t nplanes, Tm *mp) {
+    if (rx <= 16) return launch_xsyrk_ka<Tm, 4>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
+    if (rx <= 32) return launch_xsyrk_ka<Tm, 8>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
+    if (rx <= 52) return launch_xsyrk_ka<Tm, 13>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
+    return launch_xsyrk_ka<Tm, 16>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
+}
+
 // E[qy][pair(b,b')] = (b == b' ? 1 : 2) By[qy,b] By[qy,b'];  By column-major py x R
 template <typename Tm>
 __global__ void pair_table_kernel(const double *__restrict__ By, int32_t py, int32_t R,
@@ -302,7 +525,13 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
     const int64_t np = (int64_t)ry * (ry + 1) / 2;
     constexpr int64_t ES = sizeof(Tm);
     const bool f32 = ES == 4;
-    const int64_t per_plane = (rxy * K + (int64_t)px * ry * K + (int64_t)px * np) * ES +
+    // steps 3 and 4 fused (T2 never stored) unless a rank exceeds 64;
+    // TTDFT_DENSITY_FUSED=0 selects the two-step form
+    bool fused = xsyrk_fits(rx, ry);
+    if (const char *env = std::getenv("TTDFT_DENSITY_FUSED"))
+        if (env[0] == '0') fused = false;
+    const int64_t t2_plane = fused ? 0 : (int64_t)px * ry * K;
+    const int64_t per_plane = (rxy * K + t2_plane + (int64_t)px * np) * ES +
                               (f32 ? (int64_t)px * py * 4 : 0);
     const int64_t budget = (int64_t)6 << 30;  // bytes for the chunk buffers
     int64_t QC = budget / per_plane;
@@ -313,13 +542,17 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
         const long v = std::strtol(env, nullptr, 10);
         if (v > 0 && v < QC) QC = v;
     }
+    {  // equal chunks: the last one is not a small tail
+        const int64_t nchunk = (pz + QC - 1) / QC;
+        QC = (pz + nchunk - 1) / nchunk;
+    }
     auto align = [](int64_t b) { return (b + 255) / 256 * 256; };
     const int64_t o_cg = 0, o_e = align(N * K * ES), o_sw = o_e + align(py * np * ES),
                   o_idx = o_sw + align(K * 8), o_bx = o_idx + align(K * 8),
                   o_bz = o_bx + (f32 ? align((int64_t)px * rx * ES) : 0),
                   o_t1 = o_bz + (f32 ? align((int64_t)pz * rz * ES) : 0),
                   o_t2 = o_t1 + align(QC * rxy * K * ES),
-                  o_mp = o_t2 + align(QC * px * ry * K * ES + 16 * ry * ES),
+                  o_mp = o_t2 + (fused ? 0 : align(QC * px * ry * K * ES + 16 * ry * ES)),
                   o_rf = o_mp + align(QC * px * np * ES),
                   total = o_rf + (f32 ? align(QC * px * py * 4) : 0);
     TT_TRY(ctx->scratch[5].ensure((size_t)total));
@@ -364,13 +597,18 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
         TT_BLAS(gemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
                      (rocblas_int)(rxy * K), qc, rz, Cg, (rocblas_int)(rxy * K), Bzm + q0, pz, T1,
                      (rocblas_int)(rxy * K)));
-        // 3. T2[qzl][qx][(s, b)] = sum_a T1[qzl][a][(s, b)] Bx[qx, a], one GEMM per plane
-        TT_BLAS(gemm_batched(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
-                             (rocblas_int)((int64_t)ry * K), px, rx, T1,
-                             (rocblas_int)((int64_t)ry * K), rxy * K, Bxm, px, 0, T2,
-                             (rocblas_int)((int64_t)ry * K), (int64_t)px * ry * K, qc));
-        // 4. packed upper triangles of T2[qzl][qx] T2[qzl][qx]^T
-        TT_TRY(launch_syrk_pack<Tm>(ctx, T2, ry, K, (int64_t)qc * px, mp));
+        if (fused) {
+            // 3 + 4. packed M[qzl][qx] with T2 formed on chip
+            TT_TRY(launch_xsyrk<Tm>(ctx, T1, Bxm, px, rx, ry, K, qc, mp));
+        } else {
+            // 3. T2[qzl][qx][(s, b)] = sum_a T1[qzl][a][(s, b)] Bx[qx, a], one GEMM per plane
+            TT_BLAS(gemm_batched(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
+                                 (rocblas_int)((int64_t)ry * K), px, rx, T1,
+                                 (rocblas_int)((int64_t)ry * K), rxy * K, Bxm, px, 0, T2,
+                                 (rocblas_int)((int64_t)ry * K), (int64_t)px * ry * K, qc));
+            // 4. packed upper triangles of T2[qzl][qx] T2[qzl][qx]^T
+            TT_TRY(launch_syrk_pack<Tm>(ctx, T2, ry, K, (int64_t)qc * px, mp));
+        }
         // 5. rho[q0 + qzl](qx, qy) = sum_pair m[qzl][qx][pair] E[qy][pair]
         if constexpr (sizeof(Tm) == 8) {
             TT_BLAS(gemm_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
