@@ -219,7 +219,7 @@ int launch_syrk_pack(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t 
 // Work item = (plane, 16-qx tile); items are renumbered so that each XCD
 // takes a contiguous range (dispatch deals workgroups round-robin over the 8
 // XCDs): the 16-qx tiles of one plane then read that plane's T1 from one L2.
-// x step (per chunk of 8 states): the chunk's (s, b) pairs are flattened,
+// x step (per chunk of 8 FP64 or 16 FP32 states): the chunk's (s, b) pairs are flattened,
 //   n = s_local R + b, and cut into 16-wide tiles; wave w takes tiles w,
 //   w + W, ...  MFMA D[qx][n] += Bx[qx][a] T1[a][n] over a in steps of 4,
 //   with the Bx fragments read from LDS (written once per workgroup) and the
@@ -236,7 +236,9 @@ int launch_syrk_pack(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t 
 // read the other buffer, before it reached the barrier of chunk c).
 // KA = a steps of 4 (rx <= 4 KA), NB = 16-row blocks of b (R <= 16 NB).
 constexpr int kXsQ = 16;   // qx points per workgroup
-constexpr int kXsSC = 8;   // states per chunk
+// states per chunk: 8 (FP64) or 16 (FP32; half the barriers, same LDS bytes)
+template <typename Tm>
+constexpr int xs_states() { return sizeof(Tm) == 8 ? 8 : 16; }
 constexpr int kXsPB = 64;  // LDS pitch of b
 // waves per workgroup: 8 (FP64: two per SIMD, M tiles of two qx points per
 // wave) or 16 (FP32: four per SIMD, one qx point per wave)
@@ -262,6 +264,7 @@ __global__ __launch_bounds__(64 * xs_waves<Tm>(), sizeof(Tm) == 8 ? 1 : 2) void 
     typedef typename std::conditional<sizeof(Tm) == 8, d4_t, f4_t>::type acc_t;
     constexpr int NP = NB * (NB + 1) / 2;
     constexpr int kXsW = xs_waves<Tm>(), kXsCW = kXsQ / kXsW;  // qx points per wave, y step
+    constexpr int kXsSC = xs_states<Tm>();
     __shared__ __align__(16) Tm V[2][kXsSC][kXsQ][kXsPB];
     __shared__ __align__(16) Tm Af[KA][64];
     const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
@@ -500,11 +503,20 @@ __global__ void to_float_kernel(const double *__restrict__ a, float *__restrict_
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) b[i] = (float)a[i];
 }
-__global__ void to_double_kernel(const float *__restrict__ a, double *__restrict__ b,
-                                 int64_t count) {
+// rho[qzl][qy][qx] = a[(qzl px + qx) + qc px qy]: the FP32 expansion onto the
+// y points runs as one product over all (qz, qx) columns of a chunk, and this
+// pass widens its column-major result into the planes' layout (reads and
+// writes run along qx)
+__global__ void widen_planes_kernel(const float *__restrict__ a, double *__restrict__ rho,
+                                    int32_t qc, int32_t px, int32_t py) {
+    const int64_t count = (int64_t)qc * py * px;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-         i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = (double)a[i];
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int qx = (int)(i % px);
+        const int64_t r = i / px;
+        const int qy = (int)(r % py), qz = (int)(r / py);
+        rho[i] = (double)a[(int64_t)qz * px + qx + (int64_t)qc * px * qy];
+    }
 }
 
 // Steps 1-5 with the T1 / T2 intermediates and their contractions in Tm.
@@ -616,12 +628,16 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
                                  E, (rocblas_int)np, 0, rho + q0 * (int64_t)px * py, px,
                                  (int64_t)px * py, qc));
         } else {
-            TT_BLAS(gemm_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
-                                 px, py, (rocblas_int)np, mp, (rocblas_int)np, (int64_t)px * np,
-                                 E, (rocblas_int)np, 0, rho_f, px, (int64_t)px * py, qc));
+            // one product over the chunk's qc px columns: rocBLAS runs it
+            // with a wider FP32 tile than the per-plane batch (600^3, k = 300
+            // density 22.2 -> 18.6 ms together with 16-state chunks above)
+            TT_BLAS(gemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+                         (rocblas_int)((int64_t)qc * px), py, (rocblas_int)np, mp,
+                         (rocblas_int)np, E, (rocblas_int)np, rho_f,
+                         (rocblas_int)((int64_t)qc * px)));
             const int64_t cnt = (int64_t)qc * px * py;
-            hipLaunchKernelGGL(to_double_kernel, dim3(grid_for(cnt)), dim3(256), 0, ctx->stream,
-                               rho_f, rho + q0 * (int64_t)px * py, cnt);
+            hipLaunchKernelGGL(widen_planes_kernel, dim3(grid_for(cnt)), dim3(256), 0,
+                               ctx->stream, rho_f, rho + q0 * (int64_t)px * py, qc, px, py);
             TT_LAUNCH_CHECK();
         }
     }
