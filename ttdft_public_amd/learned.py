@@ -473,6 +473,8 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--n", type=int, default=64, help="uniform grid points per axis")
+    ap.add_argument("--nmin", type=int, default=13, help="fewest atoms of a cluster")
+    ap.add_argument("--nmax", type=int, default=19, help="most atoms of a cluster")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--math", choices=["fp32", "bf16"], default="bf16",
                     help="convolution arithmetic: FP32 vector math or BF16 MFMA")
@@ -484,7 +486,7 @@ def main(argv=None):
     t0 = time.perf_counter()
     samples, failed = [], 0
     while len(samples) < a.train:
-        xyz = random_cluster(rng)
+        xyz = random_cluster(rng, a.nmin, a.nmax)
         sup, corr, r = converged_sample(ctx, xyz, data_config(3 * len(xyz)), a.n)
         if not r.converged:
             failed += 1
@@ -501,12 +503,13 @@ def main(argv=None):
     test_rng = np.random.default_rng(a.seed + 1)
     evals = []
     for i in range(a.test):
-        res = compare_starts(ctx, net, random_cluster(test_rng))
+        res = compare_starts(ctx, net, random_cluster(test_rng, a.nmin, a.nmax))
         evals.append(res)
         print(json.dumps(res), flush=True)
     sup_it = [e["superposition"]["to_1e-6"] or 80 for e in evals]
     lrn_it = [e["learned"]["to_1e-6"] or 80 for e in evals]
-    report = {"train_clusters": a.train, "failed_scf": failed, "epochs": a.epochs,
+    report = {"train_clusters": a.train, "atoms": [a.nmin, a.nmax], "failed_scf": failed,
+              "epochs": a.epochs,
               "math": a.math,
               "data_seconds": t_data, "train_seconds": t_train, "loss_history": net.history,
               "params": net.nparams, "test": evals,
