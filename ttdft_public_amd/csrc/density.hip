@@ -17,7 +17,8 @@
 //      3 + 4 fused in xsyrk_kernel (rx, ry <= 64): T2 lives in LDS only.
 //      Above 64: a rocBLAS GEMM per plane writes T2, syrk_pack_kernel reads it.
 //   5. rho[qz]^T       = m[qz] E^T,  E[qy][pair] = (2 - [b=b']) By[qy,b] By[qy,b']
-//                        (rocBLAS GEMM per plane)
+//                        (one rocBLAS GEMM over all columns of a chunk, then
+//                         widen_planes_kernel into the planes' layout)
 // Steps 2-5 run over chunks of qz planes to bound the workspace.  All steps
 // run in FP64 for an FP64 block and in FP32 (FP32 accumulation) for an FP32
 // block, which runs the products at the FP32 rate; an FP32 plane's rho is
@@ -507,7 +508,8 @@ __global__ void to_float_kernel(const double *__restrict__ a, float *__restrict_
 // y points runs as one product over all (qz, qx) columns of a chunk, and this
 // pass widens its column-major result into the planes' layout (reads and
 // writes run along qx)
-__global__ void widen_planes_kernel(const float *__restrict__ a, double *__restrict__ rho,
+template <typename Tm>
+__global__ void widen_planes_kernel(const Tm *__restrict__ a, double *__restrict__ rho,
                                     int32_t qc, int32_t px, int32_t py) {
     const int64_t count = (int64_t)qc * py * px;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
@@ -543,8 +545,16 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
     if (const char *env = std::getenv("TTDFT_DENSITY_FUSED"))
         if (env[0] == '0') fused = false;
     const int64_t t2_plane = fused ? 0 : (int64_t)px * ry * K;
+    // the expansion onto the y points as one product over all (qz, qx) columns
+    // of a chunk, through a column-major buffer that widen_planes_kernel turns
+    // into the planes: rocBLAS runs the one product with a wider tile than the
+    // per-plane batch (600^3, k = 300: FP64 35.9 -> 35.3 ms, FP32 -2.8 ms).
+    // TTDFT_DENSITY_PAIR=0 selects the batch for FP64.
+    bool pair_one = true;
+    if (const char *env = std::getenv("TTDFT_DENSITY_PAIR"))
+        if (env[0] == '0' && !f32) pair_one = false;
     const int64_t per_plane = (rxy * K + t2_plane + (int64_t)px * np) * ES +
-                              (f32 ? (int64_t)px * py * 4 : 0);
+                              (pair_one ? (int64_t)px * py * ES : 0);
     const int64_t budget = (int64_t)6 << 30;  // bytes for the chunk buffers
     int64_t QC = budget / per_plane;
     if (QC < 1) QC = 1;
@@ -566,7 +576,7 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
                   o_t2 = o_t1 + align(QC * rxy * K * ES),
                   o_mp = o_t2 + (fused ? 0 : align(QC * px * ry * K * ES + 16 * ry * ES)),
                   o_rf = o_mp + align(QC * px * np * ES),
-                  total = o_rf + (f32 ? align(QC * px * py * 4) : 0);
+                  total = o_rf + (pair_one ? align(QC * px * py * ES) : 0);
     TT_TRY(ctx->scratch[5].ensure((size_t)total));
     char *base = (char *)ctx->scratch[5].ptr;
     Tm *Cg = (Tm *)(base + o_cg);
@@ -587,7 +597,7 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
     Tm *T1 = (Tm *)(base + o_t1);
     Tm *T2 = (Tm *)(base + o_t2);
     Tm *mp = (Tm *)(base + o_mp);
-    float *rho_f = (float *)(base + o_rf);  // FP32 only
+    Tm *rho_c = (Tm *)(base + o_rf);  // one-product expansion only
 
     TT_HIP(hipMemcpyAsync(swd, sw.data(), sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
     TT_HIP(hipMemcpyAsync(idxd, idx.data(), sizeof(int32_t) * K, hipMemcpyHostToDevice,
@@ -623,21 +633,20 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
         }
         // 5. rho[q0 + qzl](qx, qy) = sum_pair m[qzl][qx][pair] E[qy][pair]
         if constexpr (sizeof(Tm) == 8) {
-            TT_BLAS(gemm_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
-                                 px, py, (rocblas_int)np, mp, (rocblas_int)np, (int64_t)px * np,
-                                 E, (rocblas_int)np, 0, rho + q0 * (int64_t)px * py, px,
-                                 (int64_t)px * py, qc));
-        } else {
-            // one product over the chunk's qc px columns: rocBLAS runs it
-            // with a wider FP32 tile than the per-plane batch (600^3, k = 300
-            // density 22.2 -> 18.6 ms together with 16-state chunks above)
+            if (!pair_one)
+                TT_BLAS(gemm_batched(ctx->blas, rocblas_operation_transpose,
+                                     rocblas_operation_none, px, py, (rocblas_int)np, mp,
+                                     (rocblas_int)np, (int64_t)px * np, E, (rocblas_int)np, 0,
+                                     rho + q0 * (int64_t)px * py, px, (int64_t)px * py, qc));
+        }
+        if (pair_one) {
             TT_BLAS(gemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
                          (rocblas_int)((int64_t)qc * px), py, (rocblas_int)np, mp,
-                         (rocblas_int)np, E, (rocblas_int)np, rho_f,
+                         (rocblas_int)np, E, (rocblas_int)np, rho_c,
                          (rocblas_int)((int64_t)qc * px)));
             const int64_t cnt = (int64_t)qc * px * py;
-            hipLaunchKernelGGL(widen_planes_kernel, dim3(grid_for(cnt)), dim3(256), 0,
-                               ctx->stream, rho_f, rho + q0 * (int64_t)px * py, qc, px, py);
+            hipLaunchKernelGGL(widen_planes_kernel<Tm>, dim3(grid_for(cnt)), dim3(256), 0,
+                               ctx->stream, rho_c, rho + q0 * (int64_t)px * py, qc, px, py);
             TT_LAUNCH_CHECK();
         }
     }
