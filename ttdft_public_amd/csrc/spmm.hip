@@ -393,29 +393,34 @@ __global__ __launch_bounds__(512) void dense_hx_kernel(
 // 16-row tile per wave, 4 waves) by 16 JT states over part p of the inner
 // dimension l (nparts parts of whole 32-wide steps, so there are enough
 // workgroups to fill the chip).  Per step the workgroup copies X[l0 .. l0 +
-// 31][its states] into LDS once (coalesced loads), so every X value read
-// from L2 serves four row tiles; each wave loads its 16 rows of H as before
+// 31][its states] into LDS once (coalesced loads of VX consecutive states,
+// pairs in FP32 when aligned), so every X value read from L2 serves four
+// row tiles; each wave loads its 16 rows of H as before
 // (lane (i, q): H[i][l0 + 8q .. 8q + 7]) and MFMA step u pairs l = l0 + 8q + u
 // in both operands, the X operand read from LDS (row 8q + u, 16 consecutive
 // states per quarter-wave; the row pitch of 16 JT + 18 elements puts the
 // quarter-waves' rows 8 apart on distinct banks: 8 pitches are 16 mod 64
-// dwords in FP32, 32 in FP64).  Even and odd steps accumulate separately.
-// The next step's X and H loads are issued before this step's products; the
-// X stage alternates between two buffers with one barrier per step.  The
+// dwords in FP32, 32 in FP64).  One accumulator per state tile.
+// The next step's X and H loads are issued before this step's products
+// (pinned there by a scheduling barrier, and unconditional so that the wait
+// before the products covers only their LDS reads); the X stage alternates
+// between two buffers with one barrier per step.  The
 // part's sums go to P[p][r][c] in FP64.  Rows past n read row n - 1 (never
 // stored); X rows past n meet zero H padding.
 // dense_combine_kernel: Y = alpha (sum_p P[p] + C^T W) (+ beta X) (+ delta Z),
 // the parts summed in a fixed order.
-template <typename T, int JT>
+template <typename T, int JT, int VX>
 __global__ __launch_bounds__(256) void dense_hx_part_kernel(int64_t n, int64_t ldh, int32_t k,
                                                             const T *__restrict__ H,
                                                             const T *__restrict__ X,
                                                             int64_t ldx, int32_t nparts,
                                                             double *__restrict__ P) {
     typedef typename DenseMma<T>::acc_t acc_t;
+    typedef T xv_t __attribute__((ext_vector_type(VX)));
     constexpr int NS = 16 * JT;                 // states per workgroup
     constexpr int XR = NS + 18;                 // LDS row pitch (elements)
-    constexpr int NX = (32 * NS + 255) / 256;   // X elements per thread per step
+    constexpr int NX = 32 * NS / (256 * VX);    // X vectors per thread per step
+    static_assert(32 * NS % (256 * VX) == 0, "every thread stages the same count");
     typedef T v4 __attribute__((ext_vector_type(4)));
     __shared__ __align__(16) T xs[2][32 * XR];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -427,43 +432,36 @@ __global__ __launch_bounds__(256) void dense_hx_part_kernel(int64_t n, int64_t l
     const int64_t sb = nst * part / nparts, se = nst * (part + 1) / nparts;
     const int64_t row = i0 + li < n ? i0 + li : n - 1;
     const T *hrow = H + row * ldh + 8 * q;
-    int xoff[NX], xlds[NX], xrow[NX];
-    bool xon[NX];
-#pragma unroll
-    for (int e = 0; e < NX; ++e) {
-        const int idx = t + 256 * e;
-        xon[e] = idx < 32 * NS;
-        const int r = xon[e] ? idx / NS : 0, col = xon[e] ? idx % NS : 0;
-        const int c = c0 + col < k ? c0 + col : k - 1;
-        xoff[e] = c;
-        xrow[e] = r;
-        xlds[e] = r * XR + col;
-    }
-    T xv[NX];
+    // X vector e of this thread: row idx / (NS / VX) of the step, VX
+    // consecutive states from col (the last vector of a ragged k is clamped
+    // inside the row: states past k are never stored, and with VX = 2 k is even)
+    auto xrow = [&](int e) { return (t + 256 * e) / (NS / VX); };
+    auto xcol = [&](int e) { return VX * ((t + 256 * e) % (NS / VX)); };
+    xv_t xv[NX];
     v4 ha[2], hb[2];
     auto load_x = [&](int64_t l0) {
 #pragma unroll
         for (int e = 0; e < NX; ++e) {
-            if (!xon[e]) continue;
-            int64_t l = l0 + xrow[e];
+            int64_t l = l0 + xrow(e);
             l = l < n ? l : n - 1;
-            xv[e] = X[l * ldx + xoff[e]];
+            const int c = c0 + xcol(e) < k ? c0 + xcol(e) : k - VX;
+            xv[e] = *reinterpret_cast<const xv_t *>(X + l * ldx + c);
         }
     };
     auto store_x = [&](int buf) {
 #pragma unroll
         for (int e = 0; e < NX; ++e)
-            if (xon[e]) xs[buf][xlds[e]] = xv[e];
+            *reinterpret_cast<xv_t *>(&xs[buf][xrow(e) * XR + xcol(e)]) = xv[e];
     };
     auto load_h = [&](int64_t l0, v4 (&h)[2]) {
         h[0] = *reinterpret_cast<const v4 *>(hrow + l0);
         h[1] = *reinterpret_cast<const v4 *>(hrow + l0 + 4);
     };
-    acc_t acc[2][JT];
+    // one accumulator per state tile: consecutive products go to different
+    // tiles, so a tile's chain has JT - 1 other products between its steps
+    acc_t acc[JT];
 #pragma unroll
-    for (int e = 0; e < 2; ++e)
-#pragma unroll
-        for (int jt = 0; jt < JT; ++jt) acc[e][jt] = acc_t{0, 0, 0, 0};
+    for (int jt = 0; jt < JT; ++jt) acc[jt] = acc_t{0, 0, 0, 0};
     auto mma = [&](int buf, const v4 (&h)[2]) {
         const T *xb = xs[buf] + 8 * q * XR + li;
 #pragma unroll
@@ -471,27 +469,33 @@ __global__ __launch_bounds__(256) void dense_hx_part_kernel(int64_t n, int64_t l
             const T hu = h[u >> 2][u & 3];
 #pragma unroll
             for (int jt = 0; jt < JT; ++jt)
-                acc[u & 1][jt] = DenseMma<T>::step(xb[u * XR + 16 * jt], hu, acc[u & 1][jt]);
+                acc[jt] = DenseMma<T>::step(xb[u * XR + 16 * jt], hu, acc[jt]);
         }
     };
     if (sb < se) {
         load_x(32 * sb);
-        load_h(32 * sb, ha);
+        load_h(32 * sb, hb);
         store_x(0);
+        // (ha is copied, not loaded, at the loop entry too: the wait the
+        // copy needs then stays here and not before every step's products)
+        ha[0] = hb[0];
+        ha[1] = hb[1];
         __syncthreads();
         int buf = 0;
         for (int64_t st = sb; st < se; ++st) {
-            const bool more = st + 1 < se;
-            if (more) {
-                load_x(32 * (st + 1));
-                load_h(32 * (st + 1), hb);
-            }
+            // the last step reloads its own operands (unconditional loads keep
+            // the wait before the products to this step's LDS reads)
+            const int64_t nx = st + 1 < se ? st + 1 : st;
+            load_x(32 * nx);
+            load_h(32 * nx, hb);
+            // keep the next step's loads ahead of this step's products (the
+            // scheduler would otherwise sink the H loads past them into the
+            // registers the products free, exposing their latency)
+            __builtin_amdgcn_sched_barrier(0);
             mma(buf, ha);
-            if (more) {
-                store_x(buf ^ 1);
-                ha[0] = hb[0];
-                ha[1] = hb[1];
-            }
+            store_x(buf ^ 1);
+            ha[0] = hb[0];
+            ha[1] = hb[1];
             __syncthreads();
             buf ^= 1;
         }
@@ -505,7 +509,7 @@ __global__ __launch_bounds__(256) void dense_hx_part_kernel(int64_t n, int64_t l
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = c0 + 16 * jt + DenseMma<T>::state(q, v);
-            if (c < k) Pr[c] = (double)acc[0][jt][v] + (double)acc[1][jt][v];
+            if (c < k) Pr[c] = (double)acc[jt][v];
         }
 }
 
@@ -773,8 +777,19 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
         if (nparts > nst) nparts = (int)nst;
         if (nparts > kDenseParts) nparts = kDenseParts;
         const dim3 g((unsigned)((n + 63) / 64), grid.y, (unsigned)nparts);
-        hipLaunchKernelGGL((dense_hx_part_kernel<T, JT>), g, dim3(256), 0, ctx->stream, n,
-                           H->dense_ld, k, Hd, (const T *)X, ldx, nparts, P);
+        // X staged in pairs when every pair is aligned and whole (FP32)
+        bool pairs = false;
+        if constexpr (sizeof(T) == 4) {
+            pairs = k % 2 == 0 && ldx % 2 == 0 &&
+                    reinterpret_cast<uintptr_t>(X) % (2 * sizeof(T)) == 0;
+            if (pairs)
+                hipLaunchKernelGGL((dense_hx_part_kernel<T, JT, 2>), g, dim3(256), 0,
+                                   ctx->stream, n, H->dense_ld, k, Hd, (const T *)X, ldx, nparts,
+                                   P);
+        }
+        if (!pairs)
+            hipLaunchKernelGGL((dense_hx_part_kernel<T, JT, 1>), g, dim3(256), 0, ctx->stream,
+                               n, H->dense_ld, k, Hd, (const T *)X, ldx, nparts, P);
         const int64_t total = n * k;
         const unsigned cb = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
         hipLaunchKernelGGL((dense_combine_kernel<T>), dim3(cb), dim3(256), 0, ctx->stream, n, k,
