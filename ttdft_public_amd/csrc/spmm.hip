@@ -763,10 +763,10 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
     // six-state runs per lane, fetched as pairs, when every tile is full and
     // every row of X starts on a pair boundary (even leading dimension and a
     // pair-aligned X: a filter block may start at an odd column)
-    // FP32: the split product with X staged in LDS (27,000 x 96: 2.54 -> 1.80
-    // ms); FP64 keeps the one-kernel product, whose registers the split form
-    // exceeds (3.27 against 4.73 ms).  TTDFT_DENSE_LDS=0 turns the split off,
-    // 1 picks the parts, 2..4 fixes them.
+    // FP32: the split product with X staged in LDS (27,000 x 96: 2.54 -> 1.41
+    // ms); FP64 keeps the one-kernel product (3.26 ms against 3.33 for the
+    // split form at its chosen three parts).  TTDFT_DENSE_LDS=0 turns the
+    // split off, 1 picks the parts, 2..4 fixes them.
     const int lds_form = dense_split_form(sizeof(T) == 8 ? TTDFT_F64 : TTDFT_F32);
     if (lds_form >= 1 && P) {
         // parts of l for about 4.5 workgroups per CU (measured best at 27,000)
