@@ -508,11 +508,12 @@ def test_density_wide_y(ctx, ry, dtype):
 
 @pytest.mark.parametrize("dtype", [E.F64, E.F32])
 @pytest.mark.parametrize("rx,ry,k", [(13, 16, 9), (30, 20, 17), (50, 40, 8), (61, 64, 33),
-                                     (4, 50, 3)])
+                                     (4, 50, 3), (70, 70, 11), (80, 72, 5)])
 def test_density_fused_shapes(ctx, monkeypatch, rx, ry, k, dtype):
-    """The fused x + y-factor step over its tile classes (a steps of 4, 8, 13, 16; 1-4 blocks
-    of 16 b), a ragged last qx tile and ragged state chunks, against the oracle and against
-    the two-step form (x GEMM to memory, then the packed y-factor products)."""
+    """The fused x + y-factor step over its tile classes (a steps of 4, 8, 13, 16, 20; 1-5
+    blocks of 16 b; ranks past 64 are fused in FP32 only), a ragged last qx tile and ragged
+    state chunks, against the oracle and against the two-step form (x GEMM to memory, then
+    the packed y-factor products)."""
     rng = np.random.default_rng(rx * 100 + ry)
     rz = 2
     px, py, pz = 37, 11, 3

@@ -14,8 +14,9 @@
 //   2. T1[qz][a][s][b] = sum_c Bz[qz,c] Cg[c][a][s][b]   (rocBLAS GEMM)
 //   3. T2[qz][qx][s][b]= sum_a Bx[qx,a] T1[qz][a][s][b]
 //   4. m[qz][qx][pair] = upper triangle of T2[qz][qx] T2[qz][qx]^T, packed
-//      3 + 4 fused in xsyrk_kernel (rx, ry <= 64): T2 lives in LDS only.
-//      Above 64: a rocBLAS GEMM per plane writes T2, syrk_pack_kernel reads it.
+//      3 + 4 fused in xsyrk_kernel (rx, ry <= 64 in FP64, <= 80 in FP32): T2
+//      lives in LDS only.  Above: a rocBLAS GEMM per plane writes T2,
+//      syrk_pack_kernel reads it.
 //   5. rho[qz]^T       = m[qz] E^T,  E[qy][pair] = (2 - [b=b']) By[qy,b] By[qy,b']
 //                        (one rocBLAS GEMM over all columns of a chunk, then
 //                         widen_planes_kernel into the planes' layout)
@@ -237,14 +238,18 @@ int launch_syrk_pack(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t 
 // read the other buffer, before it reached the barrier of chunk c).
 // KA = a steps of 4 (rx <= 4 KA), NB = 16-row blocks of b (R <= 16 NB).
 constexpr int kXsQ = 16;   // qx points per workgroup
-// states per chunk: 8 (FP64) or 16 (FP32; half the barriers, same LDS bytes)
-template <typename Tm>
-constexpr int xs_states() { return sizeof(Tm) == 8 ? 8 : 16; }
-constexpr int kXsPB = 64;  // LDS pitch of b
-// waves per workgroup: 8 (FP64: two per SIMD, M tiles of two qx points per
-// wave) or 16 (FP32: four per SIMD, one qx point per wave)
-template <typename Tm>
-constexpr int xs_waves() { return sizeof(Tm) == 8 ? 8 : 16; }
+// states per chunk: 8, or 16 in FP32 up to 4 row blocks (half the barriers,
+// the same LDS bytes)
+template <typename Tm, int NB>
+constexpr int xs_states() { return sizeof(Tm) == 4 && NB <= 4 ? 16 : 8; }
+// LDS pitch of b: 64, or 80 for 5 row blocks (FP32 only)
+template <int NB>
+constexpr int xs_pitch() { return NB <= 4 ? 64 : 16 * NB; }
+// waves per workgroup: 8 (two per SIMD, M tiles of two qx points per wave:
+// FP64, and FP32 at 5 row blocks) or 16 (FP32 up to 4 row blocks: four per
+// SIMD, one qx point per wave)
+template <typename Tm, int NB>
+constexpr int xs_waves() { return sizeof(Tm) == 8 || NB > 4 ? 8 : 16; }
 
 __device__ __forceinline__ d4_t mfma16(double a, double b, d4_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -259,13 +264,13 @@ __device__ __forceinline__ int drow(int l, int v) {
 }
 
 template <typename Tm, int KA, int NB>
-__global__ __launch_bounds__(64 * xs_waves<Tm>(), sizeof(Tm) == 8 ? 1 : 2) void xsyrk_kernel(
+__global__ __launch_bounds__((64 * xs_waves<Tm, NB>()), (xs_waves<Tm, NB>() == 16 ? 2 : 1)) void xsyrk_kernel(
     const Tm *__restrict__ T1, const Tm *__restrict__ Bx, int32_t px, int32_t rx, int32_t R,
     int32_t K, int32_t nplanes, int32_t ntx, int32_t per_xcd, Tm *__restrict__ mp) {
     typedef typename std::conditional<sizeof(Tm) == 8, d4_t, f4_t>::type acc_t;
     constexpr int NP = NB * (NB + 1) / 2;
-    constexpr int kXsW = xs_waves<Tm>(), kXsCW = kXsQ / kXsW;  // qx points per wave, y step
-    constexpr int kXsSC = xs_states<Tm>();
+    constexpr int kXsW = xs_waves<Tm, NB>(), kXsCW = kXsQ / kXsW;  // qx points per wave, y step
+    constexpr int kXsSC = xs_states<Tm, NB>(), kXsPB = xs_pitch<NB>();
     __shared__ __align__(16) Tm V[2][kXsSC][kXsQ][kXsPB];
     __shared__ __align__(16) Tm Af[KA][64];
     const int64_t item = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
@@ -393,7 +398,12 @@ __global__ __launch_bounds__(64 * xs_waves<Tm>(), sizeof(Tm) == 8 ? 1 : 2) void 
 }
 
 // the fused path covers rx, ry <= 64 (register budget of the M tiles)
-bool xsyrk_fits(int32_t rx, int32_t ry) { return rx >= 1 && rx <= 64 && ry >= 1 && ry <= 64; }
+// the fused path covers ranks up to 64 (FP64) or 80 (FP32: the M tiles of
+// two qx points at 5 row blocks fit half a SIMD's registers in FP32 only)
+bool xsyrk_fits(int32_t rx, int32_t ry, bool f32) {
+    const int m = f32 ? 80 : 64;
+    return rx >= 1 && rx <= m && ry >= 1 && ry <= m;
+}
 
 template <typename Tm, int KA>
 int launch_xsyrk_ka(ttdft_ctx ctx, const Tm *T1, const Tm *Bx, int32_t px, int32_t rx,
@@ -403,7 +413,8 @@ int launch_xsyrk_ka(ttdft_ctx ctx, const Tm *T1, const Tm *Bx, int32_t px, int32
     const int32_t ntx = (px + kXsQ - 1) / kXsQ;
     const int64_t items = (int64_t)nplanes * ntx;
     const int32_t per_xcd = (int32_t)((items + 7) / 8);
-    const dim3 grid((unsigned)(8 * (int64_t)per_xcd)), block(64 * xs_waves<Tm>());
+    const dim3 grid((unsigned)(8 * (int64_t)per_xcd));
+    const dim3 block(64 * (R > 64 ? xs_waves<Tm, 5>() : xs_waves<Tm, 4>()));
     switch ((R + 15) / 16) {
     case 1:
         hipLaunchKernelGGL((xsyrk_kernel<Tm, KA, 1>), grid, block, 0, ctx->stream, T1, Bx, px, rx,
@@ -417,23 +428,35 @@ int launch_xsyrk_ka(ttdft_ctx ctx, const Tm *T1, const Tm *Bx, int32_t px, int32
         hipLaunchKernelGGL((xsyrk_kernel<Tm, KA, 3>), grid, block, 0, ctx->stream, T1, Bx, px, rx,
                            R, K, nplanes, ntx, per_xcd, mp);
         break;
-    default:
+    case 4:
         hipLaunchKernelGGL((xsyrk_kernel<Tm, KA, 4>), grid, block, 0, ctx->stream, T1, Bx, px, rx,
                            R, K, nplanes, ntx, per_xcd, mp);
         break;
+    default:
+        if constexpr (sizeof(Tm) == 4) {
+            hipLaunchKernelGGL((xsyrk_kernel<Tm, KA, 5>), grid, block, 0, ctx->stream, T1, Bx, px,
+                               rx, R, K, nplanes, ntx, per_xcd, mp);
+            break;
+        }
+        set_error("density: fused y step with %d y ranks", R);
+        return TTDFT_EBADARG;
     }
     TT_LAUNCH_CHECK();
     return TTDFT_OK;
 }
 
-// a steps rounded up to 4, 8, 13 or 16 (rx <= 16, 32, 52, 64)
+// a steps rounded up to 4, 8, 13, 16 or 20 (rx <= 16, 32, 52, 64, 80)
 template <typename Tm>
 int launch_xsyrk(ttdft_ctx ctx, const Tm *T1, const Tm *Bx, int32_t px, int32_t rx, int32_t R,
                  int32_t K, int32_t nplanes, Tm *mp) {
     if (rx <= 16) return launch_xsyrk_ka<Tm, 4>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
     if (rx <= 32) return launch_xsyrk_ka<Tm, 8>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
     if (rx <= 52) return launch_xsyrk_ka<Tm, 13>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
-    return launch_xsyrk_ka<Tm, 16>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
+    if (rx <= 64) return launch_xsyrk_ka<Tm, 16>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
+    if constexpr (sizeof(Tm) == 4)
+        return launch_xsyrk_ka<Tm, 20>(ctx, T1, Bx, px, rx, R, K, nplanes, mp);
+    set_error("density: fused x step with %d x ranks", rx);
+    return TTDFT_EBADARG;
 }
 
 // E[qy][pair(b,b')] = (b == b' ? 1 : 2) By[qy,b] By[qy,b'];  By column-major py x R
@@ -539,9 +562,9 @@ int density_run(ttdft_ctx ctx, const void *X, int64_t ldx, int dtype, const std:
     const int64_t np = (int64_t)ry * (ry + 1) / 2;
     constexpr int64_t ES = sizeof(Tm);
     const bool f32 = ES == 4;
-    // steps 3 and 4 fused (T2 never stored) unless a rank exceeds 64;
+    // steps 3 and 4 fused (T2 never stored) unless a rank exceeds 64 (FP64) or 80;
     // TTDFT_DENSITY_FUSED=0 selects the two-step form
-    bool fused = xsyrk_fits(rx, ry);
+    bool fused = xsyrk_fits(rx, ry, f32);
     if (const char *env = std::getenv("TTDFT_DENSITY_FUSED"))
         if (env[0] == '0') fused = false;
     const int64_t t2_plane = fused ? 0 : (int64_t)px * ry * K;
