@@ -214,23 +214,25 @@ int launch_syrk_pack(ttdft_ctx ctx, const Tm *T2, int32_t R, int32_t K, int64_t 
 // ---- fused x contraction + y-factor density matrices (steps 3 and 4) ----
 // For one qz plane of T1 and 16 consecutive qx points the kernel forms
 //   V[qx][s][b] = sum_a Bx[qx,a] T1[qz][a][s][b]
-// eight states at a time on the matrix cores, hands it to the y step through
+// a chunk of states at a time on the matrix cores, hands it to the y step through
 // LDS and accumulates the packed upper triangle of M[qx] = V[qx]^T V[qx] over
 // all states in registers, so the x-contracted block never reaches memory.
 //
 // Work item = (plane, 16-qx tile); items are renumbered so that each XCD
 // takes a contiguous range (dispatch deals workgroups round-robin over the 8
 // XCDs): the 16-qx tiles of one plane then read that plane's T1 from one L2.
-// x step (per chunk of 8 FP64 or 16 FP32 states): the chunk's (s, b) pairs are flattened,
-//   n = s_local R + b, and cut into 16-wide tiles; wave w takes tiles w,
-//   w + W, ...  MFMA D[qx][n] += Bx[qx][a] T1[a][n] over a in steps of 4,
-//   with the Bx fragments read from LDS (written once per workgroup) and the
-//   T1 operand 16 consecutive values of one row a per lane group (one
-//   coalesced load); the next tile's loads are issued before the current
-//   tile's MFMAs, across chunk boundaries too.  Each finished tile is stored to LDS as V[s_local][qx][b]; b runs
-//   to 64 with zeros past R.
-// y step: wave w owns qx = CW w .. CW w + CW - 1 (CW = 2 FP64, 1 FP32).
-//   Per 4 states lane (i, kq) reads
+// x step (per chunk of SC states: 16 in FP32 up to rank 64, else 8): the
+//   chunk's (s, b) pairs are flattened, n = s_local R + b, and cut into
+//   16-wide tiles; wave w takes tiles w, w + W, ... (W waves).  MFMA D[qx][n]
+//   += Bx[qx][a] T1[a][n] over a in steps of 4, with the Bx fragments read
+//   from LDS (written once per workgroup) and the T1 operand 16 consecutive
+//   values of one row a per lane group (one coalesced load).  Operand k of
+//   the wave's next tile, across chunk boundaries too, is loaded right after
+//   MFMA k of the current one.  Each finished tile is stored to LDS as
+//   V[s_local][qx][b]; b runs to the pitch (64, or 80 at five row blocks)
+//   with zeros past R.
+// y step: wave w owns qx = CW w .. CW w + CW - 1 (CW = 16 / W: two qx points
+//   with 8 waves, one with 16).  Per 4 states lane (i, kq) reads
 //   V[4 ks + kq][q][16 bi + i] and accumulates the M tiles (bi <= bj) as in
 //   syrk_pack_kernel (both MFMA operands from the same registers).
 // Two LDS buffers: the x step of chunk c writes buffer c & 1, so one barrier
@@ -304,7 +306,7 @@ __global__ __launch_bounds__((64 * xs_waves<Tm, NB>()), (xs_waves<Tm, NB>() == 1
         return *reinterpret_cast<const Tm *>(
             T1p + (uint32_t)((col + (uint32_t)a * (uint32_t)KR) * sizeof(Tm)));
     };
-    // this wave's tiles in order: t = w, w + 8, ... of every chunk; operand k
+    // this wave's tiles in order: t = w, w + W, ... of every chunk; operand k
     // of the next tile is loaded into the register that MFMA k of the current
     // tile has just read
     auto next = [&](int &c, int &t) {
@@ -397,7 +399,6 @@ __global__ __launch_bounds__((64 * xs_waves<Tm, NB>()), (xs_waves<Tm, NB>() == 1
     }
 }
 
-// the fused path covers rx, ry <= 64 (register budget of the M tiles)
 // the fused path covers ranks up to 64 (FP64) or 80 (FP32: the M tiles of
 // two qx points at 5 row blocks fit half a SIMD's registers in FP32 only)
 bool xsyrk_fits(int32_t rx, int32_t ry, bool f32) {
