@@ -748,8 +748,9 @@ constexpr int kDenseParts = 4;  // most parts of l in the split dense product
 // 0: one-kernel dense product; 1: split product, parts picked; 2..4: split
 // product with that many parts
 int dense_split_form(int dtype) {
+    (void)dtype;  // both precisions take the split product by default
     const char *env = std::getenv("TTDFT_DENSE_LDS");
-    return env ? std::atoi(env) : (dtype == TTDFT_F32 ? 1 : 0);
+    return env ? std::atoi(env) : 1;
 }
 
 template <typename T, int JT>
@@ -763,15 +764,20 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
     // six-state runs per lane, fetched as pairs, when every tile is full and
     // every row of X starts on a pair boundary (even leading dimension and a
     // pair-aligned X: a filter block may start at an odd column)
-    // FP32: the split product with X staged in LDS (27,000 x 96: 2.54 -> 1.41
-    // ms); FP64 keeps the one-kernel product (3.26 ms against 3.33 for the
-    // split form at its chosen three parts).  TTDFT_DENSE_LDS=0 turns the
-    // split off, 1 picks the parts, 2..4 fixes them.
+    // The split product with X staged in LDS, against the one-kernel form:
+    // FP32 27,000 x 96 1.41 against 2.54 ms, 27,000 x 48 0.74 against 1.26,
+    // 8,000 x 24 0.077 against 0.089; FP64 27,000 x 96 3.17 against 3.26,
+    // 27,000 x 48 1.62 against 2.13, 8,000 x 24 0.141 against 0.154.
+    // TTDFT_DENSE_LDS=0 turns the split off, 1 picks the parts, 2..4 fixes
+    // them.
     const int lds_form = dense_split_form(sizeof(T) == 8 ? TTDFT_F64 : TTDFT_F32);
     if (lds_form >= 1 && P) {
-        // parts of l for about 4.5 workgroups per CU (measured best at 27,000)
+        // parts of l for about 4.5 workgroups per CU, 6 for FP64 at 80-96
+        // states (measured best at 27,000: FP64 x 96 3.17 ms at four parts,
+        // 3.33 at three; FP64 x 48 1.62 ms at three, 1.74 at four)
         const int64_t wgs = (n + 63) / 64 * grid.y;
-        int nparts = (int)((1152 + wgs - 1) / wgs);
+        const int64_t target = sizeof(T) == 8 && JT >= 5 ? 1536 : 1152;
+        int nparts = (int)((target + wgs - 1) / wgs);
         if (lds_form > 1) nparts = lds_form;
         const int64_t nst = H->dense_ld / 32;
         if (nparts > nst) nparts = (int)nst;
