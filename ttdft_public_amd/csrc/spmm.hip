@@ -765,9 +765,9 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
     // every row of X starts on a pair boundary (even leading dimension and a
     // pair-aligned X: a filter block may start at an odd column)
     // The split product with X staged in LDS, against the one-kernel form:
-    // FP32 27,000 x 96 1.41 against 2.54 ms, 27,000 x 48 0.74 against 1.26,
-    // 8,000 x 24 0.077 against 0.089; FP64 27,000 x 96 3.17 against 3.26,
-    // 27,000 x 48 1.62 against 2.13, 8,000 x 24 0.141 against 0.154.
+    // FP32 27,000 x 96 1.39 against 2.54 ms, 27,000 x 48 0.74 against 1.26,
+    // 8,000 x 24 0.077 against 0.089; FP64 27,000 x 96 3.02 against 3.26,
+    // 27,000 x 48 1.58 against 2.14, 8,000 x 24 0.133 against 0.153.
     // TTDFT_DENSE_LDS=0 turns the split off, 1 picks the parts, 2..4 fixes
     // them.
     const int lds_form = dense_split_form(sizeof(T) == 8 ? TTDFT_F64 : TTDFT_F32);
@@ -783,17 +783,13 @@ static void dense_launch(ttdft_ctx ctx, const ttdft_ham_s *H, const T *Hd, const
         if (nparts > nst) nparts = (int)nst;
         if (nparts > kDenseParts) nparts = kDenseParts;
         const dim3 g((unsigned)((n + 63) / 64), grid.y, (unsigned)nparts);
-        // X staged in pairs when every pair is aligned and whole (FP32)
-        bool pairs = false;
-        if constexpr (sizeof(T) == 4) {
-            pairs = k % 2 == 0 && ldx % 2 == 0 &&
-                    reinterpret_cast<uintptr_t>(X) % (2 * sizeof(T)) == 0;
-            if (pairs)
-                hipLaunchKernelGGL((dense_hx_part_kernel<T, JT, 2>), g, dim3(256), 0,
-                                   ctx->stream, n, H->dense_ld, k, Hd, (const T *)X, ldx, nparts,
-                                   P);
-        }
-        if (!pairs)
+        // X staged in pairs when every pair is aligned and whole
+        const bool pairs = k % 2 == 0 && ldx % 2 == 0 &&
+                           reinterpret_cast<uintptr_t>(X) % (2 * sizeof(T)) == 0;
+        if (pairs)
+            hipLaunchKernelGGL((dense_hx_part_kernel<T, JT, 2>), g, dim3(256), 0, ctx->stream,
+                               n, H->dense_ld, k, Hd, (const T *)X, ldx, nparts, P);
+        else
             hipLaunchKernelGGL((dense_hx_part_kernel<T, JT, 1>), g, dim3(256), 0, ctx->stream,
                                n, H->dense_ld, k, Hd, (const T *)X, ldx, nparts, P);
         const int64_t total = n * k;
