@@ -533,10 +533,9 @@ __global__ __launch_bounds__(G > 1 ? 64 * G * GXM : 1024) void brick_spmm_kernel
 //   FP64: 3 x-bricks x 5 groups of 64 states, one line in flight (two 61 KB
 //         stages of 320 states at KS = 4);
 //   FP32: 4 x-bricks x 4 groups of 80 states, one line in flight (two 36 KB
-//         stages).
+//         stages), two with two planes (launch_db).
 // Measured alternatives on the 172-atom product: FP64 with 2 x-bricks and two
-// lines in flight 6.48 ms against 5.80; FP32 with two lines in flight 3.87
-// against 3.49 ms.
+// lines in flight 6.48 ms against 5.80.
 template <typename T, int V>
 struct CoShape;
 template <>
@@ -563,6 +562,20 @@ int launch_db(ttdft_ctx ctx, const BrickArgs<T> &a, int cogroups) {
     }
     if (cogroups == 1) {
         typedef CoShape<T, 0> C;
+        // FP32 two-plane workgroups keep two lines in flight (three 36 KB
+        // stages): 172-atom product 3.45 against 3.54 ms.  With one plane
+        // (KS = 5) one line measured faster (500-atom product 37.0 against
+        // 39.0 ms).  TTDFT_BRICK_DEPTH=1 keeps one line.
+        if constexpr (sizeof(T) == 4 && ZB == 2) {
+            const char *env = std::getenv("TTDFT_BRICK_DEPTH");
+            if (!(env && env[0] == '1')) {
+                hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, 2, C::GXM, C::G, ZB>),
+                                   dim3((unsigned)blocks), dim3(64 * C::G * C::GXM), 0,
+                                   ctx->stream, a);
+                TT_LAUNCH_CHECK();
+                return TTDFT_OK;
+            }
+        }
         hipLaunchKernelGGL((brick_spmm_kernel<T, KS, NST, DB, C::DEPTH, C::GXM, C::G, ZB>),
                            dim3((unsigned)blocks), dim3(64 * C::G * C::GXM), 0, ctx->stream, a);
         TT_LAUNCH_CHECK();
