@@ -186,8 +186,9 @@ int ttdft_ham_apply(ttdft_ctx ctx, ttdft_ham ham, const void *X, int64_t ldx,
  * device work space of 32 MiB per GPU (held until the process ends): the
  * call fails with TTDFT_EBADARG when p * k * (1 + splits) * sizeof(dtype)
  * exceeds it (k = 300 and p = 172 need at most 2.5 MB in FP64).  A product on
- * dense storage also keeps its partial sums there (4 n k doubles: n k up to
- * about 1e6).  Every call on one
+ * dense storage also keeps its partial sums there (4 n k doubles) when they
+ * fit (n k up to about 1e6); above that it runs in one kernel, which agrees
+ * with ttdft_ham_apply up to the order of the additions.  Every call on one
  * GPU shares that work space, so calls that overlap on one GPU (different
  * contexts or streams) race; calls on one context are stream-ordered and
  * safe.  Not recorded by the context's kernel timing. */

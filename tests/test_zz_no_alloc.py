@@ -2,7 +2,8 @@
 
 The allocation-free entry point runs the same kernels with the nonlocal
 projections in the library's static per-GPU work space, so its result must
-equal ttdft_ham_apply's bit for bit.  Two calls in a row, the second on a
+equal ttdft_ham_apply's bit for bit (a dense product whose partial sums do not
+fit that work space runs in one kernel and agrees to rounding).  Two calls in a row, the second on a
 smaller input (fewer basis functions, projectors and states), show that the
 work space left by one call does not leak into the next.
 """
@@ -63,6 +64,27 @@ def test_noalloc_dense_storage():
         for dtype in (E.F64, E.F32):
             Y, Z = _apply_both(c, ham, s.n, 96, 96, dtype, 3)
             assert np.array_equal(Y, Z)
+        ham.free()
+    finally:
+        c.close()
+
+
+def test_noalloc_dense_storage_past_the_work_space():
+    """A dense product whose partial sums (4 n k doubles, 41 MB here) exceed the
+    static work space runs in one kernel: it agrees with ttdft_ham_apply (the
+    split product) up to the order of the additions."""
+    c = E.Context(0)
+    try:
+        s = E.SynthHam(30, 30, 30, 14, 14, 14, 55, seed=3)
+        ham = c.ham_upload(s, E.HAM_F64 | E.HAM_F32, tucker=False)
+        assert ham.format()[0] == 2
+        k = 48
+        assert 4 * s.n * k * 8 > 32 << 20
+        for dtype, tol in ((E.F64, 1e-13), (E.F32, 1e-5)):
+            print(f"noalloc dense past the work space: n={s.n} k={k} dtype={dtype}", flush=True)
+            Y, Z = _apply_both(c, ham, s.n, k, k, dtype, 4)
+            Y, Z = Y.astype(np.float64), Z.astype(np.float64)
+            assert np.abs(Y - Z).max() <= tol * np.abs(Y).max()
         ham.free()
     finally:
         c.close()

@@ -882,17 +882,15 @@ int ham_step(ttdft_ctx ctx, const ttdft_ham_s *H, const void *X, int64_t ldx, vo
                                 alpha, beta, delta, use_beta, use_delta));
     else if (dense) {
         // partial sums of the split product: after the projections in the
-        // caller's work space, else from the context
+        // caller's work space, else from the context.  When the caller's work
+        // space cannot hold them the product runs in one kernel (no partial
+        // sums; the same result up to the order of the additions).
         double *P = nullptr;
         if (dense_split_form(dtype) >= 1) {
             const size_t need_p = (size_t)kDenseParts * H->n * k * sizeof(double);
             if (ws) {
                 const size_t off = W ? ((size_t)nl_workspace(H, k) * es + 255) / 256 * 256 : 0;
-                TT_CHECK(off + need_p <= ws_bytes, TTDFT_EBADARG,
-                         "the dense product of %lld functions x %d states needs %zu bytes of "
-                         "work space, more than the %zu available",
-                         (long long)H->n, k, off + need_p, ws_bytes);
-                P = (double *)((char *)ws + off);
+                if (off + need_p <= ws_bytes) P = (double *)((char *)ws + off);
             } else {
                 TT_TRY(ctx->scratch[8].ensure(need_p));
                 P = (double *)ctx->scratch[8].ptr;
