@@ -530,6 +530,65 @@ int ttdft_read_atom_file(const char *path, int32_t *ntype, int32_t *natoms, int3
  * nlpWaveFun_*, Density_*, locPotential_*): nrows x ncols row-major. */
 int ttdft_read_table(const char *path, int32_t *nrows, int32_t *ncols, double *values);
 
+/* ---- tabulated pseudopotentials --------------------------------------------- */
+/* Cubic spline through (x_i, y_i), x strictly ascending, whose first and last
+ * segments are parabolas (S''' = 0 there): alglib spline1dbuildcubic with no
+ * end condition given, as atoms/NonLocalPSPData.cc:382-502 calls it.  coef:
+ * 4 (n - 1) values, segment s is c0 + c1 t + c2 t^2 + c3 t^3, t = x - x_s;
+ * beyond the table the end segments extend.  Host only. */
+int ttdft_spline_parabolic(int32_t n, const double *x, const double *y, double *coef);
+/* Local pseudopotential on a tensor grid (gx, gy, gz: device coordinates,
+ * d_v: px py pz device values, x fastest; written, or added to when add != 0):
+ *   V(q) = sum_a  d_a > 75.001 ? -Z_a / d_a : vloc(|q - R_a|),  d_a = |q - R_a| + 1e-12,
+ * vloc the spline above through the table (r, vloc; host, nr rows).  atoms:
+ * natoms rows of 7 as ttdft_read_atom_file writes them (Z, x, y, z, ...).
+ * replaces NonLocalPSPData::computeLocalPart (atoms/NonLocalPSPData.cc:214-275)
+ * with createSplineObject (:382-408). */
+int ttdft_psp_local(ttdft_ctx ctx, int32_t natoms, const double *atoms, int32_t nr,
+                    const double *r, const double *vloc, int32_t px, int32_t py, int32_t pz,
+                    const double *d_gx, const double *d_gy, const double *d_gz, double *d_v,
+                    int add);
+/* Projector data of one atom type on an atom-centred tensor grid (gx, gy, gz:
+ * device coordinates relative to the atom).  Tables (host, nr rows sharing the
+ * ascending radii r): vloc, vl (lmax rows: V_l), rl (lmax rows: R_l).
+ *   dV_l = spline(V_l - V_loc), the difference formed on the table (:441-463);
+ *   phi_lm = spline(R_l) Y_lm, real harmonics of polar angle acos(z/(r+1e-12))
+ *   and azimuth atan2(y, x): m < 0: sqrt2 Im Y_l^|m|, m > 0: sqrt2 Re Y_l^m,
+ *   m = 0: Y_l^0, Condon-Shortley phase included (:527-604);
+ *   fields f = l^2 + l + m (l = 0..lmax-1, m = -l..l), each px py pz, x fastest.
+ * d_field (device, lmax^2 fields; may be NULL): dV_l phi_lm.
+ * d_phi (device, lmax^2 fields; may be NULL): phi_lm.
+ * clm (host, lmax^2; may be NULL): C_lm = sum phi_lm^2 dV_l wx wy wz with the
+ *   device weights d_wx, d_wy, d_wz (jac w of each axis); the reference prints
+ *   1 / C_lm and scales the projections by it.  The sum is exact on the
+ *   quadrature grid where the reference integrates a rank-rankNloc Tucker
+ *   form of the same field.
+ * 1 <= lmax <= 4.  replaces the constructor NonLocalPSPData::NonLocalPSPData
+ * (:33-212) up to its Tucker decompositions, computePhi_lm (:527-604) and
+ * computeC_lm (:609-645). */
+int ttdft_psp_projectors(ttdft_ctx ctx, int32_t nr, const double *r, const double *vloc,
+                         const double *vl, const double *rl, int32_t lmax, int32_t px,
+                         int32_t py, int32_t pz, const double *d_gx, const double *d_gy,
+                         const double *d_gz, const double *d_wx, const double *d_wy,
+                         const double *d_wz, double *d_field, double *d_phi, double *clm);
+
+/* ---- restart files ----------------------------------------------------------- */
+/* n x k block (device, column-major, leading dimension ldx) to or from a file
+ * of n k FP64 values in big-endian order with no header: the files
+ * PetscBinaryWrite / PetscBinaryRead of PETSC_DOUBLE data write and read
+ * (wavefunctions wfn_proc<rank>, main.cc:748-790, 1146-1170; local part
+ * potloc_proc<rank>, :541-592 with k = 1).  A file of another size is an
+ * error. */
+int ttdft_restart_write(ttdft_ctx ctx, const char *path, const double *d_x, int64_t n, int32_t k,
+                        int64_t ldx);
+int ttdft_restart_read(ttdft_ctx ctx, const char *path, double *d_x, int64_t n, int32_t k,
+                       int64_t ldx);
+/* cheby_bounds.txt: upper, lower_unwanted, lower_wanted, err_upper, occ_energy
+ * and the total energy, tab-separated, %.16e (main.cc:1150-1157, read back at
+ * :755-760).  Reading sets those five state fields and *energy (may be NULL). */
+int ttdft_restart_write_bounds(const char *path, const ttdft_chfsi_state *st, double energy);
+int ttdft_restart_read_bounds(const char *path, ttdft_chfsi_state *st, double *energy);
+
 /* ---- effective potential and grid utilities of the SCF loop ------------------ */
 enum {
     TTDFT_LDA_PZ81 = 0,          /* Slater exchange + Perdew-Zunger 1981 correlation   */

@@ -1,7 +1,8 @@
 """Copies input data files of the reference's example deck (example/) into
-tests/golden/example/: one axis of the 1-D basis functions, the atom file and
-two pseudopotential tables.  They are data (numbers the example run reads),
-used as fixtures for the readers and the 1-D matrices.
+tests/golden/example/: one axis of the 1-D basis functions, the atom file, the
+three pseudopotential tables of the atom type and its radial density.  They
+are data (numbers the example run reads), used as fixtures for the readers,
+the 1-D matrices and the pseudopotential set-up.
 
     python tests/golden/copy_example_inputs.py /root/reference
 """
@@ -9,7 +10,7 @@ import os
 import shutil
 import sys
 
-FILES = ["basis_x.txt", "shell1", "nlpV_AT0", "Density_AT0"]
+FILES = ["basis_x.txt", "shell1", "nlpV_AT0", "nlpWaveFun_AT0", "locPotential_AT0", "Density_AT0"]
 
 
 def main(ref):

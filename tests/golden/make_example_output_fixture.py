@@ -3,7 +3,8 @@
 The reference ships no tests; its only recorded results are the stdout of one
 converged 13-atom run (example/output).  For every SCF iteration this script
 collects the Ritz values that fed the Fermi-level solve, the Fermi level, the
-24 occupancies, the band energy, the total energy and the charge, and writes
+24 occupancies, the band energy, the total energy and the charge (plus the
+projector normalisations 1 / C_lm printed once at set-up), and writes
 them to ``example_output_scf.json`` next to this file.  The values are data
 (inputs and expected outputs), printed by the reference with %.18e.
 
@@ -14,6 +15,18 @@ import json
 import os
 import re
 import sys
+
+
+def parse_c_lm(path):
+    """The four "C_lm:" lines printed before the first rank (example/output:48-51):
+    1 / C_lm of the s and the three p projectors, atoms/NonLocalPSPData.cc:640-642."""
+    out = []
+    with open(path) as fh:
+        for ln in fh:
+            m = re.match(r"C_lm: ([-+0-9.eE]+)", ln)
+            if m:
+                out.append(float(m.group(1)))
+    return out
 
 
 def parse(path):
@@ -79,6 +92,7 @@ def main():
         "kb": 3.166811429e-06,
         "temperature": 500.0,
         "scf": scfs,
+        "inv_c_lm": parse_c_lm(src),
     }
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "example_output_scf.json")
     with open(out, "w") as fh:

@@ -164,6 +164,18 @@ _SIGS = {
     "ttdft_read_atom_file": (ctypes.c_int, [ctypes.c_char_p, _c_i32p, _c_i32p, _c_i32p, _c_i32p,
                                             _c_dp]),
     "ttdft_read_table": (ctypes.c_int, [ctypes.c_char_p, _c_i32p, _c_i32p, _c_dp]),
+    "ttdft_spline_parabolic": (ctypes.c_int, [_i32, _c_dp, _c_dp, _c_dp]),
+    "ttdft_psp_local": (ctypes.c_int, [_vp, _i32, _c_dp, _i32, _c_dp, _c_dp, _i32, _i32, _i32,
+                                       _vp, _vp, _vp, _vp, ctypes.c_int]),
+    "ttdft_psp_projectors": (ctypes.c_int, [_vp, _i32, _c_dp, _c_dp, _c_dp, _c_dp, _i32, _i32,
+                                            _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                            _c_dp]),
+    "ttdft_restart_write": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _i64, _i32, _i64]),
+    "ttdft_restart_read": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _i64, _i32, _i64]),
+    "ttdft_restart_write_bounds": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ChfsiState),
+                                                  _d]),
+    "ttdft_restart_read_bounds": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ChfsiState),
+                                                 _c_dp]),
     "ttdft_lda_xc": (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp, ctypes.c_int, ctypes.c_int]),
     "ttdft_local_pseudopotential": (ctypes.c_int, [_vp, _i32, _c_dp, _d, _i32, _i32, _i32, _vp,
                                                    _vp, _vp, _vp, ctypes.c_int]),
@@ -315,6 +327,30 @@ def read_table(path):
     out = np.empty((nr.value, nc.value))
     _check(lib().ttdft_read_table(p, ctypes.byref(nr), ctypes.byref(nc), _dptr(out)))
     return out
+
+
+def spline_parabolic(x, y):
+    """Cubic spline with parabolic end segments through (x, y): (n - 1, 4)
+    power-basis coefficients per segment in t = x - x_s."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    coef = np.empty((max(len(x) - 1, 1), 4))
+    _check(lib().ttdft_spline_parabolic(len(x), _dptr(x), _dptr(y), _dptr(coef)))
+    return coef
+
+
+def restart_write_bounds(path, state, energy):
+    """cheby_bounds.txt of main.cc:1150-1157 from a ChfsiState and the energy."""
+    _check(lib().ttdft_restart_write_bounds(os.fsencode(path), ctypes.byref(state), energy))
+
+
+def restart_read_bounds(path, state):
+    """Sets the five bound fields of state from cheby_bounds.txt; returns the
+    energy stored with them."""
+    e = ctypes.c_double()
+    _check(lib().ttdft_restart_read_bounds(os.fsencode(path), ctypes.byref(state),
+                                           ctypes.byref(e)))
+    return e.value
 
 
 def fermi_occupancy(evals, nelec, kb=3.166811429e-06, temperature=500.0, maxit=20):
@@ -482,6 +518,40 @@ class Context:
         _check(lib().ttdft_initial_density(self.h, len(xyz) // 3, _dptr(xyz), len(r), _dptr(r),
                                            _dptr(rho_r), pdims[0], pdims[1], pdims[2], dgx, dgy,
                                            dgz, d_rho))
+
+    def psp_local(self, atoms, r, vloc, dgx, dgy, dgz, pdims, d_v, add=False):
+        """Local pseudopotential of the atoms (rows of 7: Z, x, y, z, ...) from
+        the radial table (r, vloc) on the device grid; written or added."""
+        atoms = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1, 7)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        vloc = np.ascontiguousarray(vloc, dtype=np.float64)
+        _check(lib().ttdft_psp_local(self.h, len(atoms), _dptr(atoms), len(r), _dptr(r),
+                                     _dptr(vloc), pdims[0], pdims[1], pdims[2], dgx, dgy, dgz,
+                                     d_v, 1 if add else 0))
+
+    def psp_projectors(self, r, vloc, vl, rl, dgx, dgy, dgz, pdims, dwx=None, dwy=None,
+                       dwz=None, d_field=None, d_phi=None):
+        """dV_l phi_lm and phi_lm fields of one atom type on an atom-centred
+        device grid (written where given); returns the C_lm integrals when the
+        weights are given, else None."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        vloc = np.ascontiguousarray(vloc, dtype=np.float64)
+        vl = np.ascontiguousarray(np.atleast_2d(vl), dtype=np.float64)
+        rl = np.ascontiguousarray(np.atleast_2d(rl), dtype=np.float64)
+        lmax = vl.shape[0]
+        clm = np.empty(lmax * lmax) if dwx is not None else None
+        _check(lib().ttdft_psp_projectors(self.h, len(r), _dptr(r), _dptr(vloc), _dptr(vl),
+                                          _dptr(rl), lmax, pdims[0], pdims[1], pdims[2], dgx,
+                                          dgy, dgz, dwx, dwy, dwz, d_field, d_phi,
+                                          _dptr(clm) if clm is not None else None))
+        return clm
+
+    def restart_write(self, path, X, n, k, ldx):
+        """n x k device block (leading dimension ldx) to a big-endian FP64 file."""
+        _check(lib().ttdft_restart_write(self.h, os.fsencode(path), X, n, k, ldx))
+
+    def restart_read(self, path, X, n, k, ldx):
+        _check(lib().ttdft_restart_read(self.h, os.fsencode(path), X, n, k, ldx))
 
     def grid_integral(self, d_f, pdims, dwx, dwy, dwz):
         out = ctypes.c_double()
