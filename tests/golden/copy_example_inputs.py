@@ -1,5 +1,5 @@
 """Copies input data files of the reference's example deck (example/) into
-tests/golden/example/: one axis of the 1-D basis functions, the atom file, the
+tests/golden/example/: the 1-D basis functions of the three axes, the atom file, the
 three pseudopotential tables of the atom type and its radial density.  They
 are data (numbers the example run reads), used as fixtures for the readers,
 the 1-D matrices and the pseudopotential set-up.
@@ -10,7 +10,8 @@ import os
 import shutil
 import sys
 
-FILES = ["basis_x.txt", "shell1", "nlpV_AT0", "nlpWaveFun_AT0", "locPotential_AT0", "Density_AT0"]
+FILES = ["basis_x.txt", "basis_y.txt", "basis_z.txt", "shell1", "nlpV_AT0", "nlpWaveFun_AT0",
+         "locPotential_AT0", "Density_AT0"]
 
 
 def main(ref):

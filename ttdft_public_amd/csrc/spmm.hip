@@ -39,6 +39,11 @@ __global__ __launch_bounds__(256) void spmm_fused_kernel(
 #pragma unroll
     for (int t = 0; t < NC; ++t) acc[t] = T(0);
 
+    // U nonzeros per step: their X rows are loaded together before the
+    // multiply-adds, so a wave keeps U gathers in flight instead of one (the
+    // adds still run in the row's order).  Lanes past the chunk's count hold
+    // value 0 and column 0, so the last step reads row 0 and adds zeros.
+    constexpr int U = NC <= 2 ? 8 : (NC <= 4 ? 4 : 2);
     const int64_t e0 = rowptr[row], e1 = rowptr[row + 1];
     for (int64_t base = e0; base < e1; base += 64) {
         const int cnt = (int)((e1 - base) < 64 ? (e1 - base) : 64);
@@ -48,13 +53,20 @@ __global__ __launch_bounds__(256) void spmm_fused_kernel(
             v = val[base + lane];
             c = col[base + lane];
         }
-        for (int j = 0; j < cnt; ++j) {
-            const T a = bcast(v, j);
-            const T *xr = X + (int64_t)bcast(c, j) * ldx + lane;
+        const int steps = (cnt + U - 1) / U;
+        for (int s = 0; s < steps; ++s) {
+            T a[U], xv[U][NC];
 #pragma unroll
-            for (int t = 0; t < NC; ++t) {
-                if (lane + 64 * t < k) acc[t] += a * xr[64 * t];
+            for (int u = 0; u < U; ++u) {
+                a[u] = bcast(v, s * U + u);
+                const T *xr = X + (int64_t)bcast(c, s * U + u) * ldx + lane;
+#pragma unroll
+                for (int t = 0; t < NC; ++t) xv[u][t] = lane + 64 * t < k ? xr[64 * t] : T(0);
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int t = 0; t < NC; ++t) acc[t] += a[u] * xv[u][t];
         }
     }
     if (ct_rowptr != nullptr) {
