@@ -664,14 +664,18 @@ void drop_dense(ttdft_ham_s *H) {
     if (H->format == 2) H->format = 0;
 }
 
-// Dense storage when H_loc has no Kronecker band storage, n <= 32768 and at
-// least a fifth of the entries are nonzero: the CSR kernel then walks rows of
-// thousands of entries with the block's k states on one wavefront, while a
-// matrix product streams the n^2 values once at full bandwidth.
+// Dense storage when H_loc has no Kronecker band storage, at least a fifth of
+// its entries are nonzero and the FP64 matrix takes at most 48 GiB (n up to
+// about 80,000): the CSR kernel then walks rows of thousands of entries with
+// the block's k states on one wavefront, while a matrix product streams the
+// n^2 values once at full bandwidth.
+constexpr double kDenseMaxBytes = 48.0 * 1024 * 1024 * 1024;
 int maybe_build_dense(ttdft_ctx ctx, ttdft_ham_s *H, int copies) {
     (void)copies;
     const int64_t n = H->n;
-    if (H->format != 0 || n > 32768 || n < 1 || (double)H->nnz < 0.2 * (double)n * n) return TTDFT_OK;
+    if (H->format != 0 || n < 1 || 8.0 * (double)n * (double)n > kDenseMaxBytes ||
+        (double)H->nnz < 0.2 * (double)n * n)
+        return TTDFT_OK;
     // rows padded to 16 and the leading dimension to 32 (the product's tiles)
     const int64_t ld = (n + 31) / 32 * 32;
     const size_t n2 = (size_t)((n + 15) / 16 * 16) * (size_t)ld;
