@@ -599,6 +599,44 @@ def bench_large_step(ctx, E, with_density=True):
     return out
 
 
+def bench_example_deck(ctx):
+    """The reference's example/ deck (its own input files, kept as fixtures under
+    tests/golden/example) through the engine from set-up to convergence, against the
+    converged iteration the reference printed."""
+    from ttdft_public_amd.example_deck import ExampleDeck
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    deck_dir = os.path.join(here, "tests", "golden", "example")
+    with open(os.path.join(here, "tests", "golden", "example_output_scf.json")) as fh:
+        ref_scf = json.load(fh)["scf"]
+    ref = ref_scf[-1]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    deck = ExampleDeck(ctx, deck_dir)
+    out = deck.run()
+    ctx.synchronize()
+    seconds = time.perf_counter() - t0
+    last = out["records"][-1]
+    res = {
+        "workload": "the reference's example deck: 13 atoms, tabulated pseudopotentials "
+                    "(s, p), rank-40 file bases (64,000 functions, H stored dense), 400^3 "
+                    "quadrature grid, 24 states, degree 20, to residual 5e-7",
+        "seconds_setup_to_converged": round(seconds, 2),
+        "iterations": out["iterations"],
+        "converged": out["converged"],
+        "total_energy_Ha": last["total_energy"],
+        "reference_total_energy_Ha": ref["total_energy"],
+        "reference_iterations": len(ref_scf),
+        "delta_total_energy_per_atom_Ha": (last["total_energy"] - ref["total_energy"])
+                                          / deck.natoms,
+        "max_delta_ritz_Ha": float(np.max(np.abs(np.array(last["evals"])
+                                                 - np.array(ref["passes"][-1])))),
+        "delta_fermi_Ha": last["fermi"] - ref["fermi"],
+    }
+    log(f"example deck: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -627,6 +665,8 @@ def main():
                     help="skip the H assembly measurement")
     ap.add_argument("--no-scf", action="store_true", help="skip the 13-atom SCF measurement")
     ap.add_argument("--no-scf55", action="store_true", help="skip the 55-atom SCF measurement")
+    ap.add_argument("--no-deck", action="store_true",
+                    help="skip the reference's example deck run end to end")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the 500-atom ChFSI step (N = 343,000, k = 900, m = 40)")
     args = ap.parse_args()
@@ -904,6 +944,8 @@ def main():
         result["scf55"] = bench_scf(ctx, E, 55)
     if not args.no_large:
         result["large_step"] = bench_large_step(ctx, E, not args.no_density)
+    if not args.no_deck:
+        result["example_deck"] = bench_example_deck(ctx)
     ctx.synchronize()
     print(json.dumps(result))
 
