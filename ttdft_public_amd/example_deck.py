@@ -70,6 +70,8 @@ class DeckConfig:
     hloc_truncation: float = 1e-8  # command-line arguments of the recorded run: not recorded
     cnloc_truncation: float = 1e-8
     seed: int = 20261202
+    mixed: bool = False            # FP32 filter + split-BF16 GEMMs, then FP64 refinement
+    refine_below: float = 1e-4     # residual at which a mixed run switches to FP64
 
 
 def _dev(ctx, a):
@@ -249,7 +251,7 @@ class ExampleDeck:
                               tuple(a.ne for a in ax), [a.d_xq.ptr for a in ax], self.p,
                               self.alpha, self.omega, d_vh.ptr)
 
-    def hamiltonian(self, d_veff):
+    def hamiltonian(self, d_veff, copies=E.HAM_F64):
         ctx, cfg = self.ctx, self.cfg
         rv = cfg.veff_rank
         if not hasattr(self, "_v_bufs"):
@@ -263,7 +265,7 @@ class ExampleDeck:
         hloc = (a[0].MatX, a[0].MatDX, a[1].MatX, a[1].MatDX, a[2].MatX, a[2].MatDX, PX, PY, PZ,
                 G)
         return ctx.ham_assemble_tucker(hloc, cfg.hloc_truncation, self.proj,
-                                       cfg.cnloc_truncation, self.inv_c_lm)
+                                       cfg.cnloc_truncation, self.inv_c_lm, copies)
 
     # ---- the loop --------------------------------------------------------------------------
     def run(self, callback=None, max_iter=None):
@@ -279,6 +281,12 @@ class ExampleDeck:
         mixer = E.Mixer(ctx, n, nn, self.d_jw3.ptr, self.d_one.ptr, cfg.alpha, cfg.history)
         dX = E.DeviceArray(ctx, (N * k,))
         ctx.random_subspace(dX.ptr, k, N, k, seed=cfg.seed)
+        dt = E.F64
+        if cfg.mixed:  # the same start block, rounded to FP32
+            dX32 = E.DeviceArray(ctx, (N * k,), np.float32)
+            E._check(E.lib().ttdft_block_convert(ctx.h, dX.ptr, k, E.F64, dX32.ptr, k, E.F32, N,
+                                                 k))
+            dt = E.F32
         ev = np.zeros(k)
         st = E.new_state()
         occ_index = min(self.nelec // 2 + 10, k - 1)
@@ -292,19 +300,26 @@ class ExampleDeck:
             ctx.copy(d_veff.ptr, self.d_vloc.ptr, 8 * n)
             ctx.grid_axpby(d_veff.ptr, 1.0, d_vh.ptr, 1.0, n)
             ctx.grid_axpby(d_veff.ptr, 1.0, d_vxc.ptr, 1.0, n)
-            ham = self.hamiltonian(d_veff)
+            if dt == E.F32 and recs and recs[-1]["residual"] < cfg.refine_below:
+                # FP64 refinement from here on (the FP32 block widened to FP64)
+                E._check(E.lib().ttdft_block_convert(ctx.h, dX32.ptr, k, E.F32, dX.ptr, k, E.F64,
+                                                     N, k))
+                dt = E.F64
+            # the Lanczos bounds always run on the FP64 copy
+            ham = self.hamiltonian(d_veff, E.HAM_F64 if dt == E.F64 else E.HAM_F64 | E.HAM_F32)
             h_n, h_nnz, h_p = ham.info()
-            ctx.chfsi_step(ham, dX.ptr, k, k, cfg.degree, ev, st, it, occ_index,
+            dXc = dX if dt == E.F64 else dX32
+            ctx.chfsi_step(ham, dXc.ptr, k, k, cfg.degree, ev, st, it, occ_index,
                            cfg.restart_first if it == 0 else cfg.restart_other,
-                           maxit_lanczos=cfg.lanczos_iter)
+                           maxit_lanczos=cfg.lanczos_iter, precision=dt)
             passes = st.passes
             ham.free()
             ef, occ = E.fermi_occupancy(ev, self.nelec, KB_HARTREE, cfg.temperature)
             e_band = float(E.band_energy(ev, occ))
-            ctx.density(dX.ptr, k, k, occ, self.R, *(a.d_B.ptr for a in self.axes), self.p,
-                        d_out.ptr)
-            ctx.density(dX.ptr, k, k, occ, self.R, *(a.d_nodal.ptr for a in self.axes),
-                        self.pn, d_out_n.ptr)
+            ctx.density(dXc.ptr, k, k, occ, self.R, *(a.d_B.ptr for a in self.axes), self.p,
+                        d_out.ptr, dtype=dt)
+            ctx.density(dXc.ptr, k, k, occ, self.R, *(a.d_nodal.ptr for a in self.axes),
+                        self.pn, d_out_n.ptr, dtype=dt)
             charge = self.integral(d_out)
             # total energy at rho_out (main.cc:1368-1401)
             self.hartree(d_out_n, d_vh_out)
@@ -317,7 +332,8 @@ class ExampleDeck:
             mixer.update(d_in_n.ptr, d_in.ptr, d_out_n.ptr, d_out.ptr, it)
             rec = {"iter": it, "evals": ev.copy().tolist(), "fermi": ef, "band_energy": e_band,
                    "total_energy": energy, "charge_out": charge, "residual": res,
-                   "passes": passes, "nnz": h_nnz, "seconds": time.perf_counter() - t}
+                   "passes": passes, "nnz": h_nnz, "fp32": dt == E.F32,
+                   "seconds": time.perf_counter() - t, "elapsed": time.perf_counter() - t_start}
             recs.append(rec)
             if callback:
                 callback(rec)
@@ -339,9 +355,12 @@ def main(argv=None):
     ap.add_argument("--max-iter", type=int, default=None)
     ap.add_argument("--hloc-truncation", type=float, default=DeckConfig.hloc_truncation)
     ap.add_argument("--cnloc-truncation", type=float, default=DeckConfig.cnloc_truncation)
+    ap.add_argument("--mixed", action="store_true",
+                    help="FP32 filter and split-BF16 GEMMs, FP64 below residual 1e-4")
     ap.add_argument("--out", default=None, help="write the records as JSON here")
     a = ap.parse_args(argv)
-    cfg = DeckConfig(hloc_truncation=a.hloc_truncation, cnloc_truncation=a.cnloc_truncation)
+    cfg = DeckConfig(hloc_truncation=a.hloc_truncation, cnloc_truncation=a.cnloc_truncation,
+                     mixed=a.mixed)
     ctx = E.Context(0)
     deck = ExampleDeck(ctx, a.deck, cfg)
     print(f"set-up {deck.setup_seconds:.1f} s; 1/C_lm = {np.round(1.0 / deck.c_lm, 9).tolist()}",
