@@ -460,6 +460,62 @@ def bench_scf(ctx, E, size=13):
     return out
 
 
+def live_traffic(timeout=300):
+    """HBM bytes per launch of the FP64 band SpMM, measured now: two rocprofv3
+    counter passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --mode spmm` in
+    child processes started before this process touches the GPU.  That mode runs
+    3 FP64 H applications and 3 copies of one 125000 x 300 FP64 block; each
+    counter is scaled by what it reads for the copy (a known 3e8 bytes each
+    way).  Returns (bytes or None, detail dict)."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, {"error": "rocprofv3 not on PATH"}
+    moved = DIMS[0] * DIMS[1] * DIMS[2] * K * 8
+    detail = {"command": "rocprofv3 --pmc <FETCH_SIZE | WRITE_SIZE> --output-format csv -- "
+                         "python3 bench.py --mode spmm (one pass per counter)",
+              "calibration": f"copy_block_kernel<double,double> of {moved} B"}
+    total = 0.0
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="ttdft_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", str(timeout), exe, "--pmc", counter, "--output-format",
+               "csv", "-d", d, "-o", "spmm", "--", sys.executable,
+               os.path.join(ROOT, "bench.py"), "--mode", "spmm"]
+        log(f"traffic: {counter} counter pass")
+        r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                           stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                           timeout=timeout + 60)
+        files = glob.glob(os.path.join(d, "**", "spmm_counter_collection.csv"), recursive=True)
+        if not files:
+            shutil.rmtree(d, ignore_errors=True)
+            detail["error"] = f"{counter} pass exited {r.returncode} without counter output"
+            return None, detail
+        import csv
+        spmm, copy = [], []
+        with open(files[0]) as fh:
+            for row in csv.DictReader(fh):
+                name = row.get("Kernel_Name", "")
+                if "brick_spmm_kernel<double" in name:
+                    spmm.append(float(row["Counter_Value"]))
+                elif "copy_block_kernel<double, double>" in name:
+                    copy.append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if not spmm or not copy:
+            detail["error"] = f"{counter} pass: kernels missing from the counter output"
+            return None, detail
+        factor = moved / (sum(copy) / len(copy) * 1024.0)
+        bytes_ = sum(spmm) / len(spmm) * 1024.0 * factor
+        detail[counter] = {"raw_KiB_per_launch": round(sum(spmm) / len(spmm), 1),
+                           "copy_KiB": round(sum(copy) / len(copy), 1),
+                           "factor": round(factor, 4), "bytes_per_launch": round(bytes_)}
+        total += bytes_
+    return total, detail
+
+
 def profiled_traffic(kernel_tag):
     """HBM bytes per launch of the FP64 band SpMM from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, calibrated on a block copy)."""
@@ -696,6 +752,8 @@ def main():
                     help="skip the H assembly measurement")
     ap.add_argument("--no-scf", action="store_true", help="skip the 13-atom SCF measurement")
     ap.add_argument("--no-scf55", action="store_true", help="skip the 55-atom SCF measurement")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--no-deck", action="store_true",
                     help="skip the reference's example deck run end to end")
     ap.add_argument("--no-large", action="store_true",
@@ -737,6 +795,20 @@ def main():
         ctx = E.Context(0)
         log(json.dumps(bench_example_deck(ctx)))
         return
+
+    # roofline.traffic: counter passes in child processes, before this process
+    # uses the GPU
+    traffic, traffic_source = None, "not measured (--no-traffic)"
+    if args.mode == "step" and not args.no_traffic:
+        traffic, tdetail = live_traffic()
+        if traffic is not None:
+            traffic_source = {"measured": "live, this run", **tdetail}
+        else:
+            traffic = profiled_traffic("brick_spmm_kernel<double")
+            traffic_source = {"measured": "committed summary "
+                              "profiles/spmm_band_fp64_172atom_hbm_traffic.txt (the live "
+                              "counter pass failed)", **tdetail}
+        log(f"traffic: {traffic} ({traffic_source})")
 
     t0 = time.perf_counter()
     # h = 0.559 bohr puts the top of the spectrum near 48 Ha, the Lanczos upper
@@ -855,7 +927,8 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(spmm_flops / (spmm_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
-            "traffic": profiled_traffic("brick_spmm_kernel<double"),
+            "traffic": traffic,
+            "traffic_source": traffic_source,
             "launch_ms": round(spmm_ms, 4),
             "flops_per_launch": spmm_flops,
             "bytes_per_launch": spmm_bytes,

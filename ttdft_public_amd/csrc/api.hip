@@ -306,7 +306,7 @@ int dev_alloc(ttdft_ctx ctx, void **p, size_t bytes) {
     const size_t c = block_class(bytes);
     auto it = ctx->blk_free.find(c);
     if (it != ctx->blk_free.end()) {
-        *p = it->second;
+        *p = it->second.second;
         ctx->blk_free.erase(it);
         ctx->blk_free_bytes -= c;
     } else {
@@ -334,11 +334,21 @@ void dev_release(ttdft_ctx ctx, void *p) {
     }
     const size_t c = it->second;
     ctx->blk_size.erase(it);
-    if (ctx->blk_free_bytes + c > kBlockCacheCap) {
+    if (c > kBlockCacheCap) {
         dfree(p);
         return;
     }
-    ctx->blk_free.emplace(c, p);
+    // over the cap: give back the blocks released longest ago (an iterative
+    // caller reuses what it released last; hipFree waits for pending work)
+    while (ctx->blk_free_bytes + c > kBlockCacheCap && !ctx->blk_free.empty()) {
+        auto oldest = ctx->blk_free.begin();
+        for (auto b = ctx->blk_free.begin(); b != ctx->blk_free.end(); ++b)
+            if (b->second.first < oldest->second.first) oldest = b;
+        dfree(oldest->second.second);
+        ctx->blk_free_bytes -= oldest->first;
+        ctx->blk_free.erase(oldest);
+    }
+    ctx->blk_free.emplace(c, std::make_pair(++ctx->blk_seq, p));
     ctx->blk_free_bytes += c;
 }
 
@@ -347,7 +357,7 @@ void dev_cache_clear(ttdft_ctx ctx) {
         hipError_t e = hipStreamSynchronize(ctx->stream);  // pending work may still use them
         (void)e;
     }
-    for (auto &b : ctx->blk_free) dfree(b.second);
+    for (auto &b : ctx->blk_free) dfree(b.second.second);
     ctx->blk_free.clear();
     ctx->blk_free_bytes = 0;
 }

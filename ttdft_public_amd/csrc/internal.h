@@ -99,10 +99,11 @@ struct ttdft_ctx_s {
     int *d_info = nullptr;  // rocSOLVER info
     // device blocks handed out by dev_alloc (pointer -> size class) and the
     // released ones kept for reuse on this context's stream (size class ->
-    // blocks); see dev_alloc
+    // (release number, block)); see dev_alloc
     std::unordered_map<void *, size_t> blk_size;
-    std::multimap<size_t, void *> blk_free;
+    std::multimap<size_t, std::pair<uint64_t, void *>> blk_free;
     size_t blk_free_bytes = 0;
+    uint64_t blk_seq = 0;
     // Hamiltonians and mixers created on this context and not yet freed.
     // ttdft_ctx_destroy with any alive only marks the context; the last
     // ttdft_ham_free / ttdft_mixer_destroy then destroys it.
