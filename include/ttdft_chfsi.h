@@ -105,6 +105,12 @@ int ttdft_ctx_destroy(ttdft_ctx ctx);
  * blocks are reused in the order of one stream. */
 int ttdft_ctx_set_stream(ttdft_ctx ctx, void *hip_stream);
 int ttdft_ctx_synchronize(ttdft_ctx ctx);
+/* Returns the context's released storage blocks to the HIP runtime (up to
+ * 64 GiB are kept for reuse by later Hamiltonians and mixers), after the
+ * context's stream has finished with them: for a caller that shares the
+ * device with another allocator (torch's, say).  No counterpart in the
+ * reference, whose storage is PETSc's. */
+int ttdft_ctx_trim(ttdft_ctx ctx);
 /* Enable per-launch HIP-event timing of the kernel classes TTDFT_KSTAT_*. */
 int ttdft_ctx_set_profiling(ttdft_ctx ctx, int enable);
 /* Launch count, summed milliseconds and algorithmic bytes / flops recorded
@@ -573,16 +579,17 @@ int ttdft_psp_projectors(ttdft_ctx ctx, int32_t nr, const double *r, const doubl
                          const double *d_wz, double *d_field, double *d_phi, double *clm);
 
 /* ---- restart files ----------------------------------------------------------- */
-/* n x k block (device, column-major, leading dimension ldx) to or from a file
- * of n k FP64 values in big-endian order with no header: the files
- * PetscBinaryWrite / PetscBinaryRead of PETSC_DOUBLE data write and read
- * (wavefunctions wfn_proc<rank>, main.cc:748-790, 1146-1170; local part
- * potloc_proc<rank>, :541-592 with k = 1).  A file of another size is an
- * error. */
-int ttdft_restart_write(ttdft_ctx ctx, const char *path, const double *d_x, int64_t n, int32_t k,
-                        int64_t ldx);
-int ttdft_restart_read(ttdft_ctx ctx, const char *path, double *d_x, int64_t n, int32_t k,
-                       int64_t ldx);
+/* An X block (device, basis-major, ld >= k, TTDFT_F64 or TTDFT_F32) to or
+ * from a file of its n k values in the reference's column-major order (the n
+ * coefficients of state 0, then of state 1, ...) as big-endian FP64 with no
+ * header: the files PetscBinaryWrite / PetscBinaryRead of PETSC_DOUBLE data
+ * write and read from a MATDENSE array (wavefunctions wfn_proc<rank>,
+ * main.cc:748-790, 1146-1170).  A grid field is the case k = 1, ldx = 1 (local
+ * part potloc_proc<rank>, :541-592).  A file of another size is an error. */
+int ttdft_restart_write(ttdft_ctx ctx, const char *path, const void *dX, int64_t ldx, int dtype,
+                        int64_t n, int32_t k);
+int ttdft_restart_read(ttdft_ctx ctx, const char *path, void *dX, int64_t ldx, int dtype,
+                       int64_t n, int32_t k);
 /* cheby_bounds.txt: upper, lower_unwanted, lower_wanted, err_upper, occ_energy
  * and the total energy, tab-separated, %.16e (main.cc:1150-1157, read back at
  * :755-760).  Reading sets those five state fields and *energy (may be NULL). */

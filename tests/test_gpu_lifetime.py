@@ -5,6 +5,7 @@
   its block cache).
 * Switching the context's stream waits for the outgoing stream, so storage
   released on it and reused on the new stream is never overwritten early.
+* Trimming the context's block cache between Hamiltonians.
 * A filter split into column blocks whose second block starts at an odd
   column takes the dense product's unpaired path and still matches the
   oracle (ChebyShevFilter.cc:57-60 chunking).
@@ -84,6 +85,27 @@ def test_stream_switch_then_rebuild():
     dY.free()
     c.close()
     assert hip.hipStreamDestroy(st) == 0
+
+
+def test_trim_between_hamiltonians():
+    """ttdft_ctx_trim gives the cached blocks back; the next Hamiltonian allocates anew and
+    its product still matches the oracle."""
+    c = E.Context(0)
+    s1 = E.SynthHam(10, 9, 8, 3, 3, 3, 4, seed=21)
+    ham = c.ham_upload(s1, E.HAM_F64)
+    X = np.random.default_rng(4).standard_normal((s1.n, 24))
+    dX, dY = E.DeviceArray.from_host(c, X), E.DeviceArray(c, (s1.n, 24))
+    c.ham_apply(ham, dX.ptr, 24, dY.ptr, 24, 24, E.F64)  # queued work on the storage
+    ham.free()
+    c.trim()
+    s2 = E.SynthHam(10, 9, 8, 3, 3, 3, 4, seed=22)
+    ham2 = c.ham_upload(s2, E.HAM_F64)
+    c.ham_apply(ham2, dX.ptr, 24, dY.ptr, 24, 24, E.F64)
+    assert rel(dY.to_host(), oracle.ham_apply(oracle.Ham.from_synth(s2), X)) < 1e-13
+    ham2.free()
+    dX.free()
+    dY.free()
+    c.close()
 
 
 @pytest.mark.parametrize("dtype", [E.F64, E.F32])

@@ -135,21 +135,36 @@ def test_psp_local_matches_oracle(ctx):
 
 
 def test_restart_block_round_trip(ctx, tmp_path):
-    print("restart_write/read: 1000 x 7 block, ldx 1003", flush=True)
-    n, k, ldx = 1000, 7, 1003
+    print("restart_write/read: 1000 x 7 basis-major block, ld 9; FP32; a grid field", flush=True)
+    n, k, ld = 1000, 7, 9
     rng = np.random.default_rng(11)
-    X = rng.standard_normal((k, ldx))  # column-major: one column per row here
+    X = rng.standard_normal((n, ld))  # basis-major: X[i, j] = state j of function i
     dX = dev(ctx, X)
     p = str(tmp_path / "wfn_proc0")
-    ctx.restart_write(p, dX.ptr, n, k, ldx)
+    ctx.restart_write(p, dX.ptr, ld, n, k)
     raw = open(p, "rb").read()
-    assert raw == np.ascontiguousarray(X[:, :n]).astype(">f8").tobytes()
+    # the reference's MATDENSE order: n values of state 0, then state 1, ... big-endian
+    assert raw == np.ascontiguousarray(X[:, :k].T).astype(">f8").tobytes()
     dY = E.DeviceArray(ctx, (n * k,))
-    ctx.restart_read(p, dY.ptr, n, k, n)
-    assert np.array_equal(dY.to_host().reshape(k, n), X[:, :n])
+    ctx.restart_read(p, dY.ptr, k, n, k)
+    assert np.array_equal(dY.to_host().reshape(n, k), X[:, :k])
+    # FP32 blocks are widened on the way out and rounded back exactly
+    X32 = X[:, :k].astype(np.float32)
+    d32 = E.DeviceArray.from_host(ctx, np.ascontiguousarray(X32))
+    ctx.restart_write(p, d32.ptr, k, n, k, dtype=E.F32)
+    assert open(p, "rb").read() == np.ascontiguousarray(X32.T).astype(">f8").tobytes()
+    back = E.DeviceArray(ctx, (n * k,), np.float32)
+    ctx.restart_read(p, back.ptr, k, n, k, dtype=E.F32)
+    assert np.array_equal(back.to_host().reshape(n, k), X32)
+    # a grid field (the local part's potloc file): k = 1, ld = 1
+    g = rng.standard_normal(4096)
+    dg = dev(ctx, g)
+    q = str(tmp_path / "potloc_proc0")
+    ctx.restart_write(q, dg.ptr, 1, 4096, 1)
+    assert open(q, "rb").read() == g.astype(">f8").tobytes()
     with pytest.raises(RuntimeError):
-        ctx.restart_read(p, dY.ptr, n, k - 1, n)  # file holds more
+        ctx.restart_read(p, dY.ptr, k - 1, n, k - 1)  # file holds more
     with pytest.raises(RuntimeError):
-        ctx.restart_read(p, dY.ptr, n + 1, k, n + 1)  # file holds fewer
+        ctx.restart_read(p, dY.ptr, k, n + 1, k)  # file holds fewer
     with pytest.raises(RuntimeError):
-        ctx.restart_read(str(tmp_path / "missing"), dY.ptr, n, k, n)
+        ctx.restart_read(str(tmp_path / "missing"), dY.ptr, k, n, k)

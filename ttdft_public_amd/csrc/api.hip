@@ -503,6 +503,13 @@ int ttdft_ctx_destroy(ttdft_ctx ctx) {
     return TTDFT_OK;
 }
 
+int ttdft_ctx_trim(ttdft_ctx ctx) {
+    TT_CHECK(ctx, TTDFT_EBADARG, "ctx is NULL");
+    TT_HIP(hipSetDevice(ctx->device));
+    dev_cache_clear(ctx);  // waits for the stream before the blocks go back
+    return TTDFT_OK;
+}
+
 int ttdft_ctx_set_stream(ttdft_ctx ctx, void *hip_stream) {
     TT_CHECK(ctx, TTDFT_EBADARG, "ctx is NULL");
     hipStream_t next = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;

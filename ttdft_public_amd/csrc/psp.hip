@@ -17,7 +17,8 @@
 //
 // Restart files (main.cc:541-592, 748-806, 1146-1170, 1212-1230): raw FP64 in
 // big-endian byte order without a header (PetscBinaryWrite / PetscBinaryRead
-// of PETSC_DOUBLE data), and cheby_bounds.txt with six values.
+// of PETSC_DOUBLE data) in the MATDENSE column-major order, converted from and
+// to the engine's basis-major blocks, and cheby_bounds.txt with six values.
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -348,29 +349,31 @@ extern "C" int ttdft_psp_projectors(ttdft_ctx ctx, int32_t nr, const double *r,
     return TTDFT_OK;
 }
 
-extern "C" int ttdft_restart_write(ttdft_ctx ctx, const char *path, const double *d_x, int64_t n,
-                                   int32_t k, int64_t ldx) {
+extern "C" int ttdft_restart_write(ttdft_ctx ctx, const char *path, const void *dX, int64_t ldx,
+                                   int dtype, int64_t n, int32_t k) {
     clear_error();
-    TT_CHECK(ctx && path && d_x && n >= 1 && k >= 1 && ldx >= n, TTDFT_EBADARG,
-             "restart_write: bad arguments");
+    TT_CHECK(ctx && path && dX && n >= 1 && k >= 1 && ldx >= k &&
+                 (dtype == TTDFT_F64 || dtype == TTDFT_F32),
+             TTDFT_EBADARG, "restart_write: bad arguments");
+    // the reference's column-major FP64 layout (n values of state 0, then state 1, ...)
     std::vector<double> h((size_t)n * k);
-    TT_HIP(hipMemcpy2DAsync(h.data(), sizeof(double) * n, d_x, sizeof(double) * ldx,
-                            sizeof(double) * n, k, hipMemcpyDeviceToHost, ctx->stream));
-    TT_HIP(hipStreamSynchronize(ctx->stream));
+    TT_TRY(ttdft_block_download_colmajor(ctx, dX, ldx, dtype, n, k, h.data(), n));
     swap_bytes(h.data(), h.size());
     FILE *f = std::fopen(path, "wb");
     TT_CHECK(f, TTDFT_EBADARG, "restart_write: cannot open %s: %s", path, std::strerror(errno));
     const size_t put = std::fwrite(h.data(), sizeof(double), h.size(), f);
     const int closed = std::fclose(f);
-    TT_CHECK(put == h.size() && closed == 0, TTDFT_EBADARG, "restart_write: short write to %s", path);
+    TT_CHECK(put == h.size() && closed == 0, TTDFT_EBADARG, "restart_write: short write to %s",
+             path);
     return TTDFT_OK;
 }
 
-extern "C" int ttdft_restart_read(ttdft_ctx ctx, const char *path, double *d_x, int64_t n,
-                                  int32_t k, int64_t ldx) {
+extern "C" int ttdft_restart_read(ttdft_ctx ctx, const char *path, void *dX, int64_t ldx,
+                                  int dtype, int64_t n, int32_t k) {
     clear_error();
-    TT_CHECK(ctx && path && d_x && n >= 1 && k >= 1 && ldx >= n, TTDFT_EBADARG,
-             "restart_read: bad arguments");
+    TT_CHECK(ctx && path && dX && n >= 1 && k >= 1 && ldx >= k &&
+                 (dtype == TTDFT_F64 || dtype == TTDFT_F32),
+             TTDFT_EBADARG, "restart_read: bad arguments");
     FILE *f = std::fopen(path, "rb");
     TT_CHECK(f, TTDFT_EBADARG, "restart_read: cannot open %s: %s", path, std::strerror(errno));
     std::vector<double> h((size_t)n * k);
@@ -381,10 +384,7 @@ extern "C" int ttdft_restart_read(ttdft_ctx ctx, const char *path, double *d_x, 
              "restart_read: %s holds %s than the %lld values asked for", path,
              extra ? "more" : "fewer", (long long)h.size());
     swap_bytes(h.data(), h.size());
-    TT_HIP(hipMemcpy2DAsync(d_x, sizeof(double) * ldx, h.data(), sizeof(double) * n,
-                            sizeof(double) * n, k, hipMemcpyHostToDevice, ctx->stream));
-    TT_HIP(hipStreamSynchronize(ctx->stream));  // host staging goes out of scope
-    return TTDFT_OK;
+    return ttdft_block_upload_colmajor(ctx, h.data(), n, n, k, dX, ldx, dtype);
 }
 
 extern "C" int ttdft_restart_write_bounds(const char *path, const ttdft_chfsi_state *st,

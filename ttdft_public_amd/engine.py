@@ -170,8 +170,11 @@ _SIGS = {
     "ttdft_psp_projectors": (ctypes.c_int, [_vp, _i32, _c_dp, _c_dp, _c_dp, _c_dp, _i32, _i32,
                                             _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                             _c_dp]),
-    "ttdft_restart_write": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _i64, _i32, _i64]),
-    "ttdft_restart_read": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _i64, _i32, _i64]),
+    "ttdft_restart_write": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _i64, ctypes.c_int, _i64,
+                                           _i32]),
+    "ttdft_restart_read": (ctypes.c_int, [_vp, ctypes.c_char_p, _vp, _i64, ctypes.c_int, _i64,
+                                          _i32]),
+    "ttdft_ctx_trim": (ctypes.c_int, [_vp]),
     "ttdft_restart_write_bounds": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ChfsiState),
                                                   _d]),
     "ttdft_restart_read_bounds": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ChfsiState),
@@ -452,6 +455,10 @@ class Context:
     def synchronize(self):
         _check(lib().ttdft_ctx_synchronize(self.h))
 
+    def trim(self):
+        """Give the context's cached storage blocks back to the HIP runtime."""
+        _check(lib().ttdft_ctx_trim(self.h))
+
     def set_profiling(self, on=True):
         _check(lib().ttdft_ctx_set_profiling(self.h, 1 if on else 0))
 
@@ -546,12 +553,13 @@ class Context:
                                           _dptr(clm) if clm is not None else None))
         return clm
 
-    def restart_write(self, path, X, n, k, ldx):
-        """n x k device block (leading dimension ldx) to a big-endian FP64 file."""
-        _check(lib().ttdft_restart_write(self.h, os.fsencode(path), X, n, k, ldx))
+    def restart_write(self, path, X, ldx, n, k, dtype=F64):
+        """Basis-major device block to the reference's column-major big-endian
+        FP64 restart file."""
+        _check(lib().ttdft_restart_write(self.h, os.fsencode(path), X, ldx, dtype, n, k))
 
-    def restart_read(self, path, X, n, k, ldx):
-        _check(lib().ttdft_restart_read(self.h, os.fsencode(path), X, n, k, ldx))
+    def restart_read(self, path, X, ldx, n, k, dtype=F64):
+        _check(lib().ttdft_restart_read(self.h, os.fsencode(path), X, ldx, dtype, n, k))
 
     def grid_integral(self, d_f, pdims, dwx, dwy, dwz):
         out = ctypes.c_double()
