@@ -79,3 +79,35 @@ def test_example_deck_mixed_precision_reaches_the_fp64_energy(deck_fp64):
     assert any(r["fp32"] for r in out["records"]) and not out["records"][-1]["fp32"]
     e64 = out64["records"][-1]["total_energy"]
     assert abs(out["records"][-1]["total_energy"] - e64) < 1e-5 * deck.natoms
+
+
+def test_deck_with_two_atom_types_matches_one(ctx, tmp_path):
+    """The same deck written with two atom types (the first 7 atoms type 0, the rest type 1,
+    each type with its own copy of the tables, as main.cc:501-519 reads them): three SCF
+    iterations give the one-type run's energies and Ritz values to rounding (the local part
+    and the initial density are summed per type, in another order)."""
+    import shutil
+
+    from ttdft_public_amd.example_deck import DeckConfig
+
+    for name in os.listdir(EX):
+        shutil.copy(os.path.join(EX, name), tmp_path / name)
+        if name.endswith("_AT0"):
+            shutil.copy(os.path.join(EX, name), tmp_path / (name[:-1] + "1"))
+    lines = open(os.path.join(EX, "shell1")).read().splitlines()
+    head = lines[0].split()
+    atoms = [ln for ln in lines[2:] if ln.strip()]
+    out = [f"2 {head[1]} {head[2]}", f"{lines[1].split()[0]} {lines[1].split()[0]}"]
+    for i, ln in enumerate(atoms):
+        cols = ln.rsplit(",", 1)
+        out.append(f"{cols[0]}, {0 if i < 7 else 1}.0")
+    (tmp_path / "shell2").write_text("\n".join(out) + "\n")
+    print("example deck as two atom types, 3 iterations", flush=True)
+    cfg = DeckConfig(max_iter=3)
+    one = ExampleDeck(ctx, EX, cfg).run(show)
+    two_deck = ExampleDeck(ctx, str(tmp_path), cfg, system="shell2")
+    assert list(two_deck.types) == [0] * 7 + [1] * 6
+    two = two_deck.run(show)
+    for a, b in zip(one["records"], two["records"]):
+        assert abs(a["total_energy"] - b["total_energy"]) < 1e-9
+        assert np.max(np.abs(np.array(a["evals"]) - np.array(b["evals"]))) < 1e-10

@@ -20,9 +20,9 @@ main.cc's SCF settings.  One iteration follows main.cc:846-1449:
      rho_out (main.cc:1368-1401) with V_H[rho_out] from the nodal rho_out;
   6. residual int (rho_out - rho_in)^2 and the mixer update.
 
-Set-up (main.cc:440-592, 674-680): V_loc by ttdft_psp_local; the projectors
-of each atom from the Tucker form (rank rankNloc) of dV_l phi_lm on the
-nonlocal mesh, contracted with the basis functions evaluated at the atom's
+Set-up (main.cc:440-592, 674-680), for every atom type with its own tables:
+V_loc by ttdft_psp_local; the projectors of each atom from the Tucker form
+(rank rankNloc) of its type's dV_l phi_lm on the nonlocal mesh, contracted with the basis functions evaluated at the atom's
 shifted nonlocal quadrature points (Create_Cnloc,
 ProjectHamiltonianSparse.cc:83-328), D = 1 / C_lm; the initial density from
 Density_AT0 renormalised to the electron count.
@@ -127,18 +127,18 @@ class DeckAxis:
 
 
 class ExampleDeck:
-    """Set-up and SCF of the example deck found in `deck` (a directory)."""
+    """Set-up and SCF of the deck found in `deck` (a directory): the atom file `system`,
+    basis_{x,y,z}.txt and the four tables of each atom type."""
 
-    def __init__(self, ctx, deck, cfg=None):
+    def __init__(self, ctx, deck, cfg=None, system="shell1"):
         self.ctx = ctx
         self.cfg = cfg = cfg or DeckConfig()
         t0 = time.perf_counter()
         f = lambda name: os.path.join(deck, name)  # noqa: E731
-        self.nelec, lmax, atoms = E.read_atom_file(f("shell1"))
-        if len(lmax) != 1:
-            raise ValueError("the deck runner handles one atom type")
-        self.lmax = int(lmax[0])
-        self.atoms = atoms
+        self.nelec, lmax, atoms = E.read_atom_file(f(system))
+        self.lmax = [int(v) for v in lmax]
+        self.atoms = atoms  # grouped by type, as all_nuclei
+        self.types = atoms[:, 6].astype(int)
         self.xyz = atoms[:, 1:4].copy()
         self.zval = atoms[:, 0].copy()
         self.natoms = len(atoms)
@@ -150,19 +150,29 @@ class ExampleDeck:
         self.ngrid = int(np.prod(self.p))
         self.nnodal = int(np.prod(self.pn))
         ax = self.axes
-        # tables of the atom type
-        V = E.read_table(f("nlpV_AT0"))
-        Rt = E.read_table(f("nlpWaveFun_AT0"))
-        L = E.read_table(f("locPotential_AT0"))
-        if not (np.array_equal(V[:, 0], Rt[:, 0]) and np.array_equal(V[:, 0], L[:, 0])):
-            raise ValueError("pseudopotential tables on different radial grids")
-        r, vloc = L[:, 0], L[:, 1]
-        vl = V[:, 1:1 + self.lmax].T.copy()
-        rl = Rt[:, 1:1 + self.lmax].T.copy()
-        # local part on the quadrature grid (computeLocalPart)
+        # tables of each atom type (main.cc:501-519: locPotential_AT<i>, nlpV_AT<i>,
+        # nlpWaveFun_AT<i>; Density_AT<i> for the initial density)
+        self.tables = []
+        for t in range(len(self.lmax)):
+            V = E.read_table(f(f"nlpV_AT{t}"))
+            Rt = E.read_table(f(f"nlpWaveFun_AT{t}"))
+            L = E.read_table(f(f"locPotential_AT{t}"))
+            if not (np.array_equal(V[:, 0], Rt[:, 0]) and np.array_equal(V[:, 0], L[:, 0])):
+                raise ValueError(f"type {t}: pseudopotential tables on different radial grids")
+            lm = self.lmax[t]
+            self.tables.append((L[:, 0], L[:, 1], V[:, 1:1 + lm].T.copy(),
+                                Rt[:, 1:1 + lm].T.copy(), E.read_table(f(f"Density_AT{t}"))))
+        # local part on the quadrature grid (computeLocalPart, summed over the types)
         self.d_vloc = E.DeviceArray(ctx, (self.ngrid,))
-        ctx.psp_local(atoms, r, vloc, *(a.d_xq.ptr for a in ax), self.p, self.d_vloc.ptr)
-        self.proj, self.inv_c_lm, self.c_lm = self._projectors(r, vloc, vl, rl)
+        first = True
+        for t, (r, vloc, _, _, _) in enumerate(self.tables):
+            sel = self.types == t
+            if sel.any():
+                ctx.psp_local(atoms[sel], r, vloc, *(a.d_xq.ptr for a in ax), self.p,
+                              self.d_vloc.ptr, add=not first)
+                first = False
+        self.proj, self.inv_c_lm, self.c_lm_by_type = self._projectors()
+        self.c_lm = self.c_lm_by_type[0]
         # weights
         self.d_jw3 = E.DeviceArray(ctx, (self.ngrid,))
         ctx.grid_weights(self.p, *(a.d_jw.ptr for a in ax), self.d_jw3.ptr)
@@ -173,54 +183,72 @@ class ExampleDeck:
         rr = np.sqrt((d ** 2).sum(-1)) + np.eye(self.natoms)
         self.e_nn = 0.5 * float(((self.zval[:, None] * self.zval[None, :]) / rr)
                                 [~np.eye(self.natoms, dtype=bool)].sum())
-        self.density_table = E.read_table(f("Density_AT0"))
         self.setup_seconds = time.perf_counter() - t0
 
     # ---- set-up pieces -----------------------------------------------------------------
-    def _projectors(self, r, vloc, vl, rl):
-        """Tucker form (rank rankNloc) of every dV_l phi_lm on the nonlocal mesh and
-        each atom's projector factors mx[I, a] = sum_q B_I(R_x + x_q) U_a(x_q) jw_q."""
+    def _projectors(self):
+        """Tucker form (rank rankNloc) of every dV_l phi_lm of each type on the nonlocal
+        mesh, and each atom's projector factors mx[I, a] = sum_q B_I(R_x + x_q) U_a(x_q) jw_q,
+        atoms in all_nuclei order, (l, m) within an atom."""
         ctx, cfg = self.ctx, self.cfg
         xi, w = E.quadrature_rule(cfg.quad)
         nl_nodes = E.fem_mesh("uniform", cfg.nl_nel, cfg.npe, -cfg.nl_radius, cfg.nl_radius)
         jw_nl, _, xq_nl = ctx.fem_interpolate(cfg.nl_nel, cfg.npe, xi, w, nl_nodes)
         m = len(xq_nl)
-        d_g, d_w = _dev(ctx, xq_nl), _dev(ctx, jw_nl)
-        F = self.lmax * self.lmax
         npts = m ** 3
-        d_field = E.DeviceArray(ctx, (F * npts,))
-        c_lm = ctx.psp_projectors(r, vloc, vl, rl, d_g.ptr, d_g.ptr, d_g.ptr, (m, m, m), d_w.ptr,
-                                  d_w.ptr, d_w.ptr, d_field=d_field.ptr)
+        d_g, d_w = _dev(ctx, xq_nl), _dev(ctx, jw_nl)
         rk = cfg.rank_nloc
         d_core = E.DeviceArray(ctx, (rk ** 3,))
         d_U = [E.DeviceArray(ctx, (m * rk,)) for _ in range(3)]
-        forms = []
-        for fi in range(F):
-            ctx.sthosvd(d_field.ptr + 8 * fi * npts, (m, m, m), (rk,) * 3, d_core.ptr,
-                        [u.ptr for u in d_U])
-            G = d_core.to_host().reshape(rk, rk, rk).transpose(2, 1, 0)  # [a, b, c]
-            U = [u.to_host().reshape(rk, m) for u in d_U]  # rows: factor functions
-            forms.append((G, U))
-        cores, mx, my, mz = [], [], [], []
+        forms, c_lms = [], []
+        for t, (r, vloc, vl, rl, _) in enumerate(self.tables):
+            F = self.lmax[t] ** 2
+            d_field = E.DeviceArray(ctx, (F * npts,))
+            c_lms.append(ctx.psp_projectors(r, vloc, vl, rl, d_g.ptr, d_g.ptr, d_g.ptr,
+                                            (m, m, m), d_w.ptr, d_w.ptr, d_w.ptr,
+                                            d_field=d_field.ptr))
+            ft = []
+            for fi in range(F):
+                ctx.sthosvd(d_field.ptr + 8 * fi * npts, (m, m, m), (rk,) * 3, d_core.ptr,
+                            [u.ptr for u in d_U])
+                G = d_core.to_host().reshape(rk, rk, rk).transpose(2, 1, 0)  # [a, b, c]
+                U = [u.to_host().reshape(rk, m) for u in d_U]  # rows: factor functions
+                ft.append((G, U))
+            forms.append(ft)
+        cores, mx, my, mz, inv = [], [], [], [], []
         for a in range(self.natoms):
+            t = self.types[a]
             Bq = [eval_nodal(ax.nodes, self.cfg.npe, ax.basis, self.xyz[a, d] + xq_nl)
                   for d, ax in enumerate(self.axes)]  # (R, m) per axis
-            for G, U in forms:
+            for G, U in forms[t]:
                 cores.append(G)
                 mx.append(Bq[0] @ (U[0] * jw_nl).T)
                 my.append(Bq[1] @ (U[1] * jw_nl).T)
                 mz.append(Bq[2] @ (U[2] * jw_nl).T)
-        inv_c_lm = np.tile(1.0 / c_lm, self.natoms)
-        return (cores, mx, my, mz), inv_c_lm, c_lm
+            inv.extend(1.0 / c_lms[t])
+        return (cores, mx, my, mz), np.array(inv), c_lms
 
     def initial_density(self, d_grid, d_nodal):
-        """Superposition of Density_AT0 at the quadrature points and the nodes, both
-        scaled so that the quadrature grid holds the electron count (main.cc:674-680)."""
+        """Superposition of each type's Density_AT<i> over its atoms at the quadrature
+        points and the nodes, both scaled so that the quadrature grid holds the electron
+        count (main.cc:674-680)."""
         ctx, ax = self.ctx, self.axes
-        r, rho_r = self.density_table[:, 0], self.density_table[:, 1]
-        ctx.initial_density(self.xyz, r, rho_r, *(a.d_xq.ptr for a in ax), self.p, d_grid.ptr)
-        ctx.initial_density(self.xyz, r, rho_r, *(a.d_nodes.ptr for a in ax), self.pn,
-                            d_nodal.ptr)
+        targets = ((d_grid, [a.d_xq for a in ax], self.p, self.ngrid),
+                   (d_nodal, [a.d_nodes for a in ax], self.pn, self.nnodal))
+        first, tmp = True, None
+        for t, (_, _, _, _, dens) in enumerate(self.tables):
+            sel = self.types == t
+            if not sel.any():
+                continue
+            if not first and tmp is None:
+                tmp = [E.DeviceArray(ctx, (count,)) for _, _, _, count in targets]
+            for i, (d_out, pts, dims, count) in enumerate(targets):
+                dst = d_out if first else tmp[i]  # the first type writes, later ones add
+                ctx.initial_density(self.xyz[sel], dens[:, 0], dens[:, 1],
+                                    *(g.ptr for g in pts), dims, dst.ptr)
+                if not first:
+                    ctx.grid_axpby(d_out.ptr, 1.0, dst.ptr, 1.0, count)
+            first = False
         q = self.integral(d_grid)
         s = self.nelec / q
         ctx.grid_scale(d_grid.ptr, self.ngrid, s)
