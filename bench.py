@@ -460,7 +460,7 @@ def bench_scf(ctx, E, size=13):
     return out
 
 
-def live_traffic(timeout=300):
+def live_traffic(timeout=180):
     """HBM bytes per launch of the FP64 band SpMM, measured now: two rocprofv3
     counter passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --mode spmm` in
     child processes started before this process touches the GPU.  That mode runs
